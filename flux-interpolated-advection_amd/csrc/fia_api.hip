@@ -1,0 +1,801 @@
+// fia_api.hip - the C ABI of libfia.so (include/fia.h): context, device scratch, argument
+// checks and the order of kernels for each entry point.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/fia.h"
+#include "fia_internal.hpp"
+
+using namespace fia;
+
+namespace {
+
+thread_local std::string g_last_error = "no error";
+
+fia_status fail(fia_status s, const std::string &msg) {
+    g_last_error = msg;
+    return s;
+}
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t _e = (expr);                                                                \
+        if (_e != hipSuccess)                                                                  \
+            return fail(FIA_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e));    \
+    } while (0)
+
+template <class T> struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    hipError_t ensure(size_t want) {
+        if (want <= n && p) return hipSuccess;
+        if (p) { hipError_t e = hipFree(p); if (e != hipSuccess) return e; p = nullptr; n = 0; }
+        hipError_t e = hipMalloc((void **)&p, std::max<size_t>(want, 1) * sizeof(T));
+        if (e == hipSuccess) n = want;
+        return e;
+    }
+    void release() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+const int kGaussSizes[] = {1, 3, 5, 7, 9, 11, 13, 25, 49};
+
+bool gauss_supported(int n) {
+    for (int s : kGaussSizes) if (s == n) return true;
+    return false;
+}
+
+// Gauss-Legendre rule with weights summing to 1 (reference src/gauss_quadrature_points.cpp:18-299)
+GaussTable make_gauss(int n) {
+    GaussTable g;
+    memset(&g, 0, sizeof(g));
+    g.n = n;
+    if (n == 1) { g.x[0] = 0.0; g.w[0] = 1.0; return g; }
+    for (int i = 0; i < n; ++i) {
+        double z = cos(M_PI * (i + 0.75) / (n + 0.5)), dp = 1.0;
+        for (int it = 0; it < 100; ++it) {
+            double p0 = 1.0, p1 = 0.0;
+            for (int k = 1; k <= n; ++k) { double p2 = p1; p1 = p0; p0 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p2) / k; }
+            dp = n * (z * p0 - p1) / (z * z - 1.0);
+            const double dz = p0 / dp;
+            z -= dz;
+            if (fabs(dz) < 1e-16) break;
+        }
+        double p0 = 1.0, p1 = 0.0;
+        for (int k = 1; k <= n; ++k) { double p2 = p1; p1 = p0; p0 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p2) / k; }
+        dp = n * (z * p0 - p1) / (z * z - 1.0);
+        g.x[n - 1 - i] = z;
+        g.w[n - 1 - i] = 1.0 / ((1.0 - z * z) * dp * dp);
+    }
+    return g;
+}
+
+}  // namespace
+
+struct fia_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DevBuf<double> psi;
+    DevBuf<float> disp[3];
+    DevBuf<double> fa[3], fb[3];
+    DevBuf<float> work[3];
+    DevBuf<int> twisted;
+    DevBuf<int> counters;   // [0] twisted count, [1] error
+    DevBuf<float> grid[4];  // device mirrors of host grids: rho, u, v, w
+    DevBuf<float> gen[3];   // scratch for test entry points
+    DevBuf<double> genD[2];
+    DevBuf<int> genI;
+    DevBuf<double> pcg[8];  // b, diag, x, r, z, p, q, partials
+    DevBuf<double> scal;
+    DevBuf<unsigned int> bits;
+    DevBuf<uint8_t> solid;
+    // instrumentation
+    bool timing = false;
+    DevBuf<unsigned long long> stats;   // [0] pieces, [1] density points
+    struct Span { hipEvent_t a, b; int kind; };
+    std::vector<Span> spans;            // recorded, not yet read
+    fia_kernel_stats acc{};
+};
+
+namespace {
+
+size_t ctx_bytes(const fia_ctx *c) {
+    size_t b = c->psi.bytes() + c->twisted.bytes() + c->counters.bytes() + c->scal.bytes() + c->bits.bytes() +
+               c->solid.bytes() + c->genI.bytes();
+    for (int a = 0; a < 3; ++a) b += c->disp[a].bytes() + c->fa[a].bytes() + c->fb[a].bytes() + c->work[a].bytes() + c->gen[a].bytes();
+    for (int a = 0; a < 4; ++a) b += c->grid[a].bytes();
+    for (int a = 0; a < 2; ++a) b += c->genD[a].bytes();
+    for (int a = 0; a < 8; ++a) b += c->pcg[a].bytes();
+    return b;
+}
+
+hipError_t set_device(fia_ctx *c) { return hipSetDevice(c->device); }
+
+enum SpanKind { SK_FACE = 0, SK_VERTEX = 1, SK_PSI = 2, SK_GATHER = 3 };
+
+// Opens a timed span on the context's stream (no-op unless timing is enabled).
+hipError_t span_begin(fia_ctx *c, int kind) {
+    if (!c->timing) return hipSuccess;
+    fia_ctx::Span sp;
+    sp.kind = kind;
+    hipError_t e = hipEventCreate(&sp.a);
+    if (e != hipSuccess) return e;
+    e = hipEventCreate(&sp.b);
+    if (e != hipSuccess) return e;
+    e = hipEventRecord(sp.a, c->stream);
+    c->spans.push_back(sp);
+    return e;
+}
+hipError_t span_end(fia_ctx *c) {
+    if (!c->timing || c->spans.empty()) return hipSuccess;
+    return hipEventRecord(c->spans.back().b, c->stream);
+}
+
+struct PassSpec {
+    int split, interp, psi_mode, zero_velocity, integral, ng;
+    double min_area;
+};
+
+// Reads the kernel error flag (synchronises the stream) and maps it to a status.
+fia_status check_kernel_errors(fia_ctx *c) {
+    int h[2] = {0, 0};
+    HIP_TRY(hipMemcpyAsync(h, c->counters.p, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (h[1] == KE_CAPACITY) return fail(FIA_ERR_CAPACITY, "face cutter capacity exceeded (polygon or stack)");
+    if (h[1] == KE_NONFINITE) return fail(FIA_ERR_NONFINITE, "non-finite backtraced vertex (NaN/Inf velocity or dt)");
+    if (h[1] != 0) return fail(FIA_ERR_DEVICE, "kernel reported error " + std::to_string(h[1]));
+    return FIA_OK;
+}
+
+__global__ void zero_ints_kernel(int *a, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0;
+}
+__global__ void zero_uints_kernel(unsigned int *a, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0u;
+}
+__global__ void copy_floats_kernel(const float *a, float *b, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
+fia_status ensure_common(fia_ctx *c) {
+    HIP_TRY(c->counters.ensure(4));
+    return FIA_OK;
+}
+
+fia_status reset_error(fia_ctx *c) {
+    hipLaunchKernelGGL(zero_ints_kernel, dim3(1), dim3(64), 0, c->stream, c->counters.p, 4);
+    HIP_TRY(hipGetLastError());
+    return FIA_OK;
+}
+
+fia_status do_psi(fia_ctx *c, const CompGrid &g, const float *q, int mode, int integral, int ng) {
+    HIP_TRY(c->psi.ensure(g.psi_size()));
+    PsiParams p;
+    memset(&p, 0, sizeof(p));
+    p.g = g; p.q = q; p.psi = c->psi.p; p.mode = mode; p.integral = integral; p.kappa = 1.0f;
+    p.gauss = make_gauss(ng);
+    HIP_TRY(span_begin(c, SK_PSI));
+    HIP_TRY(launch_psi(p, c->stream));
+    HIP_TRY(span_end(c));
+    return FIA_OK;
+}
+
+fia_status do_vertices(fia_ctx *c, const CompGrid &g, const float *u, const float *v, const float *w, double dt_over_h,
+                       int interp) {
+    for (int a = 0; a < 3; ++a) HIP_TRY(c->disp[a].ensure(g.lattice()));
+    VertexParams p;
+    memset(&p, 0, sizeof(p));
+    p.g = g; p.u = u; p.v = v; p.w = w; p.dt_over_h = (float)dt_over_h; p.interp = interp;
+    for (int a = 0; a < 3; ++a) p.disp[a] = c->disp[a].p;
+    HIP_TRY(span_begin(c, SK_VERTEX));
+    HIP_TRY(launch_vertices(p, c->stream));
+    HIP_TRY(span_end(c));
+    if (c->timing) c->acc.vertices += (long long)g.lattice();
+    return FIA_OK;
+}
+
+fia_status do_faces(fia_ctx *c, const CompGrid &g, const float *q, bool moving, DevBuf<double> *flux,
+                    const PassSpec &s, int fam_mask, double h) {
+    for (int f = 0; f < 3; ++f) {
+        if (g.faces(f) >= (size_t(1) << 30)) return fail(FIA_ERR_INVALID_ARG, "grid too large for the face index");
+        HIP_TRY(flux[f].ensure(g.faces(f)));
+    }
+    HIP_TRY(c->twisted.ensure(g.faces(0) + g.faces(1) + g.faces(2)));
+    hipLaunchKernelGGL(zero_ints_kernel, dim3(1), dim3(64), 0, c->stream, c->counters.p, 1);
+    HIP_TRY(hipGetLastError());
+    FaceParams p;
+    memset(&p, 0, sizeof(p));
+    p.g = g; p.q = q; p.psi = c->psi.p;
+    for (int a = 0; a < 3; ++a) { p.disp[a] = moving ? c->disp[a].p : nullptr; p.flux[a] = flux[a].p; }
+    p.twisted = c->twisted.p; p.twisted_count = c->counters.p; p.error = c->counters.p + 1;
+    p.split = s.split; p.interp = s.interp; p.integral = s.integral; p.fam_mask = fam_mask;
+    p.general_all = s.min_area < 1.0;
+    p.kappa_q = 1.0f;
+    p.kappa_psi = (float)(h * h);   // psi is stored in units of q*cells; S in physical units
+    p.min_area = s.min_area;
+    p.gauss = make_gauss(s.ng);
+    if (c->timing) {
+        HIP_TRY(c->stats.ensure(2));
+        p.stats = c->stats.p;
+    }
+    HIP_TRY(span_begin(c, SK_FACE));
+    HIP_TRY(launch_faces(p, c->stream));
+    HIP_TRY(launch_faces_general(p, c->stream));
+    HIP_TRY(span_end(c));
+    if (c->timing) c->acc.face_launches += 1;
+    return FIA_OK;
+}
+
+hipError_t timed_gather(fia_ctx *c, const CompGrid &g, double *const f1[3], double *const f2[3], const float *base,
+                        float *out, int mode, int fm, hipStream_t s) {
+    hipError_t e = span_begin(c, SK_GATHER);
+    if (e != hipSuccess) return e;
+    e = launch_gather(g, f1, f2, base, out, mode, fm, s);
+    if (e != hipSuccess) return e;
+    if (c->timing) c->acc.gather_cells += (long long)g.cells();
+    return span_end(c);
+}
+
+// Complete pass: q_out = flux-advected q_in (q_out may alias q_in).
+fia_status run_pass(fia_ctx *c, const CompGrid &g, const float *q_in, float *q_out, const float *u, const float *v,
+                    const float *w, double h, double dt, const PassSpec &s) {
+    fia_status st;
+    if ((st = do_psi(c, g, q_in, s.psi_mode, s.integral, s.ng)) != FIA_OK) return st;
+    if (!s.zero_velocity && (st = do_vertices(c, g, u, v, w, dt / h, s.interp)) != FIA_OK) return st;
+    const int fm = s.zero_velocity ? 2 : 7;   // undeformed x and z faces have exactly zero y area
+    if ((st = do_faces(c, g, q_in, !s.zero_velocity, c->fa, s, fm, h)) != FIA_OK) return st;
+    double *f[3] = {c->fa[0].p, c->fa[1].p, c->fa[2].p};
+    HIP_TRY(timed_gather(c, g,f, nullptr, nullptr, q_out, 0, fm, c->stream));
+    return FIA_OK;
+}
+
+// psi mode implied by the interpolation method (reference src/move_substances_3d.cpp:61-83,
+// src/advect_velocity_flux_advection_3d.cpp:601-620)
+int psi_mode_for(int interp, int &integral) {
+    if (interp == FIA_INTERP_LINEAR) return FIA_PSI_LINEAR;
+    if (interp == FIA_INTERP_1DY_LINEAR) { integral = 0; return FIA_PSI_CONST; }
+    return FIA_PSI_WENO6;
+}
+
+// Full step on a component grid: main pass plus the zero-velocity correction (reference
+// src/move_substances_3d.cpp:84-226 for density, src/advect_velocity_flux_advection_3d.cpp:
+// 580-759 per velocity component).  q (device, G-sized) is read; result goes to out.
+fia_status run_step(fia_ctx *c, const CompGrid &g, const float *q, float *out, const float *u, const float *v,
+                    const float *w, double h, double dt, const fia_advect_opts *o, int split_main, int split_err) {
+    fia_status st;
+    int integral = o->use_integral ? 1 : 0;
+    const int pm = psi_mode_for(o->interpolation_method, integral);
+    PassSpec p1{split_main, (int)o->interpolation_method, pm, 0, integral, o->num_gauss_points_integrate_density,
+                o->minimal_area};
+    if ((st = do_psi(c, g, q, p1.psi_mode, p1.integral, p1.ng)) != FIA_OK) return st;
+    if ((st = do_vertices(c, g, u, v, w, dt / h, p1.interp)) != FIA_OK) return st;
+    if ((st = do_faces(c, g, q, true, c->fa, p1, 7, h)) != FIA_OK) return st;
+    double *fa[3] = {c->fa[0].p, c->fa[1].p, c->fa[2].p};
+    double *fb[3] = {c->fb[0].p, c->fb[1].p, c->fb[2].p};
+    if (!o->enable_zero_velocity_correction) {
+        HIP_TRY(timed_gather(c, g,fa, nullptr, nullptr, out, 0, 7, c->stream));
+        return FIA_OK;
+    }
+    // pass 2: zero velocity, non-integral psi (reference src/move_substances_3d.cpp:133-153)
+    HIP_TRY(c->work[2].ensure(g.cells()));
+    PassSpec p2{split_main, (int)o->interpolation_method, FIA_PSI_CONST, 1, 0, o->num_gauss_points_integrate_density,
+                o->minimal_area};
+    if ((st = do_psi(c, g, q, p2.psi_mode, p2.integral, p2.ng)) != FIA_OK) return st;
+    if ((st = do_faces(c, g, q, false, c->fb, p2, 2, h)) != FIA_OK) return st;
+    fb[0] = c->fb[0].p; fb[1] = c->fb[1].p; fb[2] = c->fb[2].p;
+    float *err = c->work[2].p;
+    HIP_TRY(timed_gather(c, g,fb, nullptr, q, err, 2, 2, c->stream));   // err = q - pass2(q)
+    // pass 3: error advected with 1Dy_linear (reference src/move_substances_3d.cpp:177-225)
+    PassSpec p3{split_err, FIA_INTERP_1DY_LINEAR, FIA_PSI_CONST, 0, 0, o->num_gauss_points_integrate_density,
+                o->minimal_area};
+    if ((st = do_psi(c, g, err, p3.psi_mode, p3.integral, p3.ng)) != FIA_OK) return st;
+    if ((st = do_vertices(c, g, u, v, w, dt / h, p3.interp)) != FIA_OK) return st;
+    if ((st = do_faces(c, g, err, true, c->fb, p3, 7, h)) != FIA_OK) return st;
+    fb[0] = c->fb[0].p; fb[1] = c->fb[1].p; fb[2] = c->fb[2].p;
+    HIP_TRY(timed_gather(c, g,fa, fb, nullptr, out, 1, 7, c->stream));
+    return FIA_OK;
+}
+
+fia_status check_grid(const fia_grid3d *g, bool need_density) {
+    if (!g) return fail(FIA_ERR_INVALID_ARG, "grid is NULL");
+    if (g->nx < 1 || g->ny < 1 || g->nz < 1) return fail(FIA_ERR_INVALID_ARG, "grid sizes must be >= 1");
+    if (!(g->dx > 0.0) || !isfinite(g->dx)) return fail(FIA_ERR_INVALID_ARG, "dx must be positive and finite");
+    if (!g->u || !g->v || !g->w) return fail(FIA_ERR_INVALID_ARG, "velocity pointer is NULL");
+    if (need_density && !g->density) return fail(FIA_ERR_INVALID_ARG, "density pointer is NULL");
+    return FIA_OK;
+}
+
+fia_status check_opts(const fia_advect_opts *o) {
+    if (!o) return fail(FIA_ERR_INVALID_ARG, "opts is NULL");
+    if (!o->use_flux_advection) return fail(FIA_ERR_UNSUPPORTED, "semi-Lagrangian advection is not implemented");
+    if (o->use_maccormack) return fail(FIA_ERR_UNSUPPORTED, "MacCormack is not implemented");
+    if (o->enable_cell_volume_correction) return fail(FIA_ERR_UNSUPPORTED, "cell volume correction is not implemented");
+    if (o->interpolation_method < FIA_INTERP_LINEAR || o->interpolation_method > FIA_INTERP_1DY_WENO6)
+        return fail(FIA_ERR_INVALID_ARG, "unknown interpolation method");
+    if (o->interpolation_method == FIA_INTERP_1DY_WENO6)
+        return fail(FIA_ERR_UNSUPPORTED, "1Dy_WENO6 has no psi rule in the step (reference leaves it undefined)");
+    if (o->split_method < FIA_SPLIT_NONE || o->split_method > FIA_SPLIT_X_AXIS_Z_CELLFACE)
+        return fail(FIA_ERR_INVALID_ARG, "unknown split method");
+    if (!gauss_supported(o->num_gauss_points_integrate_density))
+        return fail(FIA_ERR_INVALID_ARG, "num_gauss_points_integrate_density must be 1,3,5,7,9,11,13,25 or 49");
+    if (!(o->minimal_area > 0.0)) return fail(FIA_ERR_INVALID_ARG, "minimal_area must be > 0");
+    return FIA_OK;
+}
+
+// Device views of a grid: the caller's pointers or mirrors filled from host memory.
+struct GridView {
+    float *rho, *u, *v, *w;
+};
+
+fia_status upload_grid(fia_ctx *c, const fia_grid3d *g, GridView &gv, bool density) {
+    const size_t nc = (size_t)g->nx * g->ny * g->nz;
+    const size_t nu = (size_t)(g->nx + 1) * g->ny * g->nz, nv = (size_t)g->nx * (g->ny + 1) * g->nz,
+                 nw = (size_t)g->nx * g->ny * (g->nz + 1);
+    if (g->on_device) {
+        gv.rho = g->density; gv.u = g->u; gv.v = g->v; gv.w = g->w;
+        return FIA_OK;
+    }
+    HIP_TRY(c->grid[1].ensure(nu)); HIP_TRY(c->grid[2].ensure(nv)); HIP_TRY(c->grid[3].ensure(nw));
+    HIP_TRY(hipMemcpyAsync(c->grid[1].p, g->u, nu * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->grid[2].p, g->v, nv * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->grid[3].p, g->w, nw * 4, hipMemcpyHostToDevice, c->stream));
+    gv.u = c->grid[1].p; gv.v = c->grid[2].p; gv.w = c->grid[3].p; gv.rho = nullptr;
+    if (density) {
+        HIP_TRY(c->grid[0].ensure(nc));
+        HIP_TRY(hipMemcpyAsync(c->grid[0].p, g->density, nc * 4, hipMemcpyHostToDevice, c->stream));
+        gv.rho = c->grid[0].p;
+    }
+    return FIA_OK;
+}
+
+// Split of each velocity component's main pass and error pass (reference
+// src/advect_velocity_flux_advection_3d.cpp:622-628, 804-810, 997-1003; 721, 913, 1096).
+int velocity_split(int comp, bool one_dy, bool error_pass) {
+    const bool flip = one_dy != error_pass;
+    if (comp == 0) return flip ? FIA_SPLIT_X_CELLFACE_Z_AXIS : FIA_SPLIT_X_AXIS_Z_CELLFACE;
+    if (comp == 1) return flip ? FIA_SPLIT_XZ_AXIS : FIA_SPLIT_XZ_CELLFACE;
+    return flip ? FIA_SPLIT_X_AXIS_Z_CELLFACE : FIA_SPLIT_X_CELLFACE_Z_AXIS;
+}
+
+}  // namespace
+
+// =============================================================================================
+extern "C" {
+
+void fia_default_opts(fia_advect_opts *o) {
+    if (!o) return;
+    memset(o, 0, sizeof(*o));
+    o->use_flux_advection = 1;
+    o->use_maccormack = 0;
+    o->use_clamping_in_maccormack = 1;
+    o->num_gauss_quad_boundary = 1;
+    o->num_gauss_quad_bulk = 1;
+    o->enable_zero_velocity_correction = 1;
+    o->split_method = FIA_SPLIT_NONE;
+    o->num_gauss_quadrature_points = 1;
+    o->interpolation_method = FIA_INTERP_WENO6;
+    o->use_integral = 1;
+    o->num_gauss_points_integrate_density = 1;
+    o->enable_cell_volume_correction = 0;
+    o->minimal_area = 1.0;
+}
+
+const char *fia_last_error(void) { return g_last_error.c_str(); }
+
+int fia_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+fia_status fia_ctx_create(int device, fia_ctx **out) {
+    if (!out) return fail(FIA_ERR_INVALID_ARG, "out is NULL");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(FIA_ERR_NO_DEVICE, "no GPU visible");
+    int dev = device;
+    if (dev < 0) HIP_TRY(hipGetDevice(&dev));
+    if (dev >= n) return fail(FIA_ERR_INVALID_ARG, "device index out of range");
+    fia_ctx *c = new fia_ctx();
+    c->device = dev;
+    if (hipSetDevice(dev) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(FIA_ERR_DEVICE, "cannot create the stream");
+    }
+    if (ensure_common(c) != FIA_OK) { (void)hipStreamDestroy(c->stream); delete c; return FIA_ERR_DEVICE; }
+    *out = c;
+    return FIA_OK;
+}
+
+fia_status fia_ctx_destroy(fia_ctx *c) {
+    if (!c) return FIA_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    c->psi.release(); c->twisted.release(); c->counters.release(); c->scal.release(); c->bits.release();
+    c->solid.release(); c->genI.release();
+    for (int a = 0; a < 3; ++a) { c->disp[a].release(); c->fa[a].release(); c->fb[a].release(); c->work[a].release(); c->gen[a].release(); }
+    for (int a = 0; a < 4; ++a) c->grid[a].release();
+    for (int a = 0; a < 2; ++a) c->genD[a].release();
+    for (int a = 0; a < 8; ++a) c->pcg[a].release();
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return FIA_OK;
+}
+
+size_t fia_ctx_device_bytes(const fia_ctx *c) { return c ? ctx_bytes(c) : 0; }
+
+fia_status fia_ctx_set_timing(fia_ctx *c, int enable) {
+    if (!c) return fail(FIA_ERR_INVALID_ARG, "ctx is NULL");
+    HIP_TRY(set_device(c));
+    c->timing = enable != 0;
+    if (c->timing) {
+        HIP_TRY(c->stats.ensure(2));
+        HIP_TRY(hipMemsetAsync(c->stats.p, 0, 2 * sizeof(unsigned long long), c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    return FIA_OK;
+}
+
+fia_status fia_ctx_reset_stats(fia_ctx *c) {
+    if (!c) return fail(FIA_ERR_INVALID_ARG, "ctx is NULL");
+    HIP_TRY(set_device(c));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (auto &sp : c->spans) { (void)hipEventDestroy(sp.a); (void)hipEventDestroy(sp.b); }
+    c->spans.clear();
+    c->acc = fia_kernel_stats{};
+    if (c->stats.p) {
+        HIP_TRY(hipMemsetAsync(c->stats.p, 0, 2 * sizeof(unsigned long long), c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    return FIA_OK;
+}
+
+fia_status fia_ctx_get_stats(fia_ctx *c, fia_kernel_stats *out) {
+    if (!c || !out) return fail(FIA_ERR_INVALID_ARG, "NULL argument");
+    HIP_TRY(set_device(c));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (auto &sp : c->spans) {
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, sp.a, sp.b));
+        if (sp.kind == SK_FACE) c->acc.face_ms += ms;
+        else if (sp.kind == SK_VERTEX) c->acc.vertex_ms += ms;
+        else if (sp.kind == SK_PSI) c->acc.psi_ms += ms;
+        else c->acc.gather_ms += ms;
+        (void)hipEventDestroy(sp.a);
+        (void)hipEventDestroy(sp.b);
+    }
+    c->spans.clear();
+    *out = c->acc;
+    if (c->stats.p) {
+        unsigned long long h[2] = {0, 0};
+        HIP_TRY(hipMemcpy(h, c->stats.p, sizeof(h), hipMemcpyDeviceToHost));
+        out->pieces = (long long)h[0];
+        out->density_points = (long long)h[1];
+        out->psi_points = (long long)h[0];
+    }
+    return FIA_OK;
+}
+
+fia_status fia_move_substances_3d(fia_ctx *c, fia_grid3d *g, int i_frame, double dt, const fia_advect_opts *o) {
+    (void)i_frame;
+    if (!c) return fail(FIA_ERR_INVALID_ARG, "ctx is NULL");
+    fia_status st;
+    if ((st = check_grid(g, true)) != FIA_OK) return st;
+    if ((st = check_opts(o)) != FIA_OK) return st;
+    if (!isfinite(dt)) return fail(FIA_ERR_NONFINITE, "dt is not finite");
+    HIP_TRY(set_device(c));
+    if ((st = reset_error(c)) != FIA_OK) return st;
+    GridView gv;
+    if ((st = upload_grid(c, g, gv, true)) != FIA_OK) return st;
+    CompGrid cg;
+    cg.init(-1, g->nx, g->ny, g->nz);
+    if ((st = run_step(c, cg, gv.rho, gv.rho, gv.u, gv.v, gv.w, g->dx, dt, o, o->split_method, FIA_SPLIT_XZ_AXIS)) != FIA_OK)
+        return st;
+    if (!g->on_device)
+        HIP_TRY(hipMemcpyAsync(g->density, gv.rho, cg.cells() * 4, hipMemcpyDeviceToHost, c->stream));
+    return check_kernel_errors(c);
+}
+
+fia_status fia_advect_flux_pass(fia_ctx *c, const fia_grid3d *g, float *q, double dt, fia_split split,
+                                fia_interp interp, fia_psi_mode psi_mode, int zero_velocity, int use_integral,
+                                int num_gauss_density, double minimal_area) {
+    if (!c) return fail(FIA_ERR_INVALID_ARG, "ctx is NULL");
+    fia_status st;
+    if ((st = check_grid(g, false)) != FIA_OK) return st;
+    if (!q) return fail(FIA_ERR_INVALID_ARG, "q is NULL");
+    if (!gauss_supported(num_gauss_density)) return fail(FIA_ERR_INVALID_ARG, "unsupported Gauss rule");
+    if (interp < FIA_INTERP_LINEAR || interp > FIA_INTERP_1DY_WENO6) return fail(FIA_ERR_INVALID_ARG, "unknown interpolation");
+    if (split < FIA_SPLIT_NONE || split > FIA_SPLIT_X_AXIS_Z_CELLFACE) return fail(FIA_ERR_INVALID_ARG, "unknown split");
+    if (psi_mode < FIA_PSI_CONST || psi_mode > FIA_PSI_WENO6) return fail(FIA_ERR_INVALID_ARG, "unknown psi mode");
+    if (!(minimal_area > 0.0)) return fail(FIA_ERR_INVALID_ARG, "minimal_area must be > 0");
+    if (!isfinite(dt)) return fail(FIA_ERR_NONFINITE, "dt is not finite");
+    HIP_TRY(set_device(c));
+    if ((st = reset_error(c)) != FIA_OK) return st;
+    GridView gv;
+    if ((st = upload_grid(c, g, gv, false)) != FIA_OK) return st;
+    CompGrid cg;
+    cg.init(-1, g->nx, g->ny, g->nz);
+    float *qd = q;
+    if (!g->on_device) {
+        HIP_TRY(c->work[0].ensure(cg.cells()));
+        HIP_TRY(hipMemcpyAsync(c->work[0].p, q, cg.cells() * 4, hipMemcpyHostToDevice, c->stream));
+        qd = c->work[0].p;
+    }
+    PassSpec s{(int)split, (int)interp, (int)psi_mode, zero_velocity ? 1 : 0, use_integral ? 1 : 0,
+               num_gauss_density, minimal_area};
+    if ((st = run_pass(c, cg, qd, qd, gv.u, gv.v, gv.w, g->dx, dt, s)) != FIA_OK) return st;
+    if (!g->on_device) HIP_TRY(hipMemcpyAsync(q, qd, cg.cells() * 4, hipMemcpyDeviceToHost, c->stream));
+    return check_kernel_errors(c);
+}
+
+fia_status fia_advect_velocity_3d(fia_ctx *c, fia_grid3d *g, int dim, double dt, const fia_advect_opts *o) {
+    if (!c) return fail(FIA_ERR_INVALID_ARG, "ctx is NULL");
+    fia_status st;
+    if ((st = check_grid(g, false)) != FIA_OK) return st;
+    if ((st = check_opts(o)) != FIA_OK) return st;
+    if (dim < -1 || dim > 2) return fail(FIA_ERR_INVALID_ARG, "dim must be -1, 0, 1 or 2");
+    if (!isfinite(dt)) return fail(FIA_ERR_NONFINITE, "dt is not finite");
+    HIP_TRY(set_device(c));
+    if ((st = reset_error(c)) != FIA_OK) return st;
+    GridView gv;
+    if ((st = upload_grid(c, g, gv, false)) != FIA_OK) return st;
+    float *comp_ptr[3] = {gv.u, gv.v, gv.w};
+    float *host_ptr[3] = {g->u, g->v, g->w};
+    const bool one_dy = o->interpolation_method == FIA_INTERP_1DY_LINEAR;
+    for (int comp = 0; comp < 3; ++comp) {
+        if (dim >= 0 && comp != dim) continue;
+        CompGrid cg;
+        cg.init(comp, g->nx, g->ny, g->nz);
+        HIP_TRY(c->work[comp].ensure(cg.cells()));
+        if ((st = run_step(c, cg, comp_ptr[comp], c->work[comp].p, gv.u, gv.v, gv.w, g->dx, dt, o,
+                           velocity_split(comp, one_dy, false), velocity_split(comp, false, true))) != FIA_OK)
+            return st;
+    }
+    // write back after all components: every component backtraces through the pre-step velocity
+    // (reference src/advect_velocity_flux_advection_3d.cpp:1247-1271)
+    for (int comp = 0; comp < 3; ++comp) {
+        if (dim >= 0 && comp != dim) continue;
+        CompGrid cg;
+        cg.init(comp, g->nx, g->ny, g->nz);
+        const size_t n = cg.cells();
+        if (g->on_device) {
+            hipLaunchKernelGGL(copy_floats_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                               c->work[comp].p, comp_ptr[comp], n);
+            HIP_TRY(hipGetLastError());
+        } else {
+            HIP_TRY(hipMemcpyAsync(host_ptr[comp], c->work[comp].p, n * 4, hipMemcpyDeviceToHost, c->stream));
+        }
+    }
+    return check_kernel_errors(c);
+}
+
+fia_status fia_calc_psi_y(fia_ctx *c, int nx, int ny, int nz, double dx, const float *q, double *psi,
+                          fia_psi_mode mode, int use_integral, int num_gauss, int on_device) {
+    if (!c || !q || !psi) return fail(FIA_ERR_INVALID_ARG, "NULL argument");
+    if (nx < 1 || ny < 1 || nz < 1 || !(dx > 0)) return fail(FIA_ERR_INVALID_ARG, "bad grid");
+    if (!gauss_supported(num_gauss)) return fail(FIA_ERR_INVALID_ARG, "unsupported Gauss rule");
+    if (mode < FIA_PSI_CONST || mode > FIA_PSI_WENO6) return fail(FIA_ERR_INVALID_ARG, "unknown psi mode");
+    HIP_TRY(set_device(c));
+    CompGrid cg;
+    cg.init(-1, nx, ny, nz);
+    const float *qd = q;
+    if (!on_device) {
+        HIP_TRY(c->gen[0].ensure(cg.cells()));
+        HIP_TRY(hipMemcpyAsync(c->gen[0].p, q, cg.cells() * 4, hipMemcpyHostToDevice, c->stream));
+        qd = c->gen[0].p;
+    }
+    fia_status st;
+    if ((st = do_psi(c, cg, qd, mode, use_integral, num_gauss)) != FIA_OK) return st;
+    HIP_TRY(hipMemcpyAsync(psi, c->psi.p, cg.psi_size() * 8, on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                           c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return FIA_OK;
+}
+
+fia_status fia_interpolate(fia_ctx *c, const float *field, int mx, int my, int mz, double ox, double oy, double oz,
+                           double dx, fia_interp method, const float *points_xyz, size_t n, float *out,
+                           int on_device) {
+    if (!c || !field || (!points_xyz && n) || (!out && n)) return fail(FIA_ERR_INVALID_ARG, "NULL argument");
+    if (mx < 1 || my < 1 || mz < 1 || !(dx > 0)) return fail(FIA_ERR_INVALID_ARG, "bad field");
+    if (method < FIA_INTERP_LINEAR || method > FIA_INTERP_1DY_WENO6) return fail(FIA_ERR_INVALID_ARG, "unknown method");
+    HIP_TRY(set_device(c));
+    const size_t nf = (size_t)mx * my * mz;
+    HIP_TRY(c->gen[0].ensure(nf)); HIP_TRY(c->gen[1].ensure(3 * n)); HIP_TRY(c->gen[2].ensure(n));
+    HIP_TRY(hipMemcpyAsync(c->gen[0].p, field, nf * 4, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream));
+    // points to cell units
+    std::vector<float> pts;
+    if (!on_device) {
+        pts.resize(3 * n);
+        for (size_t i = 0; i < 3 * n; ++i) pts[i] = (float)((double)points_xyz[i] / dx);
+        HIP_TRY(hipMemcpyAsync(c->gen[1].p, pts.data(), 3 * n * 4, hipMemcpyHostToDevice, c->stream));
+    } else {
+        return fail(FIA_ERR_UNSUPPORTED, "fia_interpolate takes host points");
+    }
+    HIP_TRY(launch_interpolate(c->gen[0].p, mx, my, mz, ox, oy, oz, (int)method, 1.0f, c->gen[1].p, n, c->gen[2].p,
+                               c->stream));
+    HIP_TRY(hipMemcpyAsync(out, c->gen[2].p, n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return FIA_OK;
+}
+
+fia_status fia_backtrace_vertices(fia_ctx *c, const fia_grid3d *g, int dim, double dt, fia_interp interp,
+                                  float *disp_xyz, int on_device) {
+    if (!c || !disp_xyz) return fail(FIA_ERR_INVALID_ARG, "NULL argument");
+    fia_status st;
+    if ((st = check_grid(g, false)) != FIA_OK) return st;
+    if (dim < -1 || dim > 2) return fail(FIA_ERR_INVALID_ARG, "dim must be -1..2");
+    if (interp < FIA_INTERP_LINEAR || interp > FIA_INTERP_1DY_WENO6) return fail(FIA_ERR_INVALID_ARG, "unknown interp");
+    HIP_TRY(set_device(c));
+    GridView gv;
+    if ((st = upload_grid(c, g, gv, false)) != FIA_OK) return st;
+    CompGrid cg;
+    cg.init(dim, g->nx, g->ny, g->nz);
+    if ((st = do_vertices(c, cg, gv.u, gv.v, gv.w, dt / g->dx, (int)interp)) != FIA_OK) return st;
+    const size_t nl = cg.lattice();
+    for (int a = 0; a < 3; ++a)
+        HIP_TRY(hipMemcpyAsync(disp_xyz + a * nl, c->disp[a].p, nl * 4,
+                               on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return FIA_OK;
+}
+
+fia_status fia_split_faces(fia_ctx *c, const float *quads_xyz, size_t n, double dx, fia_split split,
+                           double minimal_area, int max_pieces, int *piece_count, double *centroid_xyz,
+                           double *area_xyz, int on_device) {
+    if (!c || (n && (!quads_xyz || !piece_count || !centroid_xyz || !area_xyz)))
+        return fail(FIA_ERR_INVALID_ARG, "NULL argument");
+    if (max_pieces < 1 || !(dx > 0) || !(minimal_area > 0)) return fail(FIA_ERR_INVALID_ARG, "bad argument");
+    if (on_device) return fail(FIA_ERR_UNSUPPORTED, "fia_split_faces takes host arrays");
+    if (n == 0) return FIA_OK;
+    HIP_TRY(set_device(c));
+    fia_status st;
+    if ((st = reset_error(c)) != FIA_OK) return st;
+    std::vector<double> q(12 * n);
+    for (size_t i = 0; i < 12 * n; ++i) q[i] = (double)quads_xyz[i] / dx;
+    HIP_TRY(c->pcg[0].ensure(12 * n));
+    HIP_TRY(c->genI.ensure(n));
+    HIP_TRY(c->genD[0].ensure(3 * n * max_pieces));
+    HIP_TRY(c->genD[1].ensure(3 * n * max_pieces));
+    HIP_TRY(hipMemcpyAsync(c->pcg[0].p, q.data(), 12 * n * 8, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(launch_split_faces(c->pcg[0].p, n, (int)split, minimal_area, max_pieces, c->genI.p, c->genD[0].p,
+                               c->genD[1].p, c->counters.p + 1, c->stream));
+    HIP_TRY(hipMemcpyAsync(piece_count, c->genI.p, n * 4, hipMemcpyDeviceToHost, c->stream));
+    std::vector<double> cen(3 * n * max_pieces), ar(3 * n * max_pieces);
+    HIP_TRY(hipMemcpyAsync(cen.data(), c->genD[0].p, cen.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(ar.data(), c->genD[1].p, ar.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    if ((st = check_kernel_errors(c)) != FIA_OK) return st;
+    for (size_t i = 0; i < 3 * n * max_pieces; ++i) { centroid_xyz[i] = cen[i] * dx; area_xyz[i] = ar[i] * dx * dx; }
+    return FIA_OK;
+}
+
+fia_status fia_cfl_time_step(fia_ctx *c, const fia_grid3d *g, double now_time, int i_movie_frame, int cfl_number,
+                             double movie_fps, double *dt_out, int *write_frame) {
+    if (!c || !dt_out || !write_frame) return fail(FIA_ERR_INVALID_ARG, "NULL argument");
+    fia_status st;
+    if ((st = check_grid(g, false)) != FIA_OK) return st;
+    if (!(movie_fps > 0)) return fail(FIA_ERR_INVALID_ARG, "movie_fps must be > 0");
+    HIP_TRY(set_device(c));
+    GridView gv;
+    if ((st = upload_grid(c, g, gv, false)) != FIA_OK) return st;
+    HIP_TRY(c->bits.ensure(1));
+    hipLaunchKernelGGL(zero_uints_kernel, dim3(1), dim3(64), 0, c->stream, c->bits.p, 1);
+    HIP_TRY(hipGetLastError());
+    const size_t nu = (size_t)(g->nx + 1) * g->ny * g->nz, nv = (size_t)g->nx * (g->ny + 1) * g->nz,
+                 nw = (size_t)g->nx * g->ny * (g->nz + 1);
+    HIP_TRY(launch_absmax(gv.u, nu, c->bits.p, c->stream));
+    HIP_TRY(launch_absmax(gv.v, nv, c->bits.p, c->stream));
+    HIP_TRY(launch_absmax(gv.w, nw, c->bits.p, c->stream));
+    unsigned int bits = 0;
+    HIP_TRY(hipMemcpyAsync(&bits, c->bits.p, 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    float m;
+    memcpy(&m, &bits, 4);
+    if (m != m) return fail(FIA_ERR_NONFINITE, "velocity contains NaN");
+    // reference src/calc_time_step_length_from_CFL_number.cpp:122-131 (FP32 there)
+    const double dt = (double)cfl_number * g->dx / (double)m;
+    const double next = (i_movie_frame + 1) / movie_fps;
+    if (now_time + dt > next) { *write_frame = 1; *dt_out = next - now_time; }
+    else { *write_frame = 0; *dt_out = dt; }
+    return FIA_OK;
+}
+
+fia_status fia_pressure_project(fia_ctx *c, fia_grid3d *g, const uint8_t *solid, float *pressure, int warm_start,
+                                double rel_tol, int max_iters, int *iters, double *rel_residual) {
+    if (!c) return fail(FIA_ERR_INVALID_ARG, "ctx is NULL");
+    fia_status st;
+    if ((st = check_grid(g, false)) != FIA_OK) return st;
+    if (!(rel_tol > 0) || max_iters < 0) return fail(FIA_ERR_INVALID_ARG, "bad tolerance or iteration limit");
+    HIP_TRY(set_device(c));
+    GridView gv;
+    if ((st = upload_grid(c, g, gv, false)) != FIA_OK) return st;
+    const size_t n = (size_t)g->nx * g->ny * g->nz;
+    PcgGrid pg;
+    pg.nx = g->nx; pg.ny = g->ny; pg.nz = g->nz; pg.solid = nullptr; pg.pin = (long)n - 1;
+    if (solid) {
+        HIP_TRY(c->solid.ensure(n));
+        HIP_TRY(hipMemcpyAsync(c->solid.p, solid, n, g->on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                               c->stream));
+        pg.solid = c->solid.p;
+        // last fluid cell is pinned (the reference pins the last cell, which has no obstacles)
+        std::vector<uint8_t> hs(n);
+        HIP_TRY(hipMemcpyAsync(hs.data(), c->solid.p, n, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        long pin = -1;
+        for (long i = (long)n - 1; i >= 0; --i) if (!hs[i]) { pin = i; break; }
+        if (pin < 0) return fail(FIA_ERR_INVALID_ARG, "every cell is solid");
+        pg.pin = pin;
+        HIP_TRY(launch_zero_solid_faces(pg, gv.u, gv.v, gv.w, c->stream));
+    }
+    for (int a = 0; a < 7; ++a) HIP_TRY(c->pcg[a].ensure(n));
+    const int nparts = 1024;
+    HIP_TRY(c->pcg[7].ensure(nparts));
+    HIP_TRY(c->scal.ensure(16));
+    double *b = c->pcg[0].p, *diag = c->pcg[1].p, *x = c->pcg[2].p, *r = c->pcg[3].p, *z = c->pcg[4].p,
+           *p = c->pcg[5].p, *q = c->pcg[6].p, *part = c->pcg[7].p, *sc = c->scal.p;
+    HIP_TRY(launch_pcg_setup(pg, gv.u, gv.v, gv.w, g->dx, b, diag, c->stream));
+    float *pd = nullptr;
+    if (pressure) {
+        HIP_TRY(c->gen[0].ensure(n));
+        pd = c->gen[0].p;
+        if (g->on_device) pd = pressure;
+        else if (warm_start) HIP_TRY(hipMemcpyAsync(pd, pressure, n * 4, hipMemcpyHostToDevice, c->stream));
+    }
+    if (pressure && warm_start) HIP_TRY(launch_pressure_in(pg, pd, x, c->stream));
+    else HIP_TRY(launch_fill_double(x, 0.0, n, c->stream));
+    HIP_TRY(launch_pcg_residual(pg, diag, b, x, r, c->stream));
+    HIP_TRY(launch_pcg_precond(r, diag, z, n, c->stream));
+    HIP_TRY(hipMemcpyAsync(p, z, n * 8, hipMemcpyDeviceToDevice, c->stream));
+    const double tol2 = rel_tol * rel_tol;
+    HIP_TRY(hipMemcpyAsync(sc + 5, &tol2, 8, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(launch_dot(r, z, n, part, nparts, c->stream)); HIP_TRY(launch_finish_dot(part, nparts, sc + 2, c->stream));
+    HIP_TRY(launch_dot(r, r, n, part, nparts, c->stream)); HIP_TRY(launch_finish_dot(part, nparts, sc + 3, c->stream));
+    HIP_TRY(launch_dot(b, b, n, part, nparts, c->stream)); HIP_TRY(launch_finish_dot(part, nparts, sc + 4, c->stream));
+    HIP_TRY(launch_pcg_scalars(sc, 0, c->stream));
+    double hs[8] = {0};
+    int it = 0;
+    const int poll = 16;
+    while (true) {
+        HIP_TRY(hipMemcpyAsync(hs, sc, 8 * 8, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        if (hs[6] != 0.0 || it >= max_iters) break;
+        const int batch = std::min(poll, max_iters - it);
+        for (int k = 0; k < batch; ++k) {
+            HIP_TRY(launch_pcg_apply(pg, diag, p, q, c->stream));
+            HIP_TRY(launch_dot(p, q, n, part, nparts, c->stream)); HIP_TRY(launch_finish_dot(part, nparts, sc + 1, c->stream));
+            HIP_TRY(launch_pcg_update_xr(x, r, p, q, sc, n, c->stream));
+            HIP_TRY(launch_pcg_precond(r, diag, z, n, c->stream));
+            HIP_TRY(launch_dot(r, z, n, part, nparts, c->stream)); HIP_TRY(launch_finish_dot(part, nparts, sc + 2, c->stream));
+            HIP_TRY(launch_dot(r, r, n, part, nparts, c->stream)); HIP_TRY(launch_finish_dot(part, nparts, sc + 3, c->stream));
+            HIP_TRY(launch_pcg_scalars(sc, 1, c->stream));
+            HIP_TRY(launch_pcg_update_p(p, z, sc, n, c->stream));
+            HIP_TRY(launch_pcg_scalars(sc, 2, c->stream));
+        }
+        it += batch;
+    }
+    const int iters_done = (int)hs[7];
+    const double bb = hs[4] > 0 ? hs[4] : 1.0;
+    const double res = sqrt(hs[3] / bb);
+    HIP_TRY(launch_project(pg, x, g->dx, gv.u, gv.v, gv.w, pd, c->stream));
+    if (!g->on_device) {
+        const size_t nu = (size_t)(g->nx + 1) * g->ny * g->nz, nv = (size_t)g->nx * (g->ny + 1) * g->nz,
+                     nw = (size_t)g->nx * g->ny * (g->nz + 1);
+        HIP_TRY(hipMemcpyAsync(g->u, gv.u, nu * 4, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipMemcpyAsync(g->v, gv.v, nv * 4, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipMemcpyAsync(g->w, gv.w, nw * 4, hipMemcpyDeviceToHost, c->stream));
+        if (pressure) HIP_TRY(hipMemcpyAsync(pressure, pd, n * 4, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (iters) *iters = iters_done;
+    if (rel_residual) *rel_residual = res;
+    if (hs[6] == 0.0) return fail(FIA_ERR_NOT_CONVERGED, "PCG reached max_iters before rel_tol");
+    return FIA_OK;
+}
+
+}  // extern "C"

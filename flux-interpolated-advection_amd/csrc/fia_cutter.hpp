@@ -1,0 +1,318 @@
+// fia_cutter.hpp - face cutting in xi-eta space, FP64 predicates (gfx950).
+//
+// The reference cuts a backtraced face recursively (src/facecutter.h:59-224): the unit xi-eta
+// square carries a "colour" (cell coordinates of the backtraced corners, src/split_face_3D.cpp:
+// 26-43) that is interpolated linearly along polygon edges; each polygon is split at the first
+// integer strictly inside its colour range, x colours first, then z.  The pieces are therefore
+// the regions  k_s <= c_x < k_{s+1},  m <= c_z < m+1  of the square (sgn(0) = +1: a vertex on an
+// iso line belongs to the upper side).
+//
+// Fast path (no arrays, everything in registers): each piece (s, m) is the square clipped by up
+// to four colour half-planes, streamed corner by corner through a four-stage Sutherland-Hodgman
+// pipeline whose last stage accumulates the xi-eta area moments and the y component of the 3D
+// vector area of the mapped loop.  When every cut meets a polygon in exactly two points this is
+// the reference's piece; each stage counts the sign changes it sees and reports a face whose
+// colour crosses an iso line more than twice (a twisted face), which is then cut by the exact
+// recursive restatement in fia_cutter_general (fallback kernel).
+#pragma once
+#include "fia_math.hpp"
+
+namespace fia {
+
+struct CutVert {
+    double xi, eta, cx, cz;
+};
+
+// Face geometry: the four backtraced corners in cell units (physical frame) and the bilinear map
+// (reference src/cell_face_3D.cpp:68-73), written around corner 0 so that a face whose corners
+// share a coordinate maps every point to exactly that coordinate.
+struct FaceGeom {
+    double p0[3], e1[3], e3[3], e13[3];   // p(xi,eta) = p0 + xi e1 + eta e3 + xi eta e13
+    FIA_HD void init(const double v[4][3]) {
+        for (int a = 0; a < 3; ++a) {
+            p0[a] = v[0][a];
+            e1[a] = v[1][a] - v[0][a];
+            e3[a] = v[3][a] - v[0][a];
+            e13[a] = v[0][a] - v[1][a] + v[2][a] - v[3][a];
+        }
+    }
+    // point relative to corner 0
+    FIA_HD void rel(double xi, double eta, double out[3]) const {
+        for (int a = 0; a < 3; ++a) out[a] = xi * e1[a] + eta * e3[a] + xi * eta * e13[a];
+    }
+};
+
+// Last stage: loop moments.  area2 = sum(xi_k eta_{k+1} - xi_{k+1} eta_k) (twice the signed
+// area), mxi/meta the centroid moments (reference src/facecutter.h:226-253), a?2 = twice the
+// vector area of the mapped loop (reference src/split_face_3D.cpp:78-92; for a
+// closed loop the fan around the centre equals the loop sum).
+struct Accum {
+    int n;
+    CutVert first, prev;
+    double fr[3], pr[3];
+    double area2, mxi, meta, ax2, ay2, az2;
+    FIA_HD void reset() { n = 0; area2 = mxi = meta = ax2 = ay2 = az2 = 0.0; }
+    FIA_HD void edge(const CutVert &a, const double ra[3], const CutVert &b, const double rb[3]) {
+        const double cr = a.xi * b.eta - b.xi * a.eta;
+        area2 += cr;
+        mxi += cr * (a.xi + b.xi);
+        meta += cr * (a.eta + b.eta);
+        ax2 += ra[1] * rb[2] - ra[2] * rb[1];
+        ay2 += ra[2] * rb[0] - ra[0] * rb[2];
+        az2 += ra[0] * rb[1] - ra[1] * rb[0];
+    }
+    FIA_HD void put(const FaceGeom &g, const CutVert &v) {
+        double r[3];
+        g.rel(v.xi, v.eta, r);
+        if (n == 0) { first = v; fr[0] = r[0]; fr[1] = r[1]; fr[2] = r[2]; }
+        else edge(prev, pr, v, r);
+        prev = v; pr[0] = r[0]; pr[1] = r[1]; pr[2] = r[2];
+        ++n;
+    }
+    FIA_HD void close() { if (n > 1) edge(prev, pr, first, fr); }
+};
+
+// One Sutherland-Hodgman stage: keeps c_dim >= iso (upper = true) or c_dim < iso (upper = false).
+// A vertex exactly on the iso line counts as upper (sgn(0) = +1, reference src/facecutter.h:55-57).
+struct Stage {
+    bool active, upper;
+    int dim;
+    double iso;
+    int n, crossings;
+    CutVert first, prev;
+    bool prev_in, first_in;
+    FIA_HD void reset(bool act, int d, double k, bool up) {
+        active = act; dim = d; iso = k; upper = up; n = 0; crossings = 0;
+    }
+    FIA_HD bool inside(const CutVert &v) const {
+        const double c = (dim == 0 ? v.cx : v.cz) - iso;
+        return upper ? (c >= 0.0) : (c < 0.0);
+    }
+    // crossing point on edge a -> b (reference src/facecutter.h:151-161)
+    FIA_HD CutVert cross(const CutVert &a, const CutVert &b) const {
+        const double v0 = (dim == 0 ? a.cx : a.cz) - iso, v1 = (dim == 0 ? b.cx : b.cz) - iso;
+        const double inv = 1.0 / (v1 - v0);
+        const double t0 = v1 * inv, t1 = -v0 * inv;
+        CutVert r;
+        r.xi = t0 * a.xi + t1 * b.xi;
+        r.eta = t0 * a.eta + t1 * b.eta;
+        if (dim == 0) { r.cx = iso; r.cz = t0 * a.cz + t1 * b.cz; }
+        else { r.cz = iso; r.cx = t0 * a.cx + t1 * b.cx; }
+        return r;
+    }
+};
+
+// Four stages and the accumulator.  Forwarding is compile-time recursion over the stage index,
+// so every stage's state stays in registers (no dynamically indexed arrays).
+struct Pipeline {
+    Stage st[4];
+    Accum acc;
+
+    template <int S> FIA_HD void push(const FaceGeom &g, const CutVert &v) {
+        if constexpr (S == 4) {
+            acc.put(g, v);
+        } else {
+            Stage &T = st[S];
+            if (!T.active) { push<S + 1>(g, v); return; }
+            const bool in = T.inside(v);
+            if (T.n == 0) {
+                T.first = v; T.first_in = in;
+            } else if (T.prev_in != in) {
+                ++T.crossings;
+                push<S + 1>(g, T.cross(T.prev, v));
+            }
+            if (in) push<S + 1>(g, v);
+            T.prev = v; T.prev_in = in; ++T.n;
+        }
+    }
+    template <int S> FIA_HD void close(const FaceGeom &g) {
+        if constexpr (S == 4) {
+            acc.close();
+        } else {
+            Stage &T = st[S];
+            if (T.active && T.n > 0 && T.prev_in != T.first_in) {
+                ++T.crossings;
+                push<S + 1>(g, T.cross(T.prev, T.first));
+            }
+            close<S + 1>(g);
+        }
+    }
+    FIA_HD bool twisted() const {
+        return (st[0].active && st[0].crossings > 2) || (st[1].active && st[1].crossings > 2) ||
+               (st[2].active && st[2].crossings > 2) || (st[3].active && st[3].crossings > 2);
+    }
+};
+
+// Interior integers of the open colour range (min, max) (reference src/facecutter.h:128-131).
+FIA_HD void interior_integers(double lo, double hi, int &first, int &last) {
+    first = (int)ceil(lo);
+    last = (int)floor(hi);
+    if (lo - first == 0.0) first += 1;
+    if (hi - last == 0.0) last -= 1;
+}
+
+// ---------------------------------------------------------------------------------------------
+// General cutter: the reference's recursion (src/facecutter.h:59-224) on an explicit stack, and
+// subdivide (src/facecutter.h:255-324), in FP64.  Used for twisted faces and minimal_area < 1.
+// Pieces come out in a different order than the reference's depth-first order; the set is the
+// same.  Capacities: kGenVerts vertices per polygon (the reference's MAX_TMP_POINTS), kGenStack
+// pending polygons, kGenTris pending triangles; exceeding one returns false.
+// ---------------------------------------------------------------------------------------------
+constexpr int kGenVerts = 32;
+constexpr int kGenStack = 8;
+constexpr int kGenTris = 40;
+
+struct GenPoly {
+    int n, dim;
+    CutVert v[kGenVerts];
+};
+
+// centroid and |area| of a xi-eta polygon (reference src/facecutter.h:226-253)
+FIA_HD void gen_center(const double (*p)[2], int num, double &cx, double &cy, double &area) {
+    double a2 = 0.0, mx = 0.0, my = 0.0;
+    for (int n = 0; n < num; ++n) {
+        const int m = (n + 1 == num) ? 0 : n + 1;
+        const double a = p[n][0] * p[m][1] - p[m][0] * p[n][1];
+        a2 += a;
+        mx += a * (p[n][0] + p[m][0]);
+        my += a * (p[n][1] + p[m][1]);
+    }
+    area = fabs(0.5 * a2);
+    if (area != 0.0) {
+        const double s = 1.0 / (6.0 * area);
+        cx = mx * s; cy = my * s;
+    } else {
+        cx = cy = 0.0;
+        for (int n = 0; n < num; ++n) { cx += p[n][0]; cy += p[n][1]; }
+        cx /= num; cy /= num;
+    }
+}
+
+// emit(pts, num, centre_xi, centre_eta) for every piece.
+template <class Emit>
+FIA_HD bool general_cut(const double col[4][2], double min_area, Emit &emit) {
+    GenPoly stack[kGenStack];
+    int sp = 0;
+    {
+        GenPoly &q = stack[sp++];
+        q.n = 4; q.dim = 0;
+        const double sq[4][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};
+        for (int i = 0; i < 4; ++i) { q.v[i].xi = sq[i][0]; q.v[i].eta = sq[i][1]; q.v[i].cx = col[i][0]; q.v[i].cz = col[i][1]; }
+    }
+    GenPoly cur, arr0, arr1;
+    while (sp > 0) {
+        cur = stack[--sp];
+        const int dim = cur.dim, num = cur.n;
+        double lo = 1e300, hi = -1e300;
+        for (int n = 0; n < num; ++n) {
+            const double c = dim == 0 ? cur.v[n].cx : cur.v[n].cz;
+            lo = fmin(lo, c); hi = fmax(hi, c);
+        }
+        int first, last;
+        interior_integers(lo, hi, first, last);
+        if (first <= last) {
+            const double iso = first;
+            arr0.n = 0; arr1.n = 0; arr0.dim = dim; arr1.dim = dim;
+            int slot = 0;
+            arr0.v[arr0.n++] = cur.v[0];
+            for (int n = 0; n < num; ++n) {
+                const int m = (n + 1 == num) ? 0 : n + 1;
+                const double v0 = (dim == 0 ? cur.v[n].cx : cur.v[n].cz) - iso;
+                const double v1 = (dim == 0 ? cur.v[m].cx : cur.v[m].cz) - iso;
+                const bool neg0 = v0 < 0.0, neg1 = v1 < 0.0;
+                if (neg0 != neg1) {
+                    const double inv = 1.0 / (v1 - v0);
+                    const double t0 = v1 * inv, t1 = -v0 * inv;
+                    CutVert add;
+                    add.xi = t0 * cur.v[n].xi + t1 * cur.v[m].xi;
+                    add.eta = t0 * cur.v[n].eta + t1 * cur.v[m].eta;
+                    if (dim == 0) { add.cx = iso; add.cz = t0 * cur.v[n].cz + t1 * cur.v[m].cz; }
+                    else { add.cz = iso; add.cx = t0 * cur.v[n].cx + t1 * cur.v[m].cx; }
+                    if (slot == 0) {
+                        arr0.v[arr0.n++] = add;
+                        slot = 1; arr1.n = 0;
+                        if (t1 < 1.0) arr1.v[arr1.n++] = add;
+                    } else {
+                        arr1.v[arr1.n++] = add;
+                        if (sp >= kGenStack) return false;
+                        stack[sp++] = arr1;
+                        slot = 0;
+                        if (t1 < 1.0) arr0.v[arr0.n++] = add;
+                    }
+                }
+                if (n < num - 1) {
+                    GenPoly &dst = slot == 0 ? arr0 : arr1;
+                    dst.v[dst.n++] = cur.v[m];
+                }
+                if (arr0.n >= kGenVerts - 2 || arr1.n >= kGenVerts - 2) return false;
+            }
+            if (slot != 0) return false;   // odd number of sign changes: cannot happen on a loop
+            if (sp >= kGenStack) return false;
+            stack[sp++] = arr0;
+        } else if (dim == 0) {
+            cur.dim = 1;
+            if (sp >= kGenStack) return false;
+            stack[sp++] = cur;
+        } else if (num >= 3) {
+            // subdivide (reference src/facecutter.h:255-324)
+            double pts[kGenVerts][2];
+            for (int n = 0; n < num; ++n) { pts[n][0] = cur.v[n].xi; pts[n][1] = cur.v[n].eta; }
+            double tri[kGenTris][3][2];
+            int tp = 0;
+            if (num == 3) {
+                for (int k = 0; k < 3; ++k) { tri[0][k][0] = pts[k][0]; tri[0][k][1] = pts[k][1]; }
+                tp = 1;
+            } else {
+                double cx, cy, area;
+                gen_center(pts, num, cx, cy, area);
+                if (area > min_area) {
+                    for (int n = num - 1; n >= 0; --n) {   // pushed in reverse: popped in polygon order
+                        const int m = (n + 1 == num) ? 0 : n + 1;
+                        if (tp >= kGenTris) return false;
+                        tri[tp][0][0] = cx; tri[tp][0][1] = cy;
+                        tri[tp][1][0] = pts[n][0]; tri[tp][1][1] = pts[n][1];
+                        tri[tp][2][0] = pts[m][0]; tri[tp][2][1] = pts[m][1];
+                        ++tp;
+                    }
+                } else {
+                    if (area != 0.0) emit(pts, num, cx, cy);
+                    continue;
+                }
+            }
+            while (tp > 0) {
+                double t[3][2];
+                --tp;
+                for (int k = 0; k < 3; ++k) { t[k][0] = tri[tp][k][0]; t[k][1] = tri[tp][k][1]; }
+                double cx, cy, area;
+                gen_center(t, 3, cx, cy, area);
+                if (area > min_area) {
+                    int imax = 0;
+                    double maxlen = -1.0;
+                    for (int n = 0; n < 3; ++n) {
+                        const int m = (n + 1) % 3;
+                        const double dx = t[n][0] - t[m][0], dy = t[n][1] - t[m][1];
+                        const double len = dx * dx + dy * dy;
+                        if (maxlen < len) { maxlen = len; imax = n; }
+                    }
+                    const int i1 = (imax + 1) % 3, i2 = (imax + 2) % 3;
+                    const double mx = 0.5 * (t[imax][0] + t[i1][0]), my = 0.5 * (t[imax][1] + t[i1][1]);
+                    if (tp + 2 > kGenTris) return false;
+                    // tri1 = {mid, p1, p2} pushed first so tri0 = {p0, mid, p2} is processed first
+                    tri[tp][0][0] = mx; tri[tp][0][1] = my;
+                    tri[tp][1][0] = t[i1][0]; tri[tp][1][1] = t[i1][1];
+                    tri[tp][2][0] = t[i2][0]; tri[tp][2][1] = t[i2][1];
+                    ++tp;
+                    tri[tp][0][0] = t[imax][0]; tri[tp][0][1] = t[imax][1];
+                    tri[tp][1][0] = mx; tri[tp][1][1] = my;
+                    tri[tp][2][0] = t[i2][0]; tri[tp][2][1] = t[i2][1];
+                    ++tp;
+                } else if (area != 0.0) {
+                    emit(t, 3, cx, cy);
+                }
+            }
+        }
+    }
+    return true;
+}
+
+}  // namespace fia

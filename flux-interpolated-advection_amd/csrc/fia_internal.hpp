@@ -1,0 +1,114 @@
+// fia_internal.hpp - kernel parameter blocks and launchers shared by fia_kernels.hip and
+// fia_api.hip.  Not part of the public ABI (include/fia.h is).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fia {
+
+constexpr int kMaxGauss = 49;
+
+// A component grid G: density (comp = -1, dims n) or one MAC velocity component (comp = 0..2,
+// dims n + e_comp).  Its vertex lattice has dims m + 1 and sits at lattice - e_comp/2 (cells).
+struct CompGrid {
+    int comp;
+    int n[3];   // base grid
+    int m[3];   // G dims
+    int e[3];
+    __host__ __device__ void init(int c, int nx, int ny, int nz) {
+        comp = c; n[0] = nx; n[1] = ny; n[2] = nz;
+        for (int a = 0; a < 3; ++a) { e[a] = (a == c); m[a] = n[a] + e[a]; }
+    }
+    __host__ __device__ size_t cells() const { return (size_t)m[0] * m[1] * m[2]; }
+    __host__ __device__ size_t lattice() const { return (size_t)(m[0] + 1) * (m[1] + 1) * (m[2] + 1); }
+    __host__ __device__ size_t psi_size() const { return (size_t)m[0] * (m[1] + 1) * m[2]; }
+    __host__ __device__ size_t faces(int fam) const {
+        return (size_t)(m[0] + (fam == 0)) * (m[1] + (fam == 1)) * (m[2] + (fam == 2));
+    }
+};
+
+struct GaussTable {
+    int n;
+    double x[kMaxGauss];   // nodes on [-1, 1]
+    double w[kMaxGauss];   // weights, sum 1
+};
+
+struct PsiParams {
+    CompGrid g;
+    const float *q;
+    double *psi;
+    int mode;       // 0 const, 1 linear, 2 WENO6
+    int integral;
+    float kappa;    // WENO smoothness scale for q (physical units: 1)
+    GaussTable gauss;
+};
+
+struct VertexParams {
+    CompGrid g;
+    const float *u, *v, *w;
+    float dt_over_h;
+    int interp;
+    float *disp[3];
+};
+
+struct FaceParams {
+    CompGrid g;
+    const float *q;          // advected field on G (density points)
+    const double *psi;       // psi on the y-faces of G, in units of q * cells
+    const float *disp[3];    // vertex displacements (cells); null = zero velocity
+    double *flux[3];         // per family face fluxes (normalised: q * cells^3 / cells^3)
+    int *twisted;            // list of faces for the general cutter: fam << 30 | index
+    int *twisted_count;
+    int *error;              // first error code seen (0 = none)
+    int split, interp, integral;
+    int fam_mask;            // families computed (bit 0 x, 1 y, 2 z)
+    int general_all;         // every face goes through the general cutter (minimal_area < 1)
+    float kappa_q, kappa_psi;
+    double min_area;
+    unsigned long long *stats;   // optional work counters: [0] pieces, [1] density points
+    GaussTable gauss;
+};
+
+// errors reported by kernels through FaceParams::error
+enum KernelError { KE_NONE = 0, KE_CAPACITY = 3, KE_NONFINITE = 4 };
+
+hipError_t launch_psi(const PsiParams &p, hipStream_t s);
+hipError_t launch_vertices(const VertexParams &p, hipStream_t s);
+hipError_t launch_faces(const FaceParams &p, hipStream_t s);
+hipError_t launch_faces_general(const FaceParams &p, hipStream_t s);
+// out = sum over families of flux differences (scaled); mode 0: out = G1; 1: out = G1 + G2;
+// 2: out = base - G1 (y family only).
+hipError_t launch_gather(const CompGrid &g, double *const f1[3], double *const f2[3], const float *base,
+                         float *out, int mode, int fam_mask, hipStream_t s);
+hipError_t launch_interpolate(const float *field, int mx, int my, int mz, double ox, double oy, double oz,
+                              int method, float kappa, const float *pts_cells, size_t n, float *out,
+                              hipStream_t s);
+hipError_t launch_split_faces(const double *quads_cells, size_t n, int split, double min_area, int max_pieces,
+                              int *count, double *cen, double *area, int *error, hipStream_t s);
+hipError_t launch_absmax(const float *a, size_t n, unsigned int *out_bits, hipStream_t s);
+
+// Jacobi-PCG pieces (FP64)
+struct PcgGrid {
+    int nx, ny, nz;
+    const uint8_t *solid;   // may be null
+    long pin;               // pinned cell (p = 0)
+};
+hipError_t launch_pcg_setup(const PcgGrid &g, const float *u, const float *v, const float *w, double h,
+                            double *b, double *diag, hipStream_t s);
+hipError_t launch_pcg_apply(const PcgGrid &g, const double *diag, const double *x, double *y, hipStream_t s);
+hipError_t launch_dot(const double *a, const double *b, size_t n, double *partials, int nparts, hipStream_t s);
+hipError_t launch_finish_dot(const double *partials, int nparts, double *out, hipStream_t s);
+hipError_t launch_pcg_update_xr(double *x, double *r, const double *p, const double *q, const double *scal,
+                                size_t n, hipStream_t s);
+hipError_t launch_pcg_precond(const double *r, const double *diag, double *z, size_t n, hipStream_t s);
+hipError_t launch_pcg_update_p(double *p, const double *z, const double *scal, size_t n, hipStream_t s);
+hipError_t launch_pcg_residual(const PcgGrid &g, const double *diag, const double *b, const double *x, double *r,
+                               hipStream_t s);
+hipError_t launch_pcg_scalars(double *scal, int stage, hipStream_t s);
+hipError_t launch_project(const PcgGrid &g, const double *p, double h, float *u, float *v, float *w,
+                          float *pressure_out, hipStream_t s);
+hipError_t launch_zero_solid_faces(const PcgGrid &g, float *u, float *v, float *w, hipStream_t s);
+hipError_t launch_fill_double(double *a, double v, size_t n, hipStream_t s);
+hipError_t launch_pressure_in(const PcgGrid &g, const float *p_in, double *x, hipStream_t s);
+
+}  // namespace fia

@@ -1,0 +1,527 @@
+// fia_kernels.hip - gfx950 kernels of the flux-interpolated advection step.
+//
+// One flux pass over a component grid G (density, or one staggered velocity component):
+//   psi_kernel       y-prefix psi on the y-faces of G, one thread per (x,z) column, FP64 sums
+//   vertex_kernel    backtraced displacement of every lattice vertex, one thread per vertex
+//   face_kernel      one thread per unique face: FP64 cut into pieces + quadrature -> face flux
+//   general_kernel   the exact recursive cutter for the few twisted faces (small fixed grid)
+//   gather_kernel    per cell: sum of its six signed face fluxes (deterministic, no atomics)
+// Each interior face is computed once; its flux enters the two neighbouring cells with opposite
+// signs, so nothing is lost or created inside the domain.
+#include "fia_cutter.hpp"
+#include "fia_internal.hpp"
+#include "fia_math.hpp"
+
+namespace fia {
+
+constexpr int kBlock = 256;
+
+static inline unsigned grid_for(size_t n, int block) { return (unsigned)((n + block - 1) / block); }
+
+// ---------------------------------------------------------------------------------------------
+// psi (reference src/calc_psi_3D.cpp:10-308 density, :311-537 velocity)
+// ---------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) psi_kernel(PsiParams P) {
+    const int mx = P.g.m[0], my = P.g.m[1], mz = P.g.m[2];
+    const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= (long)mx * mz) return;
+    const int i = (int)(col / mz), k = (int)(col % mz);
+    const Loader<float> Q{P.q, mx, my, mz};
+    double acc = 0.0;
+    P.psi[idx3(i, 0, k, my + 1, mz)] = 0.0;
+    for (int j = 1; j <= my; ++j) {
+        const int jj = j - 1;   // cell below face j
+        float inc;
+        if (!P.integral || P.mode == 0) {
+            inc = Q(i, jj, k);
+        } else if (P.mode == 1) {
+            // trilinear at (x_c, jj + 1/4, z_c) and (x_c, jj + 3/4, z_c): x, z on nodes
+            const float a = 0.75f * Q(i, jj - 1, k) + 0.25f * Q(i, jj, k);   // sample jj - 1/4
+            const float b = 0.75f * Q(i, jj, k) + 0.25f * Q(i, jj + 1, k);   // sample jj + 1/4
+            inc = 0.5f * a + 0.5f * b;
+        } else {
+            // Gauss-Legendre on [jj, jj+1/2] and [jj+1/2, jj+1] over WENO6 in y
+            inc = 0.0f;
+            for (int half = 0; half < 2; ++half) {
+                const double a = jj + 0.5 * half;
+                for (int gq = 0; gq < P.gauss.n; ++gq) {
+                    const double y = 0.25 * P.gauss.x[gq] + 0.25 + a;   // G coords
+                    int by; float ty;
+                    split_coord(y - 0.5, by, ty);
+                    inc += 0.5f * (float)P.gauss.w[gq] * weno1dy<float>(Q, i, by, k, ty, P.kappa);
+                }
+            }
+        }
+        acc += (double)inc;
+        P.psi[idx3(i, j, k, my + 1, mz)] = acc;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// vertex backtrace (reference src/calc_backtraced_face_3D.cpp:9-159)
+// ---------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) vertex_kernel(VertexParams P) {
+    const int lx = P.g.m[0] + 1, ly = P.g.m[1] + 1, lz = P.g.m[2] + 1;
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long)lx * ly * lz) return;
+    const int k = (int)(id % lz), j = (int)((id / lz) % ly), i = (int)(id / ((long)ly * lz));
+    const int lat[3] = {i, j, k};
+    double p[3];
+    for (int a = 0; a < 3; ++a) p[a] = lat[a] - 0.5 * P.g.e[a];
+    bool pinned = false;
+    if (P.g.comp >= 0) {
+        const int li = lat[P.g.comp], n = P.g.n[P.g.comp];
+        pinned = li <= 1 || li >= n;
+    }
+    float d[3] = {0.0f, 0.0f, 0.0f};
+    if (!pinned) {
+        const int nx = P.g.n[0], ny = P.g.n[1], nz = P.g.n[2];
+        const Loader<float> U{P.u, nx + 1, ny, nz}, V{P.v, nx, ny + 1, nz}, W{P.w, nx, ny, nz + 1};
+        const float vu = interp_point<float>(U, P.interp, p[0], p[1], p[2], 0.0, 0.5, 0.5, 1.0f);
+        const float vv = interp_point<float>(V, P.interp, p[0], p[1], p[2], 0.5, 0.0, 0.5, 1.0f);
+        const float vw = interp_point<float>(W, P.interp, p[0], p[1], p[2], 0.5, 0.5, 0.0, 1.0f);
+        d[0] = -P.dt_over_h * vu;
+        d[1] = -P.dt_over_h * vv;
+        d[2] = -P.dt_over_h * vw;
+    }
+    for (int a = 0; a < 3; ++a) {
+        // clamp the position to [-e_a/2, n_a + e_a/2] (reference :131-159) as a bound on d
+        const float lo = (float)(-0.5 * P.g.e[a] - p[a]);
+        const float hi = (float)(P.g.n[a] + 0.5 * P.g.e[a] - p[a]);
+        float v = d[a];
+        if (!(v >= lo)) v = (v != v) ? v : lo;   // NaN passes through and is caught by the face kernel
+        if (v > hi) v = hi;
+        P.disp[a][id] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Face setup: corners of face (fam, i, j, k) of G in "+" orientation, backtraced, in the
+// physical frame (cell units), plus corner colours.  Orientation of the reference's faces 1, 2, 5
+// (src/backtrace_and_calc_all_quadrature_point_list_for_integrate_density.cpp:83-112).
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ void face_corner_offsets(int fam, int c, int o[3]) {
+    // x-face: (j,k),(j+1,k),(j+1,k+1),(j,k+1); y-face: (i,k),(i,k+1),(i+1,k+1),(i+1,k);
+    // z-face: (i,j),(i+1,j),(i+1,j+1),(i,j+1)
+    const int a = (c == 1 || c == 2), b = (c == 2 || c == 3);
+    o[0] = o[1] = o[2] = 0;
+    if (fam == 0) { o[1] = a; o[2] = b; }
+    else if (fam == 1) { o[2] = a; o[0] = b; }
+    else { o[0] = a; o[1] = b; }
+}
+
+__device__ __forceinline__ bool load_face(const FaceParams &P, int fam, int i, int j, int k, double v[4][3],
+                                          double col[4][2]) {
+    const int ly = P.g.m[1] + 1, lz = P.g.m[2] + 1;
+    bool finite = true;
+    for (int c = 0; c < 4; ++c) {
+        int o[3];
+        face_corner_offsets(fam, c, o);
+        const int li = i + o[0], lj = j + o[1], lk = k + o[2];
+        v[c][0] = li - 0.5 * P.g.e[0];
+        v[c][1] = lj - 0.5 * P.g.e[1];
+        v[c][2] = lk - 0.5 * P.g.e[2];
+        if (P.disp[0]) {
+            const size_t vi = idx3(li, lj, lk, ly, lz);
+            v[c][0] += (double)P.disp[0][vi];
+            v[c][1] += (double)P.disp[1][vi];
+            v[c][2] += (double)P.disp[2][vi];
+        }
+        finite = finite && isfinite(v[c][0]) && isfinite(v[c][1]) && isfinite(v[c][2]);
+        // colours (reference src/split_face_3D.cpp:26-43)
+        const double x = v[c][0], z = v[c][2];
+        double cx = 0.0, cz = 0.0;
+        switch (P.split) {
+            case 1: cx = x - 0.5; cz = z - 0.5; break;
+            case 2: cx = x; cz = z; break;
+            case 3: cx = x - 0.5; cz = z; break;
+            case 4: cx = x; cz = z - 0.5; break;
+            default: break;
+        }
+        col[c][0] = cx; col[c][1] = cz;
+    }
+    return finite;
+}
+
+// Flux contribution of one piece with centre c (physical frame, cells) and y vector area ay
+// (cells^2), in units of q (reference quadrature src/polygon_3D.cpp:57-77 and 257-487,
+// consumers src/calc_density_in_cell_from_quadrature_point_list_for_integrate_density.cpp:119-201
+// and src/calc_density_in_cell_from_quadrature_point_list.cpp:98-132).
+__device__ __forceinline__ double piece_flux(const FaceParams &P, const double c[3], double ay, int &npts) {
+    const int mx = P.g.m[0], my = P.g.m[1], mz = P.g.m[2];
+    const double ex = 0.5 * P.g.e[0], ey = 0.5 * P.g.e[1], ez = 0.5 * P.g.e[2];
+    // field offsets in the physical frame: q at G cell centres, psi on G y-faces
+    const double oqx = 0.5 - ex, oqy = 0.5 - ey, oqz = 0.5 - ez;
+    const double opx = 0.5 - ex, opy = -ey, opz = 0.5 - ez;
+    const Loader<double> PS{P.psi, mx, my + 1, mz};
+    if (!P.integral) {
+        return ay * interp_point<double>(PS, P.interp, c[0], c[1], c[2], opx, opy, opz, P.kappa_psi);
+    }
+    const Loader<float> Q{P.q, mx, my, mz};
+    const double gy = c[1] + ey;             // G coordinate along y
+    const double jf = floor(gy);
+    const double f = gy - jf;
+    const double level = jf - ey;            // physical-frame y of G face level j
+    const double psi = interp_point<double>(PS, P.interp, c[0], level, c[2], opx, opy, opz, P.kappa_psi);
+    float dens = 0.0f;
+    const int nh = f < 0.5 ? 1 : 2;
+    for (int half = 0; half < nh; ++half) {
+        double a, len, frac;
+        if (nh == 1) { a = level; len = f; frac = f; }
+        else if (half == 0) { a = level; len = 0.5; frac = 0.5; }
+        else { a = level + 0.5; len = f - 0.5; frac = f - 0.5; }
+        for (int gq = 0; gq < P.gauss.n; ++gq) {
+            const double y = (0.5 * len) * P.gauss.x[gq] + 0.5 * len + a;
+            const float val = interp_point<float>(Q, P.interp, c[0], y, c[2], oqx, oqy, oqz, P.kappa_q);
+            dens += val * (float)(frac * P.gauss.w[gq]);
+        }
+    }
+    npts += nh * P.gauss.n;
+    return ay * (psi + (double)dens);
+}
+
+__device__ __forceinline__ void decode_face(const CompGrid &g, int fam, long id, int &i, int &j, int &k) {
+    const int fy = g.m[1] + (fam == 1), fz = g.m[2] + (fam == 2);
+    k = (int)(id % fz);
+    j = (int)((id / fz) % fy);
+    i = (int)(id / ((long)fy * fz));
+}
+
+__device__ __forceinline__ void report(int *err, int code) { atomicCAS(err, 0, code); }
+
+// One face: cut with the streaming pipeline and integrate every piece; twisted faces are queued
+// for general_kernel.  npieces/npts count the quadrature work (instrumentation).
+template <int FAM> __device__ __forceinline__ void face_body(const FaceParams &P, long id, int &npieces, int &npts) {
+    int i, j, k;
+    decode_face(P.g, FAM, id, i, j, k);
+    double v[4][3], col[4][2];
+    if (!load_face(P, FAM, i, j, k, v, col)) {
+        report(P.error, KE_NONFINITE);
+        P.flux[FAM][id] = 0.0;
+        return;
+    }
+    if (P.general_all) {
+        const int slot = atomicAdd(P.twisted_count, 1);
+        P.twisted[slot] = (FAM << 30) | (int)id;
+        return;
+    }
+    FaceGeom geom;
+    geom.init(v);
+    double lx = col[0][0], hx = col[0][0], lz = col[0][1], hz = col[0][1];
+    for (int c = 1; c < 4; ++c) {
+        lx = fmin(lx, col[c][0]); hx = fmax(hx, col[c][0]);
+        lz = fmin(lz, col[c][1]); hz = fmax(hz, col[c][1]);
+    }
+    int x0, x1, z0, z1;
+    interior_integers(lx, hx, x0, x1);
+    interior_integers(lz, hz, z0, z1);
+    const int nxc = x1 >= x0 ? x1 - x0 + 1 : 0, nzc = z1 >= z0 ? z1 - z0 + 1 : 0;
+    double flux = 0.0;
+    for (int s = 0; s <= nxc; ++s) {
+        for (int m = 0; m <= nzc; ++m) {
+            Pipeline pp;
+            pp.st[0].reset(s > 0, 0, (double)(x0 + s - 1), true);
+            pp.st[1].reset(s < nxc, 0, (double)(x0 + s), false);
+            pp.st[2].reset(m > 0, 1, (double)(z0 + m - 1), true);
+            pp.st[3].reset(m < nzc, 1, (double)(z0 + m), false);
+            pp.acc.reset();
+            const double sq[4][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};
+            for (int c = 0; c < 4; ++c) {
+                CutVert cv;
+                cv.xi = sq[c][0]; cv.eta = sq[c][1]; cv.cx = col[c][0]; cv.cz = col[c][1];
+                pp.push<0>(geom, cv);
+            }
+            pp.close<0>(geom);
+            if (pp.twisted()) {
+                const int slot = atomicAdd(P.twisted_count, 1);
+                P.twisted[slot] = (FAM << 30) | (int)id;
+                return;
+            }
+            if (pp.acc.n >= 3 && pp.acc.area2 != 0.0) {
+                const double inv3a = 1.0 / (3.0 * pp.acc.area2);
+                const double xi = pp.acc.mxi * inv3a, eta = pp.acc.meta * inv3a;
+                double r[3];
+                geom.rel(xi, eta, r);
+                const double c3[3] = {geom.p0[0] + r[0], geom.p0[1] + r[1], geom.p0[2] + r[2]};
+                const double ay = 0.5 * pp.acc.ay2;
+                ++npieces;
+                // a piece with no y area contributes exactly zero (only psi_y and A_y enter)
+                if (ay != 0.0) flux += piece_flux(P, c3, ay, npts);
+            }
+        }
+    }
+    P.flux[FAM][id] = flux;
+}
+
+__device__ __forceinline__ void add_stats(unsigned long long *stats, int a, int b) {
+    unsigned long long x = (unsigned long long)a, y = (unsigned long long)b;
+    for (int off = 32; off > 0; off >>= 1) {
+        x += __shfl_down(x, off, 64);
+        y += __shfl_down(y, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(stats, x);
+        atomicAdd(stats + 1, y);
+    }
+}
+
+template <int FAM> __global__ void __launch_bounds__(kBlock) face_kernel(FaceParams P) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int npieces = 0, npts = 0;
+    if (id < (long)P.g.faces(FAM)) face_body<FAM>(P, id, npieces, npts);
+    if (P.stats) add_stats(P.stats, npieces, npts);
+}
+
+// Exact recursive cutter for the faces listed by face_kernel.  Small fixed grid: the private
+// stack is large and only a few faces need it.
+__global__ void __launch_bounds__(64) general_kernel(FaceParams P) {
+    const int count = *P.twisted_count;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < count; t += gridDim.x * blockDim.x) {
+        const int code = P.twisted[t];
+        const int fam = code >> 30;
+        const long id = code & ((1 << 30) - 1);
+        int i, j, k;
+        decode_face(P.g, fam, id, i, j, k);
+        double v[4][3], col[4][2];
+        load_face(P, fam, i, j, k, v, col);
+        FaceGeom geom;
+        geom.init(v);
+        double flux = 0.0;
+        auto emit = [&](const double (*pts)[2], int num, double cxi, double ceta) {
+            double ay2 = 0.0;
+            double r0[3], rp[3];
+            geom.rel(pts[0][0], pts[0][1], r0);
+            rp[0] = r0[0]; rp[1] = r0[1]; rp[2] = r0[2];
+            for (int n = 1; n <= num; ++n) {
+                double rn[3];
+                if (n == num) { rn[0] = r0[0]; rn[1] = r0[1]; rn[2] = r0[2]; }
+                else geom.rel(pts[n][0], pts[n][1], rn);
+                ay2 += rp[2] * rn[0] - rp[0] * rn[2];
+                rp[0] = rn[0]; rp[1] = rn[1]; rp[2] = rn[2];
+            }
+            double r[3];
+            geom.rel(cxi, ceta, r);
+            const double c3[3] = {geom.p0[0] + r[0], geom.p0[1] + r[1], geom.p0[2] + r[2]};
+            const double ay = 0.5 * ay2;
+            int npts = 0;
+            if (ay != 0.0) flux += piece_flux(P, c3, ay, npts);
+        };
+        if (!general_cut(col, P.min_area, emit)) report(P.error, KE_CAPACITY);
+        P.flux[fam][id] = flux;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// gather: q(i,j,k) = sum over the six faces of +-flux (the '+' face of a cell is its own
+// orientation, the '-' face is the neighbour's '+' face reversed).
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ double cell_sum(const CompGrid &g, double *const f[3], int i, int j, int k, int fm) {
+    const int my = g.m[1], mz = g.m[2];
+    double s = 0.0;
+    if (fm & 1) s += f[0][idx3(i + 1, j, k, my, mz)] - f[0][idx3(i, j, k, my, mz)];
+    if (fm & 2) s += f[1][idx3(i, j + 1, k, my + 1, mz)] - f[1][idx3(i, j, k, my + 1, mz)];
+    if (fm & 4) s += f[2][idx3(i, j, k + 1, my, mz + 1)] - f[2][idx3(i, j, k, my, mz + 1)];
+    return s;
+}
+
+struct GatherArgs {
+    CompGrid g;
+    double *f1[3];
+    double *f2[3];
+    const float *base;
+    float *out;
+    int mode, fam_mask;
+};
+
+__global__ void __launch_bounds__(kBlock) gather_kernel(GatherArgs A) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long)A.g.cells()) return;
+    const int mz = A.g.m[2], my = A.g.m[1];
+    const int k = (int)(id % mz), j = (int)((id / mz) % my), i = (int)(id / ((long)my * mz));
+    double s = cell_sum(A.g, A.f1, i, j, k, A.fam_mask);
+    if (A.mode == 1) s += cell_sum(A.g, A.f2, i, j, k, 7);
+    if (A.mode == 2) s = (double)A.base[id] - s;
+    A.out[id] = (float)s;
+}
+
+// ---------------------------------------------------------------------------------------------
+// utility kernels
+// ---------------------------------------------------------------------------------------------
+__global__ void interpolate_kernel(const float *field, int mx, int my, int mz, double ox, double oy, double oz,
+                                   int method, float kappa, const float *pts, size_t n, float *out) {
+    const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n) return;
+    const Loader<float> L{field, mx, my, mz};
+    out[id] = interp_point<float>(L, method, pts[3 * id], pts[3 * id + 1], pts[3 * id + 2], ox, oy, oz, kappa);
+}
+
+__global__ void __launch_bounds__(64) split_faces_kernel(const double *quads, size_t n, int split, double min_area,
+                                                         int max_pieces, int *count, double *cen, double *area,
+                                                         int *error) {
+    const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n) return;
+    double v[4][3], col[4][2];
+    for (int c = 0; c < 4; ++c) {
+        for (int a = 0; a < 3; ++a) v[c][a] = quads[12 * id + 3 * c + a];
+        const double x = v[c][0], z = v[c][2];
+        double cx = 0.0, cz = 0.0;
+        switch (split) {
+            case 1: cx = x - 0.5; cz = z - 0.5; break;
+            case 2: cx = x; cz = z; break;
+            case 3: cx = x - 0.5; cz = z; break;
+            case 4: cx = x; cz = z - 0.5; break;
+            default: break;
+        }
+        col[c][0] = cx; col[c][1] = cz;
+    }
+    FaceGeom geom;
+    geom.init(v);
+    int cnt = 0;
+    bool over = false;
+    auto put = [&](const double r[3], double ax2, double ay2, double az2) {
+        if (cnt >= max_pieces) { over = true; return; }
+        const size_t o = (id * max_pieces + cnt) * 3;
+        cen[o] = geom.p0[0] + r[0]; cen[o + 1] = geom.p0[1] + r[1]; cen[o + 2] = geom.p0[2] + r[2];
+        area[o] = 0.5 * ax2; area[o + 1] = 0.5 * ay2; area[o + 2] = 0.5 * az2;
+        ++cnt;
+    };
+    bool twisted = min_area < 1.0;
+    if (!twisted) {
+        double lx = col[0][0], hx = col[0][0], lz = col[0][1], hz = col[0][1];
+        for (int c = 1; c < 4; ++c) {
+            lx = fmin(lx, col[c][0]); hx = fmax(hx, col[c][0]);
+            lz = fmin(lz, col[c][1]); hz = fmax(hz, col[c][1]);
+        }
+        int x0, x1, z0, z1;
+        interior_integers(lx, hx, x0, x1);
+        interior_integers(lz, hz, z0, z1);
+        const int nxc = x1 >= x0 ? x1 - x0 + 1 : 0, nzc = z1 >= z0 ? z1 - z0 + 1 : 0;
+        for (int s = 0; s <= nxc && !twisted; ++s) {
+            for (int m = 0; m <= nzc; ++m) {
+                Pipeline pp;
+                pp.st[0].reset(s > 0, 0, (double)(x0 + s - 1), true);
+                pp.st[1].reset(s < nxc, 0, (double)(x0 + s), false);
+                pp.st[2].reset(m > 0, 1, (double)(z0 + m - 1), true);
+                pp.st[3].reset(m < nzc, 1, (double)(z0 + m), false);
+                pp.acc.reset();
+                const double sq[4][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};
+                for (int c = 0; c < 4; ++c) {
+                    CutVert cv;
+                    cv.xi = sq[c][0]; cv.eta = sq[c][1]; cv.cx = col[c][0]; cv.cz = col[c][1];
+                    pp.push<0>(geom, cv);
+                }
+                pp.close<0>(geom);
+                if (pp.twisted()) { twisted = true; break; }
+                if (pp.acc.n >= 3 && pp.acc.area2 != 0.0) {
+                    const double inv3a = 1.0 / (3.0 * pp.acc.area2);
+                    double r[3];
+                    geom.rel(pp.acc.mxi * inv3a, pp.acc.meta * inv3a, r);
+                    put(r, pp.acc.ax2, pp.acc.ay2, pp.acc.az2);
+                }
+            }
+        }
+    }
+    if (twisted) {
+        cnt = 0;
+        over = false;
+        auto emit = [&](const double (*pts)[2], int num, double cxi, double ceta) {
+            double a2[3] = {0, 0, 0};
+            double r0[3], rp[3];
+            geom.rel(pts[0][0], pts[0][1], r0);
+            rp[0] = r0[0]; rp[1] = r0[1]; rp[2] = r0[2];
+            for (int q = 1; q <= num; ++q) {
+                double rn[3];
+                if (q == num) { rn[0] = r0[0]; rn[1] = r0[1]; rn[2] = r0[2]; }
+                else geom.rel(pts[q][0], pts[q][1], rn);
+                a2[0] += rp[1] * rn[2] - rp[2] * rn[1];
+                a2[1] += rp[2] * rn[0] - rp[0] * rn[2];
+                a2[2] += rp[0] * rn[1] - rp[1] * rn[0];
+                rp[0] = rn[0]; rp[1] = rn[1]; rp[2] = rn[2];
+            }
+            double r[3];
+            geom.rel(cxi, ceta, r);
+            put(r, a2[0], a2[1], a2[2]);
+        };
+        if (!general_cut(col, min_area, emit)) over = true;
+    }
+    count[id] = cnt;
+    if (over) atomicCAS(error, 0, (int)KE_CAPACITY);
+}
+
+__global__ void absmax_kernel(const float *a, size_t n, unsigned int *out_bits) {
+    float m = 0.0f;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const float v = fabsf(a[i]);
+        m = (v > m || v != v) ? v : m;
+    }
+    // wave reduction then one atomic per wave; |x| bit patterns order like the floats
+    for (int off = 32; off > 0; off >>= 1) {
+        const float o = __shfl_down(m, off, 64);
+        m = (o > m || o != o) ? o : m;
+    }
+    if ((threadIdx.x & 63) == 0) atomicMax(out_bits, __float_as_uint(m));
+}
+
+// ---------------------------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------------------------
+hipError_t launch_psi(const PsiParams &p, hipStream_t s) {
+    const size_t cols = (size_t)p.g.m[0] * p.g.m[2];
+    hipLaunchKernelGGL(psi_kernel, dim3(grid_for(cols, kBlock)), dim3(kBlock), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_vertices(const VertexParams &p, hipStream_t s) {
+    hipLaunchKernelGGL(vertex_kernel, dim3(grid_for(p.g.lattice(), kBlock)), dim3(kBlock), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_faces(const FaceParams &p, hipStream_t s) {
+    if (p.fam_mask & 1)
+        hipLaunchKernelGGL(face_kernel<0>, dim3(grid_for(p.g.faces(0), kBlock)), dim3(kBlock), 0, s, p);
+    if (p.fam_mask & 2)
+        hipLaunchKernelGGL(face_kernel<1>, dim3(grid_for(p.g.faces(1), kBlock)), dim3(kBlock), 0, s, p);
+    if (p.fam_mask & 4)
+        hipLaunchKernelGGL(face_kernel<2>, dim3(grid_for(p.g.faces(2), kBlock)), dim3(kBlock), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_faces_general(const FaceParams &p, hipStream_t s) {
+    hipLaunchKernelGGL(general_kernel, dim3(256), dim3(64), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather(const CompGrid &g, double *const f1[3], double *const f2[3], const float *base, float *out,
+                         int mode, int fam_mask, hipStream_t s) {
+    GatherArgs A;
+    A.g = g;
+    for (int a = 0; a < 3; ++a) { A.f1[a] = f1[a]; A.f2[a] = f2 ? f2[a] : nullptr; }
+    A.base = base; A.out = out; A.mode = mode; A.fam_mask = fam_mask;
+    hipLaunchKernelGGL(gather_kernel, dim3(grid_for(g.cells(), kBlock)), dim3(kBlock), 0, s, A);
+    return hipGetLastError();
+}
+
+hipError_t launch_interpolate(const float *field, int mx, int my, int mz, double ox, double oy, double oz, int method,
+                              float kappa, const float *pts, size_t n, float *out, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(interpolate_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, field, mx, my, mz, ox, oy,
+                       oz, method, kappa, pts, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_split_faces(const double *quads, size_t n, int split, double min_area, int max_pieces, int *count,
+                              double *cen, double *area, int *error, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(split_faces_kernel, dim3(grid_for(n, 64)), dim3(64), 0, s, quads, n, split, min_area,
+                       max_pieces, count, cen, area, error);
+    return hipGetLastError();
+}
+
+hipError_t launch_absmax(const float *a, size_t n, unsigned int *out_bits, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const unsigned blocks = (unsigned)std::min<size_t>(grid_for(n, kBlock), 2048);
+    hipLaunchKernelGGL(absmax_kernel, dim3(blocks), dim3(kBlock), 0, s, a, n, out_bits);
+    return hipGetLastError();
+}
+
+}  // namespace fia

@@ -1,0 +1,217 @@
+/*
+ * fia.h - C ABI of the MI355X (gfx950) flux-interpolated advection library (libfia.so).
+ *
+ * Every entry point is a plain C function over plain pointers and sizes.  Arrays use the
+ * reference's Grid_3D MAC layout: [x][y][z] with z fastest (reference include/utils.h:92-165):
+ *   density    nx     * ny     * nz
+ *   u (x-face) (nx+1) * ny     * nz
+ *   v (y-face) nx     * (ny+1) * nz
+ *   w (z-face) nx     * ny     * (nz+1)
+ * Values are FP32 (the reference's MY_FLOAT_TYPE, reference include/define_float_type.h:5).
+ *
+ * Pointers in fia_grid3d are either all host pointers (on_device = 0: the call copies them to the
+ * GPU, runs, and copies the results back, matching the reference's in-place update of Grid_3D's
+ * std::vectors) or all device pointers (on_device = 1: the call runs in place on the GPU).
+ *
+ * Errors: every function returns fia_status.  The reference returns void and calls exit(0) or
+ * assert() on failures (reference src/facecutter.h:188-204, src/split_face_3D.cpp:54,73); here the
+ * same conditions come back as status codes and fia_last_error() gives a message.
+ */
+#ifndef FIA_H
+#define FIA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum fia_status {
+    FIA_OK = 0,
+    FIA_ERR_INVALID_ARG = 1,   /* bad size, null pointer, unknown option value            */
+    FIA_ERR_UNSUPPORTED = 2,   /* option the flux path does not implement (MacCormack...)  */
+    FIA_ERR_CAPACITY = 3,      /* a face produced more polygon vertices or pieces than fit */
+    FIA_ERR_NONFINITE = 4,     /* NaN/Inf in an input field or in the time step            */
+    FIA_ERR_DEVICE = 5,        /* HIP runtime error                                        */
+    FIA_ERR_NO_DEVICE = 6,     /* no GPU visible                                           */
+    FIA_ERR_NOT_CONVERGED = 7  /* pressure solve hit its iteration limit                   */
+} fia_status;
+
+/* Interpolation methods (the reference's interpolation_method strings). */
+typedef enum fia_interp {
+    FIA_INTERP_LINEAR = 0,      /* "linear"     : trilinear, clamp-to-edge                    */
+    FIA_INTERP_WENO6 = 1,       /* "WENO6"      : 6^3 WENO6, z then y then x passes           */
+    FIA_INTERP_1DY_LINEAR = 2,  /* "1Dy_linear" : linear in y inside the containing column    */
+    FIA_INTERP_1DY_WENO6 = 3    /* "1Dy_WENO6"  : WENO6 in y inside the containing column     */
+} fia_interp;
+
+/* Face split methods (reference src/split_face_3D.cpp:26-43).  Any other string of the
+ * reference (including its default "y-CellFaceAligned") leaves faces uncut: FIA_SPLIT_NONE. */
+typedef enum fia_split {
+    FIA_SPLIT_NONE = 0,
+    FIA_SPLIT_XZ_CELLFACE = 1,           /* "xz-CellFaceAligned"              */
+    FIA_SPLIT_XZ_AXIS = 2,               /* "xz-AxisAligned"                  */
+    FIA_SPLIT_X_CELLFACE_Z_AXIS = 3,     /* "x-CellFaceAligned_z-AxisAligned" */
+    FIA_SPLIT_X_AXIS_Z_CELLFACE = 4      /* "x-AxisAligned_z-CellFaceAligned" */
+} fia_split;
+
+/* How the y-prefix psi is built (interpolation_method_in_calc_psi of the reference). */
+typedef enum fia_psi_mode {
+    FIA_PSI_CONST = 0,   /* psi(j) = psi(j-1) + q(j-1) dx                                   */
+    FIA_PSI_LINEAR = 1,  /* trilinear q at y = j-3/4, j-1/4, half-cell weights               */
+    FIA_PSI_WENO6 = 2    /* Gauss-Legendre over 1D-y WENO6 of q on both half cells           */
+} fia_psi_mode;
+
+/* Options of move_substances_3D / advect_velocity_*_flux_advection_3D_std_thread, one field per
+ * argument of the reference signature (reference include/move_substances_3d.h:48-67,
+ * include/advect_velocity_flux_advection_3d.h:14-124).  Strings are parsed once by the caller
+ * (the C++ shim in fia_smoke_simulation.hpp does it for the reference strings). */
+typedef struct fia_advect_opts {
+    int use_flux_advection;                 /* 0 selects semi-Lagrangian: FIA_ERR_UNSUPPORTED    */
+    int use_maccormack;                     /* nonzero: FIA_ERR_UNSUPPORTED                      */
+    int use_clamping_in_maccormack;         /* ignored (MacCormack only)                         */
+    int num_gauss_quad_boundary;            /* ignored by the 3D flux path                       */
+    int num_gauss_quad_bulk;                /* ignored by the 3D flux path                       */
+    int enable_zero_velocity_correction;    /* enable_eliminate_zero_velocity_false_diffusion    */
+    fia_split split_method;
+    int num_gauss_quadrature_points;        /* ignored by the optimized split                    */
+    fia_interp interpolation_method;
+    int use_integral;
+    int num_gauss_points_integrate_density; /* 1,3,5,7,9,11,13,25 or 49                          */
+    int enable_cell_volume_correction;      /* nonzero: FIA_ERR_UNSUPPORTED                      */
+    double minimal_area;                    /* triangle subdivision threshold in xi-eta space    */
+} fia_advect_opts;
+
+/* Grid_3D (reference include/grid_3d.h:14-88): sizes, cell length and the four fields. */
+typedef struct fia_grid3d {
+    int nx, ny, nz;
+    double dx;        /* _cell_length; the domain is [0, n*dx] on every axis (min_pos = 0) */
+    float *density;   /* substance_density                                                  */
+    float *u;         /* velocity_in_voxel_face_x                                           */
+    float *v;         /* velocity_in_voxel_face_y                                           */
+    float *w;         /* velocity_in_voxel_face_z                                           */
+    int on_device;    /* 1: device pointers, 0: host pointers                                */
+} fia_grid3d;
+
+typedef struct fia_ctx fia_ctx;
+
+/* Fills opts with the reference's command-line defaults (reference src/main.cpp:48-83) and
+ * interpolation_method = WENO6, split_method = the reference's default string (FIA_SPLIT_NONE). */
+void fia_default_opts(fia_advect_opts *opts);
+
+/* Context: owns the device scratch (psi, backtraced vertices, face fluxes) for the last grid
+ * size it saw, and one HIP stream of default priority.  device < 0 picks the current device. */
+fia_status fia_ctx_create(int device, fia_ctx **out);
+fia_status fia_ctx_destroy(fia_ctx *ctx);
+/* Bytes of device memory the context holds now. */
+size_t fia_ctx_device_bytes(const fia_ctx *ctx);
+/* Message of the last error of this thread (never NULL). */
+const char *fia_last_error(void);
+/* Number of visible GPUs (0 when none); does not create a context. */
+int fia_device_count(void);
+
+/* One density step: replaces smoke_simulation::move_substances_3D
+ * (reference include/move_substances_3d.h:48-67, src/move_substances_3d.cpp:38-240).
+ * Main flux pass, then, if opts->enable_zero_velocity_correction, the zero-velocity pass and
+ * the error advection pass (reference src/move_substances_3d.cpp:115-226).  i_frame is unused
+ * by the reference's 3D path and kept for signature parity. */
+fia_status fia_move_substances_3d(fia_ctx *ctx, fia_grid3d *grid, int i_frame, double dt,
+                                  const fia_advect_opts *opts);
+
+/* One flux pass over a cell-centred field q (nx*ny*nz), in place: replaces
+ * advect_density_flux_advection_3D_std_thread (reference src/move_substances_3d.cpp:585-841).
+ * interp drives velocity and q interpolation, psi_mode the psi build, zero_velocity the
+ * use_zero_velocity_for_backtrace flag.  q and the grid pointers follow grid->on_device. */
+fia_status fia_advect_flux_pass(fia_ctx *ctx, const fia_grid3d *grid, float *q, double dt,
+                                fia_split split, fia_interp interp, fia_psi_mode psi_mode,
+                                int zero_velocity, int use_integral, int num_gauss_density,
+                                double minimal_area);
+
+/* Velocity self-advection of one component (dim 0,1,2) or of all three (dim = -1, every
+ * component backtraced through the pre-step velocity): replaces
+ * advect_velocity_{x,y,z,xyz}_flux_advection_3D_std_thread
+ * (reference include/advect_velocity_flux_advection_3d.h:38-124,
+ * src/advect_velocity_flux_advection_3d.cpp:580-1272). */
+fia_status fia_advect_velocity_3d(fia_ctx *ctx, fia_grid3d *grid, int dim, double dt,
+                                  const fia_advect_opts *opts);
+
+/* psi_y of a cell-centred field of size nx*ny*nz into psi (nx*(ny+1)*nz, FP64, in units of
+ * q*dx): replaces calc_psi_on_cell_face_from_density_on_cell_center_3D
+ * (reference src/calc_psi_3D.cpp:10-308).  Pointers follow on_device. */
+fia_status fia_calc_psi_y(fia_ctx *ctx, int nx, int ny, int nz, double dx, const float *q,
+                          double *psi, fia_psi_mode mode, int use_integral, int num_gauss,
+                          int on_device);
+
+/* Interpolates one staggered field at n points: replaces WENO6_interpolation_3D_*_values,
+ * linear_interpolation_3D_*, linear_interpolation_1Dy_* and WENO6_interpolation_1Dy_*
+ * (reference src/WENO6_interpolation_3d.cpp:7-1020, src/linear_interpolation_3d.cpp:7-567,
+ * src/linear_interpolation_1d.cpp:314-702, src/WENO6_interpolation_1d.cpp:315-779).
+ * The field has dims (mx,my,mz) and sample point (i,j,k) sits at ((i+ox)dx,(j+oy)dx,(k+oz)dx);
+ * points are xyz triples in physical units.  Pointers follow on_device. */
+fia_status fia_interpolate(fia_ctx *ctx, const float *field, int mx, int my, int mz,
+                           double ox, double oy, double oz, double dx, fia_interp method,
+                           const float *points_xyz, size_t n, float *out, int on_device);
+
+/* Backtraced positions of every vertex of the (component) lattice: replaces
+ * calc_backtraced_face_3D + clamp_vertex_position_by_grid_3D applied to all faces
+ * (reference src/calc_backtraced_face_3D.cpp:9-159).  dim = -1 for density, 0..2 for the
+ * velocity component lattices.  out holds 3 arrays of (nx+1+ex)(ny+1+ey)(nz+1+ez) floats:
+ * the backtraced positions minus the original ones, in units of dx. */
+fia_status fia_backtrace_vertices(fia_ctx *ctx, const fia_grid3d *grid, int dim, double dt,
+                                  fia_interp interp, float *disp_xyz, int on_device);
+
+/* Splits n faces given by 4 vertex positions each (n*12 floats, physical units, reference
+ * vertex order) with the face cutter (reference src/split_face_3D.cpp:17-93,
+ * src/facecutter.h:59-324).  For each face writes at most max_pieces pieces:
+ * piece_count[n], centroid_xyz[n*max_pieces*3], area_xyz[n*max_pieces*3] (vector areas).
+ * FIA_ERR_CAPACITY when a face has more pieces than max_pieces.  Pointers follow on_device. */
+fia_status fia_split_faces(fia_ctx *ctx, const float *quads_xyz, size_t n, double dx,
+                           fia_split split, double minimal_area, int max_pieces,
+                           int *piece_count, double *centroid_xyz, double *area_xyz,
+                           int on_device);
+
+/* CFL time step: replaces calc_time_step_length_from_CFL_number_3D
+ * (reference src/calc_time_step_length_from_CFL_number.cpp:81-132).  cfl_number is an int as
+ * in the reference (include/calc_time_step_length_from_CFL_number.h:38).  When now_time+dt
+ * passes the next movie frame the step is shortened to it and *write_frame is set to 1. */
+fia_status fia_cfl_time_step(fia_ctx *ctx, const fia_grid3d *grid, double now_time,
+                             int i_movie_frame, int cfl_number, double movie_fps,
+                             double *dt_out, int *write_frame);
+
+/* Pressure projection: replaces calc_pressure_gradient_term_3D
+ * (reference src/calc_pressure_3d.cpp:18-269).  Solves the reference's 7-point Neumann system
+ * (last cell pinned to p = 0) with Jacobi-preconditioned CG, FP64 dots and residuals, to
+ * ||b - A p|| <= rel_tol * ||b||, then u -= grad p / dx on interior faces.  solid (optional,
+ * nx*ny*nz bytes, nonzero = obstacle cell) turns faces next to obstacles into Neumann walls and
+ * zeroes the velocity on them.  pressure (optional, nx*ny*nz floats) receives p and, when
+ * warm_start is nonzero, is read as the initial guess.  iters/rel_residual are outputs. */
+fia_status fia_pressure_project(fia_ctx *ctx, fia_grid3d *grid, const uint8_t *solid,
+                                float *pressure, int warm_start, double rel_tol, int max_iters,
+                                int *iters, double *rel_residual);
+
+/* Instrumentation (no counterpart in the reference, which only prints wall-clock phases at
+ * src/move_substances_3d.cpp:607-840): when enabled, the library records HIP events around every
+ * face-kernel launch on its own stream and counts the quadrature work those launches did. */
+typedef struct fia_kernel_stats {
+    double face_ms;              /* summed face-kernel time (HIP events on the context's stream)  */
+    long long face_launches;     /* number of face-kernel launches                                */
+    long long pieces;            /* face pieces integrated                                        */
+    long long psi_points;        /* psi interpolation points (one per piece)                      */
+    long long density_points;    /* density/velocity quadrature points                            */
+    long long vertices;          /* backtraced lattice vertices                                   */
+    double vertex_ms;            /* summed vertex-kernel time                                     */
+    double psi_ms;               /* summed psi-kernel time                                        */
+    double gather_ms;            /* summed gather-kernel time                                     */
+    long long gather_cells;      /* cells written by gather kernels                               */
+} fia_kernel_stats;
+
+fia_status fia_ctx_set_timing(fia_ctx *ctx, int enable);
+fia_status fia_ctx_get_stats(fia_ctx *ctx, fia_kernel_stats *out);   /* synchronises */
+fia_status fia_ctx_reset_stats(fia_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FIA_H */
