@@ -76,14 +76,21 @@ GaussTable make_gauss(int n) {
 
 }  // namespace
 
+// Per-family face fluxes of one pass and the corrections of faces whose two evaluations differ.
+struct FluxSet {
+    DevBuf<double> f[3], c[3];
+    size_t bytes() const { size_t b = 0; for (int a = 0; a < 3; ++a) b += f[a].bytes() + c[a].bytes(); return b; }
+    void release() { for (int a = 0; a < 3; ++a) { f[a].release(); c[a].release(); } }
+};
+
 struct fia_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     DevBuf<double> psi;
     DevBuf<float> disp[3];
-    DevBuf<double> fa[3], fb[3];
+    FluxSet fsa, fsb;      // face fluxes (and orientation corrections) of two passes
     DevBuf<float> work[3];
-    DevBuf<int> twisted;
+    DevBuf<unsigned> twisted;
     DevBuf<int> counters;   // [0] twisted count, [1] error
     DevBuf<float> grid[4];  // device mirrors of host grids: rho, u, v, w
     DevBuf<float> gen[3];   // scratch for test entry points
@@ -106,7 +113,8 @@ namespace {
 size_t ctx_bytes(const fia_ctx *c) {
     size_t b = c->psi.bytes() + c->twisted.bytes() + c->counters.bytes() + c->scal.bytes() + c->bits.bytes() +
                c->solid.bytes() + c->genI.bytes();
-    for (int a = 0; a < 3; ++a) b += c->disp[a].bytes() + c->fa[a].bytes() + c->fb[a].bytes() + c->work[a].bytes() + c->gen[a].bytes();
+    b += c->fsa.bytes() + c->fsb.bytes();
+    for (int a = 0; a < 3; ++a) b += c->disp[a].bytes() + c->work[a].bytes() + c->gen[a].bytes();
     for (int a = 0; a < 4; ++a) b += c->grid[a].bytes();
     for (int a = 0; a < 2; ++a) b += c->genD[a].bytes();
     for (int a = 0; a < 8; ++a) b += c->pcg[a].bytes();
@@ -201,11 +209,12 @@ fia_status do_vertices(fia_ctx *c, const CompGrid &g, const float *u, const floa
     return FIA_OK;
 }
 
-fia_status do_faces(fia_ctx *c, const CompGrid &g, const float *q, bool moving, DevBuf<double> *flux,
-                    const PassSpec &s, int fam_mask, double h) {
+fia_status do_faces(fia_ctx *c, const CompGrid &g, const float *q, bool moving, FluxSet &fs, const PassSpec &s,
+                    int fam_mask, double h) {
     for (int f = 0; f < 3; ++f) {
         if (g.faces(f) >= (size_t(1) << 30)) return fail(FIA_ERR_INVALID_ARG, "grid too large for the face index");
-        HIP_TRY(flux[f].ensure(g.faces(f)));
+        HIP_TRY(fs.f[f].ensure(g.faces(f)));
+        HIP_TRY(fs.c[f].ensure(g.faces(f)));
     }
     HIP_TRY(c->twisted.ensure(g.faces(0) + g.faces(1) + g.faces(2)));
     hipLaunchKernelGGL(zero_ints_kernel, dim3(1), dim3(64), 0, c->stream, c->counters.p, 1);
@@ -213,7 +222,11 @@ fia_status do_faces(fia_ctx *c, const CompGrid &g, const float *q, bool moving, 
     FaceParams p;
     memset(&p, 0, sizeof(p));
     p.g = g; p.q = q; p.psi = c->psi.p;
-    for (int a = 0; a < 3; ++a) { p.disp[a] = moving ? c->disp[a].p : nullptr; p.flux[a] = flux[a].p; }
+    for (int a = 0; a < 3; ++a) {
+        p.disp[a] = moving ? c->disp[a].p : nullptr;
+        p.flux[a] = fs.f[a].p;
+        p.corr[a] = fs.c[a].p;
+    }
     p.twisted = c->twisted.p; p.twisted_count = c->counters.p; p.error = c->counters.p + 1;
     p.split = s.split; p.interp = s.interp; p.integral = s.integral; p.fam_mask = fam_mask;
     p.general_all = s.min_area < 1.0;
@@ -233,11 +246,15 @@ fia_status do_faces(fia_ctx *c, const CompGrid &g, const float *q, bool moving, 
     return FIA_OK;
 }
 
-hipError_t timed_gather(fia_ctx *c, const CompGrid &g, double *const f1[3], double *const f2[3], const float *base,
-                        float *out, int mode, int fm, hipStream_t s) {
+hipError_t timed_gather(fia_ctx *c, const CompGrid &g, FluxSet &s1, FluxSet *s2, const float *base, float *out,
+                        int mode, int fm) {
     hipError_t e = span_begin(c, SK_GATHER);
     if (e != hipSuccess) return e;
-    e = launch_gather(g, f1, f2, base, out, mode, fm, s);
+    double *f1[3] = {s1.f[0].p, s1.f[1].p, s1.f[2].p}, *c1[3] = {s1.c[0].p, s1.c[1].p, s1.c[2].p};
+    double *f2[3] = {nullptr, nullptr, nullptr}, *c2[3] = {nullptr, nullptr, nullptr};
+    if (s2)
+        for (int a = 0; a < 3; ++a) { f2[a] = s2->f[a].p; c2[a] = s2->c[a].p; }
+    e = launch_gather(g, f1, c1, s2 ? f2 : nullptr, s2 ? c2 : nullptr, base, out, mode, fm, c->stream);
     if (e != hipSuccess) return e;
     if (c->timing) c->acc.gather_cells += (long long)g.cells();
     return span_end(c);
@@ -250,9 +267,8 @@ fia_status run_pass(fia_ctx *c, const CompGrid &g, const float *q_in, float *q_o
     if ((st = do_psi(c, g, q_in, s.psi_mode, s.integral, s.ng)) != FIA_OK) return st;
     if (!s.zero_velocity && (st = do_vertices(c, g, u, v, w, dt / h, s.interp)) != FIA_OK) return st;
     const int fm = s.zero_velocity ? 2 : 7;   // undeformed x and z faces have exactly zero y area
-    if ((st = do_faces(c, g, q_in, !s.zero_velocity, c->fa, s, fm, h)) != FIA_OK) return st;
-    double *f[3] = {c->fa[0].p, c->fa[1].p, c->fa[2].p};
-    HIP_TRY(timed_gather(c, g,f, nullptr, nullptr, q_out, 0, fm, c->stream));
+    if ((st = do_faces(c, g, q_in, !s.zero_velocity, c->fsa, s, fm, h)) != FIA_OK) return st;
+    HIP_TRY(timed_gather(c, g, c->fsa, nullptr, nullptr, q_out, 0, fm));
     return FIA_OK;
 }
 
@@ -276,11 +292,9 @@ fia_status run_step(fia_ctx *c, const CompGrid &g, const float *q, float *out, c
                 o->minimal_area};
     if ((st = do_psi(c, g, q, p1.psi_mode, p1.integral, p1.ng)) != FIA_OK) return st;
     if ((st = do_vertices(c, g, u, v, w, dt / h, p1.interp)) != FIA_OK) return st;
-    if ((st = do_faces(c, g, q, true, c->fa, p1, 7, h)) != FIA_OK) return st;
-    double *fa[3] = {c->fa[0].p, c->fa[1].p, c->fa[2].p};
-    double *fb[3] = {c->fb[0].p, c->fb[1].p, c->fb[2].p};
+    if ((st = do_faces(c, g, q, true, c->fsa, p1, 7, h)) != FIA_OK) return st;
     if (!o->enable_zero_velocity_correction) {
-        HIP_TRY(timed_gather(c, g,fa, nullptr, nullptr, out, 0, 7, c->stream));
+        HIP_TRY(timed_gather(c, g, c->fsa, nullptr, nullptr, out, 0, 7));
         return FIA_OK;
     }
     // pass 2: zero velocity, non-integral psi (reference src/move_substances_3d.cpp:133-153)
@@ -288,18 +302,16 @@ fia_status run_step(fia_ctx *c, const CompGrid &g, const float *q, float *out, c
     PassSpec p2{split_main, (int)o->interpolation_method, FIA_PSI_CONST, 1, 0, o->num_gauss_points_integrate_density,
                 o->minimal_area};
     if ((st = do_psi(c, g, q, p2.psi_mode, p2.integral, p2.ng)) != FIA_OK) return st;
-    if ((st = do_faces(c, g, q, false, c->fb, p2, 2, h)) != FIA_OK) return st;
-    fb[0] = c->fb[0].p; fb[1] = c->fb[1].p; fb[2] = c->fb[2].p;
+    if ((st = do_faces(c, g, q, false, c->fsb, p2, 2, h)) != FIA_OK) return st;
     float *err = c->work[2].p;
-    HIP_TRY(timed_gather(c, g,fb, nullptr, q, err, 2, 2, c->stream));   // err = q - pass2(q)
+    HIP_TRY(timed_gather(c, g, c->fsb, nullptr, q, err, 2, 2));   // err = q - pass2(q)
     // pass 3: error advected with 1Dy_linear (reference src/move_substances_3d.cpp:177-225)
     PassSpec p3{split_err, FIA_INTERP_1DY_LINEAR, FIA_PSI_CONST, 0, 0, o->num_gauss_points_integrate_density,
                 o->minimal_area};
     if ((st = do_psi(c, g, err, p3.psi_mode, p3.integral, p3.ng)) != FIA_OK) return st;
     if ((st = do_vertices(c, g, u, v, w, dt / h, p3.interp)) != FIA_OK) return st;
-    if ((st = do_faces(c, g, err, true, c->fb, p3, 7, h)) != FIA_OK) return st;
-    fb[0] = c->fb[0].p; fb[1] = c->fb[1].p; fb[2] = c->fb[2].p;
-    HIP_TRY(timed_gather(c, g,fa, fb, nullptr, out, 1, 7, c->stream));
+    if ((st = do_faces(c, g, err, true, c->fsb, p3, 7, h)) != FIA_OK) return st;
+    HIP_TRY(timed_gather(c, g, c->fsa, &c->fsb, nullptr, out, 1, 7));
     return FIA_OK;
 }
 
@@ -420,7 +432,8 @@ fia_status fia_ctx_destroy(fia_ctx *c) {
     (void)hipStreamSynchronize(c->stream);
     c->psi.release(); c->twisted.release(); c->counters.release(); c->scal.release(); c->bits.release();
     c->solid.release(); c->genI.release();
-    for (int a = 0; a < 3; ++a) { c->disp[a].release(); c->fa[a].release(); c->fb[a].release(); c->work[a].release(); c->gen[a].release(); }
+    c->fsa.release(); c->fsb.release();
+    for (int a = 0; a < 3; ++a) { c->disp[a].release(); c->work[a].release(); c->gen[a].release(); }
     for (int a = 0; a < 4; ++a) c->grid[a].release();
     for (int a = 0; a < 2; ++a) c->genD[a].release();
     for (int a = 0; a < 8; ++a) c->pcg[a].release();

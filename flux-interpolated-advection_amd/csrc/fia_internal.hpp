@@ -57,7 +57,9 @@ struct FaceParams {
     const double *psi;       // psi on the y-faces of G, in units of q * cells
     const float *disp[3];    // vertex displacements (cells); null = zero velocity
     double *flux[3];         // per family face fluxes (normalised: q * cells^3 / cells^3)
-    int *twisted;            // list of faces for the general cutter: fam << 30 | index
+    double *corr[3];         // per family: (upper cell's own evaluation) + flux, nonzero only for
+                             // faces the general cutter handled (kept zero elsewhere)
+    unsigned *twisted;       // list of faces for the general cutter: fam << 30 | index (unsigned)
     int *twisted_count;
     int *error;              // first error code seen (0 = none)
     int split, interp, integral;
@@ -78,8 +80,8 @@ hipError_t launch_faces(const FaceParams &p, hipStream_t s);
 hipError_t launch_faces_general(const FaceParams &p, hipStream_t s);
 // out = sum over families of flux differences (scaled); mode 0: out = G1; 1: out = G1 + G2;
 // 2: out = base - G1 (y family only).
-hipError_t launch_gather(const CompGrid &g, double *const f1[3], double *const f2[3], const float *base,
-                         float *out, int mode, int fam_mask, hipStream_t s);
+hipError_t launch_gather(const CompGrid &g, double *const f1[3], double *const c1[3], double *const f2[3],
+                         double *const c2[3], const float *base, float *out, int mode, int fam_mask, hipStream_t s);
 hipError_t launch_interpolate(const float *field, int mx, int my, int mz, double ox, double oy, double oz,
                               int method, float kappa, const float *pts_cells, size_t n, float *out,
                               hipStream_t s);

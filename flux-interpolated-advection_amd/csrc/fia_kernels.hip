@@ -36,7 +36,7 @@ __global__ void __launch_bounds__(kBlock) psi_kernel(PsiParams P) {
             inc = Q(i, jj, k);
         } else if (P.mode == 1) {
             // trilinear at (x_c, jj + 1/4, z_c) and (x_c, jj + 3/4, z_c): x, z on nodes
-            const float a = 0.75f * Q(i, jj - 1, k) + 0.25f * Q(i, jj, k);   // sample jj - 1/4
+            const float a = 0.25f * Q(i, jj - 1, k) + 0.75f * Q(i, jj, k);   // sample jj - 1/4
             const float b = 0.75f * Q(i, jj, k) + 0.25f * Q(i, jj + 1, k);   // sample jj + 1/4
             inc = 0.5f * a + 0.5f * b;
         } else {
@@ -198,11 +198,12 @@ template <int FAM> __device__ __forceinline__ void face_body(const FaceParams &P
     if (!load_face(P, FAM, i, j, k, v, col)) {
         report(P.error, KE_NONFINITE);
         P.flux[FAM][id] = 0.0;
+        P.corr[FAM][id] = 0.0;
         return;
     }
     if (P.general_all) {
         const int slot = atomicAdd(P.twisted_count, 1);
-        P.twisted[slot] = (FAM << 30) | (int)id;
+        P.twisted[slot] = ((unsigned)FAM << 30) | (unsigned)id;
         return;
     }
     FaceGeom geom;
@@ -234,7 +235,7 @@ template <int FAM> __device__ __forceinline__ void face_body(const FaceParams &P
             pp.close<0>(geom);
             if (pp.twisted()) {
                 const int slot = atomicAdd(P.twisted_count, 1);
-                P.twisted[slot] = (FAM << 30) | (int)id;
+                P.twisted[slot] = ((unsigned)FAM << 30) | (unsigned)id;
                 return;
             }
             if (pp.acc.n >= 3 && pp.acc.area2 != 0.0) {
@@ -251,6 +252,7 @@ template <int FAM> __device__ __forceinline__ void face_body(const FaceParams &P
         }
     }
     P.flux[FAM][id] = flux;
+    P.corr[FAM][id] = 0.0;
 }
 
 __device__ __forceinline__ void add_stats(unsigned long long *stats, int a, int b) {
@@ -272,42 +274,64 @@ template <int FAM> __global__ void __launch_bounds__(kBlock) face_kernel(FacePar
     if (P.stats) add_stats(P.stats, npieces, npts);
 }
 
-// Exact recursive cutter for the faces listed by face_kernel.  Small fixed grid: the private
-// stack is large and only a few faces need it.
+// Flux of one face with the exact recursive cutter, corners in the given order.
+__device__ __forceinline__ double general_face_flux(const FaceParams &P, const double v[4][3], const double col[4][2],
+                                                    bool &ok) {
+    FaceGeom geom;
+    geom.init(v);
+    double flux = 0.0;
+    auto emit = [&](const double (*pts)[2], int num, double cxi, double ceta) {
+        double ay2 = 0.0;
+        double r0[3], rp[3];
+        geom.rel(pts[0][0], pts[0][1], r0);
+        rp[0] = r0[0]; rp[1] = r0[1]; rp[2] = r0[2];
+        for (int n = 1; n <= num; ++n) {
+            double rn[3];
+            if (n == num) { rn[0] = r0[0]; rn[1] = r0[1]; rn[2] = r0[2]; }
+            else geom.rel(pts[n][0], pts[n][1], rn);
+            ay2 += rp[2] * rn[0] - rp[0] * rn[2];
+            rp[0] = rn[0]; rp[1] = rn[1]; rp[2] = rn[2];
+        }
+        double r[3];
+        geom.rel(cxi, ceta, r);
+        const double c3[3] = {geom.p0[0] + r[0], geom.p0[1] + r[1], geom.p0[2] + r[2]};
+        const double ay = 0.5 * ay2;
+        int npts = 0;
+        if (ay != 0.0) flux += piece_flux(P, c3, ay, npts);
+    };
+    ok = general_cut(col, P.min_area, emit);
+    return flux;
+}
+
+// Exact recursive cutter for the faces listed by face_kernel.  The reference evaluates every
+// face twice, once per neighbouring cell with the corner order reversed
+// (src/backtrace_and_calc_all_quadrature_point_list_for_integrate_density.cpp:77-112); for a
+// twisted face its cutter's pieces depend on that order, so both evaluations are done here: the
+// lower cell's ('+' order v0 v1 v2 v3) goes to flux, the upper cell's (v0 v3 v2 v1) enters
+// through corr = upper + flux.  Small fixed grid: the private stack is large and only a few
+// faces need it.
 __global__ void __launch_bounds__(64) general_kernel(FaceParams P) {
     const int count = *P.twisted_count;
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < count; t += gridDim.x * blockDim.x) {
-        const int code = P.twisted[t];
-        const int fam = code >> 30;
-        const long id = code & ((1 << 30) - 1);
+        const unsigned code = P.twisted[t];
+        const int fam = (int)(code >> 30);   // 0, 1 or 2
+        const long id = (long)(code & ((1u << 30) - 1u));
         int i, j, k;
         decode_face(P.g, fam, id, i, j, k);
         double v[4][3], col[4][2];
         load_face(P, fam, i, j, k, v, col);
-        FaceGeom geom;
-        geom.init(v);
-        double flux = 0.0;
-        auto emit = [&](const double (*pts)[2], int num, double cxi, double ceta) {
-            double ay2 = 0.0;
-            double r0[3], rp[3];
-            geom.rel(pts[0][0], pts[0][1], r0);
-            rp[0] = r0[0]; rp[1] = r0[1]; rp[2] = r0[2];
-            for (int n = 1; n <= num; ++n) {
-                double rn[3];
-                if (n == num) { rn[0] = r0[0]; rn[1] = r0[1]; rn[2] = r0[2]; }
-                else geom.rel(pts[n][0], pts[n][1], rn);
-                ay2 += rp[2] * rn[0] - rp[0] * rn[2];
-                rp[0] = rn[0]; rp[1] = rn[1]; rp[2] = rn[2];
-            }
-            double r[3];
-            geom.rel(cxi, ceta, r);
-            const double c3[3] = {geom.p0[0] + r[0], geom.p0[1] + r[1], geom.p0[2] + r[2]};
-            const double ay = 0.5 * ay2;
-            int npts = 0;
-            if (ay != 0.0) flux += piece_flux(P, c3, ay, npts);
-        };
-        if (!general_cut(col, P.min_area, emit)) report(P.error, KE_CAPACITY);
-        P.flux[fam][id] = flux;
+        bool ok1, ok2;
+        const double lower = general_face_flux(P, v, col, ok1);
+        double vr[4][3], cr[4][2];
+        const int order[4] = {0, 3, 2, 1};
+        for (int c = 0; c < 4; ++c) {
+            for (int a = 0; a < 3; ++a) vr[c][a] = v[order[c]][a];
+            cr[c][0] = col[order[c]][0]; cr[c][1] = col[order[c]][1];
+        }
+        const double upper = general_face_flux(P, vr, cr, ok2);
+        if (!ok1 || !ok2) report(P.error, KE_CAPACITY);
+        P.flux[fam][id] = lower;
+        P.corr[fam][id] = upper + lower;
     }
 }
 
@@ -315,19 +339,29 @@ __global__ void __launch_bounds__(64) general_kernel(FaceParams P) {
 // gather: q(i,j,k) = sum over the six faces of +-flux (the '+' face of a cell is its own
 // orientation, the '-' face is the neighbour's '+' face reversed).
 // ---------------------------------------------------------------------------------------------
-__device__ __forceinline__ double cell_sum(const CompGrid &g, double *const f[3], int i, int j, int k, int fm) {
+__device__ __forceinline__ double cell_sum(const CompGrid &g, double *const f[3], double *const c[3], int i, int j,
+                                           int k, int fm) {
     const int my = g.m[1], mz = g.m[2];
     double s = 0.0;
-    if (fm & 1) s += f[0][idx3(i + 1, j, k, my, mz)] - f[0][idx3(i, j, k, my, mz)];
-    if (fm & 2) s += f[1][idx3(i, j + 1, k, my + 1, mz)] - f[1][idx3(i, j, k, my + 1, mz)];
-    if (fm & 4) s += f[2][idx3(i, j, k + 1, my, mz + 1)] - f[2][idx3(i, j, k, my, mz + 1)];
+    if (fm & 1) {
+        const size_t lo = idx3(i, j, k, my, mz);
+        s += f[0][idx3(i + 1, j, k, my, mz)] - f[0][lo] + c[0][lo];
+    }
+    if (fm & 2) {
+        const size_t lo = idx3(i, j, k, my + 1, mz);
+        s += f[1][idx3(i, j + 1, k, my + 1, mz)] - f[1][lo] + c[1][lo];
+    }
+    if (fm & 4) {
+        const size_t lo = idx3(i, j, k, my, mz + 1);
+        s += f[2][idx3(i, j, k + 1, my, mz + 1)] - f[2][lo] + c[2][lo];
+    }
     return s;
 }
 
 struct GatherArgs {
     CompGrid g;
-    double *f1[3];
-    double *f2[3];
+    double *f1[3], *c1[3];
+    double *f2[3], *c2[3];
     const float *base;
     float *out;
     int mode, fam_mask;
@@ -338,8 +372,8 @@ __global__ void __launch_bounds__(kBlock) gather_kernel(GatherArgs A) {
     if (id >= (long)A.g.cells()) return;
     const int mz = A.g.m[2], my = A.g.m[1];
     const int k = (int)(id % mz), j = (int)((id / mz) % my), i = (int)(id / ((long)my * mz));
-    double s = cell_sum(A.g, A.f1, i, j, k, A.fam_mask);
-    if (A.mode == 1) s += cell_sum(A.g, A.f2, i, j, k, 7);
+    double s = cell_sum(A.g, A.f1, A.c1, i, j, k, A.fam_mask);
+    if (A.mode == 1) s += cell_sum(A.g, A.f2, A.c2, i, j, k, 7);
     if (A.mode == 2) s = (double)A.base[id] - s;
     A.out[id] = (float)s;
 }
@@ -491,11 +525,14 @@ hipError_t launch_faces_general(const FaceParams &p, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_gather(const CompGrid &g, double *const f1[3], double *const f2[3], const float *base, float *out,
-                         int mode, int fam_mask, hipStream_t s) {
+hipError_t launch_gather(const CompGrid &g, double *const f1[3], double *const c1[3], double *const f2[3],
+                         double *const c2[3], const float *base, float *out, int mode, int fam_mask, hipStream_t s) {
     GatherArgs A;
     A.g = g;
-    for (int a = 0; a < 3; ++a) { A.f1[a] = f1[a]; A.f2[a] = f2 ? f2[a] : nullptr; }
+    for (int a = 0; a < 3; ++a) {
+        A.f1[a] = f1[a]; A.c1[a] = c1[a];
+        A.f2[a] = f2 ? f2[a] : nullptr; A.c2[a] = c2 ? c2[a] : nullptr;
+    }
     A.base = base; A.out = out; A.mode = mode; A.fam_mask = fam_mask;
     hipLaunchKernelGGL(gather_kernel, dim3(grid_for(g.cells(), kBlock)), dim3(kBlock), 0, s, A);
     return hipGetLastError();

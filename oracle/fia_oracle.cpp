@@ -467,12 +467,18 @@ static void face_colours(const R vert[4][3], R h, int split, C col[4][2]) {
     }
 }
 
-// Bilinear xi-eta -> real-space map (reference src/cell_face_3D.cpp:68-73).
+// Bilinear xi-eta -> real-space map (reference src/cell_face_3D.cpp:68-73).  The reference
+// writes it as (1-xi)(1-eta) v0 + xi(1-eta) v1 + xi eta v2 + (1-xi) eta v3; here it is the same
+// polynomial written around corner 0, v0 + xi (v1-v0) + eta (v3-v0) + xi eta (v0-v1+v2-v3), so
+// that a coordinate shared by all four corners comes out exactly.  That matters on faces that
+// stay on a y-level (every face on the top and bottom walls): the integral-psi rule switches
+// branch at floor(c_y) with a jump of the order of the interpolation error, and the reference's
+// form can round c_y to either side of the level.
 template <class R> static void bilinear(const R v[4][3], R xi, R eta, R out[3]) {
-    // the reference forms the scalar weights in double (1.0 literals) and rounds them to R
-    const double x = xi, e = eta;
-    const R a = R((1.0 - x) * (1.0 - e)), b = R(x * (1.0 - e)), c = R(x * e), d = R((1.0 - x) * e);
-    for (int k = 0; k < 3; ++k) out[k] = a * v[0][k] + b * v[1][k] + c * v[2][k] + d * v[3][k];
+    for (int k = 0; k < 3; ++k) {
+        const R e1 = v[1][k] - v[0][k], e3 = v[3][k] - v[0][k], e13 = v[0][k] - v[1][k] + v[2][k] - v[3][k];
+        out[k] = v[0][k] + (xi * e1 + eta * e3 + xi * eta * e13);
+    }
 }
 
 // Piece centre and area-scaled normal (reference src/split_face_3D.cpp:67-93).
@@ -557,18 +563,89 @@ struct PassOpts {
 // src/calc_velocity_from_quadrature_point_list.cpp:8-129).
 // The velocity field used for backtracing is vel; q (G-sized) is advected in place.
 // Returns 0 on success, 3 on cutter capacity overflow.
+// Everything one pass needs to integrate one face: the fields, psi and the options.
+template <class R, class C> struct FaceIntegrator {
+    const Comp &G;
+    const Velocity<R> &vel;
+    R h, dt;
+    const PassOpts &o;
+    Field<R> Q, PS;
+    GaussRule gr;
+    R oy, h3;
+
+    FaceIntegrator(const Comp &g, const Velocity<R> &v, R h_, R dt_, const R *q, const R *psi, const PassOpts &opt)
+        : G(g), vel(v), h(h_), dt(dt_), o(opt),
+          Q{q, g.m[0], g.m[1], g.m[2], 0.5 - 0.5 * g.e[0], 0.5 - 0.5 * g.e[1], 0.5 - 0.5 * g.e[2]},
+          PS{psi, g.m[0], g.m[1] + 1, g.m[2], 0.5 - 0.5 * g.e[0], -0.5 * g.e[1], 0.5 - 0.5 * g.e[2]},
+          gr(gauss_legendre(opt.ng)), oy(R(g.origin(1)) * h_), h3(h_ * h_ * h_) {}
+
+    // Contribution of the face with corner lattice indices lat[4] (reference vertex order) to the
+    // cell that owns it, split into the density-point and psi-point parts.  Returns false on
+    // cutter overflow.
+    bool face(const int lat[4][3], R &acc_density, R &acc_psi, long &pieces) const {
+        R vert[4][3];
+        for (int k = 0; k < 4; ++k)
+            backtrace_vertex<R>(G, vel, o.interp, h, dt, lat[k][0], lat[k][1], lat[k][2], o.zero_velocity != 0,
+                                vert[k]);
+        C col[4][2];
+        face_colours<C, R>(vert, h, o.split, col);
+        auto emit = [&](const P2<C> *poly, int num, const P2<C> &cen, C) {
+            ++pieces;
+            R pos[3], an[3];
+            piece_geometry<R, C>(vert, poly, num, cen, pos, an);
+            if (o.use_integral) {
+                // reference src/polygon_3D.cpp:275-478
+                const int j = (int)std::floor((pos[1] - oy) / h);
+                const R over = ((pos[1] - oy) / h) - R(j);
+                if (over < R(0.5)) {
+                    const R a = R(j) * h + oy, b = pos[1], len = b - a;
+                    for (int g = 0; g < o.ng; ++g) {
+                        const R yg = (len / R(2)) * R(gr.x[g]) + (len / R(2)) + a;
+                        const R wy = an[1] * over * h * R(gr.w[g]);
+                        acc_density += interp(Q, o.interp, pos[0], yg, pos[2], h) * wy / h3;
+                    }
+                } else {
+                    for (int half = 0; half < 2; ++half) {
+                        const R a = half == 0 ? R(j) * h + oy : (R(j) + R(0.5)) * h + oy;
+                        const R b = half == 0 ? (R(j) + R(0.5)) * h + oy : (R(j) + over) * h + oy;
+                        const R frac = half == 0 ? R(0.5) : over - R(0.5);
+                        const R len = b - a;
+                        for (int g = 0; g < o.ng; ++g) {
+                            const R yg = (len / R(2)) * R(gr.x[g]) + (len / R(2)) + a;
+                            const R wy = an[1] * frac * h * R(gr.w[g]);
+                            acc_density += interp(Q, o.interp, pos[0], yg, pos[2], h) * wy / h3;
+                        }
+                    }
+                }
+                const R py = R(j) * h + oy;
+                acc_psi += interp(PS, o.interp, pos[0], py, pos[2], h) * an[1] / h3;
+            } else {
+                acc_psi += interp(PS, o.interp, pos[0], pos[1], pos[2], h) * an[1] / h3;
+            }
+        };
+        return split_xieta<C>(col, C(o.min_area), emit);
+    }
+};
+
+// One flux pass over G: psi, all faces of all cells, accumulation (reference
+// src/move_substances_3d.cpp:585-841 and src/advect_velocity_flux_advection_3d.cpp:252-576,
+// producers src/backtrace_and_calc_all_quadrature_point_list_for_integrate_density.cpp:7-597,
+// src/backtrace_and_calc_all_quadrature_point_list.cpp:7-280, quadrature
+// src/polygon_3D.cpp:57-77,257-487, consumers
+// src/calc_density_in_cell_from_quadrature_point_list_for_integrate_density.cpp:119-201,
+// src/calc_density_in_cell_from_quadrature_point_list.cpp:98-132,
+// src/calc_velocity_from_quadrature_point_list_use_integral.cpp:12-181,
+// src/calc_velocity_from_quadrature_point_list.cpp:8-129).
+// The velocity field used for backtracing is vel; q (G-sized) is advected in place.
+// Returns 0 on success, 3 on cutter capacity overflow.
 template <class R, class C>
 static int advect_pass(const Comp &G, const Velocity<R> &vel, R h, R dt, R *q, const PassOpts &o,
                        int nthreads, double *stats) {
     const int mx = G.m[0], my = G.m[1], mz = G.m[2];
     std::vector<R> psi((size_t)mx * (my + 1) * mz);
     calc_psi<R>(G, q, h, o.psi_mode, o.use_integral, o.ng, psi.data());
-    const Field<R> Q{q, mx, my, mz, 0.5 - 0.5 * G.e[0], 0.5 - 0.5 * G.e[1], 0.5 - 0.5 * G.e[2]};
-    const Field<R> PS{psi.data(), mx, my + 1, mz, 0.5 - 0.5 * G.e[0], -0.5 * G.e[1], 0.5 - 0.5 * G.e[2]};
+    const FaceIntegrator<R, C> fi(G, vel, h, dt, q, psi.data(), o);
     std::vector<R> result((size_t)mx * my * mz, R(0));
-    const GaussRule gr = gauss_legendre(o.ng);
-    const R oy = R(G.origin(1)) * h;
-    const R h3 = h * h * h;
     std::atomic<int> fail{0};
     std::atomic<long> npieces{0};
 
@@ -579,47 +656,12 @@ static int advect_pass(const Comp &G, const Velocity<R> &vel, R h, R dt, R *q, c
         for (int iz = 0; iz < mz; ++iz) {
             R acc_density = R(0), acc_psi = R(0);
             for (int f = 0; f < 6; ++f) {
-                R vert[4][3];
-                for (int k = 0; k < 4; ++k)
-                    backtrace_vertex<R>(G, vel, o.interp, h, dt, ix + kFaceVerts[f][k][0], iy + kFaceVerts[f][k][1],
-                                        iz + kFaceVerts[f][k][2], o.zero_velocity != 0, vert[k]);
-                C col[4][2];
-                face_colours<C, R>(vert, h, o.split, col);
-                auto emit = [&](const P2<C> *poly, int num, const P2<C> &cen, C) {
-                    ++local_pieces;
-                    R pos[3], an[3];
-                    piece_geometry<R, C>(vert, poly, num, cen, pos, an);
-                    if (o.use_integral) {
-                        // reference src/polygon_3D.cpp:275-478
-                        const int j = (int)std::floor((pos[1] - oy) / h);
-                        const R over = ((pos[1] - oy) / h) - R(j);
-                        if (over < R(0.5)) {
-                            const R a = R(j) * h + oy, b = pos[1], len = b - a;
-                            for (int g = 0; g < o.ng; ++g) {
-                                const R yg = (len / R(2)) * R(gr.x[g]) + (len / R(2)) + a;
-                                const R wy = an[1] * over * h * R(gr.w[g]);
-                                acc_density += interp(Q, o.interp, pos[0], yg, pos[2], h) * wy / h3;
-                            }
-                        } else {
-                            for (int half = 0; half < 2; ++half) {
-                                const R a = half == 0 ? R(j) * h + oy : (R(j) + R(0.5)) * h + oy;
-                                const R b = half == 0 ? (R(j) + R(0.5)) * h + oy : (R(j) + over) * h + oy;
-                                const R frac = half == 0 ? R(0.5) : over - R(0.5);
-                                const R len = b - a;
-                                for (int g = 0; g < o.ng; ++g) {
-                                    const R yg = (len / R(2)) * R(gr.x[g]) + (len / R(2)) + a;
-                                    const R wy = an[1] * frac * h * R(gr.w[g]);
-                                    acc_density += interp(Q, o.interp, pos[0], yg, pos[2], h) * wy / h3;
-                                }
-                            }
-                        }
-                        const R py = R(j) * h + oy;
-                        acc_psi += interp(PS, o.interp, pos[0], py, pos[2], h) * an[1] / h3;
-                    } else {
-                        acc_psi += interp(PS, o.interp, pos[0], pos[1], pos[2], h) * an[1] / h3;
-                    }
-                };
-                if (!split_xieta<C>(col, C(o.min_area), emit)) { fail = 3; return; }
+                int lat[4][3];
+                for (int k = 0; k < 4; ++k) {
+                    lat[k][0] = ix + kFaceVerts[f][k][0]; lat[k][1] = iy + kFaceVerts[f][k][1];
+                    lat[k][2] = iz + kFaceVerts[f][k][2];
+                }
+                if (!fi.face(lat, acc_density, acc_psi, local_pieces)) { fail = 3; return; }
             }
             result[idx3(ix, iy, iz, my, mz)] = acc_density + acc_psi;
         }
@@ -635,6 +677,38 @@ static int advect_pass(const Comp &G, const Velocity<R> &vel, R h, R dt, R *q, c
     if (fail) return fail;
     std::copy(result.begin(), result.end(), q);
     if (stats) stats[0] += (double)npieces.load();
+    return 0;
+}
+
+// Flux through every unique face of G in the orientation of the reference's faces 1 (x+),
+// 2 (y+) and 5 (z+) of the cell below it, normalised to units of q (the contribution to that
+// cell times h^3 / h^3): fx ((mx+1) my mz), fy (mx (my+1) mz), fz (mx my (mz+1)).
+template <class R, class C>
+static int face_fluxes(const Comp &G, const Velocity<R> &vel, R h, R dt, const R *q, const PassOpts &o,
+                       double *fx, double *fy, double *fz) {
+    const int mx = G.m[0], my = G.m[1], mz = G.m[2];
+    std::vector<R> psi((size_t)mx * (my + 1) * mz);
+    calc_psi<R>(G, q, h, o.psi_mode, o.use_integral, o.ng, psi.data());
+    const FaceIntegrator<R, C> fi(G, vel, h, dt, q, psi.data(), o);
+    double *out[3] = {fx, fy, fz};
+    const int face_type[3] = {1, 2, 4 + 1};
+    long pieces = 0;
+    for (int fam = 0; fam < 3; ++fam) {
+        const int dx = mx + (fam == 0), dy = my + (fam == 1), dz = mz + (fam == 2);
+        for (int i = 0; i < dx; ++i) for (int j = 0; j < dy; ++j) for (int k = 0; k < dz; ++k) {
+            // the '+' face of cell (i,j,k) - e_fam: shift the reference table back by one along fam
+            int lat[4][3];
+            const int ft = face_type[fam];
+            for (int c = 0; c < 4; ++c) {
+                lat[c][0] = i + kFaceVerts[ft][c][0] - (fam == 0);
+                lat[c][1] = j + kFaceVerts[ft][c][1] - (fam == 1);
+                lat[c][2] = k + kFaceVerts[ft][c][2] - (fam == 2);
+            }
+            R a = R(0), b = R(0);
+            if (!fi.face(lat, a, b, pieces)) return 3;
+            out[fam][idx3(i, j, k, dy, dz)] = (double)((a + b) * h * h * h / (h * h * h));
+        }
+    }
     return 0;
 }
 
@@ -985,6 +1059,29 @@ extern "C" int orc_move_substances(int prec, int cprec, int nx, int ny, int nz, 
                            : move_t<float, double>(nx, ny, nz, h, rho, u, v, w, dt, split, method, integ, corr, ng, min_area, nthreads, stats);
     return cprec ? move_t<double, float>(nx, ny, nz, h, rho, u, v, w, dt, split, method, integ, corr, ng, min_area, nthreads, stats)
                  : move_t<double, double>(nx, ny, nz, h, rho, u, v, w, dt, split, method, integ, corr, ng, min_area, nthreads, stats);
+}
+
+template <class R, class C>
+static int ff_t(int comp, int nx, int ny, int nz, double h, const double *u, const double *v, const double *w,
+                const double *q, double dt, int split, int method, int psi_mode, int zero_vel, int integ, int ng,
+                double min_area, double *fx, double *fy, double *fz) {
+    std::vector<R> uu = to_r<R>(u, (size_t)(nx + 1) * ny * nz), vv = to_r<R>(v, (size_t)nx * (ny + 1) * nz),
+                   ww = to_r<R>(w, (size_t)nx * ny * (nz + 1));
+    Velocity<R> vel = make_vel<R>(nx, ny, nz, uu, vv, ww);
+    Comp G(comp, nx, ny, nz);
+    std::vector<R> qq = to_r<R>(q, (size_t)G.m[0] * G.m[1] * G.m[2]);
+    PassOpts o{comp, split, method, psi_mode, zero_vel, integ, ng, min_area};
+    return face_fluxes<R, C>(G, vel, R(h), R(dt), qq.data(), o, fx, fy, fz);
+}
+extern "C" int orc_face_fluxes(int prec, int cprec, int comp, int nx, int ny, int nz, double h, const double *u,
+                               const double *v, const double *w, const double *q, double dt, int split, int method,
+                               int psi_mode, int zero_vel, int integ, int ng, double min_area, double *fx,
+                               double *fy, double *fz) {
+    if (!gauss_n_supported(ng)) return 1;
+    if (prec) return cprec ? ff_t<float, float>(comp, nx, ny, nz, h, u, v, w, q, dt, split, method, psi_mode, zero_vel, integ, ng, min_area, fx, fy, fz)
+                           : ff_t<float, double>(comp, nx, ny, nz, h, u, v, w, q, dt, split, method, psi_mode, zero_vel, integ, ng, min_area, fx, fy, fz);
+    return cprec ? ff_t<double, float>(comp, nx, ny, nz, h, u, v, w, q, dt, split, method, psi_mode, zero_vel, integ, ng, min_area, fx, fy, fz)
+                 : ff_t<double, double>(comp, nx, ny, nz, h, u, v, w, q, dt, split, method, psi_mode, zero_vel, integ, ng, min_area, fx, fy, fz);
 }
 
 // Velocity self-advection; dim = 0,1,2 advects one component, -1 all three through the pre-step

@@ -65,6 +65,9 @@ def lib():
         L.orc_advect_velocity.argtypes = [ctypes.c_int] * 6 + [ctypes.c_double, _dp, _dp, _dp, ctypes.c_double,
                                           ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                           ctypes.c_int, _dp]
+        L.orc_face_fluxes.argtypes = [ctypes.c_int] * 6 + [ctypes.c_double, _dp, _dp, _dp, _dp, ctypes.c_double,
+                                      ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_int, ctypes.c_double, _dp, _dp, _dp]
         L.orc_cfl_time_step.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double, _dp, _dp, _dp, ctypes.c_double,
                                         ctypes.c_int, ctypes.c_int, ctypes.c_double, _ip]
         L.orc_cfl_time_step.restype = ctypes.c_double
@@ -159,6 +162,27 @@ def advect_pass(q, u, v, w, h, dt, split="xz-CellFaceAligned", method="WENO6", p
     if stats is not None:
         stats["pieces"] = stats.get("pieces", 0) + st[0]
     return q
+
+
+def face_fluxes(q, u, v, w, h, dt, split="xz-CellFaceAligned", method="WENO6", psi_mode="WENO6",
+                zero_velocity=False, use_integral=True, num_gauss=1, min_area=1.0, comp=-1, prec=0, cprec=0):
+    """Flux through every unique face (x+, y+, z+ orientation of the cell below), in units of q."""
+    q = _f64(q)
+    u, v, w = _f64(u), _f64(v), _f64(w)
+    nx, ny, nz = u.shape[0] - 1, u.shape[1], u.shape[2]
+    mx, my, mz = q.shape
+    fx = np.zeros((mx + 1, my, mz)); fy = np.zeros((mx, my + 1, mz)); fz = np.zeros((mx, my, mz + 1))
+    rc = lib().orc_face_fluxes(prec, cprec, comp, nx, ny, nz, h, _p(u), _p(v), _p(w), _p(q), dt, SPLIT[split],
+                               INTERP[method], PSI[psi_mode], int(zero_velocity), int(use_integral), num_gauss,
+                               min_area, _p(fx), _p(fy), _p(fz))
+    if rc:
+        raise RuntimeError("orc_face_fluxes failed %d" % rc)
+    return fx, fy, fz
+
+
+def gather_fluxes(fx, fy, fz):
+    """Cell values from unique-face fluxes: + face minus - face along each axis."""
+    return (fx[1:] - fx[:-1]) + (fy[:, 1:] - fy[:, :-1]) + (fz[:, :, 1:] - fz[:, :, :-1])
 
 
 def move_substances(rho, u, v, w, h, dt, split="xz-CellFaceAligned", method="WENO6", use_integral=True,

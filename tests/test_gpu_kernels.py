@@ -59,7 +59,9 @@ def test_psi_matches_oracle(fia, oracle, ctx, scenes, mode, integral, ng):
 @pytest.mark.parametrize("dim", [-1, 0, 1, 2])
 def test_backtrace_matches_oracle(fia, oracle, ctx, scenes, method, dim):
     print("case backtrace", method, dim, flush=True)
-    n = 12; h = 1.0 / n
+    # h = 2^-k: lattice points then sit exactly on the columns of the piecewise-constant 1Dy
+    # rule in both FP32 and FP64 (for other h the reference's floor(x/h) depends on rounding)
+    n = 12; h = 1.0 / 16
     u, v, w = [a.astype(np.float32) for a in scenes.curl_noise(n, seed=2)]
     dt = scenes.cfl_dt(u, v, w, h, 2.0)
     grid = fia.Grid3D(n, n, n, h, None, u, v, w)
@@ -91,7 +93,7 @@ def test_split_faces_match_oracle(fia, oracle, ctx, split, min_area):
         cen, an = oracle.split_face(quads[i].astype(np.float32).astype(np.float64), h, split, min_area=min_area)
         gc, ga = got[i]
         assert len(gc) == len(cen), i
-        key = lambda c: np.lexsort((c[:, 2], c[:, 1], c[:, 0]))
+        key = lambda c: np.lexsort(tuple(np.round(c[:, a], 9) for a in (2, 1, 0)))
         oc, oa = cen[key(cen)], an[key(cen)]
         pc, pa = gc[key(gc)], ga[key(gc)]
         assert np.abs(oc - pc).max() < 1e-9 * 8

@@ -15,7 +15,10 @@ TOL = 1e-5
 
 
 def scene(scenes, n, cfl, seed=1, sigma=0.12):
-    h = 1.0 / n
+    # h = 2^-k for every n: lattice points then sit exactly on the columns of the piecewise-
+    # constant 1Dy rule in both FP32 and FP64 (for other h the reference's floor(x/h) on a lattice
+    # point depends on rounding); the domain is [0, n h]
+    h = 1.0 / 32 if n > 16 else 1.0 / 16
     rho = scenes.gaussian_blob(n, centre=(0.5, 0.45, 0.5), sigma=sigma)
     u, v, w = scenes.curl_noise(n, seed=seed)
     dt = scenes.cfl_dt(u, v, w, h, cfl)
