@@ -89,6 +89,10 @@ struct fia_ctx {
     DevBuf<double> psi;
     DevBuf<float> disp[3];
     FluxSet fsa, fsb;      // face fluxes (and orientation corrections) of two passes
+    DevBuf<unsigned> face_n;               // piece slots per face
+    DevBuf<unsigned long long> face_start; // exclusive scan of face_n
+    DevBuf<unsigned char> scan_tmp;
+    DevBuf<double> piece_c[3], piece_ay, piece_val;
     DevBuf<float> work[3];
     DevBuf<unsigned> twisted;
     DevBuf<int> counters;   // [0] twisted count, [1] error
@@ -113,7 +117,9 @@ namespace {
 size_t ctx_bytes(const fia_ctx *c) {
     size_t b = c->psi.bytes() + c->twisted.bytes() + c->counters.bytes() + c->scal.bytes() + c->bits.bytes() +
                c->solid.bytes() + c->genI.bytes();
-    b += c->fsa.bytes() + c->fsb.bytes();
+    b += c->fsa.bytes() + c->fsb.bytes() + c->face_n.bytes() + c->face_start.bytes() + c->scan_tmp.bytes() +
+         c->piece_ay.bytes() + c->piece_val.bytes();
+    for (int a = 0; a < 3; ++a) b += c->piece_c[a].bytes();
     for (int a = 0; a < 3; ++a) b += c->disp[a].bytes() + c->work[a].bytes() + c->gen[a].bytes();
     for (int a = 0; a < 4; ++a) b += c->grid[a].bytes();
     for (int a = 0; a < 2; ++a) b += c->genD[a].bytes();
@@ -235,11 +241,36 @@ fia_status do_faces(fia_ctx *c, const CompGrid &g, const float *q, bool moving, 
     p.min_area = s.min_area;
     p.gauss = make_gauss(s.ng);
     if (c->timing) {
-        HIP_TRY(c->stats.ensure(2));
+        HIP_TRY(c->stats.ensure(2 * kStatSlots));
         p.stats = c->stats.p;
     }
+    // slots per face -> exclusive scan -> piece records -> quadrature -> ordered sums
+    const size_t nf = g.faces(0) + g.faces(1) + g.faces(2);
+    HIP_TRY(c->face_n.ensure(nf));
+    HIP_TRY(c->face_start.ensure(nf));
+    const size_t tb = scan_temp_bytes(nf);
+    HIP_TRY(c->scan_tmp.ensure(tb));
+    p.face_n = c->face_n.p;
+    p.face_start = c->face_start.p;
+    p.fam_base[0] = 0; p.fam_base[1] = g.faces(0); p.fam_base[2] = g.faces(0) + g.faces(1);
     HIP_TRY(span_begin(c, SK_FACE));
-    HIP_TRY(launch_faces(p, c->stream));
+    HIP_TRY(launch_face_count(p, c->stream));   // all families; inactive ones count zero
+    HIP_TRY(launch_scan(c->scan_tmp.p, tb, c->face_n.p, c->face_start.p, nf, c->stream));
+    unsigned long long last[2] = {0, 0};
+    unsigned lastn = 0;
+    HIP_TRY(hipMemcpyAsync(&last[0], c->face_start.p + nf - 1, 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(&lastn, c->face_n.p + nf - 1, 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const unsigned long long nslots = last[0] + lastn;
+    for (int a = 0; a < 3; ++a) HIP_TRY(c->piece_c[a].ensure(nslots));
+    HIP_TRY(c->piece_ay.ensure(nslots));
+    HIP_TRY(c->piece_val.ensure(nslots));
+    for (int a = 0; a < 3; ++a) p.piece_c[a] = c->piece_c[a].p;
+    p.piece_ay = c->piece_ay.p;
+    p.piece_val = c->piece_val.p;
+    HIP_TRY(launch_face_cut(p, c->stream));
+    HIP_TRY(launch_pieces(p, nslots, c->stream));
+    HIP_TRY(launch_face_sum(p, c->stream));
     HIP_TRY(launch_faces_general(p, c->stream));
     HIP_TRY(span_end(c));
     if (c->timing) c->acc.face_launches += 1;
@@ -432,7 +463,9 @@ fia_status fia_ctx_destroy(fia_ctx *c) {
     (void)hipStreamSynchronize(c->stream);
     c->psi.release(); c->twisted.release(); c->counters.release(); c->scal.release(); c->bits.release();
     c->solid.release(); c->genI.release();
-    c->fsa.release(); c->fsb.release();
+    c->fsa.release(); c->fsb.release(); c->face_n.release(); c->face_start.release(); c->scan_tmp.release();
+    c->piece_ay.release(); c->piece_val.release();
+    for (int a = 0; a < 3; ++a) c->piece_c[a].release();
     for (int a = 0; a < 3; ++a) { c->disp[a].release(); c->work[a].release(); c->gen[a].release(); }
     for (int a = 0; a < 4; ++a) c->grid[a].release();
     for (int a = 0; a < 2; ++a) c->genD[a].release();
@@ -449,8 +482,8 @@ fia_status fia_ctx_set_timing(fia_ctx *c, int enable) {
     HIP_TRY(set_device(c));
     c->timing = enable != 0;
     if (c->timing) {
-        HIP_TRY(c->stats.ensure(2));
-        HIP_TRY(hipMemsetAsync(c->stats.p, 0, 2 * sizeof(unsigned long long), c->stream));
+        HIP_TRY(c->stats.ensure(2 * kStatSlots));
+        HIP_TRY(hipMemsetAsync(c->stats.p, 0, 2 * kStatSlots * sizeof(unsigned long long), c->stream));
         HIP_TRY(hipStreamSynchronize(c->stream));
     }
     return FIA_OK;
@@ -464,7 +497,7 @@ fia_status fia_ctx_reset_stats(fia_ctx *c) {
     c->spans.clear();
     c->acc = fia_kernel_stats{};
     if (c->stats.p) {
-        HIP_TRY(hipMemsetAsync(c->stats.p, 0, 2 * sizeof(unsigned long long), c->stream));
+        HIP_TRY(hipMemsetAsync(c->stats.p, 0, 2 * kStatSlots * sizeof(unsigned long long), c->stream));
         HIP_TRY(hipStreamSynchronize(c->stream));
     }
     return FIA_OK;
@@ -487,11 +520,13 @@ fia_status fia_ctx_get_stats(fia_ctx *c, fia_kernel_stats *out) {
     c->spans.clear();
     *out = c->acc;
     if (c->stats.p) {
-        unsigned long long h[2] = {0, 0};
-        HIP_TRY(hipMemcpy(h, c->stats.p, sizeof(h), hipMemcpyDeviceToHost));
-        out->pieces = (long long)h[0];
-        out->density_points = (long long)h[1];
-        out->psi_points = (long long)h[0];
+        std::vector<unsigned long long> h(2 * kStatSlots);
+        HIP_TRY(hipMemcpy(h.data(), c->stats.p, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        long long np = 0, nd = 0;
+        for (int i = 0; i < kStatSlots; ++i) { np += (long long)h[2 * i]; nd += (long long)h[2 * i + 1]; }
+        out->pieces = np;
+        out->density_points = nd;
+        out->psi_points = np;
     }
     return FIA_OK;
 }

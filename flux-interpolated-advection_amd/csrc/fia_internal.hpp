@@ -7,6 +7,8 @@
 namespace fia {
 
 constexpr int kMaxGauss = 49;
+// work counters are kept as kStatSlots (pieces, density points) pairs, summed on read
+constexpr int kStatSlots = 1024;
 
 // A component grid G: density (comp = -1, dims n) or one MAC velocity component (comp = 0..2,
 // dims n + e_comp).  Its vertex lattice has dims m + 1 and sits at lattice - e_comp/2 (cells).
@@ -67,7 +69,15 @@ struct FaceParams {
     int general_all;         // every face goes through the general cutter (minimal_area < 1)
     float kappa_q, kappa_psi;
     double min_area;
-    unsigned long long *stats;   // optional work counters: [0] pieces, [1] density points
+    unsigned long long *stats;   // optional work counters: kStatSlots (pieces, density points) pairs
+    // phased face path: per face slot counts and starts over the three families (fam_base
+    // offsets), piece records (centre in cells, y area) and piece values
+    unsigned *face_n;
+    unsigned long long *face_start;
+    size_t fam_base[3];
+    double *piece_c[3];
+    double *piece_ay;
+    double *piece_val;
     GaussTable gauss;
 };
 
@@ -76,7 +86,12 @@ enum KernelError { KE_NONE = 0, KE_CAPACITY = 3, KE_NONFINITE = 4 };
 
 hipError_t launch_psi(const PsiParams &p, hipStream_t s);
 hipError_t launch_vertices(const VertexParams &p, hipStream_t s);
-hipError_t launch_faces(const FaceParams &p, hipStream_t s);
+hipError_t launch_face_count(const FaceParams &p, hipStream_t s);
+hipError_t launch_face_cut(const FaceParams &p, hipStream_t s);
+hipError_t launch_pieces(const FaceParams &p, unsigned long long nslots, hipStream_t s);
+hipError_t launch_face_sum(const FaceParams &p, hipStream_t s);
+size_t scan_temp_bytes(size_t n);
+hipError_t launch_scan(void *temp, size_t bytes, const unsigned *in, unsigned long long *out, size_t n, hipStream_t s);
 hipError_t launch_faces_general(const FaceParams &p, hipStream_t s);
 // out = sum over families of flux differences (scaled); mode 0: out = G1; 1: out = G1 + G2;
 // 2: out = base - G1 (y family only).

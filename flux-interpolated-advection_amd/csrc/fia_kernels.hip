@@ -8,6 +8,8 @@
 //   gather_kernel    per cell: sum of its six signed face fluxes (deterministic, no atomics)
 // Each interior face is computed once; its flux enters the two neighbouring cells with opposite
 // signs, so nothing is lost or created inside the domain.
+#include <hipcub/hipcub.hpp>
+
 #include "fia_cutter.hpp"
 #include "fia_internal.hpp"
 #include "fia_math.hpp"
@@ -189,9 +191,51 @@ __device__ __forceinline__ void decode_face(const CompGrid &g, int fam, long id,
 
 __device__ __forceinline__ void report(int *err, int code) { atomicCAS(err, 0, code); }
 
-// One face: cut with the streaming pipeline and integrate every piece; twisted faces are queued
-// for general_kernel.  npieces/npts count the quadrature work (instrumentation).
-template <int FAM> __device__ __forceinline__ void face_body(const FaceParams &P, long id, int &npieces, int &npts) {
+// Corner colours -> number of piece slots (the product of the x and z strip counts).
+__device__ __forceinline__ void colour_ranges(const double col[4][2], int &x0, int &nxc, int &z0, int &nzc) {
+    double lx = col[0][0], hx = col[0][0], lz = col[0][1], hz = col[0][1];
+    for (int c = 1; c < 4; ++c) {
+        lx = fmin(lx, col[c][0]); hx = fmax(hx, col[c][0]);
+        lz = fmin(lz, col[c][1]); hz = fmax(hz, col[c][1]);
+    }
+    int x1, z1;
+    interior_integers(lx, hx, x0, x1);
+    interior_integers(lz, hz, z0, z1);
+    nxc = x1 >= x0 ? x1 - x0 + 1 : 0;
+    nzc = z1 >= z0 ? z1 - z0 + 1 : 0;
+}
+
+constexpr unsigned kFaceElsewhere = 0xffffffffu;   // face handled by general_kernel
+
+__device__ __forceinline__ void queue_general(const FaceParams &P, int fam, long id) {
+    const int slot = atomicAdd(P.twisted_count, 1);
+    P.twisted[slot] = ((unsigned)fam << 30) | (unsigned)id;
+}
+
+// Phase 1: slots each face needs, (x strips) x (z strips); faces for the general cutter get 0.
+template <int FAM> __global__ void __launch_bounds__(kBlock) face_count_kernel(FaceParams P) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long)P.g.faces(FAM)) return;
+    int i, j, k;
+    decode_face(P.g, FAM, id, i, j, k);
+    double v[4][3], col[4][2];
+    unsigned n = 0;
+    if ((P.fam_mask & (1 << FAM)) && load_face(P, FAM, i, j, k, v, col) && !P.general_all) {
+        int x0, nxc, z0, nzc;
+        colour_ranges(col, x0, nxc, z0, nzc);
+        n = (unsigned)((nxc + 1) * (nzc + 1));
+    }
+    P.face_n[P.fam_base[FAM] + id] = n;
+}
+
+// Phase 2: cut each face with the streaming pipeline into its slots (start from the exclusive
+// scan of phase 1).  A slot holds the piece centre (cells, physical frame) and its y vector area;
+// empty strip x band combinations hold a zero area.  Twisted faces are queued for the general
+// cutter and marked; non-finite faces get zero flux and an error code.
+template <int FAM> __global__ void __launch_bounds__(kBlock) face_cut_kernel(FaceParams P) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long)P.g.faces(FAM)) return;
+    const size_t gid = P.fam_base[FAM] + id;
     int i, j, k;
     decode_face(P.g, FAM, id, i, j, k);
     double v[4][3], col[4][2];
@@ -199,27 +243,22 @@ template <int FAM> __device__ __forceinline__ void face_body(const FaceParams &P
         report(P.error, KE_NONFINITE);
         P.flux[FAM][id] = 0.0;
         P.corr[FAM][id] = 0.0;
+        P.face_n[gid] = kFaceElsewhere;
         return;
     }
     if (P.general_all) {
-        const int slot = atomicAdd(P.twisted_count, 1);
-        P.twisted[slot] = ((unsigned)FAM << 30) | (unsigned)id;
+        queue_general(P, FAM, id);
+        P.face_n[gid] = kFaceElsewhere;
         return;
     }
     FaceGeom geom;
     geom.init(v);
-    double lx = col[0][0], hx = col[0][0], lz = col[0][1], hz = col[0][1];
-    for (int c = 1; c < 4; ++c) {
-        lx = fmin(lx, col[c][0]); hx = fmax(hx, col[c][0]);
-        lz = fmin(lz, col[c][1]); hz = fmax(hz, col[c][1]);
-    }
-    int x0, x1, z0, z1;
-    interior_integers(lx, hx, x0, x1);
-    interior_integers(lz, hz, z0, z1);
-    const int nxc = x1 >= x0 ? x1 - x0 + 1 : 0, nzc = z1 >= z0 ? z1 - z0 + 1 : 0;
-    double flux = 0.0;
+    int x0, nxc, z0, nzc;
+    colour_ranges(col, x0, nxc, z0, nzc);
+    const unsigned long long base = P.face_start[gid];
+    unsigned long long slot = base;
     for (int s = 0; s <= nxc; ++s) {
-        for (int m = 0; m <= nzc; ++m) {
+        for (int m = 0; m <= nzc; ++m, ++slot) {
             Pipeline pp;
             pp.st[0].reset(s > 0, 0, (double)(x0 + s - 1), true);
             pp.st[1].reset(s < nxc, 0, (double)(x0 + s), false);
@@ -234,44 +273,80 @@ template <int FAM> __device__ __forceinline__ void face_body(const FaceParams &P
             }
             pp.close<0>(geom);
             if (pp.twisted()) {
-                const int slot = atomicAdd(P.twisted_count, 1);
-                P.twisted[slot] = ((unsigned)FAM << 30) | (unsigned)id;
+                // clear every slot of this face so the piece kernel skips them
+                const unsigned long long end = base + (unsigned long long)((nxc + 1) * (nzc + 1));
+                for (unsigned long long q = base; q < end; ++q) P.piece_ay[q] = 0.0;
+                queue_general(P, FAM, id);
+                P.face_n[gid] = kFaceElsewhere;
                 return;
             }
+            double ay = 0.0, c3[3] = {0.0, 0.0, 0.0};
             if (pp.acc.n >= 3 && pp.acc.area2 != 0.0) {
                 const double inv3a = 1.0 / (3.0 * pp.acc.area2);
-                const double xi = pp.acc.mxi * inv3a, eta = pp.acc.meta * inv3a;
                 double r[3];
-                geom.rel(xi, eta, r);
-                const double c3[3] = {geom.p0[0] + r[0], geom.p0[1] + r[1], geom.p0[2] + r[2]};
-                const double ay = 0.5 * pp.acc.ay2;
-                ++npieces;
-                // a piece with no y area contributes exactly zero (only psi_y and A_y enter)
-                if (ay != 0.0) flux += piece_flux(P, c3, ay, npts);
+                geom.rel(pp.acc.mxi * inv3a, pp.acc.meta * inv3a, r);
+                c3[0] = geom.p0[0] + r[0]; c3[1] = geom.p0[1] + r[1]; c3[2] = geom.p0[2] + r[2];
+                ay = 0.5 * pp.acc.ay2;
             }
+            P.piece_c[0][slot] = c3[0];
+            P.piece_c[1][slot] = c3[1];
+            P.piece_c[2][slot] = c3[2];
+            P.piece_ay[slot] = ay;   // a piece with no y area contributes exactly zero
         }
     }
-    P.flux[FAM][id] = flux;
-    P.corr[FAM][id] = 0.0;
 }
 
+// Work counters: block sum in LDS, then one atomic pair per block on one of kStatSlots
+// counter pairs (spread so the atomics do not serialise on one address).
 __device__ __forceinline__ void add_stats(unsigned long long *stats, int a, int b) {
+    __shared__ unsigned long long sa[kBlock / 64], sb[kBlock / 64];
     unsigned long long x = (unsigned long long)a, y = (unsigned long long)b;
     for (int off = 32; off > 0; off >>= 1) {
         x += __shfl_down(x, off, 64);
         y += __shfl_down(y, off, 64);
     }
-    if ((threadIdx.x & 63) == 0) {
-        atomicAdd(stats, x);
-        atomicAdd(stats + 1, y);
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { sa[wave] = x; sb[wave] = y; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long tx = 0, ty = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) { tx += sa[w]; ty += sb[w]; }
+        const int slot = (int)(blockIdx.x % kStatSlots);
+        atomicAdd(stats + 2 * slot, tx);
+        atomicAdd(stats + 2 * slot + 1, ty);
     }
 }
 
-template <int FAM> __global__ void __launch_bounds__(kBlock) face_kernel(FaceParams P) {
-    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+// Phase 3: quadrature of every piece slot, one thread per slot (the expensive WENO work runs
+// converged regardless of how many pieces each face has).
+__global__ void __launch_bounds__(kBlock) piece_kernel(FaceParams P, unsigned long long nslots) {
+    const unsigned long long q = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
     int npieces = 0, npts = 0;
-    if (id < (long)P.g.faces(FAM)) face_body<FAM>(P, id, npieces, npts);
+    if (q < nslots) {
+        const double ay = P.piece_ay[q];
+        double val = 0.0;
+        if (ay != 0.0) {
+            const double c3[3] = {P.piece_c[0][q], P.piece_c[1][q], P.piece_c[2][q]};
+            val = piece_flux(P, c3, ay, npts);
+            npieces = 1;
+        }
+        P.piece_val[q] = val;
+    }
     if (P.stats) add_stats(P.stats, npieces, npts);
+}
+
+// Phase 4: face flux = ordered sum of its slots (deterministic).
+template <int FAM> __global__ void __launch_bounds__(kBlock) face_sum_kernel(FaceParams P) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long)P.g.faces(FAM)) return;
+    const size_t gid = P.fam_base[FAM] + id;
+    const unsigned n = P.face_n[gid];
+    if (n == kFaceElsewhere) return;
+    const unsigned long long s0 = P.face_start[gid];
+    double f = 0.0;
+    for (unsigned q = 0; q < n; ++q) f += P.piece_val[s0 + q];
+    P.flux[FAM][id] = f;
+    P.corr[FAM][id] = 0.0;
 }
 
 // Flux of one face with the exact recursive cutter, corners in the given order.
@@ -510,14 +585,42 @@ hipError_t launch_vertices(const VertexParams &p, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_faces(const FaceParams &p, hipStream_t s) {
-    if (p.fam_mask & 1)
-        hipLaunchKernelGGL(face_kernel<0>, dim3(grid_for(p.g.faces(0), kBlock)), dim3(kBlock), 0, s, p);
-    if (p.fam_mask & 2)
-        hipLaunchKernelGGL(face_kernel<1>, dim3(grid_for(p.g.faces(1), kBlock)), dim3(kBlock), 0, s, p);
-    if (p.fam_mask & 4)
-        hipLaunchKernelGGL(face_kernel<2>, dim3(grid_for(p.g.faces(2), kBlock)), dim3(kBlock), 0, s, p);
+hipError_t launch_face_count(const FaceParams &p, hipStream_t s) {
+    // every family writes its entries so the scan sees no stale counts
+    hipLaunchKernelGGL(face_count_kernel<0>, dim3(grid_for(p.g.faces(0), kBlock)), dim3(kBlock), 0, s, p);
+    hipLaunchKernelGGL(face_count_kernel<1>, dim3(grid_for(p.g.faces(1), kBlock)), dim3(kBlock), 0, s, p);
+    hipLaunchKernelGGL(face_count_kernel<2>, dim3(grid_for(p.g.faces(2), kBlock)), dim3(kBlock), 0, s, p);
     return hipGetLastError();
+}
+
+hipError_t launch_face_cut(const FaceParams &p, hipStream_t s) {
+    if (p.fam_mask & 1) hipLaunchKernelGGL(face_cut_kernel<0>, dim3(grid_for(p.g.faces(0), kBlock)), dim3(kBlock), 0, s, p);
+    if (p.fam_mask & 2) hipLaunchKernelGGL(face_cut_kernel<1>, dim3(grid_for(p.g.faces(1), kBlock)), dim3(kBlock), 0, s, p);
+    if (p.fam_mask & 4) hipLaunchKernelGGL(face_cut_kernel<2>, dim3(grid_for(p.g.faces(2), kBlock)), dim3(kBlock), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_pieces(const FaceParams &p, unsigned long long nslots, hipStream_t s) {
+    if (nslots == 0) return hipSuccess;
+    hipLaunchKernelGGL(piece_kernel, dim3(grid_for(nslots, kBlock)), dim3(kBlock), 0, s, p, nslots);
+    return hipGetLastError();
+}
+
+hipError_t launch_face_sum(const FaceParams &p, hipStream_t s) {
+    if (p.fam_mask & 1) hipLaunchKernelGGL(face_sum_kernel<0>, dim3(grid_for(p.g.faces(0), kBlock)), dim3(kBlock), 0, s, p);
+    if (p.fam_mask & 2) hipLaunchKernelGGL(face_sum_kernel<1>, dim3(grid_for(p.g.faces(1), kBlock)), dim3(kBlock), 0, s, p);
+    if (p.fam_mask & 4) hipLaunchKernelGGL(face_sum_kernel<2>, dim3(grid_for(p.g.faces(2), kBlock)), dim3(kBlock), 0, s, p);
+    return hipGetLastError();
+}
+
+size_t scan_temp_bytes(size_t n) {
+    size_t bytes = 0;
+    hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (const unsigned *)nullptr, (unsigned long long *)nullptr, (int)n);
+    return bytes;
+}
+
+hipError_t launch_scan(void *temp, size_t bytes, const unsigned *in, unsigned long long *out, size_t n, hipStream_t s) {
+    return hipcub::DeviceScan::ExclusiveSum(temp, bytes, in, out, (int)n, s);
 }
 
 hipError_t launch_faces_general(const FaceParams &p, hipStream_t s) {
