@@ -24,7 +24,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "flux-advection Mcell-steps/s at 256^3; step ms NN-warm-PCG vs PCG at 1e-6 res"
 PEAK_FP32_TFLOPS = 157.3
-FLOP_PER_WENO3D = 43 * 200   # 43 1D WENO6 evaluations per 3D point, 200 flop each (reference weno.h:34-68)
+# flops of one 1D WENO6 evaluation as the quadrature kernel computes it (sample differences,
+# three Newton candidates, three smoothness indicators over 12 shared products, weights, blend;
+# an FMA counts 2, the reciprocal 1)
+FLOP_PER_WENO6 = 100
 
 
 def make_scene(scenes, n, cfl, seed=1):
@@ -92,11 +95,13 @@ def main():
     st = ctx.stats()
     cells = n ** 3
     value = cells * args.steps / sec / 1e6
-    # roofline of the face kernel (all face launches of the timed steps)
-    flop = (st["psi_points"] + st["density_points"]) * FLOP_PER_WENO3D
-    face_s = st["face_ms"] / 1e3
-    achieved = flop / face_s / 1e12 if face_s > 0 else 0.0
+    # roofline of the dominant kernel: the WENO6 integral quadrature (piece_kernel<1>), timed with
+    # HIP events on the library's stream around each of its launches
+    flop = st["weno_evals"] * FLOP_PER_WENO6
+    quad_s = st["quad_ms"] / 1e3
+    achieved = flop / quad_s / 1e12 if quad_s > 0 else 0.0
     launches = max(st["face_launches"], 1)
+    qlaunches = max(st["quad_launches"], 1)
     mass = float(rho_d.double().sum().item())
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Mcell-steps/s", "n_gpus": 1, "steps": args.steps,
@@ -107,10 +112,14 @@ def main():
                                "zero-velocity correction on" % (n, args.cfl),
                    "grid": [n, n, n], "cfl": args.cfl, "passes_per_step": 3},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None},
-        "kernels_ms_per_step": {k: round(st[k] / args.steps, 3) for k in ("face_ms", "vertex_ms", "psi_ms", "gather_ms")},
+                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                     "kernel": "piece_kernel<1> (WENO6 integral quadrature, FP32 VALU)",
+                     "avg_launch_ms": round(st["quad_ms"] / qlaunches, 3),
+                     "weno6_evals_per_launch": st["weno_evals"] / qlaunches, "flop_per_weno6": FLOP_PER_WENO6},
+        "kernels_ms_per_step": {k: round(st[k] / args.steps, 3)
+                                for k in ("face_ms", "quad_ms", "vertex_ms", "psi_ms", "gather_ms")},
         "face_work": {"launches": st["face_launches"], "pieces_per_launch": st["pieces"] / launches,
-                      "weno3d_points_per_cell_per_launch": (st["psi_points"] + st["density_points"]) / launches / cells,
+                      "density_points_per_step": st["density_points"] / args.steps,
                       "avg_face_launch_ms": round(st["face_ms"] / launches, 3)},
         "mass_after": mass,
     }

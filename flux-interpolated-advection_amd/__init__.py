@@ -64,7 +64,8 @@ class KernelStats(ctypes.Structure):
     _fields_ = [("face_ms", ctypes.c_double), ("face_launches", ctypes.c_longlong), ("pieces", ctypes.c_longlong),
                 ("psi_points", ctypes.c_longlong), ("density_points", ctypes.c_longlong),
                 ("vertices", ctypes.c_longlong), ("vertex_ms", ctypes.c_double), ("psi_ms", ctypes.c_double),
-                ("gather_ms", ctypes.c_double), ("gather_cells", ctypes.c_longlong)]
+                ("gather_ms", ctypes.c_double), ("gather_cells", ctypes.c_longlong),
+                ("weno_evals", ctypes.c_longlong), ("quad_ms", ctypes.c_double), ("quad_launches", ctypes.c_longlong)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -91,6 +91,7 @@ struct fia_ctx {
     FluxSet fsa, fsb;      // face fluxes (and orientation corrections) of two passes
     DevBuf<unsigned> face_n;               // piece slots per face
     DevBuf<unsigned long long> face_start; // exclusive scan of face_n
+    DevBuf<unsigned> slot_face;            // owner face of every piece slot
     DevBuf<unsigned char> scan_tmp;
     DevBuf<double> piece_c[3], piece_ay, piece_val;
     DevBuf<float> work[3];
@@ -117,7 +118,7 @@ namespace {
 size_t ctx_bytes(const fia_ctx *c) {
     size_t b = c->psi.bytes() + c->twisted.bytes() + c->counters.bytes() + c->scal.bytes() + c->bits.bytes() +
                c->solid.bytes() + c->genI.bytes();
-    b += c->fsa.bytes() + c->fsb.bytes() + c->face_n.bytes() + c->face_start.bytes() + c->scan_tmp.bytes() +
+    b += c->fsa.bytes() + c->fsb.bytes() + c->face_n.bytes() + c->face_start.bytes() + c->slot_face.bytes() + c->scan_tmp.bytes() +
          c->piece_ay.bytes() + c->piece_val.bytes();
     for (int a = 0; a < 3; ++a) b += c->piece_c[a].bytes();
     for (int a = 0; a < 3; ++a) b += c->disp[a].bytes() + c->work[a].bytes() + c->gen[a].bytes();
@@ -129,24 +130,27 @@ size_t ctx_bytes(const fia_ctx *c) {
 
 hipError_t set_device(fia_ctx *c) { return hipSetDevice(c->device); }
 
-enum SpanKind { SK_FACE = 0, SK_VERTEX = 1, SK_PSI = 2, SK_GATHER = 3 };
+enum SpanKind { SK_FACE = 0, SK_VERTEX = 1, SK_PSI = 2, SK_GATHER = 3, SK_QUAD = 4 };
 
-// Opens a timed span on the context's stream (no-op unless timing is enabled).
-hipError_t span_begin(fia_ctx *c, int kind) {
+// Opens a timed span on the context's stream (no-op unless timing is enabled); idx receives the
+// span's index for span_end (spans may nest).
+hipError_t span_begin(fia_ctx *c, int kind, int &idx) {
+    idx = -1;
     if (!c->timing) return hipSuccess;
     fia_ctx::Span sp;
     sp.kind = kind;
     hipError_t e = hipEventCreate(&sp.a);
     if (e != hipSuccess) return e;
     e = hipEventCreate(&sp.b);
-    if (e != hipSuccess) return e;
+    if (e != hipSuccess) { (void)hipEventDestroy(sp.a); return e; }
     e = hipEventRecord(sp.a, c->stream);
+    idx = (int)c->spans.size();
     c->spans.push_back(sp);
     return e;
 }
-hipError_t span_end(fia_ctx *c) {
-    if (!c->timing || c->spans.empty()) return hipSuccess;
-    return hipEventRecord(c->spans.back().b, c->stream);
+hipError_t span_end(fia_ctx *c, int idx) {
+    if (!c->timing || idx < 0 || idx >= (int)c->spans.size()) return hipSuccess;
+    return hipEventRecord(c->spans[idx].b, c->stream);
 }
 
 struct PassSpec {
@@ -195,9 +199,10 @@ fia_status do_psi(fia_ctx *c, const CompGrid &g, const float *q, int mode, int i
     memset(&p, 0, sizeof(p));
     p.g = g; p.q = q; p.psi = c->psi.p; p.mode = mode; p.integral = integral; p.kappa = 1.0f;
     p.gauss = make_gauss(ng);
-    HIP_TRY(span_begin(c, SK_PSI));
+    int sp;
+    HIP_TRY(span_begin(c, SK_PSI, sp));
     HIP_TRY(launch_psi(p, c->stream));
-    HIP_TRY(span_end(c));
+    HIP_TRY(span_end(c, sp));
     return FIA_OK;
 }
 
@@ -208,9 +213,10 @@ fia_status do_vertices(fia_ctx *c, const CompGrid &g, const float *u, const floa
     memset(&p, 0, sizeof(p));
     p.g = g; p.u = u; p.v = v; p.w = w; p.dt_over_h = (float)dt_over_h; p.interp = interp;
     for (int a = 0; a < 3; ++a) p.disp[a] = c->disp[a].p;
-    HIP_TRY(span_begin(c, SK_VERTEX));
+    int sp;
+    HIP_TRY(span_begin(c, SK_VERTEX, sp));
     HIP_TRY(launch_vertices(p, c->stream));
-    HIP_TRY(span_end(c));
+    HIP_TRY(span_end(c, sp));
     if (c->timing) c->acc.vertices += (long long)g.lattice();
     return FIA_OK;
 }
@@ -236,12 +242,13 @@ fia_status do_faces(fia_ctx *c, const CompGrid &g, const float *q, bool moving, 
     p.twisted = c->twisted.p; p.twisted_count = c->counters.p; p.error = c->counters.p + 1;
     p.split = s.split; p.interp = s.interp; p.integral = s.integral; p.fam_mask = fam_mask;
     p.general_all = s.min_area < 1.0;
+    p.seg_cut = s.split == FIA_SPLIT_NONE || s.split == FIA_SPLIT_XZ_CELLFACE;
     p.kappa_q = 1.0f;
     p.kappa_psi = (float)(h * h);   // psi is stored in units of q*cells; S in physical units
     p.min_area = s.min_area;
     p.gauss = make_gauss(s.ng);
     if (c->timing) {
-        HIP_TRY(c->stats.ensure(2 * kStatSlots));
+        HIP_TRY(c->stats.ensure(kStatWords * kStatSlots));
         p.stats = c->stats.p;
     }
     // slots per face -> exclusive scan -> piece records -> quadrature -> ordered sums
@@ -253,33 +260,46 @@ fia_status do_faces(fia_ctx *c, const CompGrid &g, const float *q, bool moving, 
     p.face_n = c->face_n.p;
     p.face_start = c->face_start.p;
     p.fam_base[0] = 0; p.fam_base[1] = g.faces(0); p.fam_base[2] = g.faces(0) + g.faces(1);
-    HIP_TRY(span_begin(c, SK_FACE));
+    int sp;
+    HIP_TRY(span_begin(c, SK_FACE, sp));
     HIP_TRY(launch_face_count(p, c->stream));   // all families; inactive ones count zero
     HIP_TRY(launch_scan(c->scan_tmp.p, tb, c->face_n.p, c->face_start.p, nf, c->stream));
-    unsigned long long last[2] = {0, 0};
+    // slot bounds of the families: starts of their first faces and the total
+    unsigned long long fam_slots[4] = {0, 0, 0, 0};
     unsigned lastn = 0;
-    HIP_TRY(hipMemcpyAsync(&last[0], c->face_start.p + nf - 1, 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(&fam_slots[1], c->face_start.p + p.fam_base[1], 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(&fam_slots[2], c->face_start.p + p.fam_base[2], 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(&fam_slots[3], c->face_start.p + nf - 1, 8, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipMemcpyAsync(&lastn, c->face_n.p + nf - 1, 4, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    const unsigned long long nslots = last[0] + lastn;
+    fam_slots[3] += lastn;
+    const unsigned long long nslots = fam_slots[3];
+    HIP_TRY(c->slot_face.ensure(nslots));
+    p.slot_face = c->slot_face.p;
     for (int a = 0; a < 3; ++a) HIP_TRY(c->piece_c[a].ensure(nslots));
     HIP_TRY(c->piece_ay.ensure(nslots));
     HIP_TRY(c->piece_val.ensure(nslots));
     for (int a = 0; a < 3; ++a) p.piece_c[a] = c->piece_c[a].p;
     p.piece_ay = c->piece_ay.p;
     p.piece_val = c->piece_val.p;
-    HIP_TRY(launch_face_cut(p, c->stream));
+    HIP_TRY(launch_face_cut(p, fam_slots, c->stream));
+    // the WENO6 integral quadrature launches are timed on their own (the dominant kernel)
+    const bool weno_quad = piece_quadrature_mode(p) != 0;
+    int sq = -1;
+    if (weno_quad) HIP_TRY(span_begin(c, SK_QUAD, sq));
     HIP_TRY(launch_pieces(p, nslots, c->stream));
+    if (weno_quad) HIP_TRY(span_end(c, sq));
     HIP_TRY(launch_face_sum(p, c->stream));
     HIP_TRY(launch_faces_general(p, c->stream));
-    HIP_TRY(span_end(c));
-    if (c->timing) c->acc.face_launches += 1;
+    HIP_TRY(span_end(c, sp));
+    if (c->timing) { c->acc.face_launches += 1; c->acc.quad_launches += weno_quad ? 1 : 0; }
     return FIA_OK;
 }
 
 hipError_t timed_gather(fia_ctx *c, const CompGrid &g, FluxSet &s1, FluxSet *s2, const float *base, float *out,
                         int mode, int fm) {
-    hipError_t e = span_begin(c, SK_GATHER);
+    int sp;
+    hipError_t e = span_begin(c, SK_GATHER, sp);
     if (e != hipSuccess) return e;
     double *f1[3] = {s1.f[0].p, s1.f[1].p, s1.f[2].p}, *c1[3] = {s1.c[0].p, s1.c[1].p, s1.c[2].p};
     double *f2[3] = {nullptr, nullptr, nullptr}, *c2[3] = {nullptr, nullptr, nullptr};
@@ -288,7 +308,7 @@ hipError_t timed_gather(fia_ctx *c, const CompGrid &g, FluxSet &s1, FluxSet *s2,
     e = launch_gather(g, f1, c1, s2 ? f2 : nullptr, s2 ? c2 : nullptr, base, out, mode, fm, c->stream);
     if (e != hipSuccess) return e;
     if (c->timing) c->acc.gather_cells += (long long)g.cells();
-    return span_end(c);
+    return span_end(c, sp);
 }
 
 // Complete pass: q_out = flux-advected q_in (q_out may alias q_in).
@@ -463,7 +483,7 @@ fia_status fia_ctx_destroy(fia_ctx *c) {
     (void)hipStreamSynchronize(c->stream);
     c->psi.release(); c->twisted.release(); c->counters.release(); c->scal.release(); c->bits.release();
     c->solid.release(); c->genI.release();
-    c->fsa.release(); c->fsb.release(); c->face_n.release(); c->face_start.release(); c->scan_tmp.release();
+    c->fsa.release(); c->fsb.release(); c->face_n.release(); c->face_start.release(); c->slot_face.release(); c->scan_tmp.release();
     c->piece_ay.release(); c->piece_val.release();
     for (int a = 0; a < 3; ++a) c->piece_c[a].release();
     for (int a = 0; a < 3; ++a) { c->disp[a].release(); c->work[a].release(); c->gen[a].release(); }
@@ -482,8 +502,8 @@ fia_status fia_ctx_set_timing(fia_ctx *c, int enable) {
     HIP_TRY(set_device(c));
     c->timing = enable != 0;
     if (c->timing) {
-        HIP_TRY(c->stats.ensure(2 * kStatSlots));
-        HIP_TRY(hipMemsetAsync(c->stats.p, 0, 2 * kStatSlots * sizeof(unsigned long long), c->stream));
+        HIP_TRY(c->stats.ensure(kStatWords * kStatSlots));
+        HIP_TRY(hipMemsetAsync(c->stats.p, 0, kStatWords * kStatSlots * sizeof(unsigned long long), c->stream));
         HIP_TRY(hipStreamSynchronize(c->stream));
     }
     return FIA_OK;
@@ -497,7 +517,7 @@ fia_status fia_ctx_reset_stats(fia_ctx *c) {
     c->spans.clear();
     c->acc = fia_kernel_stats{};
     if (c->stats.p) {
-        HIP_TRY(hipMemsetAsync(c->stats.p, 0, 2 * kStatSlots * sizeof(unsigned long long), c->stream));
+        HIP_TRY(hipMemsetAsync(c->stats.p, 0, kStatWords * kStatSlots * sizeof(unsigned long long), c->stream));
         HIP_TRY(hipStreamSynchronize(c->stream));
     }
     return FIA_OK;
@@ -513,6 +533,7 @@ fia_status fia_ctx_get_stats(fia_ctx *c, fia_kernel_stats *out) {
         if (sp.kind == SK_FACE) c->acc.face_ms += ms;
         else if (sp.kind == SK_VERTEX) c->acc.vertex_ms += ms;
         else if (sp.kind == SK_PSI) c->acc.psi_ms += ms;
+        else if (sp.kind == SK_QUAD) c->acc.quad_ms += ms;
         else c->acc.gather_ms += ms;
         (void)hipEventDestroy(sp.a);
         (void)hipEventDestroy(sp.b);
@@ -520,13 +541,15 @@ fia_status fia_ctx_get_stats(fia_ctx *c, fia_kernel_stats *out) {
     c->spans.clear();
     *out = c->acc;
     if (c->stats.p) {
-        std::vector<unsigned long long> h(2 * kStatSlots);
+        std::vector<unsigned long long> h(kStatWords * kStatSlots);
         HIP_TRY(hipMemcpy(h.data(), c->stats.p, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
-        long long np = 0, nd = 0;
-        for (int i = 0; i < kStatSlots; ++i) { np += (long long)h[2 * i]; nd += (long long)h[2 * i + 1]; }
-        out->pieces = np;
-        out->density_points = nd;
-        out->psi_points = np;
+        long long sum[kStatWords] = {0, 0, 0};
+        for (int i = 0; i < kStatSlots; ++i)
+            for (int w = 0; w < kStatWords; ++w) sum[w] += (long long)h[kStatWords * i + w];
+        out->pieces = sum[0];
+        out->psi_points = sum[0];
+        out->density_points = sum[1];
+        out->weno_evals = sum[2];
     }
     return FIA_OK;
 }

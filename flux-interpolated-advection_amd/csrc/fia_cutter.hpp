@@ -152,6 +152,167 @@ FIA_HD void interior_integers(double lo, double hi, int &first, int &last) {
 }
 
 // ---------------------------------------------------------------------------------------------
+// Piece moments from boundary segments (untwisted faces, no branches on the piece's shape).
+//
+// On an untwisted face every cut meets the polygon it cuts in two points, so the piece (s, m)
+// is the convex region of the square with  xl <= c_x < xh  (x strip s) and  zl <= c_z < zh
+// (z band m) whose boundary is: the parts of the four square edges inside strip and band (the
+// colour is linear along an edge), the parts of the x chords (cut lines of xl and xh, between
+// their crossings on the square's edges, c_z linear along them) inside the band, and the z
+// chords of the strip (between the crossings of zl and zh with the strip's boundary).  Area
+// moments and the mapped loop's vector area are sums over oriented boundary segments, so each
+// piece is the same fixed sequence of segment terms for every face: no vertex lists, and in
+// exact arithmetic the same polygon as the recursive cut.  Orientation is counter-clockwise
+// (the square's corner order); a chord runs from where the boundary leaves its region to where
+// it re-enters.  Inactive bounds are +-kNoBound.
+// ---------------------------------------------------------------------------------------------
+constexpr double kNoBound = 1e300;
+
+// [lo, hi] &= {u in [0,1]: L <= c0 + u (c1 - c0) <= H}; a constant colour keeps everything or
+// nothing, decided with the half-open rule L <= c < H.
+FIA_HD void clip_band(double c0, double c1, double L, double H, double &lo, double &hi) {
+    const double d = c1 - c0;
+    if (d == 0.0) {
+        if (!(c0 >= L && c0 < H)) { lo = 1.0; hi = 0.0; }
+        return;
+    }
+    const double r = 1.0 / d;
+    const double uL = (L - c0) * r, uH = (H - c0) * r;
+    lo = fmax(lo, d > 0.0 ? uL : uH);
+    hi = fmin(hi, d > 0.0 ? uH : uL);
+}
+
+template <bool ALL> struct SegAcc {
+    double area2 = 0.0, mxi = 0.0, meta = 0.0, ax2 = 0.0, ay2 = 0.0, az2 = 0.0;
+    FIA_HD void add(const FaceGeom &g, double xa, double ea, double xb, double eb) {
+        const double cr = xa * eb - xb * ea;
+        area2 += cr;
+        mxi += cr * (xa + xb);
+        meta += cr * (ea + eb);
+        double ra[3], rb[3];
+        g.rel(xa, ea, ra);
+        g.rel(xb, eb, rb);
+        ay2 += ra[2] * rb[0] - ra[0] * rb[2];
+        if (ALL) {
+            ax2 += ra[1] * rb[2] - ra[2] * rb[1];
+            az2 += ra[0] * rb[1] - ra[1] * rb[0];
+        }
+    }
+};
+
+// Where the boundary leaves / re-enters one region (last one seen of each).
+struct Chord {
+    bool has_exit = false, has_entry = false;
+    double xe = 0.0, ee = 0.0, ze = 0.0, xn = 0.0, en = 0.0, zn = 0.0;
+    // boundary segment a -> b with region membership ia, ib; the crossing is at a + t (b - a)
+    FIA_HD void see(bool ia, bool ib, double t, double xa, double ea, double za, double xb, double eb, double zb) {
+        if (ia == ib) return;
+        const double x = xa + t * (xb - xa), e = ea + t * (eb - ea), z = za + t * (zb - za);
+        if (ia) { has_exit = true; xe = x; ee = e; ze = z; }
+        else { has_entry = true; xn = x; en = e; zn = z; }
+    }
+};
+
+// z-chord crossings of the strip boundary segment a -> b (colour z linear along it)
+FIA_HD void see_z(Chord &lo, Chord &hi, double zl, double zh, double xa, double ea, double za, double xb, double eb,
+                  double zb) {
+    const bool la = za >= zl, lb = zb >= zl, ha = za < zh, hb = zb < zh;
+    if (la == lb && ha == hb) return;
+    const double r = 1.0 / (zb - za);
+    lo.see(la, lb, (zl - za) * r, xa, ea, za, xb, eb, zb);
+    hi.see(ha, hb, (zh - za) * r, xa, ea, za, xb, eb, zb);
+}
+
+template <bool ALL>
+FIA_HD void piece_moments(const double col[4][2], const FaceGeom &g, double xl, double xh, double zl, double zh,
+                          SegAcc<ALL> &acc) {
+    const double sx[4] = {0.0, 1.0, 1.0, 0.0}, se[4] = {0.0, 0.0, 1.0, 1.0};
+    Chord cxl, cxh, czl, czh;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int j = (i + 1) & 3;
+        const double xa = sx[i], ea = se[i], xb = sx[j], eb = se[j];
+        const double ca = col[i][0], cb = col[j][0], za = col[i][1], zb = col[j][1];
+        // crossings of the x cut lines with this edge (the x chords' ends)
+        {
+            const bool la = ca >= xl, lb = cb >= xl, ha = ca < xh, hb = cb < xh;
+            if (la != lb || ha != hb) {
+                const double r = 1.0 / (cb - ca);
+                cxl.see(la, lb, (xl - ca) * r, xa, ea, za, xb, eb, zb);
+                cxh.see(ha, hb, (xh - ca) * r, xa, ea, za, xb, eb, zb);
+            }
+        }
+        // the edge's part inside the strip: z chord crossings, then its part inside the band
+        double lo = 0.0, hi = 1.0;
+        clip_band(ca, cb, xl, xh, lo, hi);
+        if (lo < hi) {
+            const double x0 = xa + lo * (xb - xa), e0 = ea + lo * (eb - ea), z0 = za + lo * (zb - za);
+            const double x1 = xa + hi * (xb - xa), e1 = ea + hi * (eb - ea), z1 = za + hi * (zb - za);
+            see_z(czl, czh, zl, zh, x0, e0, z0, x1, e1, z1);
+            double lo2 = lo, hi2 = hi;
+            clip_band(za, zb, zl, zh, lo2, hi2);
+            if (lo2 < hi2)
+                acc.add(g, xa + lo2 * (xb - xa), ea + lo2 * (eb - ea), xa + hi2 * (xb - xa), ea + hi2 * (eb - ea));
+        }
+    }
+    // x chords (exit -> entry of their regions), their parts inside the band
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const Chord &c = k == 0 ? cxl : cxh;
+        if (c.has_exit && c.has_entry) {
+            see_z(czl, czh, zl, zh, c.xe, c.ee, c.ze, c.xn, c.en, c.zn);
+            double lo = 0.0, hi = 1.0;
+            clip_band(c.ze, c.zn, zl, zh, lo, hi);
+            if (lo < hi)
+                acc.add(g, c.xe + lo * (c.xn - c.xe), c.ee + lo * (c.en - c.ee), c.xe + hi * (c.xn - c.xe),
+                        c.ee + hi * (c.en - c.ee));
+        }
+    }
+    // z chords of the strip
+    if (czl.has_exit && czl.has_entry) acc.add(g, czl.xe, czl.ee, czl.xn, czl.en);
+    if (czh.has_exit && czh.has_entry) acc.add(g, czh.xe, czh.ee, czh.xn, czh.en);
+}
+
+// True if an integer strictly inside the range of one colour has the corners on alternating
+// sides (c0, c2 >= k > c1, c3 or the reverse): that iso line crosses the square's boundary four
+// times and the face is twisted.  Without such a line for either colour no cut of any piece
+// meets more than two crossings (cutting a loop replaces a run of vertices on one side of a
+// later line by two points on the same side, so it never adds sign changes) and the pieces are
+// those of piece_moments.
+FIA_HD bool alternating_iso(double c0, double c1, double c2, double c3) {
+    auto side = [](double a0, double a2, double b1, double b3) {
+        const double hi = fmax(a0, a2);
+        double k = floor(fmin(a0, a2));
+        if (k == hi) k -= 1.0;   // a line through the colour maximum cuts nothing
+        return k > fmax(b1, b3);
+    };
+    return side(c0, c2, c1, c3) || side(c1, c3, c0, c2);
+}
+
+FIA_HD bool face_twisted(const double col[4][2]) {
+    return alternating_iso(col[0][0], col[1][0], col[2][0], col[3][0]) ||
+           alternating_iso(col[0][1], col[1][1], col[2][1], col[3][1]);
+}
+
+// True if a corner colour is exactly an integer strictly inside its colour's range.  The cut
+// line of that integer then runs through the corner, and where an edge of constant colour lies on
+// it the recursive cut decides which side gets the edge from the rounding of interpolated
+// colours; such faces take the recursive cutter, as twisted ones do.
+FIA_HD bool face_degenerate(const double col[4][2]) {
+#pragma unroll
+    for (int d = 0; d < 2; ++d) {
+        const double lo = fmin(fmin(col[0][d], col[1][d]), fmin(col[2][d], col[3][d]));
+        const double hi = fmax(fmax(col[0][d], col[1][d]), fmax(col[2][d], col[3][d]));
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const double v = col[c][d];
+            if (v == floor(v) && v > lo && v < hi) return true;
+        }
+    }
+    return false;
+}
+
+// ---------------------------------------------------------------------------------------------
 // General cutter: the reference's recursion (src/facecutter.h:59-224) on an explicit stack, and
 // subdivide (src/facecutter.h:255-324), in FP64.  Used for twisted faces and minimal_area < 1.
 // Pieces come out in a different order than the reference's depth-first order; the set is the

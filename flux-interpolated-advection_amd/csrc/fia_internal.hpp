@@ -7,8 +7,10 @@
 namespace fia {
 
 constexpr int kMaxGauss = 49;
-// work counters are kept as kStatSlots (pieces, density points) pairs, summed on read
+// work counters are kept as kStatSlots triples (pieces, density points, 1D WENO6 evaluations),
+// summed on read
 constexpr int kStatSlots = 1024;
+constexpr int kStatWords = 3;
 
 // A component grid G: density (comp = -1, dims n) or one MAC velocity component (comp = 0..2,
 // dims n + e_comp).  Its vertex lattice has dims m + 1 and sits at lattice - e_comp/2 (cells).
@@ -67,13 +69,16 @@ struct FaceParams {
     int split, interp, integral;
     int fam_mask;            // families computed (bit 0 x, 1 y, 2 z)
     int general_all;         // every face goes through the general cutter (minimal_area < 1)
+    int seg_cut;             // pieces from boundary segments (both colours cell-face aligned, or no
+                             // split); otherwise the streaming clipping pipeline
     float kappa_q, kappa_psi;
     double min_area;
-    unsigned long long *stats;   // optional work counters: kStatSlots (pieces, density points) pairs
+    unsigned long long *stats;   // optional work counters: kStatSlots x kStatWords
     // phased face path: per face slot counts and starts over the three families (fam_base
     // offsets), piece records (centre in cells, y area) and piece values
     unsigned *face_n;
     unsigned long long *face_start;
+    unsigned *slot_face;     // owner face (global index over the families) of every slot
     size_t fam_base[3];
     double *piece_c[3];
     double *piece_ay;
@@ -87,7 +92,11 @@ enum KernelError { KE_NONE = 0, KE_CAPACITY = 3, KE_NONFINITE = 4 };
 hipError_t launch_psi(const PsiParams &p, hipStream_t s);
 hipError_t launch_vertices(const VertexParams &p, hipStream_t s);
 hipError_t launch_face_count(const FaceParams &p, hipStream_t s);
-hipError_t launch_face_cut(const FaceParams &p, hipStream_t s);
+// fam_slots: first slot of each family and the total (slots of family f: [fam_slots[f], fam_slots[f+1]))
+hipError_t launch_face_cut(const FaceParams &p, const unsigned long long fam_slots[4], hipStream_t s);
+// 1 or 2: the WENO6 integral quadrature with that many Gauss points per half (piece_kernel<1|2>);
+// 0: the general quadrature (piece_kernel<0>)
+int piece_quadrature_mode(const FaceParams &p);
 hipError_t launch_pieces(const FaceParams &p, unsigned long long nslots, hipStream_t s);
 hipError_t launch_face_sum(const FaceParams &p, hipStream_t s);
 size_t scan_temp_bytes(size_t n);

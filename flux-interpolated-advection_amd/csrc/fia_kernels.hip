@@ -20,13 +20,31 @@ constexpr int kBlock = 256;
 
 static inline unsigned grid_for(size_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
+// Every kernel is an index computation around a __host__ __device__ body, so the bodies can also
+// run serially over host arrays (a CPU emulation of the kernels used to check their logic).
+// Their atomics are plain read-modify-writes in that host build.
+FIA_HD int atomic_add_i32(int *p, int v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return atomicAdd(p, v);
+#else
+    const int o = *p;
+    *p = o + v;
+    return o;
+#endif
+}
+FIA_HD void atomic_cas_i32(int *p, int cmp, int v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    atomicCAS(p, cmp, v);
+#else
+    if (*p == cmp) *p = v;
+#endif
+}
+
 // ---------------------------------------------------------------------------------------------
 // psi (reference src/calc_psi_3D.cpp:10-308 density, :311-537 velocity)
 // ---------------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) psi_kernel(PsiParams P) {
+FIA_HD void psi_body(const PsiParams &P, long col) {
     const int mx = P.g.m[0], my = P.g.m[1], mz = P.g.m[2];
-    const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (col >= (long)mx * mz) return;
     const int i = (int)(col / mz), k = (int)(col % mz);
     const Loader<float> Q{P.q, mx, my, mz};
     double acc = 0.0;
@@ -58,14 +76,17 @@ __global__ void __launch_bounds__(kBlock) psi_kernel(PsiParams P) {
         P.psi[idx3(i, j, k, my + 1, mz)] = acc;
     }
 }
+__global__ void __launch_bounds__(kBlock) psi_kernel(PsiParams P) {
+    const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= (long)P.g.m[0] * P.g.m[2]) return;
+    psi_body(P, col);
+}
 
 // ---------------------------------------------------------------------------------------------
 // vertex backtrace (reference src/calc_backtraced_face_3D.cpp:9-159)
 // ---------------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) vertex_kernel(VertexParams P) {
-    const int lx = P.g.m[0] + 1, ly = P.g.m[1] + 1, lz = P.g.m[2] + 1;
-    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= (long)lx * ly * lz) return;
+FIA_HD void vertex_body(const VertexParams &P, long id) {
+    const int ly = P.g.m[1] + 1, lz = P.g.m[2] + 1;
     const int k = (int)(id % lz), j = (int)((id / lz) % ly), i = (int)(id / ((long)ly * lz));
     const int lat[3] = {i, j, k};
     double p[3];
@@ -96,13 +117,18 @@ __global__ void __launch_bounds__(kBlock) vertex_kernel(VertexParams P) {
         P.disp[a][id] = v;
     }
 }
+__global__ void __launch_bounds__(kBlock) vertex_kernel(VertexParams P) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long)(P.g.m[0] + 1) * (P.g.m[1] + 1) * (P.g.m[2] + 1)) return;
+    vertex_body(P, id);
+}
 
 // ---------------------------------------------------------------------------------------------
 // Face setup: corners of face (fam, i, j, k) of G in "+" orientation, backtraced, in the
 // physical frame (cell units), plus corner colours.  Orientation of the reference's faces 1, 2, 5
 // (src/backtrace_and_calc_all_quadrature_point_list_for_integrate_density.cpp:83-112).
 // ---------------------------------------------------------------------------------------------
-__device__ __forceinline__ void face_corner_offsets(int fam, int c, int o[3]) {
+FIA_HD void face_corner_offsets(int fam, int c, int o[3]) {
     // x-face: (j,k),(j+1,k),(j+1,k+1),(j,k+1); y-face: (i,k),(i,k+1),(i+1,k+1),(i+1,k);
     // z-face: (i,j),(i+1,j),(i+1,j+1),(i,j+1)
     const int a = (c == 1 || c == 2), b = (c == 2 || c == 3);
@@ -112,7 +138,7 @@ __device__ __forceinline__ void face_corner_offsets(int fam, int c, int o[3]) {
     else { o[0] = a; o[1] = b; }
 }
 
-__device__ __forceinline__ bool load_face(const FaceParams &P, int fam, int i, int j, int k, double v[4][3],
+FIA_HD bool load_face(const FaceParams &P, int fam, int i, int j, int k, double v[4][3],
                                           double col[4][2]) {
     const int ly = P.g.m[1] + 1, lz = P.g.m[2] + 1;
     bool finite = true;
@@ -149,7 +175,7 @@ __device__ __forceinline__ bool load_face(const FaceParams &P, int fam, int i, i
 // (cells^2), in units of q (reference quadrature src/polygon_3D.cpp:57-77 and 257-487,
 // consumers src/calc_density_in_cell_from_quadrature_point_list_for_integrate_density.cpp:119-201
 // and src/calc_density_in_cell_from_quadrature_point_list.cpp:98-132).
-__device__ __forceinline__ double piece_flux(const FaceParams &P, const double c[3], double ay, int &npts) {
+FIA_HD double piece_flux(const FaceParams &P, const double c[3], double ay, int &npts) {
     const int mx = P.g.m[0], my = P.g.m[1], mz = P.g.m[2];
     const double ex = 0.5 * P.g.e[0], ey = 0.5 * P.g.e[1], ez = 0.5 * P.g.e[2];
     // field offsets in the physical frame: q at G cell centres, psi on G y-faces
@@ -182,17 +208,101 @@ __device__ __forceinline__ double piece_flux(const FaceParams &P, const double c
     return ay * (psi + (double)dens);
 }
 
-__device__ __forceinline__ void decode_face(const CompGrid &g, int fam, long id, int &i, int &j, int &k) {
+// WENO6 integral rule (the same quadrature as piece_flux with IM_WENO6 and integral psi) with
+// the work shared between the points of a piece.  Every point has the piece's x and z, so the
+// sample bases and fractions along x and z are common to psi and q.  psi is evaluated on the G
+// face level J = floor(c_y + e_y) where its y sample coordinate is exactly J: only the z and x
+// passes run.  The density points of the lower half (sample coordinate in [J - 1/2, J)) use
+// sample rows J-3..J+2, those of the upper half rows J-2..J+3, so the z pass runs once over the
+// seven rows J-3..J+3 and each point adds its own y and x passes.  Pass order z, y, x and every
+// 1D evaluation are those of weno3d, so each value equals the per-point rule's (up to the
+// t == 0 skips, which return sample f_2 where this evaluates WENO6 at t = 0).
+template <int NG>
+FIA_HD double piece_flux_weno_integral(const FaceParams &P, const double c[3], double ay, int &npts,
+                                                           int &nevals) {
+    const int mx = P.g.m[0], my = P.g.m[1], mz = P.g.m[2];
+    const double ex = 0.5 * P.g.e[0], ey = 0.5 * P.g.e[1], ez = 0.5 * P.g.e[2];
+    int bx, bz;
+    float tx, tz;
+    split_coord(c[0] - (0.5 - ex), bx, tx);
+    split_coord(c[2] - (0.5 - ez), bz, tz);
+    const double gy = c[1] + ey;
+    const double jf = floor(gy);
+    const double f = gy - jf;
+    const int J = (int)jf;
+    // psi on level J
+    double psi;
+    {
+        const Loader<double> PS{P.psi, mx, my + 1, mz};
+        const WenoCoef wz(tz, P.kappa_psi), wx(tx, P.kappa_psi);
+        double xr[6];
+#pragma unroll
+        for (int ix = 0; ix < 6; ++ix) {
+            double fz[6];
+#pragma unroll
+            for (int iz = 0; iz < 6; ++iz) fz[iz] = PS(bx - 2 + ix, J, bz - 2 + iz);
+            xr[ix] = weno6<double>(fz, wz);
+        }
+        psi = weno6<double>(xr, wx);
+    }
+    // density points: half h, Gauss node g -> point p = h * NG + g
+    const Loader<float> Q{P.q, mx, my, mz};
+    const int nh = f < 0.5 ? 1 : 2;
+    const double level = jf - ey, oqy = 0.5 - ey;
+    const int B = J - 1;   // sample row base of the lower-half points
+    float ty[2 * NG], wt[2 * NG];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            double a, len, frac;
+            if (nh == 1) { a = level; len = f; frac = f; }
+            else if (h == 0) { a = level; len = 0.5; frac = 0.5; }
+            else { a = level + 0.5; len = f - 0.5; frac = f - 0.5; }
+            const double y = (0.5 * len) * P.gauss.x[g] + 0.5 * len + a;
+            ty[h * NG + g] = (float)((y - oqy) - (double)(B + h));
+            wt[h * NG + g] = (float)(frac * P.gauss.w[g]);
+        }
+    }
+    const WenoCoef wz(tz, P.kappa_q), wx(tx, P.kappa_q);
+    float yv[2 * NG][6];
+#pragma unroll
+    for (int ix = 0; ix < 6; ++ix) {
+        float zr[7];
+#pragma unroll
+        for (int r = 0; r < 7; ++r) {
+            if (r == 6 && nh == 1) { zr[6] = 0.0f; continue; }
+            float fz[6];
+#pragma unroll
+            for (int iz = 0; iz < 6; ++iz) fz[iz] = Q(bx - 2 + ix, B - 2 + r, bz - 2 + iz);
+            zr[r] = weno6<float>(fz, wz);
+        }
+#pragma unroll
+        for (int p = 0; p < 2 * NG; ++p) {
+            const int h = p / NG;
+            if (h < nh) yv[p][ix] = weno6<float>(&zr[h], WenoCoef(ty[p], P.kappa_q));
+        }
+    }
+    float dens = 0.0f;
+#pragma unroll
+    for (int p = 0; p < 2 * NG; ++p)
+        if (p / NG < nh) dens += weno6<float>(yv[p], wx) * wt[p];
+    npts += nh * NG;
+    nevals += 7 + 6 * (nh == 2 ? 7 : 6) + 7 * nh * NG;
+    return ay * (psi + (double)dens);
+}
+
+FIA_HD void decode_face(const CompGrid &g, int fam, long id, int &i, int &j, int &k) {
     const int fy = g.m[1] + (fam == 1), fz = g.m[2] + (fam == 2);
     k = (int)(id % fz);
     j = (int)((id / fz) % fy);
     i = (int)(id / ((long)fy * fz));
 }
 
-__device__ __forceinline__ void report(int *err, int code) { atomicCAS(err, 0, code); }
+FIA_HD void report(int *err, int code) { atomic_cas_i32(err, 0, code); }
 
 // Corner colours -> number of piece slots (the product of the x and z strip counts).
-__device__ __forceinline__ void colour_ranges(const double col[4][2], int &x0, int &nxc, int &z0, int &nzc) {
+FIA_HD void colour_ranges(const double col[4][2], int &x0, int &nxc, int &z0, int &nzc) {
     double lx = col[0][0], hx = col[0][0], lz = col[0][1], hz = col[0][1];
     for (int c = 1; c < 4; ++c) {
         lx = fmin(lx, col[c][0]); hx = fmax(hx, col[c][0]);
@@ -205,152 +315,175 @@ __device__ __forceinline__ void colour_ranges(const double col[4][2], int &x0, i
     nzc = z1 >= z0 ? z1 - z0 + 1 : 0;
 }
 
-constexpr unsigned kFaceElsewhere = 0xffffffffu;   // face handled by general_kernel
-
-__device__ __forceinline__ void queue_general(const FaceParams &P, int fam, long id) {
-    const int slot = atomicAdd(P.twisted_count, 1);
+FIA_HD void queue_general(const FaceParams &P, int fam, long id) {
+    const int slot = atomic_add_i32(P.twisted_count, 1);
     P.twisted[slot] = ((unsigned)fam << 30) | (unsigned)id;
 }
 
-// Phase 1: slots each face needs, (x strips) x (z strips); faces for the general cutter get 0.
-template <int FAM> __global__ void __launch_bounds__(kBlock) face_count_kernel(FaceParams P) {
-    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= (long)P.g.faces(FAM)) return;
-    int i, j, k;
-    decode_face(P.g, FAM, id, i, j, k);
-    double v[4][3], col[4][2];
+// Phase 1: slots each face needs, (x strips) x (z strips).  Faces for the general cutter (twisted
+// ones, degenerate ones where the segment cutter runs, or all when minimal_area < 1) are queued
+// and get no slots; a non-finite face reports an error and gets none (zero flux).
+template <int FAM> FIA_HD void face_count_body(const FaceParams &P, long id) {
     unsigned n = 0;
-    if ((P.fam_mask & (1 << FAM)) && load_face(P, FAM, i, j, k, v, col) && !P.general_all) {
-        int x0, nxc, z0, nzc;
-        colour_ranges(col, x0, nxc, z0, nzc);
-        n = (unsigned)((nxc + 1) * (nzc + 1));
+    if (P.fam_mask & (1 << FAM)) {
+        int i, j, k;
+        decode_face(P.g, FAM, id, i, j, k);
+        double v[4][3], col[4][2];
+        if (!load_face(P, FAM, i, j, k, v, col)) {
+            report(P.error, KE_NONFINITE);
+        } else if (P.general_all || face_twisted(col) || (P.seg_cut && face_degenerate(col))) {
+            queue_general(P, FAM, id);
+        } else {
+            int x0, nxc, z0, nzc;
+            colour_ranges(col, x0, nxc, z0, nzc);
+            n = (unsigned)((nxc + 1) * (nzc + 1));
+        }
     }
     P.face_n[P.fam_base[FAM] + id] = n;
 }
-
-// Phase 2: cut each face with the streaming pipeline into its slots (start from the exclusive
-// scan of phase 1).  A slot holds the piece centre (cells, physical frame) and its y vector area;
-// empty strip x band combinations hold a zero area.  Twisted faces are queued for the general
-// cutter and marked; non-finite faces get zero flux and an error code.
-template <int FAM> __global__ void __launch_bounds__(kBlock) face_cut_kernel(FaceParams P) {
+template <int FAM> __global__ void __launch_bounds__(kBlock) face_count_kernel(FaceParams P) {
     const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= (long)P.g.faces(FAM)) return;
+    face_count_body<FAM>(P, id);
+}
+
+// Phase 2a: owner of every slot (global face index), from the exclusive scan of phase 1.
+template <int FAM> FIA_HD void face_expand_body(const FaceParams &P, long id) {
     const size_t gid = P.fam_base[FAM] + id;
+    const unsigned n = P.face_n[gid];
+    const unsigned long long s0 = P.face_start[gid];
+    for (unsigned q = 0; q < n; ++q) P.slot_face[s0 + q] = (unsigned)gid;
+}
+template <int FAM> __global__ void __launch_bounds__(kBlock) face_expand_kernel(FaceParams P) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long)P.g.faces(FAM)) return;
+    face_expand_body<FAM>(P, id);
+}
+
+// Phase 2b: one thread per slot [s0, s1) of family FAM cuts its piece (x strip s, z band m of
+// the face, slot = s * (z bands) + m) with the streaming pipeline and stores its centre (cells,
+// physical frame) and y vector area; an empty strip x band combination stores a zero area.
+// SEG: piece moments from boundary segments (faces whose two colours are both cell-face aligned,
+// or no split); otherwise the streaming clipping pipeline.
+template <int FAM, bool SEG> FIA_HD void slot_cut_body(const FaceParams &P, unsigned long long q) {
+    const unsigned gid = P.slot_face[q];
+    const long id = (long)gid - (long)P.fam_base[FAM];
     int i, j, k;
     decode_face(P.g, FAM, id, i, j, k);
     double v[4][3], col[4][2];
-    if (!load_face(P, FAM, i, j, k, v, col)) {
-        report(P.error, KE_NONFINITE);
-        P.flux[FAM][id] = 0.0;
-        P.corr[FAM][id] = 0.0;
-        P.face_n[gid] = kFaceElsewhere;
-        return;
-    }
-    if (P.general_all) {
-        queue_general(P, FAM, id);
-        P.face_n[gid] = kFaceElsewhere;
-        return;
-    }
+    load_face(P, FAM, i, j, k, v, col);
     FaceGeom geom;
     geom.init(v);
     int x0, nxc, z0, nzc;
     colour_ranges(col, x0, nxc, z0, nzc);
-    const unsigned long long base = P.face_start[gid];
-    unsigned long long slot = base;
-    for (int s = 0; s <= nxc; ++s) {
-        for (int m = 0; m <= nzc; ++m, ++slot) {
-            Pipeline pp;
-            pp.st[0].reset(s > 0, 0, (double)(x0 + s - 1), true);
-            pp.st[1].reset(s < nxc, 0, (double)(x0 + s), false);
-            pp.st[2].reset(m > 0, 1, (double)(z0 + m - 1), true);
-            pp.st[3].reset(m < nzc, 1, (double)(z0 + m), false);
-            pp.acc.reset();
-            const double sq[4][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};
-            for (int c = 0; c < 4; ++c) {
-                CutVert cv;
-                cv.xi = sq[c][0]; cv.eta = sq[c][1]; cv.cx = col[c][0]; cv.cz = col[c][1];
-                pp.push<0>(geom, cv);
-            }
-            pp.close<0>(geom);
-            if (pp.twisted()) {
-                // clear every slot of this face so the piece kernel skips them
-                const unsigned long long end = base + (unsigned long long)((nxc + 1) * (nzc + 1));
-                for (unsigned long long q = base; q < end; ++q) P.piece_ay[q] = 0.0;
-                queue_general(P, FAM, id);
-                P.face_n[gid] = kFaceElsewhere;
-                return;
-            }
-            double ay = 0.0, c3[3] = {0.0, 0.0, 0.0};
-            if (pp.acc.n >= 3 && pp.acc.area2 != 0.0) {
-                const double inv3a = 1.0 / (3.0 * pp.acc.area2);
-                double r[3];
-                geom.rel(pp.acc.mxi * inv3a, pp.acc.meta * inv3a, r);
-                c3[0] = geom.p0[0] + r[0]; c3[1] = geom.p0[1] + r[1]; c3[2] = geom.p0[2] + r[2];
-                ay = 0.5 * pp.acc.ay2;
-            }
-            P.piece_c[0][slot] = c3[0];
-            P.piece_c[1][slot] = c3[1];
-            P.piece_c[2][slot] = c3[2];
-            P.piece_ay[slot] = ay;   // a piece with no y area contributes exactly zero
+    const int local = (int)(q - P.face_start[gid]);
+    const int s = local / (nzc + 1), m = local - s * (nzc + 1);
+    double area2, mxi, meta, ay2;
+    if constexpr (SEG) {
+        SegAcc<false> acc;
+        piece_moments(col, geom, s > 0 ? (double)(x0 + s - 1) : -kNoBound, s < nxc ? (double)(x0 + s) : kNoBound,
+                      m > 0 ? (double)(z0 + m - 1) : -kNoBound, m < nzc ? (double)(z0 + m) : kNoBound, acc);
+        area2 = acc.area2; mxi = acc.mxi; meta = acc.meta; ay2 = acc.ay2;
+    } else {
+        Pipeline pp;
+        pp.st[0].reset(s > 0, 0, (double)(x0 + s - 1), true);
+        pp.st[1].reset(s < nxc, 0, (double)(x0 + s), false);
+        pp.st[2].reset(m > 0, 1, (double)(z0 + m - 1), true);
+        pp.st[3].reset(m < nzc, 1, (double)(z0 + m), false);
+        pp.acc.reset();
+        const double sq[4][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};
+        for (int c = 0; c < 4; ++c) {
+            CutVert cv;
+            cv.xi = sq[c][0]; cv.eta = sq[c][1]; cv.cx = col[c][0]; cv.cz = col[c][1];
+            pp.push<0>(geom, cv);
         }
+        pp.close<0>(geom);
+        area2 = pp.acc.n >= 3 ? pp.acc.area2 : 0.0;
+        mxi = pp.acc.mxi; meta = pp.acc.meta; ay2 = pp.acc.ay2;
     }
+    double ay = 0.0, c3[3] = {0.0, 0.0, 0.0};
+    if (area2 != 0.0) {
+        const double inv3a = 1.0 / (3.0 * area2);
+        double r[3];
+        geom.rel(mxi * inv3a, meta * inv3a, r);
+        c3[0] = geom.p0[0] + r[0]; c3[1] = geom.p0[1] + r[1]; c3[2] = geom.p0[2] + r[2];
+        ay = 0.5 * ay2;
+        if (!isfinite(c3[0]) || !isfinite(c3[1]) || !isfinite(c3[2])) ay = 0.0;   // degenerate sliver
+    }
+    P.piece_c[0][q] = c3[0];
+    P.piece_c[1][q] = c3[1];
+    P.piece_c[2][q] = c3[2];
+    P.piece_ay[q] = ay;   // a piece with no y area contributes exactly zero
+}
+template <int FAM, bool SEG>
+__global__ void __launch_bounds__(kBlock) slot_cut_kernel(FaceParams P, unsigned long long s0, unsigned long long s1) {
+    const unsigned long long q = s0 + (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= s1) return;
+    slot_cut_body<FAM, SEG>(P, q);
 }
 
-// Work counters: block sum in LDS, then one atomic pair per block on one of kStatSlots
-// counter pairs (spread so the atomics do not serialise on one address).
-__device__ __forceinline__ void add_stats(unsigned long long *stats, int a, int b) {
-    __shared__ unsigned long long sa[kBlock / 64], sb[kBlock / 64];
-    unsigned long long x = (unsigned long long)a, y = (unsigned long long)b;
-    for (int off = 32; off > 0; off >>= 1) {
-        x += __shfl_down(x, off, 64);
-        y += __shfl_down(y, off, 64);
-    }
+// Work counters: block sum in LDS, then one atomic per counter and block on one of kStatSlots
+// counter triples (spread so the atomics do not serialise on one address).
+__device__ __forceinline__ void add_stats(unsigned long long *stats, int a, int b, int c) {
+    __shared__ unsigned long long sh[kStatWords][kBlock / 64];
+    unsigned long long x[kStatWords] = {(unsigned long long)a, (unsigned long long)b, (unsigned long long)c};
+#pragma unroll
+    for (int w = 0; w < kStatWords; ++w)
+        for (int off = 32; off > 0; off >>= 1) x[w] += __shfl_down(x[w], off, 64);
     const int wave = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) { sa[wave] = x; sb[wave] = y; }
+    if ((threadIdx.x & 63) == 0)
+        for (int w = 0; w < kStatWords; ++w) sh[w][wave] = x[w];
     __syncthreads();
     if (threadIdx.x == 0) {
-        unsigned long long tx = 0, ty = 0;
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) { tx += sa[w]; ty += sb[w]; }
         const int slot = (int)(blockIdx.x % kStatSlots);
-        atomicAdd(stats + 2 * slot, tx);
-        atomicAdd(stats + 2 * slot + 1, ty);
+        for (int w = 0; w < kStatWords; ++w) {
+            unsigned long long t = 0;
+            for (int v = 0; v < (int)(blockDim.x >> 6); ++v) t += sh[w][v];
+            atomicAdd(stats + kStatWords * slot + w, t);
+        }
     }
 }
 
 // Phase 3: quadrature of every piece slot, one thread per slot (the expensive WENO work runs
-// converged regardless of how many pieces each face has).
-__global__ void __launch_bounds__(kBlock) piece_kernel(FaceParams P, unsigned long long nslots) {
-    const unsigned long long q = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
-    int npieces = 0, npts = 0;
-    if (q < nslots) {
-        const double ay = P.piece_ay[q];
-        double val = 0.0;
-        if (ay != 0.0) {
-            const double c3[3] = {P.piece_c[0][q], P.piece_c[1][q], P.piece_c[2][q]};
-            val = piece_flux(P, c3, ay, npts);
-            npieces = 1;
-        }
-        P.piece_val[q] = val;
+// converged regardless of how many pieces each face has).  QM selects the quadrature: the
+// WENO6 integral rule with NG = QM Gauss points per half (1 or 2), or any mode (QM = 0).
+template <int QM>
+FIA_HD void piece_body(const FaceParams &P, unsigned long long q, int &npieces, int &npts, int &nevals) {
+    const double ay = P.piece_ay[q];
+    double val = 0.0;
+    if (ay != 0.0) {
+        const double c3[3] = {P.piece_c[0][q], P.piece_c[1][q], P.piece_c[2][q]};
+        if constexpr (QM == 0) val = piece_flux(P, c3, ay, npts);
+        else val = piece_flux_weno_integral<QM>(P, c3, ay, npts, nevals);
+        npieces = 1;
     }
-    if (P.stats) add_stats(P.stats, npieces, npts);
+    P.piece_val[q] = val;
+}
+template <int QM> __global__ void __launch_bounds__(kBlock) piece_kernel(FaceParams P, unsigned long long nslots) {
+    const unsigned long long q = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int npieces = 0, npts = 0, nevals = 0;
+    if (q < nslots) piece_body<QM>(P, q, npieces, npts, nevals);
+    if (P.stats) add_stats(P.stats, npieces, npts, nevals);
 }
 
 // Phase 4: face flux = ordered sum of its slots (deterministic).
-template <int FAM> __global__ void __launch_bounds__(kBlock) face_sum_kernel(FaceParams P) {
-    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= (long)P.g.faces(FAM)) return;
+template <int FAM> FIA_HD void face_sum_body(const FaceParams &P, long id) {
     const size_t gid = P.fam_base[FAM] + id;
-    const unsigned n = P.face_n[gid];
-    if (n == kFaceElsewhere) return;
+    const unsigned n = P.face_n[gid];   // 0 for faces of the general cutter (it runs later)
     const unsigned long long s0 = P.face_start[gid];
     double f = 0.0;
     for (unsigned q = 0; q < n; ++q) f += P.piece_val[s0 + q];
     P.flux[FAM][id] = f;
     P.corr[FAM][id] = 0.0;
 }
+template <int FAM> __global__ void __launch_bounds__(kBlock) face_sum_kernel(FaceParams P) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long)P.g.faces(FAM)) return;
+    face_sum_body<FAM>(P, id);
+}
 
 // Flux of one face with the exact recursive cutter, corners in the given order.
-__device__ __forceinline__ double general_face_flux(const FaceParams &P, const double v[4][3], const double col[4][2],
+FIA_HD double general_face_flux(const FaceParams &P, const double v[4][3], const double col[4][2],
                                                     bool &ok) {
     FaceGeom geom;
     geom.init(v);
@@ -385,36 +518,37 @@ __device__ __forceinline__ double general_face_flux(const FaceParams &P, const d
 // lower cell's ('+' order v0 v1 v2 v3) goes to flux, the upper cell's (v0 v3 v2 v1) enters
 // through corr = upper + flux.  Small fixed grid: the private stack is large and only a few
 // faces need it.
+FIA_HD void general_body(const FaceParams &P, int t) {
+    const unsigned code = P.twisted[t];
+    const int fam = (int)(code >> 30);   // 0, 1 or 2
+    const long id = (long)(code & ((1u << 30) - 1u));
+    int i, j, k;
+    decode_face(P.g, fam, id, i, j, k);
+    double v[4][3], col[4][2];
+    load_face(P, fam, i, j, k, v, col);
+    bool ok1, ok2;
+    const double lower = general_face_flux(P, v, col, ok1);
+    double vr[4][3], cr[4][2];
+    const int order[4] = {0, 3, 2, 1};
+    for (int c = 0; c < 4; ++c) {
+        for (int a = 0; a < 3; ++a) vr[c][a] = v[order[c]][a];
+        cr[c][0] = col[order[c]][0]; cr[c][1] = col[order[c]][1];
+    }
+    const double upper = general_face_flux(P, vr, cr, ok2);
+    if (!ok1 || !ok2) report(P.error, KE_CAPACITY);
+    P.flux[fam][id] = lower;
+    P.corr[fam][id] = upper + lower;
+}
 __global__ void __launch_bounds__(64) general_kernel(FaceParams P) {
     const int count = *P.twisted_count;
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < count; t += gridDim.x * blockDim.x) {
-        const unsigned code = P.twisted[t];
-        const int fam = (int)(code >> 30);   // 0, 1 or 2
-        const long id = (long)(code & ((1u << 30) - 1u));
-        int i, j, k;
-        decode_face(P.g, fam, id, i, j, k);
-        double v[4][3], col[4][2];
-        load_face(P, fam, i, j, k, v, col);
-        bool ok1, ok2;
-        const double lower = general_face_flux(P, v, col, ok1);
-        double vr[4][3], cr[4][2];
-        const int order[4] = {0, 3, 2, 1};
-        for (int c = 0; c < 4; ++c) {
-            for (int a = 0; a < 3; ++a) vr[c][a] = v[order[c]][a];
-            cr[c][0] = col[order[c]][0]; cr[c][1] = col[order[c]][1];
-        }
-        const double upper = general_face_flux(P, vr, cr, ok2);
-        if (!ok1 || !ok2) report(P.error, KE_CAPACITY);
-        P.flux[fam][id] = lower;
-        P.corr[fam][id] = upper + lower;
-    }
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < count; t += gridDim.x * blockDim.x) general_body(P, t);
 }
 
 // ---------------------------------------------------------------------------------------------
 // gather: q(i,j,k) = sum over the six faces of +-flux (the '+' face of a cell is its own
 // orientation, the '-' face is the neighbour's '+' face reversed).
 // ---------------------------------------------------------------------------------------------
-__device__ __forceinline__ double cell_sum(const CompGrid &g, double *const f[3], double *const c[3], int i, int j,
+FIA_HD double cell_sum(const CompGrid &g, double *const f[3], double *const c[3], int i, int j,
                                            int k, int fm) {
     const int my = g.m[1], mz = g.m[2];
     double s = 0.0;
@@ -442,15 +576,18 @@ struct GatherArgs {
     int mode, fam_mask;
 };
 
-__global__ void __launch_bounds__(kBlock) gather_kernel(GatherArgs A) {
-    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= (long)A.g.cells()) return;
+FIA_HD void gather_body(const GatherArgs &A, long id) {
     const int mz = A.g.m[2], my = A.g.m[1];
     const int k = (int)(id % mz), j = (int)((id / mz) % my), i = (int)(id / ((long)my * mz));
     double s = cell_sum(A.g, A.f1, A.c1, i, j, k, A.fam_mask);
     if (A.mode == 1) s += cell_sum(A.g, A.f2, A.c2, i, j, k, 7);
     if (A.mode == 2) s = (double)A.base[id] - s;
     A.out[id] = (float)s;
+}
+__global__ void __launch_bounds__(kBlock) gather_kernel(GatherArgs A) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long)A.g.cells()) return;
+    gather_body(A, id);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -494,38 +631,45 @@ __global__ void __launch_bounds__(64) split_faces_kernel(const double *quads, si
         area[o] = 0.5 * ax2; area[o + 1] = 0.5 * ay2; area[o + 2] = 0.5 * az2;
         ++cnt;
     };
-    bool twisted = min_area < 1.0;
+    // same cutter choice as the flux passes: boundary segments when both colours are cell-face
+    // aligned (or there is no split), the streaming pipeline otherwise
+    const bool seg = split == 0 || split == 1;
+    bool twisted = min_area < 1.0 || (seg && (face_twisted(col) || face_degenerate(col)));
     if (!twisted) {
-        double lx = col[0][0], hx = col[0][0], lz = col[0][1], hz = col[0][1];
-        for (int c = 1; c < 4; ++c) {
-            lx = fmin(lx, col[c][0]); hx = fmax(hx, col[c][0]);
-            lz = fmin(lz, col[c][1]); hz = fmax(hz, col[c][1]);
-        }
-        int x0, x1, z0, z1;
-        interior_integers(lx, hx, x0, x1);
-        interior_integers(lz, hz, z0, z1);
-        const int nxc = x1 >= x0 ? x1 - x0 + 1 : 0, nzc = z1 >= z0 ? z1 - z0 + 1 : 0;
+        int x0, nxc, z0, nzc;
+        colour_ranges(col, x0, nxc, z0, nzc);
         for (int s = 0; s <= nxc && !twisted; ++s) {
             for (int m = 0; m <= nzc; ++m) {
-                Pipeline pp;
-                pp.st[0].reset(s > 0, 0, (double)(x0 + s - 1), true);
-                pp.st[1].reset(s < nxc, 0, (double)(x0 + s), false);
-                pp.st[2].reset(m > 0, 1, (double)(z0 + m - 1), true);
-                pp.st[3].reset(m < nzc, 1, (double)(z0 + m), false);
-                pp.acc.reset();
-                const double sq[4][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};
-                for (int c = 0; c < 4; ++c) {
-                    CutVert cv;
-                    cv.xi = sq[c][0]; cv.eta = sq[c][1]; cv.cx = col[c][0]; cv.cz = col[c][1];
-                    pp.push<0>(geom, cv);
+                const double xl = s > 0 ? (double)(x0 + s - 1) : -kNoBound, xh = s < nxc ? (double)(x0 + s) : kNoBound;
+                const double zl = m > 0 ? (double)(z0 + m - 1) : -kNoBound, zh = m < nzc ? (double)(z0 + m) : kNoBound;
+                double area2, mxi, meta, ax2, ay2, az2;
+                if (seg) {
+                    SegAcc<true> acc;
+                    piece_moments(col, geom, xl, xh, zl, zh, acc);
+                    area2 = acc.area2; mxi = acc.mxi; meta = acc.meta; ax2 = acc.ax2; ay2 = acc.ay2; az2 = acc.az2;
+                } else {
+                    Pipeline pp;
+                    pp.st[0].reset(s > 0, 0, xl, true);
+                    pp.st[1].reset(s < nxc, 0, xh, false);
+                    pp.st[2].reset(m > 0, 1, zl, true);
+                    pp.st[3].reset(m < nzc, 1, zh, false);
+                    pp.acc.reset();
+                    const double sq[4][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};
+                    for (int c = 0; c < 4; ++c) {
+                        CutVert cv;
+                        cv.xi = sq[c][0]; cv.eta = sq[c][1]; cv.cx = col[c][0]; cv.cz = col[c][1];
+                        pp.push<0>(geom, cv);
+                    }
+                    pp.close<0>(geom);
+                    if (pp.twisted()) { twisted = true; break; }
+                    area2 = pp.acc.n >= 3 ? pp.acc.area2 : 0.0;
+                    mxi = pp.acc.mxi; meta = pp.acc.meta; ax2 = pp.acc.ax2; ay2 = pp.acc.ay2; az2 = pp.acc.az2;
                 }
-                pp.close<0>(geom);
-                if (pp.twisted()) { twisted = true; break; }
-                if (pp.acc.n >= 3 && pp.acc.area2 != 0.0) {
-                    const double inv3a = 1.0 / (3.0 * pp.acc.area2);
+                if (area2 != 0.0) {
+                    const double inv3a = 1.0 / (3.0 * area2);
                     double r[3];
-                    geom.rel(pp.acc.mxi * inv3a, pp.acc.meta * inv3a, r);
-                    put(r, pp.acc.ax2, pp.acc.ay2, pp.acc.az2);
+                    geom.rel(mxi * inv3a, meta * inv3a, r);
+                    put(r, ax2, ay2, az2);
                 }
             }
         }
@@ -593,16 +737,36 @@ hipError_t launch_face_count(const FaceParams &p, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_face_cut(const FaceParams &p, hipStream_t s) {
-    if (p.fam_mask & 1) hipLaunchKernelGGL(face_cut_kernel<0>, dim3(grid_for(p.g.faces(0), kBlock)), dim3(kBlock), 0, s, p);
-    if (p.fam_mask & 2) hipLaunchKernelGGL(face_cut_kernel<1>, dim3(grid_for(p.g.faces(1), kBlock)), dim3(kBlock), 0, s, p);
-    if (p.fam_mask & 4) hipLaunchKernelGGL(face_cut_kernel<2>, dim3(grid_for(p.g.faces(2), kBlock)), dim3(kBlock), 0, s, p);
+hipError_t launch_face_cut(const FaceParams &p, const unsigned long long fam_slots[4], hipStream_t s) {
+    if (p.fam_mask & 1) hipLaunchKernelGGL(face_expand_kernel<0>, dim3(grid_for(p.g.faces(0), kBlock)), dim3(kBlock), 0, s, p);
+    if (p.fam_mask & 2) hipLaunchKernelGGL(face_expand_kernel<1>, dim3(grid_for(p.g.faces(1), kBlock)), dim3(kBlock), 0, s, p);
+    if (p.fam_mask & 4) hipLaunchKernelGGL(face_expand_kernel<2>, dim3(grid_for(p.g.faces(2), kBlock)), dim3(kBlock), 0, s, p);
+    const unsigned long long n0 = fam_slots[1] - fam_slots[0], n1 = fam_slots[2] - fam_slots[1],
+                             n2 = fam_slots[3] - fam_slots[2];
+    if (p.seg_cut) {
+        if (n0) hipLaunchKernelGGL((slot_cut_kernel<0, true>), dim3(grid_for(n0, kBlock)), dim3(kBlock), 0, s, p, fam_slots[0], fam_slots[1]);
+        if (n1) hipLaunchKernelGGL((slot_cut_kernel<1, true>), dim3(grid_for(n1, kBlock)), dim3(kBlock), 0, s, p, fam_slots[1], fam_slots[2]);
+        if (n2) hipLaunchKernelGGL((slot_cut_kernel<2, true>), dim3(grid_for(n2, kBlock)), dim3(kBlock), 0, s, p, fam_slots[2], fam_slots[3]);
+    } else {
+        if (n0) hipLaunchKernelGGL((slot_cut_kernel<0, false>), dim3(grid_for(n0, kBlock)), dim3(kBlock), 0, s, p, fam_slots[0], fam_slots[1]);
+        if (n1) hipLaunchKernelGGL((slot_cut_kernel<1, false>), dim3(grid_for(n1, kBlock)), dim3(kBlock), 0, s, p, fam_slots[1], fam_slots[2]);
+        if (n2) hipLaunchKernelGGL((slot_cut_kernel<2, false>), dim3(grid_for(n2, kBlock)), dim3(kBlock), 0, s, p, fam_slots[2], fam_slots[3]);
+    }
     return hipGetLastError();
+}
+
+int piece_quadrature_mode(const FaceParams &p) {
+    if (p.interp == IM_WENO6 && p.integral && (p.gauss.n == 1 || p.gauss.n == 2)) return p.gauss.n;
+    return 0;
 }
 
 hipError_t launch_pieces(const FaceParams &p, unsigned long long nslots, hipStream_t s) {
     if (nslots == 0) return hipSuccess;
-    hipLaunchKernelGGL(piece_kernel, dim3(grid_for(nslots, kBlock)), dim3(kBlock), 0, s, p, nslots);
+    const dim3 grid(grid_for(nslots, kBlock));
+    const int qm = piece_quadrature_mode(p);
+    if (qm == 1) hipLaunchKernelGGL(piece_kernel<1>, grid, dim3(kBlock), 0, s, p, nslots);
+    else if (qm == 2) hipLaunchKernelGGL(piece_kernel<2>, grid, dim3(kBlock), 0, s, p, nslots);
+    else hipLaunchKernelGGL(piece_kernel<0>, grid, dim3(kBlock), 0, s, p, nslots);
     return hipGetLastError();
 }
 

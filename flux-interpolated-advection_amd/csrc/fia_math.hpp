@@ -33,39 +33,59 @@ FIA_HD float fast_rcp(float x) {
 #endif
 }
 
-// delta = WENO6(f)(t) - f_2 from the five first differences of the six samples.
-FIA_HD float weno6_delta(float g0, float g1, float g2, float g3, float g4, float t, float kappa) {
-    const float d0 = t + 2.0f, d1 = t + 1.0f, d2 = t, d3 = t - 1.0f, d4 = t - 2.0f, d5 = t - 3.0f;
+// Coefficients of one WENO6 evaluation that depend only on the fraction t (shared by every
+// evaluation of a pass).  With d_i = t + 2 - i: candidate r is f_2 + g-linear Newton terms with
+// the products a_r = d_r d_{r+1} / 2 and b_r = d_r d_{r+1} d_{r+2} / 6; the ideal weights are
+// C0 = d4 d5 / 20, C1 = -d5 d0 / 10, C2 = d0 d1 / 20 (all positive on [0,1]).
+struct WenoCoef {
+    float t, t1, a0, b0, a1, b1, a2, b2, c0, c1, c2, k;
+    FIA_HD explicit WenoCoef(float tt, float kappa) {
+        const float d0 = tt + 2.0f, d1 = tt + 1.0f, d2 = tt, d3 = tt - 1.0f, d4 = tt - 2.0f, d5 = tt - 3.0f;
+        t = tt; t1 = d1;
+        a0 = d0 * d1 * 0.5f; b0 = d0 * d1 * d2 * (1.0f / 6.0f);
+        a1 = d1 * d2 * 0.5f; b1 = d1 * d2 * d3 * (1.0f / 6.0f);
+        a2 = d2 * d3 * 0.5f; b2 = d2 * d3 * d4 * (1.0f / 6.0f);
+        c0 = d4 * d5 * (1.0f / 20.0f);
+        c1 = -d5 * d0 * (1.0f / 10.0f);
+        c2 = d0 * d1 * (1.0f / 20.0f);
+        k = kappa * (1.0f / 180.0f);
+    }
+};
+
+// delta = WENO6(f)(t) - f_2 from the five first differences g_k = f_{k+1} - f_k.
+FIA_HD float weno6_delta(float g0, float g1, float g2, float g3, float g4, const WenoCoef &w) {
     // Newton-form cubic candidates on samples {0..3}, {1..4}, {2..5}, relative to f_2
-    const float q0 = -(g0 + g1) + g0 * d0 + (g1 - g0) * (d0 * d1 * 0.5f)
-                   + (g2 - 2.0f * g1 + g0) * (d0 * d1 * d2 * (1.0f / 6.0f));
-    const float q1 = -g1 + g1 * d1 + (g2 - g1) * (d1 * d2 * 0.5f)
-                   + (g3 - 2.0f * g2 + g1) * (d1 * d2 * d3 * (1.0f / 6.0f));
-    const float q2 = g2 * d2 + (g3 - g2) * (d2 * d3 * 0.5f)
-                   + (g4 - 2.0f * g3 + g2) * (d2 * d3 * d4 * (1.0f / 6.0f));
-    const float c0 = d4 * d5 * (1.0f / 20.0f);
-    const float c1 = -d5 * d0 * (1.0f / 10.0f);
-    const float c2 = d0 * d1 * (1.0f / 20.0f);
-    const float k = kappa * (1.0f / 180.0f);
-    const float s0 = k * (244.0f * g0 * g0 + 1561.0f * g1 * g1 + 814.0f * g2 * g2
-                          - 1171.0f * g0 * g1 + 683.0f * g0 * g2 - 1951.0f * g1 * g2);
-    const float s1 = k * (244.0f * g1 * g1 + 961.0f * g2 * g2 + 244.0f * g3 * g3
-                          - 781.0f * g1 * g2 + 293.0f * g1 * g3 - 781.0f * g2 * g3);
-    const float s2 = k * (814.0f * g2 * g2 + 1561.0f * g3 * g3 + 244.0f * g4 * g4
-                          - 1951.0f * g2 * g3 + 683.0f * g2 * g4 - 1171.0f * g3 * g4);
-    const float e0 = 1e-6f + s0, e1 = 1e-6f + s1, e2 = 1e-6f + s2;
-    const float a0 = c0 * fast_rcp(e0 * e0);
-    const float a1 = c1 * fast_rcp(e1 * e1);
-    const float a2 = c2 * fast_rcp(e2 * e2);
+    const float h0 = g1 - g0, h1 = g2 - g1, h2 = g3 - g2, h3 = g4 - g3;
+    const float q0 = (g0 * w.t1 - g1) + h0 * w.a0 + (h1 - h0) * w.b0;
+    const float q1 = g1 * w.t + h1 * w.a1 + (h2 - h1) * w.b1;
+    const float q2 = g2 * w.t + h2 * w.a2 + (h3 - h2) * w.b2;
+    // smoothness indicators (x180): quadratic forms over shared products
+    const float p00 = g0 * g0, p11 = g1 * g1, p22 = g2 * g2, p33 = g3 * g3, p44 = g4 * g4;
+    const float p01 = g0 * g1, p02 = g0 * g2, p12 = g1 * g2, p13 = g1 * g3, p23 = g2 * g3;
+    const float p24 = g2 * g4, p34 = g3 * g4;
+    const float s0 = 244.0f * p00 + 1561.0f * p11 + 814.0f * p22 - 1171.0f * p01 + 683.0f * p02 - 1951.0f * p12;
+    const float s1 = 244.0f * p11 + 961.0f * p22 + 244.0f * p33 - 781.0f * p12 + 293.0f * p13 - 781.0f * p23;
+    const float s2 = 814.0f * p22 + 1561.0f * p33 + 244.0f * p44 - 1951.0f * p23 + 683.0f * p24 - 1171.0f * p34;
+    const float e0 = w.k * s0 + 1e-6f, e1 = w.k * s1 + 1e-6f, e2 = w.k * s2 + 1e-6f;
+    const float E0 = e0 * e0, E1 = e1 * e1, E2 = e2 * e2;
+    if (fmaxf(e0, fmaxf(e1, e2)) < 1e9f) {
+        // weights C_r / e_r^2 scaled by E0 E1 E2 (one reciprocal): below e = 1e9 every product
+        // stays under 1e36 and above 1e-36, and the weights are normalised before they meet q
+        const float a0 = w.c0 * (E1 * E2), a1 = w.c1 * (E0 * E2), a2 = w.c2 * (E0 * E1);
+        const float r = fast_rcp(a0 + a1 + a2);
+        return (a0 * r) * q0 + (a1 * r) * q1 + (a2 * r) * q2;
+    }
+    const float a0 = w.c0 * fast_rcp(E0), a1 = w.c1 * fast_rcp(E1), a2 = w.c2 * fast_rcp(E2);
     return (a0 * q0 + a1 * q1 + a2 * q2) * fast_rcp(a0 + a1 + a2);
 }
 
 // WENO6 of six samples of type V (float, or double for the y-prefix psi).
-template <class V> FIA_HD V weno6(const V f[6], float t, float kappa) {
+template <class V> FIA_HD V weno6(const V f[6], const WenoCoef &w) {
     const float g0 = (float)(f[1] - f[0]), g1 = (float)(f[2] - f[1]), g2 = (float)(f[3] - f[2]);
     const float g3 = (float)(f[4] - f[3]), g4 = (float)(f[5] - f[4]);
-    return f[2] + (V)weno6_delta(g0, g1, g2, g3, g4, t, kappa);
+    return f[2] + (V)weno6_delta(g0, g1, g2, g3, g4, w);
 }
+template <class V> FIA_HD V weno6(const V f[6], float t, float kappa) { return weno6<V>(f, WenoCoef(t, kappa)); }
 
 // Clamped loader of a staggered field.
 template <class V> struct Loader {

@@ -204,6 +204,9 @@ typedef struct fia_kernel_stats {
     double psi_ms;               /* summed psi-kernel time                                        */
     double gather_ms;            /* summed gather-kernel time                                     */
     long long gather_cells;      /* cells written by gather kernels                               */
+    long long weno_evals;        /* 1D WENO6 evaluations of the WENO6 integral quadrature kernel   */
+    double quad_ms;              /* summed time of the WENO6 integral quadrature launches alone    */
+    long long quad_launches;     /* number of those launches                                       */
 } fia_kernel_stats;
 
 fia_status fia_ctx_set_timing(fia_ctx *ctx, int enable);
