@@ -92,7 +92,10 @@ struct fia_ctx {
     DevBuf<unsigned> face_n;               // piece slots per face
     DevBuf<unsigned long long> face_start; // exclusive scan of face_n
     DevBuf<unsigned> slot_face;            // owner face of every piece slot
-    DevBuf<unsigned char> scan_tmp;
+    DevBuf<unsigned long long> scan_tmp;   // tile sums of the slot scan
+    DevBuf<unsigned long long> bounds;     // slot bounds of the families (device side)
+    unsigned long long *host_bounds = nullptr;   // pinned: [0] 0, [1] [2] family starts, [3] total
+    int *host_flags = nullptr;                   // pinned: twisted count, kernel error
     DevBuf<double> piece_c[3], piece_ay, piece_val;
     DevBuf<float> work[3];
     DevBuf<unsigned> twisted;
@@ -118,7 +121,7 @@ namespace {
 size_t ctx_bytes(const fia_ctx *c) {
     size_t b = c->psi.bytes() + c->twisted.bytes() + c->counters.bytes() + c->scal.bytes() + c->bits.bytes() +
                c->solid.bytes() + c->genI.bytes();
-    b += c->fsa.bytes() + c->fsb.bytes() + c->face_n.bytes() + c->face_start.bytes() + c->slot_face.bytes() + c->scan_tmp.bytes() +
+    b += c->fsa.bytes() + c->fsb.bytes() + c->face_n.bytes() + c->face_start.bytes() + c->slot_face.bytes() + c->scan_tmp.bytes() + c->bounds.bytes() +
          c->piece_ay.bytes() + c->piece_val.bytes();
     for (int a = 0; a < 3; ++a) b += c->piece_c[a].bytes();
     for (int a = 0; a < 3; ++a) b += c->disp[a].bytes() + c->work[a].bytes() + c->gen[a].bytes();
@@ -160,7 +163,7 @@ struct PassSpec {
 
 // Reads the kernel error flag (synchronises the stream) and maps it to a status.
 fia_status check_kernel_errors(fia_ctx *c) {
-    int h[2] = {0, 0};
+    int *h = c->host_flags;
     HIP_TRY(hipMemcpyAsync(h, c->counters.p, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (h[1] == KE_CAPACITY) return fail(FIA_ERR_CAPACITY, "face cutter capacity exceeded (polygon or stack)");
@@ -255,25 +258,23 @@ fia_status do_faces(fia_ctx *c, const CompGrid &g, const float *q, bool moving, 
     const size_t nf = g.faces(0) + g.faces(1) + g.faces(2);
     HIP_TRY(c->face_n.ensure(nf));
     HIP_TRY(c->face_start.ensure(nf));
-    const size_t tb = scan_temp_bytes(nf);
-    HIP_TRY(c->scan_tmp.ensure(tb));
+    HIP_TRY(c->scan_tmp.ensure(scan_temp_words(nf)));
+    HIP_TRY(c->bounds.ensure(4));
     p.face_n = c->face_n.p;
     p.face_start = c->face_start.p;
     p.fam_base[0] = 0; p.fam_base[1] = g.faces(0); p.fam_base[2] = g.faces(0) + g.faces(1);
     int sp;
     HIP_TRY(span_begin(c, SK_FACE, sp));
     HIP_TRY(launch_face_count(p, c->stream));   // all families; inactive ones count zero
-    HIP_TRY(launch_scan(c->scan_tmp.p, tb, c->face_n.p, c->face_start.p, nf, c->stream));
-    // slot bounds of the families: starts of their first faces and the total
-    unsigned long long fam_slots[4] = {0, 0, 0, 0};
-    unsigned lastn = 0;
-    HIP_TRY(hipMemcpyAsync(&fam_slots[1], c->face_start.p + p.fam_base[1], 8, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipMemcpyAsync(&fam_slots[2], c->face_start.p + p.fam_base[2], 8, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipMemcpyAsync(&fam_slots[3], c->face_start.p + nf - 1, 8, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipMemcpyAsync(&lastn, c->face_n.p + nf - 1, 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(launch_scan(c->scan_tmp.p, c->face_n.p, c->face_start.p, nf, p, c->bounds.p, c->stream));
+    // slot bounds of the families into pinned host memory
+    HIP_TRY(hipMemcpyAsync(c->host_bounds, c->bounds.p, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                           c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    fam_slots[3] += lastn;
+    unsigned long long fam_slots[4] = {0, c->host_bounds[1], c->host_bounds[2], c->host_bounds[3]};
     const unsigned long long nslots = fam_slots[3];
+    if (!(fam_slots[1] <= fam_slots[2] && fam_slots[2] <= fam_slots[3]))
+        return fail(FIA_ERR_DEVICE, "inconsistent piece slot bounds");
     HIP_TRY(c->slot_face.ensure(nslots));
     p.slot_face = c->slot_face.p;
     for (int a = 0; a < 3; ++a) HIP_TRY(c->piece_c[a].ensure(nslots));
@@ -472,7 +473,19 @@ fia_status fia_ctx_create(int device, fia_ctx **out) {
         delete c;
         return fail(FIA_ERR_DEVICE, "cannot create the stream");
     }
-    if (ensure_common(c) != FIA_OK) { (void)hipStreamDestroy(c->stream); delete c; return FIA_ERR_DEVICE; }
+    if (hipHostMalloc((void **)&c->host_bounds, 4 * sizeof(unsigned long long)) != hipSuccess ||
+        hipHostMalloc((void **)&c->host_flags, 4 * sizeof(int)) != hipSuccess) {
+        if (c->host_bounds) (void)hipHostFree(c->host_bounds);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return fail(FIA_ERR_DEVICE, "cannot allocate pinned host memory");
+    }
+    memset(c->host_bounds, 0, 4 * sizeof(unsigned long long));
+    memset(c->host_flags, 0, 4 * sizeof(int));
+    if (ensure_common(c) != FIA_OK) {
+        (void)hipHostFree(c->host_bounds); (void)hipHostFree(c->host_flags);
+        (void)hipStreamDestroy(c->stream); delete c; return FIA_ERR_DEVICE;
+    }
     *out = c;
     return FIA_OK;
 }
@@ -490,6 +503,11 @@ fia_status fia_ctx_destroy(fia_ctx *c) {
     for (int a = 0; a < 4; ++a) c->grid[a].release();
     for (int a = 0; a < 2; ++a) c->genD[a].release();
     for (int a = 0; a < 8; ++a) c->pcg[a].release();
+    c->bounds.release(); c->stats.release();
+    for (auto &sp : c->spans) { (void)hipEventDestroy(sp.a); (void)hipEventDestroy(sp.b); }
+    c->spans.clear();
+    if (c->host_bounds) (void)hipHostFree(c->host_bounds);
+    if (c->host_flags) (void)hipHostFree(c->host_flags);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return FIA_OK;

@@ -99,8 +99,11 @@ hipError_t launch_face_cut(const FaceParams &p, const unsigned long long fam_slo
 int piece_quadrature_mode(const FaceParams &p);
 hipError_t launch_pieces(const FaceParams &p, unsigned long long nslots, hipStream_t s);
 hipError_t launch_face_sum(const FaceParams &p, hipStream_t s);
-size_t scan_temp_bytes(size_t n);
-hipError_t launch_scan(void *temp, size_t bytes, const unsigned *in, unsigned long long *out, size_t n, hipStream_t s);
+// exclusive scan of the slot counts in (n faces) into out; bounds[1], bounds[2] = first slots of
+// families 1 and 2 (at p.fam_base), bounds[3] = total; tile_tmp holds scan_temp_words(n) words
+size_t scan_temp_words(size_t n);
+hipError_t launch_scan(unsigned long long *tile_tmp, const unsigned *in, unsigned long long *out, size_t n,
+                       const FaceParams &p, unsigned long long *bounds, hipStream_t s);
 hipError_t launch_faces_general(const FaceParams &p, hipStream_t s);
 // out = sum over families of flux differences (scaled); mode 0: out = G1; 1: out = G1 + G2;
 // 2: out = base - G1 (y family only).

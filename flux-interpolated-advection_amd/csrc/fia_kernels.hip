@@ -8,8 +8,6 @@
 //   gather_kernel    per cell: sum of its six signed face fluxes (deterministic, no atomics)
 // Each interior face is computed once; its flux enters the two neighbouring cells with opposite
 // signs, so nothing is lost or created inside the domain.
-#include <hipcub/hipcub.hpp>
-
 #include "fia_cutter.hpp"
 #include "fia_internal.hpp"
 #include "fia_math.hpp"
@@ -463,7 +461,7 @@ template <int QM> __global__ void __launch_bounds__(kBlock) piece_kernel(FacePar
     const unsigned long long q = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
     int npieces = 0, npts = 0, nevals = 0;
     if (q < nslots) piece_body<QM>(P, q, npieces, npts, nevals);
-    if (P.stats) add_stats(P.stats, npieces, npts, nevals);
+    if (QM != 0 && P.stats) add_stats(P.stats, npieces, npts, nevals);   // counted for the WENO6 rule only
 }
 
 // Phase 4: face flux = ordered sum of its slots (deterministic).
@@ -777,14 +775,106 @@ hipError_t launch_face_sum(const FaceParams &p, hipStream_t s) {
     return hipGetLastError();
 }
 
-size_t scan_temp_bytes(size_t n) {
-    size_t bytes = 0;
-    hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (const unsigned *)nullptr, (unsigned long long *)nullptr, (int)n);
-    return bytes;
+// ---------------------------------------------------------------------------------------------
+// Exclusive scan of the per-face slot counts into 64-bit starts.  Three plain kernels over tiles
+// of kScanTile counts: tile sums, one block scanning the tile sums, then each tile scanned in
+// LDS and offset.  No block waits on another block's progress.  The last kernel also writes the
+// slot bounds of the three families (bounds[f] = start of family f, bounds[3] = total).
+// ---------------------------------------------------------------------------------------------
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kBlock * kScanItems;
+
+// block-wide exclusive scan of one value per thread (kBlock threads, 64-wide waves); total out
+__device__ __forceinline__ unsigned long long block_exclusive_scan(unsigned long long x, unsigned long long &total) {
+    __shared__ unsigned long long wsum[kBlock / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    unsigned long long incl = x;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const unsigned long long y = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += y;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    unsigned long long before = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) {
+        if (w < wave) before += wsum[w];
+        all += wsum[w];
+    }
+    __syncthreads();   // wsum may be reused by the caller's next scan
+    total = all;
+    return before + incl - x;
 }
 
-hipError_t launch_scan(void *temp, size_t bytes, const unsigned *in, unsigned long long *out, size_t n, hipStream_t s) {
-    return hipcub::DeviceScan::ExclusiveSum(temp, bytes, in, out, (int)n, s);
+__global__ void __launch_bounds__(kBlock) scan_tile_sums_kernel(const unsigned *in, size_t n,
+                                                                unsigned long long *tile_sum) {
+    const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
+    unsigned long long s = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i)
+        if (base + i < n) s += in[base + i];
+    unsigned long long total;
+    (void)block_exclusive_scan(s, total);
+    if (threadIdx.x == 0) tile_sum[blockIdx.x] = total;
+}
+
+// one block: tile_sum[0..ntiles) -> exclusive prefix in place
+__global__ void __launch_bounds__(kBlock) scan_tile_offsets_kernel(unsigned long long *tile_sum, int ntiles) {
+    const int per = (ntiles + kBlock - 1) / kBlock;
+    const int b0 = threadIdx.x * per;
+    unsigned long long s = 0;
+    for (int i = 0; i < per; ++i)
+        if (b0 + i < ntiles) s += tile_sum[b0 + i];
+    unsigned long long total;
+    unsigned long long run = block_exclusive_scan(s, total);
+    for (int i = 0; i < per; ++i) {
+        if (b0 + i < ntiles) {
+            const unsigned long long v = tile_sum[b0 + i];
+            tile_sum[b0 + i] = run;
+            run += v;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) scan_tiles_kernel(const unsigned *in, size_t n,
+                                                            const unsigned long long *tile_off,
+                                                            unsigned long long *out, FaceParams P,
+                                                            unsigned long long *bounds) {
+    const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
+    unsigned v[kScanItems];
+    unsigned long long s = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        v[i] = base + i < n ? in[base + i] : 0u;
+        s += v[i];
+    }
+    unsigned long long total;
+    unsigned long long run = tile_off[blockIdx.x] + block_exclusive_scan(s, total);
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        const size_t k = base + i;
+        if (k < n) {
+            out[k] = run;
+            if (k == P.fam_base[1]) bounds[1] = run;
+            if (k == P.fam_base[2]) bounds[2] = run;
+            if (k == n - 1) bounds[3] = run + v[i];
+        }
+        run += v[i];
+    }
+}
+
+size_t scan_temp_words(size_t n) { return (n + kScanTile - 1) / kScanTile; }
+
+hipError_t launch_scan(unsigned long long *tile_tmp, const unsigned *in, unsigned long long *out, size_t n,
+                       const FaceParams &p, unsigned long long *bounds, hipStream_t s) {
+    const size_t ntiles = scan_temp_words(n);
+    if (ntiles == 0 || ntiles > (size_t)INT32_MAX) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(scan_tile_sums_kernel, dim3((unsigned)ntiles), dim3(kBlock), 0, s, in, n, tile_tmp);
+    hipLaunchKernelGGL(scan_tile_offsets_kernel, dim3(1), dim3(kBlock), 0, s, tile_tmp, (int)ntiles);
+    hipLaunchKernelGGL(scan_tiles_kernel, dim3((unsigned)ntiles), dim3(kBlock), 0, s, in, n,
+                       (const unsigned long long *)tile_tmp, out, p, bounds);
+    return hipGetLastError();
 }
 
 hipError_t launch_faces_general(const FaceParams &p, hipStream_t s) {

@@ -194,11 +194,11 @@ fia_status fia_pressure_project(fia_ctx *ctx, fia_grid3d *grid, const uint8_t *s
  * src/move_substances_3d.cpp:607-840): when enabled, the library records HIP events around every
  * face-kernel launch on its own stream and counts the quadrature work those launches did. */
 typedef struct fia_kernel_stats {
-    double face_ms;              /* summed face-kernel time (HIP events on the context's stream)  */
-    long long face_launches;     /* number of face-kernel launches                                */
-    long long pieces;            /* face pieces integrated                                        */
-    long long psi_points;        /* psi interpolation points (one per piece)                      */
-    long long density_points;    /* density/velocity quadrature points                            */
+    double face_ms;              /* summed face-phase time (HIP events on the context's stream)   */
+    long long face_launches;     /* number of face phases (one per flux pass)                     */
+    long long pieces;            /* face pieces integrated by the WENO6 integral quadrature       */
+    long long psi_points;        /* its psi points (one per piece)                                */
+    long long density_points;    /* its density quadrature points                                 */
     long long vertices;          /* backtraced lattice vertices                                   */
     double vertex_ms;            /* summed vertex-kernel time                                     */
     double psi_ms;               /* summed psi-kernel time                                        */

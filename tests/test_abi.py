@@ -62,7 +62,7 @@ def test_cpp_shim_header_compiles():
     shim = os.path.join(ROOT, "include", "fia_smoke_simulation.hpp")
     if not os.path.exists(shim):
         pytest.skip("no shim")
-    src = '#include "fia_smoke_simulation.hpp"\nint main(){ smoke_simulation::Grid_3D g(4,4,4,0.25f); return (int)g.substance_density.size() - 64; }\n'
+    src = '#include "fia_smoke_simulation.hpp"\nint main(){ smoke_simulation::Grid_3D g(4,4,4,0.25f); return (int)g.substance_density.size() - 64 + (int)g.velocity_in_voxel_face_x.size() - 80; }\n'
     r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"), "-x", "c++", "-"],
                        input=src, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
