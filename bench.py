@@ -1,14 +1,20 @@
-"""Benchmark: flux-interpolated density step (move_substances_3D) at 256^3, CFL 4, on one MI355X.
+"""Benchmark of the flux-interpolated advection path on one MI355X.
 
-Workload (BASELINE configuration "256^3 plume"): 256^3 MAC grid in the unit cube, density = a
-Gaussian plume blob plus a sphere source, divergence-free seeded curl-noise velocity (wavenumbers
-1-4) scaled so that dt = 4 dx / max|u| (CFL 4).  One step = one move_substances_3D: WENO6 flux
-pass with integral psi (Gauss n=1), xz-CellFaceAligned split, plus the zero-velocity correction
-passes -- the reference's default step.  Fields stay resident on the GPU.
+Workloads (--workload), each on the 256^3 plume of the measured configuration: 256^3 MAC grid in
+the unit cube, density = a Gaussian plume blob plus a sphere source, divergence-free seeded
+curl-noise velocity (wavenumbers 1-4) scaled so that dt = 4 dx / max|u| (CFL 4), WENO6, integral
+psi (Gauss n=1), zero-velocity correction on.  Fields stay resident on the GPU.
 
-Prints one JSON line: value = cell-steps per second / 1e6 over the timed steps, a roofline entry
-for the face kernel measured with HIP events on the library's stream, and a CPU baseline: the
-oracle's FP32 restatement of the reference path, multi-threaded, on a bounded 64^3 sample.
+  density     (default) one step = one move_substances_3D (xz-CellFaceAligned): 3 flux passes
+  velocity    one step = advect_velocity_xyz_flux_advection_3D (u, v, w, 3 passes each)
+  projection  one step = one pressure projection (Jacobi-PCG to 1e-6 relative residual, zero
+              initial guess) of the curl-noise velocity plus a buoyancy impulse, 8 spherical
+              obstacles as a solid mask
+
+Prints one JSON line: value = cell-steps per second / 1e6 over the timed steps; for the
+advection workloads a roofline entry for the WENO6 quadrature kernel measured with HIP events on
+the library's stream, and (density, unless --no-cpu-baseline) a CPU baseline: the oracle's FP32
+restatement of the reference path, multi-threaded, on a bounded 128^3 sample.
 """
 import argparse
 import importlib
@@ -38,6 +44,10 @@ PEAK_HBM_GBS = 8000.0
 # rocprofv3 counter passes (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a 1 GiB device copy);
 # recorded in profiles/density_step_256_cfl4_hbm_counters.md
 TRAFFIC_BYTES_256_CFL4 = 11.561e9
+# Jacobi-PCG algorithmic traffic per cell and iteration (FP64 vectors): apply 24 B (p, diag
+# read, q written), dot(p,q) 16, x/r update 48, preconditioner 24, dot(r,z) 16, dot(r,r) 8,
+# p update 24
+PCG_BYTES_PER_CELL_ITER = 160
 
 
 def make_scene(scenes, n, cfl, seed=1):
@@ -63,6 +73,134 @@ def cpu_baseline(scenes, n, cfl, threads):
                       "reference path (per-cell faces, std::thread over x-slabs), %.1f s" % (n, cfl, sec)}
 
 
+def quad_roofline(st, steps, n, cfl, density):
+    """Roofline entry of the WENO6 integral quadrature kernel (piece_kernel<1>) from the
+    library's HIP-event spans and device work counters of the timed steps."""
+    points = st["psi_points"] + st["density_points"]
+    quad_s = st["quad_ms"] / 1e3
+    qlaunches = max(st["quad_launches"], 1)
+    achieved = points * FLOP_PER_WENO3D_POINT / quad_s / 1e12 if quad_s > 0 else 0.0
+    executed = st["weno_evals"] * FLOP_PER_WENO6_EXECUTED / quad_s / 1e12 if quad_s > 0 else 0.0
+    traffic = TRAFFIC_BYTES_256_CFL4 if (density and n == 256 and cfl == 4.0) else None
+    avg_quad_s = quad_s / qlaunches
+    return {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
+            "kernel": "piece_kernel<1> (WENO6 integral quadrature)", "compute": "FP32 VALU, no MFMA",
+            "avg_launch_ms": round(st["quad_ms"] / qlaunches, 3),
+            "weno3d_points_per_launch": points / qlaunches, "flop_per_point": FLOP_PER_WENO3D_POINT,
+            "executed": {"TFLOPs": round(executed, 3), "frac": round(executed / PEAK_FP32_TFLOPS, 4),
+                         "weno6_evals_per_launch": st["weno_evals"] / qlaunches,
+                         "flop_per_weno6": FLOP_PER_WENO6_EXECUTED},
+            "hbm": None if traffic is None else {
+                "achieved_GBs": round(traffic / avg_quad_s / 1e9, 1), "peak_GBs": PEAK_HBM_GBS,
+                "frac": round(traffic / avg_quad_s / 1e9 / PEAK_HBM_GBS, 4)}}
+
+
+def run_advection(args, fia, scenes, torch, velocity):
+    n = args.n
+    h, rho, u, v, w, dt = make_scene(scenes, n, args.cfl)
+    dev = torch.device("cuda:0")
+    T = lambda a: torch.from_numpy(a).to(dev)
+    rho_d, u_d, v_d, w_d = T(rho), T(u), T(v), T(w)
+    grid = fia.Grid3D(n, n, n, h, rho_d, u_d, v_d, w_d)
+    ctx = fia.Context(0)
+    if velocity:
+        opts = dict(interpolation_method="WENO6", use_integral=True,
+                    num_gauss_quadrature_point_for_integrate_density=1,
+                    enable_eliminate_zero_velocity_false_diffusion=True)
+        step = lambda i: fia.advect_velocity_xyz_flux_advection_3d(ctx, grid, dt, **opts)
+    else:
+        opts = dict(split_method="xz-CellFaceAligned", interpolation_method="WENO6", use_integral=True,
+                    num_gauss_quadrature_point_for_integrate_density=1,
+                    enable_eliminate_zero_velocity_false_diffusion=True)
+        step = lambda i: fia.move_substances_3d(ctx, grid, i, dt, **opts)
+    for i in range(args.warmup):
+        step(i)
+    torch.cuda.synchronize()
+    ctx.set_timing(True)
+    ctx.reset_stats()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step(i)
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    st = ctx.stats()
+    launches = max(st["face_launches"], 1)
+    if velocity:
+        workload = ("advect_velocity_xyz_flux_advection_3D step (u, v, w; 3 passes each), %d^3 curl-noise velocity, "
+                    "CFL %g, WENO6, integral psi (Gauss 1), per-component splits, zero-velocity correction on"
+                    % (n, args.cfl))
+        extra = {"max_abs_velocity_after": float(max(u_d.abs().max().item(), v_d.abs().max().item(),
+                                                     w_d.abs().max().item()))}
+    else:
+        workload = ("move_substances_3D density step, %d^3 plume blob + sphere source, curl-noise velocity, CFL %g, "
+                    "WENO6, integral psi (Gauss 1), xz-CellFaceAligned, zero-velocity correction on" % (n, args.cfl))
+        extra = {"mass_after": float(rho_d.double().sum().item())}
+    out = {
+        "metric": METRIC, "value": round(n ** 3 * args.steps / sec / 1e6, 3), "unit": "Mcell-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(sec / args.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": workload, "grid": [n, n, n], "cfl": args.cfl,
+                   "passes_per_step": 9 if velocity else 3},
+        "roofline": quad_roofline(st, args.steps, n, args.cfl, not velocity),
+        "kernels_ms_per_step": {k: round(st[k] / args.steps, 3)
+                                for k in ("face_ms", "quad_ms", "vertex_ms", "psi_ms", "gather_ms")},
+        "face_work": {"launches": st["face_launches"], "weno_pieces_per_step": st["pieces"] / args.steps,
+                      "density_points_per_step": st["density_points"] / args.steps,
+                      "avg_face_launch_ms": round(st["face_ms"] / launches, 3)},
+    }
+    out.update(extra)
+    if not velocity and not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(scenes, args.cpu_n, args.cfl, min(os.cpu_count() or 1, 16))
+    ctx.close()
+    return out
+
+
+def run_projection(args, fia, scenes, torch):
+    n = args.n
+    h = 1.0 / n
+    u, v, w = scenes.curl_noise(n, seed=1)
+    rho = scenes.gaussian_blob(n, centre=(0.5, 0.3, 0.5), sigma=0.1) + scenes.sphere_source(n)
+    # buoyancy impulse on the y faces (y is up): v += dt_b * rho averaged to the face
+    vb = np.zeros_like(v)
+    vb[:, 1:-1, :] = 0.5 * (rho[:, 1:, :] + rho[:, :-1, :])
+    v = v + 0.5 * vb
+    solid = scenes.spherical_obstacles(n, count=8, seed=3)
+    dev = torch.device("cuda:0")
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).to(dev)
+    u0, v0, w0 = T(u), T(v), T(w)
+    solid_d = torch.from_numpy(np.ascontiguousarray(solid, dtype=np.uint8)).to(dev)
+    ctx = fia.Context(0)
+    grid = fia.Grid3D(n, n, n, h, None, u0.clone(), v0.clone(), w0.clone())
+    times, iters, res = [], [], []
+    for i in range(args.warmup + args.steps):
+        grid.u.copy_(u0); grid.v.copy_(v0); grid.w.copy_(w0)
+        grid.pressure = None
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        it, rr = fia.calc_pressure_gradient_term_3d(ctx, grid, solid=solid_d, rel_tol=1e-6, max_iters=20000)
+        torch.cuda.synchronize()
+        if i >= args.warmup:
+            times.append(time.perf_counter() - t0); iters.append(it); res.append(rr)
+    sec = sum(times)
+    it_mean = float(np.mean(iters))
+    gbs = PCG_BYTES_PER_CELL_ITER * n ** 3 * it_mean / (sec / args.steps) / 1e9
+    ctx.close()
+    return {
+        "metric": METRIC, "value": round(n ** 3 * args.steps / sec / 1e6, 3), "unit": "Mcell-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(sec / args.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "pressure projection, %d^3, Jacobi-PCG (FP64) to 1e-6 relative residual from a zero "
+                               "initial guess, curl-noise velocity + buoyancy impulse, 8 spherical obstacles" % n,
+                   "grid": [n, n, n]},
+        "pcg": {"iterations": iters, "rel_residual": res, "warm_start": "none (the U-Net warm start is not built)"},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": None,
+                     "kernel": "whole Jacobi-PCG iteration (%d B per cell and iteration)" % PCG_BYTES_PER_CELL_ITER},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,6 +208,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--cfl", type=float, default=4.0)
+    ap.add_argument("--workload", choices=("density", "velocity", "projection"), default="density")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=128)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -83,79 +222,16 @@ def main():
     import torch
     fia = importlib.import_module("flux-interpolated-advection_amd")
     scenes = importlib.import_module("flux-interpolated-advection_amd.scenes")
-    n = args.n
-    h, rho, u, v, w, dt = make_scene(scenes, n, args.cfl)
-    dev = torch.device("cuda:0")
     if args.calibrate_copy:
-        a = torch.ones(1 << 28, dtype=torch.float32, device=dev)   # 1 GiB read + 1 GiB written
+        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda:0")   # 1 GiB read + 1 GiB written
         b = torch.empty_like(a)
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
-    T = lambda a: torch.from_numpy(a).to(dev)
-    rho_d, u_d, v_d, w_d = T(rho), T(u), T(v), T(w)
-    grid = fia.Grid3D(n, n, n, h, rho_d, u_d, v_d, w_d)
-    ctx = fia.Context(0)
-    opts = dict(split_method="xz-CellFaceAligned", interpolation_method="WENO6", use_integral=True,
-                num_gauss_quadrature_point_for_integrate_density=1,
-                enable_eliminate_zero_velocity_false_diffusion=True)
-    for _ in range(args.warmup):
-        fia.move_substances_3d(ctx, grid, 0, dt, **opts)
-    torch.cuda.synchronize()
-    ctx.set_timing(True)
-    ctx.reset_stats()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        fia.move_substances_3d(ctx, grid, i, dt, **opts)
-    torch.cuda.synchronize()
-    sec = time.perf_counter() - t0
-    st = ctx.stats()
-    cells = n ** 3
-    value = cells * args.steps / sec / 1e6
-    # roofline of the dominant kernel: the WENO6 integral quadrature (piece_kernel<1>), timed with
-    # HIP events on the library's stream around each of its launches
-    points = st["psi_points"] + st["density_points"]
-    flop = points * FLOP_PER_WENO3D_POINT
-    flop_exec = st["weno_evals"] * FLOP_PER_WENO6_EXECUTED
-    quad_s = st["quad_ms"] / 1e3
-    achieved = flop / quad_s / 1e12 if quad_s > 0 else 0.0
-    executed = flop_exec / quad_s / 1e12 if quad_s > 0 else 0.0
-    launches = max(st["face_launches"], 1)
-    qlaunches = max(st["quad_launches"], 1)
-    default_workload = n == 256 and args.cfl == 4.0
-    traffic = TRAFFIC_BYTES_256_CFL4 if default_workload else None
-    avg_quad_s = quad_s / qlaunches
-    mass = float(rho_d.double().sum().item())
-    out = {
-        "metric": METRIC, "value": round(value, 3), "unit": "Mcell-steps/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(sec / args.steps * 1e3, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "move_substances_3D density step, %d^3 plume blob + sphere source, curl-noise "
-                               "velocity, CFL %g, WENO6, integral psi (Gauss 1), xz-CellFaceAligned, "
-                               "zero-velocity correction on" % (n, args.cfl),
-                   "grid": [n, n, n], "cfl": args.cfl, "passes_per_step": 3},
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
-                     "kernel": "piece_kernel<1> (WENO6 integral quadrature)", "compute": "FP32 VALU, no MFMA",
-                     "avg_launch_ms": round(st["quad_ms"] / qlaunches, 3),
-                     "weno3d_points_per_launch": points / qlaunches, "flop_per_point": FLOP_PER_WENO3D_POINT,
-                     "executed": {"TFLOPs": round(executed, 3), "frac": round(executed / PEAK_FP32_TFLOPS, 4),
-                                  "weno6_evals_per_launch": st["weno_evals"] / qlaunches,
-                                  "flop_per_weno6": FLOP_PER_WENO6_EXECUTED},
-                     "hbm": None if traffic is None else {
-                         "achieved_GBs": round(traffic / avg_quad_s / 1e9, 1), "peak_GBs": PEAK_HBM_GBS,
-                         "frac": round(traffic / avg_quad_s / 1e9 / PEAK_HBM_GBS, 4)}},
-        "kernels_ms_per_step": {k: round(st[k] / args.steps, 3)
-                                for k in ("face_ms", "quad_ms", "vertex_ms", "psi_ms", "gather_ms")},
-        "face_work": {"launches": st["face_launches"], "weno_pieces_per_step": st["pieces"] / args.steps,
-                      "density_points_per_step": st["density_points"] / args.steps,
-                      "avg_face_launch_ms": round(st["face_ms"] / launches, 3)},
-        "mass_after": mass,
-    }
-    if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(scenes, args.cpu_n, args.cfl, min(os.cpu_count() or 1, 16))
-    ctx.close()
+    if args.workload == "projection":
+        out = run_projection(args, fia, scenes, torch)
+    else:
+        out = run_advection(args, fia, scenes, torch, args.workload == "velocity")
     print(json.dumps(out))
 
 

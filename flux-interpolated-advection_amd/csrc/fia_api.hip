@@ -96,6 +96,7 @@ struct fia_ctx {
     DevBuf<unsigned long long> bounds;     // slot bounds of the families (device side)
     unsigned long long *host_bounds = nullptr;   // pinned: [0] 0, [1] [2] family starts, [3] total
     int *host_flags = nullptr;                   // pinned: twisted count, kernel error
+    double *host_scal = nullptr;                 // pinned: PCG scalars read back while iterating
     DevBuf<double> piece_c[3], piece_ay, piece_val;
     DevBuf<float> work[3];
     DevBuf<unsigned> twisted;
@@ -253,6 +254,15 @@ fia_status do_faces(fia_ctx *c, const CompGrid &g, const float *q, bool moving, 
     if (c->timing) {
         HIP_TRY(c->stats.ensure(kStatWords * kStatSlots));
         p.stats = c->stats.p;
+    }
+    if (!moving && !p.general_all) {
+        // undeformed faces: closed-form pieces of the y faces (the only family with y area)
+        int sp;
+        HIP_TRY(span_begin(c, SK_FACE, sp));
+        HIP_TRY(launch_zero_velocity_faces(p, c->stream));
+        HIP_TRY(span_end(c, sp));
+        if (c->timing) c->acc.face_launches += 1;
+        return FIA_OK;
     }
     // slots per face -> exclusive scan -> piece records -> quadrature -> ordered sums
     const size_t nf = g.faces(0) + g.faces(1) + g.faces(2);
@@ -474,16 +484,19 @@ fia_status fia_ctx_create(int device, fia_ctx **out) {
         return fail(FIA_ERR_DEVICE, "cannot create the stream");
     }
     if (hipHostMalloc((void **)&c->host_bounds, 4 * sizeof(unsigned long long)) != hipSuccess ||
-        hipHostMalloc((void **)&c->host_flags, 4 * sizeof(int)) != hipSuccess) {
+        hipHostMalloc((void **)&c->host_flags, 4 * sizeof(int)) != hipSuccess ||
+        hipHostMalloc((void **)&c->host_scal, 16 * sizeof(double)) != hipSuccess) {
         if (c->host_bounds) (void)hipHostFree(c->host_bounds);
+        if (c->host_flags) (void)hipHostFree(c->host_flags);
         (void)hipStreamDestroy(c->stream);
         delete c;
         return fail(FIA_ERR_DEVICE, "cannot allocate pinned host memory");
     }
     memset(c->host_bounds, 0, 4 * sizeof(unsigned long long));
     memset(c->host_flags, 0, 4 * sizeof(int));
+    memset(c->host_scal, 0, 16 * sizeof(double));
     if (ensure_common(c) != FIA_OK) {
-        (void)hipHostFree(c->host_bounds); (void)hipHostFree(c->host_flags);
+        (void)hipHostFree(c->host_bounds); (void)hipHostFree(c->host_flags); (void)hipHostFree(c->host_scal);
         (void)hipStreamDestroy(c->stream); delete c; return FIA_ERR_DEVICE;
     }
     *out = c;
@@ -508,6 +521,7 @@ fia_status fia_ctx_destroy(fia_ctx *c) {
     c->spans.clear();
     if (c->host_bounds) (void)hipHostFree(c->host_bounds);
     if (c->host_flags) (void)hipHostFree(c->host_flags);
+    if (c->host_scal) (void)hipHostFree(c->host_scal);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return FIA_OK;
@@ -841,13 +855,13 @@ fia_status fia_pressure_project(fia_ctx *c, fia_grid3d *g, const uint8_t *solid,
     HIP_TRY(launch_pcg_residual(pg, diag, b, x, r, c->stream));
     HIP_TRY(launch_pcg_precond(r, diag, z, n, c->stream));
     HIP_TRY(hipMemcpyAsync(p, z, n * 8, hipMemcpyDeviceToDevice, c->stream));
-    const double tol2 = rel_tol * rel_tol;
-    HIP_TRY(hipMemcpyAsync(sc + 5, &tol2, 8, hipMemcpyHostToDevice, c->stream));
+    double *hs = c->host_scal;   // pinned
+    hs[8] = rel_tol * rel_tol;
+    HIP_TRY(hipMemcpyAsync(sc + 5, hs + 8, 8, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(launch_dot(r, z, n, part, nparts, c->stream)); HIP_TRY(launch_finish_dot(part, nparts, sc + 2, c->stream));
     HIP_TRY(launch_dot(r, r, n, part, nparts, c->stream)); HIP_TRY(launch_finish_dot(part, nparts, sc + 3, c->stream));
     HIP_TRY(launch_dot(b, b, n, part, nparts, c->stream)); HIP_TRY(launch_finish_dot(part, nparts, sc + 4, c->stream));
     HIP_TRY(launch_pcg_scalars(sc, 0, c->stream));
-    double hs[8] = {0};
     int it = 0;
     const int poll = 16;
     while (true) {

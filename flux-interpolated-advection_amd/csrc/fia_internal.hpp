@@ -105,6 +105,8 @@ size_t scan_temp_words(size_t n);
 hipError_t launch_scan(unsigned long long *tile_tmp, const unsigned *in, unsigned long long *out, size_t n,
                        const FaceParams &p, unsigned long long *bounds, hipStream_t s);
 hipError_t launch_faces_general(const FaceParams &p, hipStream_t s);
+// zero-velocity pass: fluxes of the y faces of undeformed lattice faces (minimal_area >= 1)
+hipError_t launch_zero_velocity_faces(const FaceParams &p, hipStream_t s);
 // out = sum over families of flux differences (scaled); mode 0: out = G1; 1: out = G1 + G2;
 // 2: out = base - G1 (y family only).
 hipError_t launch_gather(const CompGrid &g, double *const f1[3], double *const c1[3], double *const f2[3],

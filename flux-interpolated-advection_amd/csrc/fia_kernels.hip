@@ -153,7 +153,7 @@ FIA_HD bool load_face(const FaceParams &P, int fam, int i, int j, int k, double 
             v[c][1] += (double)P.disp[1][vi];
             v[c][2] += (double)P.disp[2][vi];
         }
-        finite = finite && isfinite(v[c][0]) && isfinite(v[c][1]) && isfinite(v[c][2]);
+        finite = finite && std::isfinite(v[c][0]) && std::isfinite(v[c][1]) && std::isfinite(v[c][2]);
         // colours (reference src/split_face_3D.cpp:26-43)
         const double x = v[c][0], z = v[c][2];
         double cx = 0.0, cz = 0.0;
@@ -406,7 +406,7 @@ template <int FAM, bool SEG> FIA_HD void slot_cut_body(const FaceParams &P, unsi
         geom.rel(mxi * inv3a, meta * inv3a, r);
         c3[0] = geom.p0[0] + r[0]; c3[1] = geom.p0[1] + r[1]; c3[2] = geom.p0[2] + r[2];
         ay = 0.5 * ay2;
-        if (!isfinite(c3[0]) || !isfinite(c3[1]) || !isfinite(c3[2])) ay = 0.0;   // degenerate sliver
+        if (!std::isfinite(c3[0]) || !std::isfinite(c3[1]) || !std::isfinite(c3[2])) ay = 0.0;   // degenerate sliver
     }
     P.piece_c[0][q] = c3[0];
     P.piece_c[1][q] = c3[1];
@@ -418,6 +418,117 @@ __global__ void __launch_bounds__(kBlock) slot_cut_kernel(FaceParams P, unsigned
     const unsigned long long q = s0 + (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= s1) return;
     slot_cut_body<FAM, SEG>(P, q);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Zero-velocity pass (reference src/move_substances_3d.cpp:133-153: backtrace with zero velocity).
+// Every face is its own undeformed lattice face and only y faces carry y area.  A y face spans one
+// cell in x and z, so each colour range holds at most one integer strictly inside it: the cutter's
+// pieces are up to 2 x 2 axis-aligned rectangles, whose xi-eta centroids map to their centres and
+// whose y vector areas are their sizes.  One thread per y face sums them in the cutter's order
+// (x strips outer, z bands inner).  Not used when minimal_area < 1 (pieces are subdivided then).
+// ---------------------------------------------------------------------------------------------
+// colour offset of the split along x (d = 0) or z (d = 1): c = position - offset
+// (reference src/split_face_3D.cpp:26-43); false: no cut along that axis
+FIA_HD bool split_offset(int split, int d, double &off) {
+    switch (split) {
+        case 1: off = 0.5; return true;
+        case 2: off = 0.0; return true;
+        case 3: off = d == 0 ? 0.5 : 0.0; return true;
+        case 4: off = d == 0 ? 0.0 : 0.5; return true;
+        default: off = 0.0; return false;
+    }
+}
+
+// pieces of the unit interval [a, a+1] cut at the integer colour strictly inside (c = x - off)
+FIA_HD int unit_interval_pieces(double a, bool cut, double off, double lo[2], double hi[2]) {
+    if (cut) {
+        const double c0 = a - off;
+        const double k = floor(c0) + 1.0;   // smallest integer above c0; equals c0 + 1 when c0 is one
+        if (k < c0 + 1.0) {
+            const double x = k + off;
+            lo[0] = a; hi[0] = x; lo[1] = x; hi[1] = a + 1.0;
+            return 2;
+        }
+    }
+    lo[0] = a; hi[0] = a + 1.0;
+    return 1;
+}
+
+// WENO6 psi of the 2 x 2 pieces at once (non-integral rule): the four centres lie on psi's sample
+// level j, so each value is a z pass then an x pass over the 6 x 6 samples of that level around it.
+// The two centres along an axis are half a cell apart, so their sample bases differ by 0 or 1 and
+// the four stencils fit in 7 x 7 samples.  Columns are visited in x order: each gets one z pass
+// per z centre (14 evaluations in all) into a 6-column window, and a piece's x pass runs when the
+// window holds its six columns (4 more), the same arithmetic as weno3d.
+FIA_HD double zero_velocity_weno4(const FaceParams &P, int j, const double xl[2], const double xh[2],
+                                  const double zl[2], const double zh[2]) {
+    const double opx = 0.5 - 0.5 * P.g.e[0], opz = 0.5 - 0.5 * P.g.e[2];
+    int bx[2], bz[2];
+    float tx[2], tz[2];
+    for (int a = 0; a < 2; ++a) {
+        split_coord(0.5 * (xl[a] + xh[a]) - opx, bx[a], tx[a]);
+        split_coord(0.5 * (zl[a] + zh[a]) - opz, bz[a], tz[a]);
+    }
+    const int dx = bx[1] - bx[0], dz = bz[1] - bz[0];   // 0 or 1
+    const Loader<double> PS{P.psi, P.g.m[0], P.g.m[1] + 1, P.g.m[2]};
+    const WenoCoef wz0(tz[0], P.kappa_psi), wz1(tz[1], P.kappa_psi);
+    double w0[6] = {0, 0, 0, 0, 0, 0}, w1[6] = {0, 0, 0, 0, 0, 0};   // z passes of the last 6 columns
+    double flux = 0.0;
+#pragma unroll 1
+    for (int c = 0; c < 7; ++c) {
+        double f[7];
+#pragma unroll
+        for (int r = 0; r < 7; ++r) f[r] = PS(bx[0] - 2 + c, j, bz[0] - 2 + r);
+        double g[6];
+#pragma unroll
+        for (int r = 0; r < 6; ++r) g[r] = dz ? f[r + 1] : f[r];
+        const double v0 = weno6<double>(f, wz0), v1 = weno6<double>(g, wz1);
+#pragma unroll
+        for (int r = 0; r < 5; ++r) { w0[r] = w0[r + 1]; w1[r] = w1[r + 1]; }
+        w0[5] = v0; w1[5] = v1;
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            if (c != 5 + (a == 1 ? dx : 0)) continue;
+            const WenoCoef wx(tx[a], P.kappa_psi);
+            flux += ((xh[a] - xl[a]) * (zh[0] - zl[0])) * weno6<double>(w0, wx);
+            flux += ((xh[a] - xl[a]) * (zh[1] - zl[1])) * weno6<double>(w1, wx);
+        }
+    }
+    return flux;
+}
+
+// FAST: WENO6 non-integral rule with both axes cut in two (the host decides from the split and the
+// grid's staggering); otherwise the general quadrature per piece.
+template <bool FAST> FIA_HD void zero_velocity_face_body(const FaceParams &P, long id) {
+    int i, j, k;
+    decode_face(P.g, 1, id, i, j, k);
+    const double x0 = i - 0.5 * P.g.e[0], y = j - 0.5 * P.g.e[1], z0 = k - 0.5 * P.g.e[2];
+    double ox, oz;
+    const bool cx = split_offset(P.split, 0, ox), cz = split_offset(P.split, 1, oz);
+    double xl[2], xh[2], zl[2], zh[2];
+    const int nx = unit_interval_pieces(x0, cx, ox, xl, xh), nz = unit_interval_pieces(z0, cz, oz, zl, zh);
+    if constexpr (FAST) {
+        P.flux[1][id] = (nx == 2 && nz == 2) ? zero_velocity_weno4(P, j, xl, xh, zl, zh) : 0.0;
+        P.corr[1][id] = 0.0;
+        return;
+    } else {
+        double f = 0.0;
+        int npts = 0;
+#pragma unroll 1
+        for (int q = 0; q < nx * nz; ++q) {
+            const int a = q / nz, b = q - a * nz;   // x strips outer, z bands inner
+            const double c3[3] = {0.5 * (xl[a] + xh[a]), y, 0.5 * (zl[b] + zh[b])};
+            f += piece_flux(P, c3, (xh[a] - xl[a]) * (zh[b] - zl[b]), npts);
+        }
+        P.flux[1][id] = f;
+        P.corr[1][id] = 0.0;
+    }
+}
+template <bool FAST> __global__ void __launch_bounds__(kBlock) zero_velocity_face_kernel(FaceParams P) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long)P.g.faces(1)) return;
+    zero_velocity_face_body<FAST>(P, id);
 }
 
 // Work counters: block sum in LDS, then one atomic per counter and block on one of kStatSlots
@@ -874,6 +985,24 @@ hipError_t launch_scan(unsigned long long *tile_tmp, const unsigned *in, unsigne
     hipLaunchKernelGGL(scan_tile_offsets_kernel, dim3(1), dim3(kBlock), 0, s, tile_tmp, (int)ntiles);
     hipLaunchKernelGGL(scan_tiles_kernel, dim3((unsigned)ntiles), dim3(kBlock), 0, s, in, n,
                        (const unsigned long long *)tile_tmp, out, p, bounds);
+    return hipGetLastError();
+}
+
+// Both axes of every undeformed y face are cut in two when the colour at the face's low corner is
+// not an integer: position i - e/2 minus the split offset.
+static bool zero_velocity_cuts_both(const FaceParams &p) {
+    double ox, oz;
+    const bool cx = split_offset(p.split, 0, ox), cz = split_offset(p.split, 1, oz);
+    const double fx = 0.5 * p.g.e[0] + ox, fz = 0.5 * p.g.e[2] + oz;   // fractional part of -colour
+    return cx && cz && fx != floor(fx) && fz != floor(fz);
+}
+
+hipError_t launch_zero_velocity_faces(const FaceParams &p, hipStream_t s) {
+    const dim3 grid(grid_for(p.g.faces(1), kBlock));
+    if (p.interp == IM_WENO6 && !p.integral && zero_velocity_cuts_both(p))
+        hipLaunchKernelGGL(zero_velocity_face_kernel<true>, grid, dim3(kBlock), 0, s, p);
+    else
+        hipLaunchKernelGGL(zero_velocity_face_kernel<false>, grid, dim3(kBlock), 0, s, p);
     return hipGetLastError();
 }
 

@@ -18,6 +18,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cmath>
+
 #define FIA_HD __host__ __device__ __forceinline__
 
 namespace fia {
