@@ -100,6 +100,7 @@ struct fia_ctx {
     DevBuf<double> piece_c[3], piece_ay, piece_val;
     DevBuf<float> work[3];
     DevBuf<unsigned> twisted;
+    DevBuf<unsigned> degen;     // degenerate faces of the segment-cutter mode
     DevBuf<int> counters;   // [0] twisted count, [1] error
     DevBuf<float> grid[4];  // device mirrors of host grids: rho, u, v, w
     DevBuf<float> gen[3];   // scratch for test entry points
@@ -120,7 +121,7 @@ struct fia_ctx {
 namespace {
 
 size_t ctx_bytes(const fia_ctx *c) {
-    size_t b = c->psi.bytes() + c->twisted.bytes() + c->counters.bytes() + c->scal.bytes() + c->bits.bytes() +
+    size_t b = c->psi.bytes() + c->twisted.bytes() + c->degen.bytes() + c->counters.bytes() + c->scal.bytes() + c->bits.bytes() +
                c->solid.bytes() + c->genI.bytes();
     b += c->fsa.bytes() + c->fsb.bytes() + c->face_n.bytes() + c->face_start.bytes() + c->slot_face.bytes() + c->scan_tmp.bytes() + c->bounds.bytes() +
          c->piece_ay.bytes() + c->piece_val.bytes();
@@ -233,7 +234,9 @@ fia_status do_faces(fia_ctx *c, const CompGrid &g, const float *q, bool moving, 
         HIP_TRY(fs.c[f].ensure(g.faces(f)));
     }
     HIP_TRY(c->twisted.ensure(g.faces(0) + g.faces(1) + g.faces(2)));
-    hipLaunchKernelGGL(zero_ints_kernel, dim3(1), dim3(64), 0, c->stream, c->counters.p, 1);
+    HIP_TRY(c->degen.ensure(g.faces(0) + g.faces(1) + g.faces(2)));
+    hipLaunchKernelGGL(zero_ints_kernel, dim3(1), dim3(64), 0, c->stream, c->counters.p, 1);       // twisted count
+    hipLaunchKernelGGL(zero_ints_kernel, dim3(1), dim3(64), 0, c->stream, c->counters.p + 2, 1);   // degenerate count
     HIP_TRY(hipGetLastError());
     FaceParams p;
     memset(&p, 0, sizeof(p));
@@ -244,9 +247,10 @@ fia_status do_faces(fia_ctx *c, const CompGrid &g, const float *q, bool moving, 
         p.corr[a] = fs.c[a].p;
     }
     p.twisted = c->twisted.p; p.twisted_count = c->counters.p; p.error = c->counters.p + 1;
+    p.degen = c->degen.p; p.degen_count = c->counters.p + 2;
     p.split = s.split; p.interp = s.interp; p.integral = s.integral; p.fam_mask = fam_mask;
     p.general_all = s.min_area < 1.0;
-    p.seg_cut = s.split == FIA_SPLIT_NONE || s.split == FIA_SPLIT_XZ_CELLFACE;
+    p.seg_cut = 1;   // boundary segments; degenerate faces take the clipping pipeline
     p.kappa_q = 1.0f;
     p.kappa_psi = (float)(h * h);   // psi is stored in units of q*cells; S in physical units
     p.min_area = s.min_area;
@@ -294,6 +298,7 @@ fia_status do_faces(fia_ctx *c, const CompGrid &g, const float *q, bool moving, 
     p.piece_ay = c->piece_ay.p;
     p.piece_val = c->piece_val.p;
     HIP_TRY(launch_face_cut(p, fam_slots, c->stream));
+    if (p.seg_cut) HIP_TRY(launch_faces_degenerate(p, c->stream));
     // the WENO6 integral quadrature launches are timed on their own (the dominant kernel)
     const bool weno_quad = piece_quadrature_mode(p) != 0;
     int sq = -1;
@@ -507,7 +512,7 @@ fia_status fia_ctx_destroy(fia_ctx *c) {
     if (!c) return FIA_OK;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    c->psi.release(); c->twisted.release(); c->counters.release(); c->scal.release(); c->bits.release();
+    c->psi.release(); c->twisted.release(); c->degen.release(); c->counters.release(); c->scal.release(); c->bits.release();
     c->solid.release(); c->genI.release();
     c->fsa.release(); c->fsb.release(); c->face_n.release(); c->face_start.release(); c->slot_face.release(); c->scan_tmp.release();
     c->piece_ay.release(); c->piece_val.release();

@@ -65,12 +65,14 @@ struct FaceParams {
                              // faces the general cutter handled (kept zero elsewhere)
     unsigned *twisted;       // list of faces for the general cutter: fam << 30 | index (unsigned)
     int *twisted_count;
+    unsigned *degen;         // degenerate faces (segment cutter): cut per face by the clipping pipeline
+    int *degen_count;
     int *error;              // first error code seen (0 = none)
     int split, interp, integral;
     int fam_mask;            // families computed (bit 0 x, 1 y, 2 z)
     int general_all;         // every face goes through the general cutter (minimal_area < 1)
-    int seg_cut;             // pieces from boundary segments (both colours cell-face aligned, or no
-                             // split); otherwise the streaming clipping pipeline
+    int seg_cut;             // pieces from boundary segments (degenerate faces: clipping pipeline);
+                             // 0: the streaming clipping pipeline for every face
     float kappa_q, kappa_psi;
     double min_area;
     unsigned long long *stats;   // optional work counters: kStatSlots x kStatWords
@@ -105,6 +107,8 @@ size_t scan_temp_words(size_t n);
 hipError_t launch_scan(unsigned long long *tile_tmp, const unsigned *in, unsigned long long *out, size_t n,
                        const FaceParams &p, unsigned long long *bounds, hipStream_t s);
 hipError_t launch_faces_general(const FaceParams &p, hipStream_t s);
+// degenerate faces of the segment-cutter mode, cut per face into their slots by the pipeline
+hipError_t launch_faces_degenerate(const FaceParams &p, hipStream_t s);
 // zero-velocity pass: fluxes of the y faces of undeformed lattice faces (minimal_area >= 1)
 hipError_t launch_zero_velocity_faces(const FaceParams &p, hipStream_t s);
 // out = sum over families of flux differences (scaled); mode 0: out = G1; 1: out = G1 + G2;

@@ -317,10 +317,15 @@ FIA_HD void queue_general(const FaceParams &P, int fam, long id) {
     const int slot = atomic_add_i32(P.twisted_count, 1);
     P.twisted[slot] = ((unsigned)fam << 30) | (unsigned)id;
 }
+FIA_HD void queue_degenerate(const FaceParams &P, int fam, long id) {
+    const int slot = atomic_add_i32(P.degen_count, 1);
+    P.degen[slot] = ((unsigned)fam << 30) | (unsigned)id;
+}
 
 // Phase 1: slots each face needs, (x strips) x (z strips).  Faces for the general cutter (twisted
-// ones, degenerate ones where the segment cutter runs, or all when minimal_area < 1) are queued
-// and get no slots; a non-finite face reports an error and gets none (zero flux).
+// ones, or all when minimal_area < 1) are queued and get no slots; a non-finite face reports an
+// error and gets none (zero flux).  With the segment cutter, degenerate faces keep their slots and
+// are queued for the clipping pipeline (degenerate_kernel).
 template <int FAM> FIA_HD void face_count_body(const FaceParams &P, long id) {
     unsigned n = 0;
     if (P.fam_mask & (1 << FAM)) {
@@ -329,12 +334,13 @@ template <int FAM> FIA_HD void face_count_body(const FaceParams &P, long id) {
         double v[4][3], col[4][2];
         if (!load_face(P, FAM, i, j, k, v, col)) {
             report(P.error, KE_NONFINITE);
-        } else if (P.general_all || face_twisted(col) || (P.seg_cut && face_degenerate(col))) {
+        } else if (P.general_all || face_twisted(col)) {
             queue_general(P, FAM, id);
         } else {
             int x0, nxc, z0, nzc;
             colour_ranges(col, x0, nxc, z0, nzc);
             n = (unsigned)((nxc + 1) * (nzc + 1));
+            if (P.seg_cut && face_degenerate(col)) queue_degenerate(P, FAM, id);
         }
     }
     P.face_n[P.fam_base[FAM] + id] = n;
@@ -370,6 +376,7 @@ template <int FAM, bool SEG> FIA_HD void slot_cut_body(const FaceParams &P, unsi
     decode_face(P.g, FAM, id, i, j, k);
     double v[4][3], col[4][2];
     load_face(P, FAM, i, j, k, v, col);
+    if (SEG && face_degenerate(col)) return;   // degenerate_kernel cuts this face
     FaceGeom geom;
     geom.init(v);
     int x0, nxc, z0, nzc;
@@ -591,6 +598,58 @@ template <int FAM> __global__ void __launch_bounds__(kBlock) face_sum_kernel(Fac
     face_sum_body<FAM>(P, id);
 }
 
+// Degenerate faces of the segment-cutter mode (a corner colour exactly on an interior integer):
+// the clipping pipeline cuts every (strip, band) piece of the face into its slot, with the
+// interpolated colours of the recursive cut, so the side an edge on a cut line goes to is decided
+// the same way.  A grid-stride loop over the queued faces (few, near walls or pinned vertices).
+FIA_HD void degenerate_body(const FaceParams &P, int t) {
+    const unsigned code = P.degen[t];
+    const int fam = (int)(code >> 30);
+    const long id = (long)(code & ((1u << 30) - 1u));
+    int i, j, k;
+    decode_face(P.g, fam, id, i, j, k);
+    double v[4][3], col[4][2];
+    load_face(P, fam, i, j, k, v, col);
+    FaceGeom geom;
+    geom.init(v);
+    int x0, nxc, z0, nzc;
+    colour_ranges(col, x0, nxc, z0, nzc);
+    unsigned long long slot = P.face_start[P.fam_base[fam] + id];
+    for (int s = 0; s <= nxc; ++s) {
+        for (int m = 0; m <= nzc; ++m, ++slot) {
+            Pipeline pp;
+            pp.st[0].reset(s > 0, 0, (double)(x0 + s - 1), true);
+            pp.st[1].reset(s < nxc, 0, (double)(x0 + s), false);
+            pp.st[2].reset(m > 0, 1, (double)(z0 + m - 1), true);
+            pp.st[3].reset(m < nzc, 1, (double)(z0 + m), false);
+            pp.acc.reset();
+            const double sq[4][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};
+            for (int c = 0; c < 4; ++c) {
+                CutVert cv;
+                cv.xi = sq[c][0]; cv.eta = sq[c][1]; cv.cx = col[c][0]; cv.cz = col[c][1];
+                pp.push<0>(geom, cv);
+            }
+            pp.close<0>(geom);
+            double ay = 0.0, c3[3] = {0.0, 0.0, 0.0};
+            if (pp.acc.n >= 3 && pp.acc.area2 != 0.0) {
+                const double inv3a = 1.0 / (3.0 * pp.acc.area2);
+                double r[3];
+                geom.rel(pp.acc.mxi * inv3a, pp.acc.meta * inv3a, r);
+                c3[0] = geom.p0[0] + r[0]; c3[1] = geom.p0[1] + r[1]; c3[2] = geom.p0[2] + r[2];
+                ay = 0.5 * pp.acc.ay2;
+            }
+            P.piece_c[0][slot] = c3[0];
+            P.piece_c[1][slot] = c3[1];
+            P.piece_c[2][slot] = c3[2];
+            P.piece_ay[slot] = ay;
+        }
+    }
+}
+__global__ void __launch_bounds__(64) degenerate_kernel(FaceParams P) {
+    const int count = *P.degen_count;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < count; t += gridDim.x * blockDim.x) degenerate_body(P, t);
+}
+
 // Flux of one face with the exact recursive cutter, corners in the given order.
 FIA_HD double general_face_flux(const FaceParams &P, const double v[4][3], const double col[4][2],
                                                     bool &ok) {
@@ -740,10 +799,10 @@ __global__ void __launch_bounds__(64) split_faces_kernel(const double *quads, si
         area[o] = 0.5 * ax2; area[o + 1] = 0.5 * ay2; area[o + 2] = 0.5 * az2;
         ++cnt;
     };
-    // same cutter choice as the flux passes: boundary segments when both colours are cell-face
-    // aligned (or there is no split), the streaming pipeline otherwise
-    const bool seg = split == 0 || split == 1;
-    bool twisted = min_area < 1.0 || (seg && (face_twisted(col) || face_degenerate(col)));
+    // same cutter choice as the flux passes: twisted faces (and every face when minimal_area < 1)
+    // take the recursive cutter, degenerate ones the streaming pipeline, the rest boundary segments
+    bool twisted = min_area < 1.0 || face_twisted(col);
+    const bool seg = !face_degenerate(col);
     if (!twisted) {
         int x0, nxc, z0, nzc;
         colour_ranges(col, x0, nxc, z0, nzc);
@@ -1003,6 +1062,11 @@ hipError_t launch_zero_velocity_faces(const FaceParams &p, hipStream_t s) {
         hipLaunchKernelGGL(zero_velocity_face_kernel<true>, grid, dim3(kBlock), 0, s, p);
     else
         hipLaunchKernelGGL(zero_velocity_face_kernel<false>, grid, dim3(kBlock), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_faces_degenerate(const FaceParams &p, hipStream_t s) {
+    hipLaunchKernelGGL(degenerate_kernel, dim3(1024), dim3(64), 0, s, p);
     return hipGetLastError();
 }
 

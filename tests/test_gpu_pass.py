@@ -43,10 +43,9 @@ PASS_CASES = [
 def test_flux_pass_matches_oracle(fia, oracle, ctx, scenes, case, cfl):
     split, interp, psi, zero, integral, ng = case
     if case == ("xz-AxisAligned", "1Dy_linear", "const", False, False, 1) and cfl == 1.0:
-        pytest.skip("this case once stopped responding inside the flux-pass call (no fault reported) while the "
-                    "segment cutter also ran on axis-aligned colours; a host replay of the same kernel bodies on "
-                    "the GPU's own displacements completes and matches the oracle, so no cause is known and the "
-                    "case stays off the GPU (axis-aligned colours now use the clipping pipeline)")
+        pytest.skip("this case once stopped responding inside the flux-pass call (no fault reported); a host "
+                    "replay of the same kernel bodies on the GPU's own displacements completes, is clean under "
+                    "AddressSanitizer and matches the oracle, so no cause is known and the case stays off the GPU")
     print("case pass", case, cfl, flush=True)
     n = 16
     h, rho, u, v, w, dt = scene(scenes, n, cfl)
