@@ -228,6 +228,7 @@ FIA_HD double piece_flux_weno_integral(const FaceParams &P, const double c[3], d
     const double jf = floor(gy);
     const double f = gy - jf;
     const int J = (int)jf;
+    const bool zin = bz - 2 >= 0 && bz + 3 <= mz - 1;   // the z rows need no clamping
     // psi on level J
     double psi;
     {
@@ -237,8 +238,7 @@ FIA_HD double piece_flux_weno_integral(const FaceParams &P, const double c[3], d
 #pragma unroll
         for (int ix = 0; ix < 6; ++ix) {
             double fz[6];
-#pragma unroll
-            for (int iz = 0; iz < 6; ++iz) fz[iz] = PS(bx - 2 + ix, J, bz - 2 + iz);
+            load_z6<double>(PS, bx - 2 + ix, J, bz - 2, zin, fz);
             xr[ix] = weno6<double>(fz, wz);
         }
         psi = weno6<double>(xr, wx);
@@ -271,8 +271,7 @@ FIA_HD double piece_flux_weno_integral(const FaceParams &P, const double c[3], d
         for (int r = 0; r < 7; ++r) {
             if (r == 6 && nh == 1) { zr[6] = 0.0f; continue; }
             float fz[6];
-#pragma unroll
-            for (int iz = 0; iz < 6; ++iz) fz[iz] = Q(bx - 2 + ix, B - 2 + r, bz - 2 + iz);
+            load_z6<float>(Q, bx - 2 + ix, B - 2 + r, bz - 2, zin, fz);
             zr[r] = weno6<float>(fz, wz);
         }
 #pragma unroll

@@ -98,6 +98,22 @@ template <class V> struct Loader {
     }
 };
 
+// Six consecutive samples of row (x, y) along z from z0.  When the whole row lies inside the field
+// (interior) they are one contiguous run of memory and come in as wide loads; otherwise every
+// sample is clamped on its own.
+template <class V> FIA_HD void load_z6(const Loader<V> &L, int x, int y, int z0, bool interior, V f[6]) {
+    if (interior) {
+        const V *p = L.d + idx3(clampi(x, 0, L.mx - 1), clampi(y, 0, L.my - 1), z0, L.my, L.mz);
+        struct Row { V v[6]; } r;
+        __builtin_memcpy(&r, p, sizeof(r));
+#pragma unroll
+        for (int i = 0; i < 6; ++i) f[i] = r.v[i];
+    } else {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) f[i] = L(x, y, z0 + i);
+    }
+}
+
 // 3D WENO6, passes z then y then x (reference src/WENO6_interpolation_3d.cpp:7-258).  A pass
 // whose sample coordinate is on a grid node (SX/SY/SZ) is skipped: WENO6 at t = 0 returns f_2
 // exactly (every candidate interpolates node 2 and the weights sum to 1).
