@@ -42,10 +42,10 @@ PEAK_HBM_GBS = 8000.0
 # rocprofv3 counter passes (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a 1 GiB device copy);
 # recorded in profiles/density_step_256_cfl4_hbm_counters.md
 TRAFFIC_BYTES_256_CFL4 = 11.565e9
-# Jacobi-PCG algorithmic traffic per cell and iteration (FP64 vectors): apply 24 B (p, diag
-# read, q written), dot(p,q) 16, x/r update 48, preconditioner 24, dot(r,z) 16, dot(r,r) 8,
-# p update 24
-PCG_BYTES_PER_CELL_ITER = 160
+# Jacobi-PCG algorithmic traffic per cell and iteration (FP64 vectors, two kernels): direction
+# kernel reads r, p_old (8 + 8) and the flag byte, writes p, q (8 + 8); update kernel reads x, r,
+# p, q (4 x 8) and the flag byte, writes x, r (2 x 8)
+PCG_BYTES_PER_CELL_ITER = 82
 
 
 def make_scene(scenes, n, cfl, seed=1):

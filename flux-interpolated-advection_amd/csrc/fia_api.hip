@@ -106,7 +106,17 @@ struct fia_ctx {
     DevBuf<float> gen[3];   // scratch for test entry points
     DevBuf<double> genD[2];
     DevBuf<int> genI;
-    DevBuf<double> pcg[8];  // b, diag, x, r, z, p, q, partials
+    DevBuf<double> pcg_b;         // right-hand side of the pressure system
+    DevBuf<uint8_t> pcg_flags;    // per-cell neighbour flags of the pressure system
+    struct PcgBufs {              // one CG solve: x, r, p (two buffers), q, partial sums, scalars
+        DevBuf<double> x, r, p0, p1, q, part, scal;
+        size_t bytes() const {
+            return x.bytes() + r.bytes() + p0.bytes() + p1.bytes() + q.bytes() + part.bytes() + scal.bytes();
+        }
+        void release() {
+            x.release(); r.release(); p0.release(); p1.release(); q.release(); part.release(); scal.release();
+        }
+    } pcgs[2];
     DevBuf<double> scal;
     DevBuf<unsigned int> bits;
     DevBuf<uint8_t> solid;
@@ -129,7 +139,7 @@ size_t ctx_bytes(const fia_ctx *c) {
     for (int a = 0; a < 3; ++a) b += c->disp[a].bytes() + c->work[a].bytes() + c->gen[a].bytes();
     for (int a = 0; a < 4; ++a) b += c->grid[a].bytes();
     for (int a = 0; a < 2; ++a) b += c->genD[a].bytes();
-    for (int a = 0; a < 8; ++a) b += c->pcg[a].bytes();
+    b += c->pcg_b.bytes() + c->pcg_flags.bytes() + c->pcgs[0].bytes() + c->pcgs[1].bytes();
     return b;
 }
 
@@ -446,6 +456,64 @@ int velocity_split(int comp, bool one_dy, bool error_pass) {
 }  // namespace
 
 // =============================================================================================
+
+// The pressure system of a grid: right-hand side and neighbour flags in the context, obstacle
+// faces zeroed, the pinned cell chosen (the last cell, or the last fluid cell with obstacles).
+static fia_status pressure_system(fia_ctx *c, const fia_grid3d *g, const GridView &gv, const uint8_t *solid, PcgGrid &pg) {
+    const size_t n = (size_t)g->nx * g->ny * g->nz;
+    pg.nx = g->nx; pg.ny = g->ny; pg.nz = g->nz; pg.solid = nullptr; pg.pin = (long)n - 1;
+    if (solid) {
+        HIP_TRY(c->solid.ensure(n));
+        HIP_TRY(hipMemcpyAsync(c->solid.p, solid, n, g->on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                               c->stream));
+        pg.solid = c->solid.p;
+        // last fluid cell is pinned (the reference pins the last cell, which has no obstacles)
+        std::vector<uint8_t> hs(n);
+        HIP_TRY(hipMemcpyAsync(hs.data(), c->solid.p, n, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        long pin = -1;
+        for (long i = (long)n - 1; i >= 0; --i) if (!hs[i]) { pin = i; break; }
+        if (pin < 0) return fail(FIA_ERR_INVALID_ARG, "every cell is solid");
+        pg.pin = pin;
+        HIP_TRY(launch_zero_solid_faces(pg, gv.u, gv.v, gv.w, c->stream));
+    }
+    HIP_TRY(c->pcg_b.ensure(n));
+    HIP_TRY(c->pcg_flags.ensure(n));
+    HIP_TRY(launch_pcg_setup(pg, gv.u, gv.v, gv.w, g->dx, c->pcg_b.p, c->pcg_flags.p, c->stream));
+    return FIA_OK;
+}
+
+// Solve k's buffers (allocated for the grid of pg) bound into a CG state over the context's system.
+static fia_status pcg_bind(fia_ctx *c, int k, const PcgGrid &pg, PcgState &ps) {
+    const size_t n = (size_t)pg.nx * pg.ny * pg.nz;
+    auto &B = c->pcgs[k];
+    const int nb = pcg_blocks(n);
+    HIP_TRY(B.x.ensure(n)); HIP_TRY(B.r.ensure(n)); HIP_TRY(B.p0.ensure(n)); HIP_TRY(B.p1.ensure(n));
+    pcg_direction_tiling(pg.nx, pg.ny, pg.nz, ps);
+    const size_t nd = (size_t)ps.dir_blocks;
+    HIP_TRY(B.q.ensure(n)); HIP_TRY(B.part.ensure(5 * (size_t)nb + nd)); HIP_TRY(B.scal.ensure(kPcgScalars));
+    ps.g = pg; ps.flags = c->pcg_flags.p; ps.b = c->pcg_b.p;
+    ps.x = B.x.p; ps.r = B.r.p; ps.p[0] = B.p0.p; ps.p[1] = B.p1.p; ps.q = B.q.p;
+    ps.scal = B.scal.p; ps.part_start = B.part.p; ps.part_update = B.part.p + 3 * (size_t)nb;
+    ps.part_dir = B.part.p + 5 * (size_t)nb; ps.nblocks = nb; ps.it = 0;
+    return FIA_OK;
+}
+
+// Iterates a started solve until it converges or has run max_iters iterations in total; hs
+// (pinned, kPcgScalars doubles) receives its scalars.  The device skips iterations after
+// convergence, so batches of them are enqueued between polls.
+static fia_status pcg_run(fia_ctx *c, PcgState &ps, int max_iters, double *hs) {
+    const int poll = 32;
+    while (true) {
+        HIP_TRY(launch_pcg_check(ps, c->stream));
+        HIP_TRY(hipMemcpyAsync(hs, ps.scal, kPcgScalars * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        if (hs[kPcgDone] != 0.0 || ps.it >= max_iters) break;
+        HIP_TRY(launch_pcg_iterations(ps, std::min(poll, max_iters - ps.it), c->stream));
+    }
+    return FIA_OK;
+}
+
 extern "C" {
 
 void fia_default_opts(fia_advect_opts *o) {
@@ -520,7 +588,7 @@ fia_status fia_ctx_destroy(fia_ctx *c) {
     for (int a = 0; a < 3; ++a) { c->disp[a].release(); c->work[a].release(); c->gen[a].release(); }
     for (int a = 0; a < 4; ++a) c->grid[a].release();
     for (int a = 0; a < 2; ++a) c->genD[a].release();
-    for (int a = 0; a < 8; ++a) c->pcg[a].release();
+    c->pcg_b.release(); c->pcg_flags.release(); c->pcgs[0].release(); c->pcgs[1].release();
     c->bounds.release(); c->stats.release();
     for (auto &sp : c->spans) { (void)hipEventDestroy(sp.a); (void)hipEventDestroy(sp.b); }
     c->spans.clear();
@@ -767,12 +835,12 @@ fia_status fia_split_faces(fia_ctx *c, const float *quads_xyz, size_t n, double 
     if ((st = reset_error(c)) != FIA_OK) return st;
     std::vector<double> q(12 * n);
     for (size_t i = 0; i < 12 * n; ++i) q[i] = (double)quads_xyz[i] / dx;
-    HIP_TRY(c->pcg[0].ensure(12 * n));
+    HIP_TRY(c->pcg_b.ensure(12 * n));
     HIP_TRY(c->genI.ensure(n));
     HIP_TRY(c->genD[0].ensure(3 * n * max_pieces));
     HIP_TRY(c->genD[1].ensure(3 * n * max_pieces));
-    HIP_TRY(hipMemcpyAsync(c->pcg[0].p, q.data(), 12 * n * 8, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(launch_split_faces(c->pcg[0].p, n, (int)split, minimal_area, max_pieces, c->genI.p, c->genD[0].p,
+    HIP_TRY(hipMemcpyAsync(c->pcg_b.p, q.data(), 12 * n * 8, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(launch_split_faces(c->pcg_b.p, n, (int)split, minimal_area, max_pieces, c->genI.p, c->genD[0].p,
                                c->genD[1].p, c->counters.p + 1, c->stream));
     HIP_TRY(hipMemcpyAsync(piece_count, c->genI.p, n * 4, hipMemcpyDeviceToHost, c->stream));
     std::vector<double> cen(3 * n * max_pieces), ar(3 * n * max_pieces);
@@ -825,29 +893,9 @@ fia_status fia_pressure_project(fia_ctx *c, fia_grid3d *g, const uint8_t *solid,
     if ((st = upload_grid(c, g, gv, false)) != FIA_OK) return st;
     const size_t n = (size_t)g->nx * g->ny * g->nz;
     PcgGrid pg;
-    pg.nx = g->nx; pg.ny = g->ny; pg.nz = g->nz; pg.solid = nullptr; pg.pin = (long)n - 1;
-    if (solid) {
-        HIP_TRY(c->solid.ensure(n));
-        HIP_TRY(hipMemcpyAsync(c->solid.p, solid, n, g->on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
-                               c->stream));
-        pg.solid = c->solid.p;
-        // last fluid cell is pinned (the reference pins the last cell, which has no obstacles)
-        std::vector<uint8_t> hs(n);
-        HIP_TRY(hipMemcpyAsync(hs.data(), c->solid.p, n, hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(hipStreamSynchronize(c->stream));
-        long pin = -1;
-        for (long i = (long)n - 1; i >= 0; --i) if (!hs[i]) { pin = i; break; }
-        if (pin < 0) return fail(FIA_ERR_INVALID_ARG, "every cell is solid");
-        pg.pin = pin;
-        HIP_TRY(launch_zero_solid_faces(pg, gv.u, gv.v, gv.w, c->stream));
-    }
-    for (int a = 0; a < 7; ++a) HIP_TRY(c->pcg[a].ensure(n));
-    const int nparts = 1024;
-    HIP_TRY(c->pcg[7].ensure(nparts));
-    HIP_TRY(c->scal.ensure(16));
-    double *b = c->pcg[0].p, *diag = c->pcg[1].p, *x = c->pcg[2].p, *r = c->pcg[3].p, *z = c->pcg[4].p,
-           *p = c->pcg[5].p, *q = c->pcg[6].p, *part = c->pcg[7].p, *sc = c->scal.p;
-    HIP_TRY(launch_pcg_setup(pg, gv.u, gv.v, gv.w, g->dx, b, diag, c->stream));
+    if ((st = pressure_system(c, g, gv, solid, pg)) != FIA_OK) return st;
+    PcgState ps;
+    if ((st = pcg_bind(c, 0, pg, ps)) != FIA_OK) return st;
     float *pd = nullptr;
     if (pressure) {
         HIP_TRY(c->gen[0].ensure(n));
@@ -855,42 +903,15 @@ fia_status fia_pressure_project(fia_ctx *c, fia_grid3d *g, const uint8_t *solid,
         if (g->on_device) pd = pressure;
         else if (warm_start) HIP_TRY(hipMemcpyAsync(pd, pressure, n * 4, hipMemcpyHostToDevice, c->stream));
     }
-    if (pressure && warm_start) HIP_TRY(launch_pressure_in(pg, pd, x, c->stream));
-    else HIP_TRY(launch_fill_double(x, 0.0, n, c->stream));
-    HIP_TRY(launch_pcg_residual(pg, diag, b, x, r, c->stream));
-    HIP_TRY(launch_pcg_precond(r, diag, z, n, c->stream));
-    HIP_TRY(hipMemcpyAsync(p, z, n * 8, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(launch_pcg_guess(ps, (pressure && warm_start) ? pd : nullptr, c->stream));
+    HIP_TRY(launch_pcg_start(ps, rel_tol * rel_tol, c->stream));
     double *hs = c->host_scal;   // pinned
-    hs[8] = rel_tol * rel_tol;
-    HIP_TRY(hipMemcpyAsync(sc + 5, hs + 8, 8, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(launch_dot(r, z, n, part, nparts, c->stream)); HIP_TRY(launch_finish_dot(part, nparts, sc + 2, c->stream));
-    HIP_TRY(launch_dot(r, r, n, part, nparts, c->stream)); HIP_TRY(launch_finish_dot(part, nparts, sc + 3, c->stream));
-    HIP_TRY(launch_dot(b, b, n, part, nparts, c->stream)); HIP_TRY(launch_finish_dot(part, nparts, sc + 4, c->stream));
-    HIP_TRY(launch_pcg_scalars(sc, 0, c->stream));
-    int it = 0;
-    const int poll = 16;
-    while (true) {
-        HIP_TRY(hipMemcpyAsync(hs, sc, 8 * 8, hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(hipStreamSynchronize(c->stream));
-        if (hs[6] != 0.0 || it >= max_iters) break;
-        const int batch = std::min(poll, max_iters - it);
-        for (int k = 0; k < batch; ++k) {
-            HIP_TRY(launch_pcg_apply(pg, diag, p, q, c->stream));
-            HIP_TRY(launch_dot(p, q, n, part, nparts, c->stream)); HIP_TRY(launch_finish_dot(part, nparts, sc + 1, c->stream));
-            HIP_TRY(launch_pcg_update_xr(x, r, p, q, sc, n, c->stream));
-            HIP_TRY(launch_pcg_precond(r, diag, z, n, c->stream));
-            HIP_TRY(launch_dot(r, z, n, part, nparts, c->stream)); HIP_TRY(launch_finish_dot(part, nparts, sc + 2, c->stream));
-            HIP_TRY(launch_dot(r, r, n, part, nparts, c->stream)); HIP_TRY(launch_finish_dot(part, nparts, sc + 3, c->stream));
-            HIP_TRY(launch_pcg_scalars(sc, 1, c->stream));
-            HIP_TRY(launch_pcg_update_p(p, z, sc, n, c->stream));
-            HIP_TRY(launch_pcg_scalars(sc, 2, c->stream));
-        }
-        it += batch;
-    }
-    const int iters_done = (int)hs[7];
-    const double bb = hs[4] > 0 ? hs[4] : 1.0;
-    const double res = sqrt(hs[3] / bb);
-    HIP_TRY(launch_project(pg, x, g->dx, gv.u, gv.v, gv.w, pd, c->stream));
+    if ((st = pcg_run(c, ps, max_iters, hs)) != FIA_OK) return st;
+    const int iters_done = (int)hs[kPcgIters];
+    const double bb = hs[kPcgBB] > 0 ? hs[kPcgBB] : 1.0;
+    const double res = sqrt(hs[kPcgRR] / bb);
+    const bool converged = hs[kPcgDone] != 0.0;
+    HIP_TRY(launch_project(pg, ps.x, g->dx, gv.u, gv.v, gv.w, pd, c->stream));
     if (!g->on_device) {
         const size_t nu = (size_t)(g->nx + 1) * g->ny * g->nz, nv = (size_t)g->nx * (g->ny + 1) * g->nz,
                      nw = (size_t)g->nx * g->ny * (g->nz + 1);
@@ -902,7 +923,7 @@ fia_status fia_pressure_project(fia_ctx *c, fia_grid3d *g, const uint8_t *solid,
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (iters) *iters = iters_done;
     if (rel_residual) *rel_residual = res;
-    if (hs[6] == 0.0) return fail(FIA_ERR_NOT_CONVERGED, "PCG reached max_iters before rel_tol");
+    if (!converged) return fail(FIA_ERR_NOT_CONVERGED, "PCG reached max_iters before rel_tol");
     return FIA_OK;
 }
 

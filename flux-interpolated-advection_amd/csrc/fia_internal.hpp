@@ -122,28 +122,46 @@ hipError_t launch_split_faces(const double *quads_cells, size_t n, int split, do
                               int *count, double *cen, double *area, int *error, hipStream_t s);
 hipError_t launch_absmax(const float *a, size_t n, unsigned int *out_bits, hipStream_t s);
 
-// Jacobi-PCG pieces (FP64)
+// Jacobi-PCG (FP64 vectors, FP64 reductions in a fixed order)
 struct PcgGrid {
     int nx, ny, nz;
     const uint8_t *solid;   // may be null
     long pin;               // pinned cell (p = 0)
 };
+// per-cell flag byte: bits 0-5 = the neighbour at -x, +x, -y, +y, -z, +z is fluid; this bit = the
+// cell is an unknown of the system (fluid, not pinned, with at least one fluid neighbour)
+constexpr unsigned kPcgUnknown = 0x80u;
+constexpr int kPcgMaxBlocks = 1024;
+// device scalars of one solve: rz of even and odd iterations, last residual r.r, b.b, tol^2,
+// converged flag, iterations done
+enum { kPcgRZ = 0, kPcgRR = 3, kPcgBB = 4, kPcgTol2 = 5, kPcgDone = 6, kPcgIters = 7, kPcgScalars = 16 };
+// one CG solve in flight: its vectors and device scalars
+struct PcgState {
+    PcgGrid g;
+    const uint8_t *flags;
+    const double *b;
+    double *x, *r, *p[2], *q;
+    double *scal;         // kPcgScalars doubles
+    double *part_start;   // 3 * nblocks partial sums (start kernel: rz, rr, bb)
+    double *part_dir;     // dir_blocks (direction kernel: p.q)
+    double *part_update;  // 2 * nblocks (update kernel: rz, rr)
+    int nblocks;          // pcg_blocks(n): start and update kernels
+    int dir_blocks, dir_tiles_y, dir_tiles_z, dir_chunk_x, dir_chunks;   // pcg_direction_tiling
+    int it;             // iterations enqueued since the start (selects the p buffer)
+};
+int pcg_blocks(size_t n);
+void pcg_direction_tiling(int nx, int ny, int nz, PcgState &st);
 hipError_t launch_pcg_setup(const PcgGrid &g, const float *u, const float *v, const float *w, double h,
-                            double *b, double *diag, hipStream_t s);
-hipError_t launch_pcg_apply(const PcgGrid &g, const double *diag, const double *x, double *y, hipStream_t s);
-hipError_t launch_dot(const double *a, const double *b, size_t n, double *partials, int nparts, hipStream_t s);
-hipError_t launch_finish_dot(const double *partials, int nparts, double *out, hipStream_t s);
-hipError_t launch_pcg_update_xr(double *x, double *r, const double *p, const double *q, const double *scal,
-                                size_t n, hipStream_t s);
-hipError_t launch_pcg_precond(const double *r, const double *diag, double *z, size_t n, hipStream_t s);
-hipError_t launch_pcg_update_p(double *p, const double *z, const double *scal, size_t n, hipStream_t s);
-hipError_t launch_pcg_residual(const PcgGrid &g, const double *diag, const double *b, const double *x, double *r,
-                               hipStream_t s);
-hipError_t launch_pcg_scalars(double *scal, int stage, hipStream_t s);
+                            double *b, uint8_t *flags, hipStream_t s);
+// x = p_in on the unknowns (0 elsewhere); p_in null: x = 0
+hipError_t launch_pcg_guess(const PcgState &st, const float *p_in, hipStream_t s);
+// r = b - L x, scalars; then `count` iterations (each a no-op once converged); the check updates
+// the polled scalars (residual, iterations, converged flag) for the iterations enqueued so far
+hipError_t launch_pcg_start(PcgState &st, double tol2, hipStream_t s);
+hipError_t launch_pcg_iterations(PcgState &st, int count, hipStream_t s);
+hipError_t launch_pcg_check(const PcgState &st, hipStream_t s);
 hipError_t launch_project(const PcgGrid &g, const double *p, double h, float *u, float *v, float *w,
                           float *pressure_out, hipStream_t s);
 hipError_t launch_zero_solid_faces(const PcgGrid &g, float *u, float *v, float *w, hipStream_t s);
-hipError_t launch_fill_double(double *a, double v, size_t n, hipStream_t s);
-hipError_t launch_pressure_in(const PcgGrid &g, const float *p_in, double *x, hipStream_t s);
 
 }  // namespace fia

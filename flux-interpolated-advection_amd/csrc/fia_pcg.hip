@@ -7,146 +7,388 @@
 // neighbour), b = -div(u) h, the last cell pinned to p = 0.  Obstacle cells (optional mask) are
 // removed from the system and their faces act as walls.
 //
-// Scalars live in device memory (scal[]): the iteration runs without host round trips; once the
-// residual test passes, every update becomes a no-op, so the host only polls now and then.
+// One CG iteration is two kernels:
+//   direction: beta from the previous residual, p = z + beta p_old (z = r / diag, not stored),
+//              q = L p (2.5-D tiled stencil), partial p.q;
+//   update:    alpha = rz / p.q, x += alpha p, r -= alpha q, partial r.z and r.r.
+// p alternates between two buffers (a thread reads its neighbours' old p while others write the
+// new one).  The matrix is not stored: one flag byte per cell holds its fluid neighbours.  Per
+// cell and iteration that is 8 (r) + 8 (p old) + 1 (flags) read and 8 + 8 (p, q) written by the
+// direction kernel, 8 x 4 + 1 read and 16 written by the update kernel: 82 bytes.
+//
+// Reductions without fences: every block writes its partial sums with plain stores, and every
+// block of the NEXT kernel sums all partials itself, in block order, before its own work (the
+// kernel boundary makes the partials visible; the sum is the same in every block and does not
+// depend on scheduling).  The CG scalars are double-buffered by iteration parity, so no block
+// reads a scalar that another block of the same kernel writes.  Once converged every later kernel
+// of the solve returns at once, so the host only polls now and then (after a 1-block check).
 #include "fia_internal.hpp"
 #include "fia_math.hpp"
 
 namespace fia {
 
 constexpr int kPB = 256;
-
-// scal layout
-enum { S_RZ = 0, S_PQ = 1, S_RZNEW = 2, S_RR = 3, S_BB = 4, S_TOL2 = 5, S_DONE = 6, S_ITERS = 7 };
+constexpr int kTileZ = 64, kTileY = 16, kRows = 4;   // column tile of the direction kernel, rows per thread
 
 __device__ __forceinline__ bool fluid(const PcgGrid &g, int i, int j, int k) {
     if (i < 0 || j < 0 || k < 0 || i >= g.nx || j >= g.ny || k >= g.nz) return false;
     return !(g.solid && g.solid[idx3(i, j, k, g.ny, g.nz)]);
 }
 
+// 1 / (number of fluid neighbours), 0 for a cell outside the system
+__device__ __forceinline__ double inv_diag(unsigned f) {
+    if (!(f & kPcgUnknown)) return 0.0;
+    const int d = __builtin_popcount(f & 63u);
+    return d == 6 ? 1.0 / 6.0 : d == 5 ? 0.2 : d == 4 ? 0.25 : d == 3 ? 1.0 / 3.0 : d == 2 ? 0.5 : d == 1 ? 1.0 : 0.0;
+}
+
 __global__ void pcg_setup_kernel(PcgGrid g, const float *u, const float *v, const float *w, double h, double *b,
-                                 double *diag) {
+                                 uint8_t *flags) {
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long n = (long)g.nx * g.ny * g.nz;
     if (c >= n) return;
     const int k = (int)(c % g.nz), j = (int)((c / g.nz) % g.ny), i = (int)(c / ((long)g.ny * g.nz));
-    if (!fluid(g, i, j, k) || c == g.pin) { b[c] = 0.0; diag[c] = 0.0; return; }
-    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    if (!fluid(g, i, j, k)) { b[c] = 0.0; flags[c] = 0; return; }
+    unsigned f = (unsigned)fluid(g, i - 1, j, k) | (unsigned)fluid(g, i + 1, j, k) << 1 |
+                 (unsigned)fluid(g, i, j - 1, k) << 2 | (unsigned)fluid(g, i, j + 1, k) << 3 |
+                 (unsigned)fluid(g, i, j, k - 1) << 4 | (unsigned)fluid(g, i, j, k + 1) << 5;
+    if (c == g.pin || (f & 63u) == 0u) { b[c] = 0.0; flags[c] = (uint8_t)f; return; }
+    const int ny = g.ny, nz = g.nz;
     const double div = ((double)u[idx3(i + 1, j, k, ny, nz)] - (double)u[idx3(i, j, k, ny, nz)])
                      + ((double)v[idx3(i, j + 1, k, ny + 1, nz)] - (double)v[idx3(i, j, k, ny + 1, nz)])
                      + ((double)w[idx3(i, j, k + 1, ny, nz + 1)] - (double)w[idx3(i, j, k, ny, nz + 1)]);
-    (void)nx;
     b[c] = -div * h;
-    diag[c] = (double)(fluid(g, i - 1, j, k) + fluid(g, i + 1, j, k) + fluid(g, i, j - 1, k) +
-                       fluid(g, i, j + 1, k) + fluid(g, i, j, k - 1) + fluid(g, i, j, k + 1));
+    flags[c] = (uint8_t)(f | kPcgUnknown);
 }
 
-// y = L x on free cells (zero elsewhere)
-__global__ void pcg_apply_kernel(PcgGrid g, const double *diag, const double *x, double *y) {
-    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long n = (long)g.nx * g.ny * g.nz;
-    if (c >= n) return;
-    const double d = diag[c];
-    if (d == 0.0) { y[c] = 0.0; return; }
-    const int k = (int)(c % g.nz), j = (int)((c / g.nz) % g.ny), i = (int)(c / ((long)g.ny * g.nz));
-    double s = d * x[c];
-    // neighbours outside the system (walls, obstacles, the pinned cell) hold x = 0
-    if (i > 0) s -= x[c - (long)g.ny * g.nz];
-    if (i + 1 < g.nx) s -= x[c + (long)g.ny * g.nz];
-    if (j > 0) s -= x[c - g.nz];
-    if (j + 1 < g.ny) s -= x[c + g.nz];
-    if (k > 0) s -= x[c - 1];
-    if (k + 1 < g.nz) s -= x[c + 1];
-    y[c] = s;
-}
-
-// r = b - L x
-__global__ void pcg_residual_kernel(PcgGrid g, const double *diag, const double *b, const double *x, double *r) {
-    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long n = (long)g.nx * g.ny * g.nz;
-    if (c >= n) return;
-    const double d = diag[c];
-    if (d == 0.0) { r[c] = 0.0; return; }
-    const int k = (int)(c % g.nz), j = (int)((c / g.nz) % g.ny), i = (int)(c / ((long)g.ny * g.nz));
-    double s = d * x[c];
-    if (i > 0) s -= x[c - (long)g.ny * g.nz];
-    if (i + 1 < g.nx) s -= x[c + (long)g.ny * g.nz];
-    if (j > 0) s -= x[c - g.nz];
-    if (j + 1 < g.ny) s -= x[c + g.nz];
-    if (k > 0) s -= x[c - 1];
-    if (k + 1 < g.nz) s -= x[c + 1];
-    r[c] = b[c] - s;
-}
-
-// fixed-order two-level dot product: partials[block], then one block sums the partials
-__global__ void __launch_bounds__(kPB) dot_kernel(const double *a, const double *b, size_t n, double *partials) {
-    __shared__ double sh[kPB];
-    double s = 0.0;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        s += a[i] * b[i];
-    sh[threadIdx.x] = s;
+// Sums NV values over the block (fixed tree order); thread 0 gets the totals.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV]) {
+    __shared__ double sh[NV][kPB];
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int a = 0; a < NV; ++a) sh[a][t] = v[a];
     __syncthreads();
     for (int off = kPB / 2; off > 0; off >>= 1) {
-        if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+        if (t < off) {
+#pragma unroll
+            for (int a = 0; a < NV; ++a) sh[a][t] += sh[a][t + off];
+        }
         __syncthreads();
     }
-    if (threadIdx.x == 0) partials[blockIdx.x] = sh[0];
+#pragma unroll
+    for (int a = 0; a < NV; ++a) v[a] = sh[a][0];
+    __syncthreads();
 }
 
-__global__ void __launch_bounds__(kPB) finish_dot_kernel(const double *partials, int nparts, double *out) {
-    __shared__ double sh[kPB];
-    double s = 0.0;
-    for (int i = threadIdx.x; i < nparts; i += kPB) s += partials[i];
-    sh[threadIdx.x] = s;
+// Writes this block's sums of v into part[a * nb + block].
+template <int NV>
+__device__ __forceinline__ void write_partials(double (&v)[NV], double *part, int nb) {
+    block_sum<NV>(v);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int a = 0; a < NV; ++a) part[(size_t)a * nb + blockIdx.x] = v[a];
+    }
+}
+
+// Totals of partials written by an earlier kernel, identical in every thread of every block.
+template <int NV>
+__device__ __forceinline__ void total_of(const double *part, int nb, double (&tot)[NV]) {
+    double v[NV];
+#pragma unroll
+    for (int a = 0; a < NV; ++a) {
+        double s = 0.0;
+        for (int i = threadIdx.x; i < nb; i += kPB) s += part[(size_t)a * nb + i];
+        v[a] = s;
+    }
+    __shared__ double sh[NV][kPB];
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int a = 0; a < NV; ++a) sh[a][t] = v[a];
     __syncthreads();
     for (int off = kPB / 2; off > 0; off >>= 1) {
-        if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+        if (t < off) {
+#pragma unroll
+            for (int a = 0; a < NV; ++a) sh[a][t] += sh[a][t + off];
+        }
         __syncthreads();
     }
-    if (threadIdx.x == 0) *out = sh[0];
+#pragma unroll
+    for (int a = 0; a < NV; ++a) tot[a] = sh[a][0];
+    __syncthreads();
 }
 
-__global__ void pcg_update_xr_kernel(double *x, double *r, const double *p, const double *q, const double *scal,
-                                     size_t n) {
-    if (scal[S_DONE] != 0.0) return;
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double alpha = scal[S_RZ] / scal[S_PQ];
-    x[i] += alpha * p[i];
-    r[i] -= alpha * q[i];
+// The converged flag, read by one thread and shared, so that every thread of the block takes the
+// same branch even when block 0 of the same kernel is setting the flag.
+__device__ __forceinline__ bool solve_done(const double *scal) {
+    __shared__ int flag;
+    if (threadIdx.x == 0) flag = scal[kPcgDone] != 0.0;
+    __syncthreads();
+    return flag != 0;
 }
 
-__global__ void pcg_precond_kernel(const double *r, const double *diag, double *z, size_t n) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double d = diag[i];
-    z[i] = d > 0.0 ? r[i] / d : 0.0;
-}
-
-__global__ void pcg_update_p_kernel(double *p, const double *z, const double *scal, size_t n) {
-    if (scal[S_DONE] != 0.0) return;
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double beta = scal[S_RZNEW] / scal[S_RZ];
-    p[i] = z[i] + beta * p[i];
-}
-
-// stage 0: after the initial residual: rz = rznew, done = rr <= tol2 * bb
-// stage 1: after an iteration's dots: iters++, done test, then beta uses rz (old) in update_p,
-//          so rz is advanced by stage 2 after update_p
-__global__ void pcg_scalars_kernel(double *scal, int stage) {
-    if (stage == 0) {
-        scal[S_RZ] = scal[S_RZNEW];
-        scal[S_DONE] = scal[S_RR] <= scal[S_TOL2] * scal[S_BB] ? 1.0 : 0.0;
-        scal[S_ITERS] = 0.0;
-    } else if (stage == 1) {
-        if (scal[S_DONE] != 0.0) return;
-        scal[S_ITERS] += 1.0;
+// The residual state after k iterations: rz, rr (and bb) from the partials of the start kernel
+// (k = 0) or of update kernel k - 1.
+__device__ __forceinline__ void residual_state(const PcgState &st, int k, double &rz, double &rr, double &bb) {
+    if (k == 0) {
+        double tot[3];
+        total_of<3>(st.part_start, st.nblocks, tot);
+        rz = tot[0]; rr = tot[1]; bb = tot[2];
     } else {
-        if (scal[S_DONE] != 0.0) return;
-        scal[S_RZ] = scal[S_RZNEW];
-        if (scal[S_RR] <= scal[S_TOL2] * scal[S_BB]) scal[S_DONE] = 1.0;
+        double tot[2];
+        total_of<2>(st.part_update, st.nblocks, tot);
+        rz = tot[0]; rr = tot[1]; bb = st.scal[kPcgBB];
     }
+}
+
+// Cells of a block.  Workgroups are dispatched to the 8 XCDs round-robin, so block b runs on
+// XCD b % 8; each XCD gets one contiguous eighth of the cells (x-slabs), which its blocks sweep
+// together, so the y and x neighbours a block reads were fetched into the same XCD's L2 by its
+// sibling blocks.  nblocks is a multiple of 8.
+struct CellRange {
+    size_t first, end, step;
+};
+__device__ __forceinline__ CellRange block_cells(size_t n, int nblocks) {
+    const int xcd = blockIdx.x % 8, slot = blockIdx.x / 8, per = nblocks / 8;
+    const size_t chunk = (n + 7) / 8;
+    const size_t lo = min(n, (size_t)xcd * chunk), hi = min(n, lo + chunk);
+    return {lo + (size_t)slot * kPB + threadIdx.x, hi, (size_t)per * kPB};
+}
+
+// The 7-point stencil sum diag * v(c) - sum of v over the fluid neighbours, for a cell with
+// flag byte f.  Every load is issued unconditionally (a missing neighbour reads the cell itself
+// and is then dropped by a select), so the seven loads of a cell are in flight together.
+template <class F>
+__device__ __forceinline__ double stencil(unsigned f, size_t c, long sx, long sy, F val) {
+    const double v0 = val(c);
+    const double v1 = val((f & 1u) ? c - sx : c), v2 = val((f & 2u) ? c + sx : c);
+    const double v3 = val((f & 4u) ? c - sy : c), v4 = val((f & 8u) ? c + sy : c);
+    const double v5 = val((f & 16u) ? c - 1 : c), v6 = val((f & 32u) ? c + 1 : c);
+    double s = (double)__builtin_popcount(f & 63u) * v0;
+    s -= (f & 1u) ? v1 : 0.0;
+    s -= (f & 2u) ? v2 : 0.0;
+    s -= (f & 4u) ? v3 : 0.0;
+    s -= (f & 8u) ? v4 : 0.0;
+    s -= (f & 16u) ? v5 : 0.0;
+    s -= (f & 32u) ? v6 : 0.0;
+    return s;
+}
+
+// r = b - L x, p_old = 0, and partial rz, rr, bb; block 0 resets the tolerance and the flags
+__global__ void __launch_bounds__(kPB) pcg_start_kernel(PcgState st, double tol2) {
+    const size_t n = (size_t)st.g.nx * st.g.ny * st.g.nz;
+    const long sx = (long)st.g.ny * st.g.nz, sy = st.g.nz;
+    const double *x = st.x;
+    double acc[3] = {0.0, 0.0, 0.0};
+    const CellRange cr = block_cells(n, st.nblocks);
+    for (size_t c = cr.first; c < cr.end; c += cr.step) {
+        const unsigned f = st.flags[c];
+        const double bc = st.b[c];
+        const double s = stencil(f, c, sx, sy, [&](size_t i) { return x[i]; });
+        const double rv = (f & kPcgUnknown) ? bc - s : 0.0;
+        st.r[c] = rv;
+        st.p[0][c] = 0.0;
+        acc[0] += rv * (rv * inv_diag(f));
+        acc[1] += rv * rv;
+        acc[2] += bc * bc;
+    }
+    write_partials<3>(acc, st.part_start, st.nblocks);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        double *sc = st.scal;
+        sc[kPcgTol2] = tol2;
+        sc[kPcgDone] = 0.0;
+        sc[kPcgIters] = 0.0;
+    }
+}
+
+// Convergence test after k iterations (one block): the state the host polls.
+__global__ void __launch_bounds__(kPB) pcg_check_kernel(PcgState st, int k) {
+    if (solve_done(st.scal)) return;
+    double rz, rr, bb;
+    residual_state(st, k, rz, rr, bb);
+    if (threadIdx.x == 0) {
+        double *sc = st.scal;
+        sc[kPcgRR] = rr;
+        if (k == 0) sc[kPcgBB] = bb;
+        sc[kPcgIters] = (double)k;
+        if (rr <= sc[kPcgTol2] * bb) sc[kPcgDone] = 1.0;
+    }
+}
+
+// cells per thread and loop step of the update kernel (independent cells, so their loads overlap)
+constexpr int kUpdCells = 4;
+
+// Iteration k: converged test on the residual of iteration k - 1, beta = rz_k / rz_(k-1),
+// p = z + beta p_old, q = L p, partial p.q.  Outside the system r = p_old = 0, so p = 0 there;
+// q is forced to 0 on cells that are not unknowns (the pinned cell has fluid neighbours).
+//
+// 2.5-D stencil: a block owns a tile of kTileZ x kTileY cell columns (each thread kRows of them,
+// kTileY / kRows rows apart) and marches along x over its chunk of planes.  The new direction is
+// computed once per cell: the x neighbours stay in registers, the y and z neighbours are read
+// from the plane in LDS (tile plus a one-cell halo).  The raw values (flags, r, p_old) of the
+// plane after next are loaded one step ahead.
+struct Raw {
+    unsigned f;
+    double r, p;
+};
+
+__device__ __forceinline__ Raw load_raw(const PcgState &st, const double *pold, int x, int y, int z) {
+    Raw a{0u, 0.0, 0.0};
+    if (x >= 0 && x < st.g.nx && y >= 0 && y < st.g.ny && z >= 0 && z < st.g.nz) {
+        const size_t c = idx3(x, y, z, st.g.ny, st.g.nz);
+        a.f = st.flags[c];
+        a.r = st.r[c];
+        a.p = pold[c];
+    }
+    return a;
+}
+
+__device__ __forceinline__ double dir_of(const Raw &a, double beta) { return a.r * inv_diag(a.f) + beta * a.p; }
+
+// the halo cell a thread loads (2 kTileZ y-halo cells, then 2 kTileY z-halo cells), or false
+__device__ __forceinline__ bool halo_cell(int t, int y0, int z0, int &y, int &z, int &ly, int &lz) {
+    if (t < kTileZ) { y = y0 - 1; z = z0 + t; ly = 0; lz = t + 1; return true; }
+    if (t < 2 * kTileZ) { y = y0 + kTileY; z = z0 + t - kTileZ; ly = kTileY + 1; lz = t - kTileZ + 1; return true; }
+    if (t < 2 * kTileZ + kTileY) { y = y0 + t - 2 * kTileZ; z = z0 - 1; ly = y - y0 + 1; lz = 0; return true; }
+    if (t < 2 * kTileZ + 2 * kTileY) { y = y0 + t - 2 * kTileZ - kTileY; z = z0 + kTileZ; ly = y - y0 + 1; lz = kTileZ + 1; return true; }
+    return false;
+}
+
+__global__ void __launch_bounds__(kPB) pcg_direction_kernel(PcgState st, int k, const double *__restrict__ pold,
+                                                            double *__restrict__ pnew) {
+    static_assert(kTileZ * kTileY == kPB * kRows && 2 * (kTileZ + kTileY) <= kPB, "tile shape");
+    if (solve_done(st.scal)) return;
+    double rz, rr, bb;
+    residual_state(st, k, rz, rr, bb);
+    const bool done = rr <= st.scal[kPcgTol2] * bb;
+    const double beta = k == 0 ? 0.0 : rz / st.scal[kPcgRZ + ((k - 1) & 1)];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        double *sc = st.scal;
+        sc[kPcgRZ + (k & 1)] = rz;
+        sc[kPcgRR] = rr;
+        if (k == 0) sc[kPcgBB] = bb;   // later kernels read it
+        sc[kPcgIters] = (double)k;
+        if (done) sc[kPcgDone] = 1.0;
+    }
+    if (done) return;
+
+    __shared__ double plane[2][kTileY + 2][kTileZ + 2];
+    // blocks are dispatched to the 8 XCDs round-robin: consecutive logical tiles go to one XCD,
+    // so an XCD sweeps whole y-z slabs of neighbouring tiles
+    const int per = st.dir_blocks / 8;
+    const int L = (blockIdx.x % 8) * per + blockIdx.x / 8;
+    constexpr int kRowStep = kTileY / kRows;
+    const int ty = threadIdx.x / kTileZ, tz = threadIdx.x % kTileZ;
+    double acc[1] = {0.0};
+    if (L < st.dir_tiles_y * st.dir_tiles_z * st.dir_chunks) {
+        const int y0 = (L % st.dir_tiles_y) * kTileY;
+        const int z0 = ((L / st.dir_tiles_y) % st.dir_tiles_z) * kTileZ;
+        const int xa = (L / (st.dir_tiles_y * st.dir_tiles_z)) * st.dir_chunk_x;
+        const int xb = min(st.g.nx, xa + st.dir_chunk_x);
+        const int z = z0 + tz;
+        int hy = 0, hz = 0, hly = 0, hlz = 0;
+        const bool has_halo = halo_cell(threadIdx.x, y0, z0, hy, hz, hly, hlz);
+        const long sx = (long)st.g.ny * st.g.nz;
+        // prologue: plane xa - 1 (own columns), plane xa (own + halo into LDS), raw plane xa + 1
+        double dm[kRows], dc[kRows];
+        unsigned fc[kRows];
+        Raw on[kRows];
+        bool active[kRows];
+        size_t c[kRows];
+#pragma unroll
+        for (int w = 0; w < kRows; ++w) {
+            const int y = y0 + ty + w * kRowStep;
+            active[w] = y < st.g.ny && z < st.g.nz;
+            dm[w] = dir_of(load_raw(st, pold, xa - 1, y, z), beta);
+            const Raw oc = load_raw(st, pold, xa, y, z);
+            on[w] = load_raw(st, pold, xa + 1, y, z);
+            fc[w] = oc.f;
+            dc[w] = dir_of(oc, beta);
+            plane[0][ty + w * kRowStep + 1][tz + 1] = dc[w];
+            c[w] = idx3(xa, min(y, st.g.ny - 1), min(z, st.g.nz - 1), st.g.ny, st.g.nz);
+        }
+        const Raw hc = has_halo ? load_raw(st, pold, xa, hy, hz) : Raw{0u, 0.0, 0.0};
+        Raw hn = has_halo ? load_raw(st, pold, xa + 1, hy, hz) : Raw{0u, 0.0, 0.0};
+        if (has_halo) plane[0][hly][hlz] = dir_of(hc, beta);
+        int cur = 0;
+        __syncthreads();
+        for (int x = xa; x < xb; ++x) {
+            Raw on2[kRows];
+#pragma unroll
+            for (int w = 0; w < kRows; ++w) on2[w] = load_raw(st, pold, x + 2, y0 + ty + w * kRowStep, z);
+            const Raw hn2 = has_halo ? load_raw(st, pold, x + 2, hy, hz) : Raw{0u, 0.0, 0.0};
+            double dn[kRows];
+#pragma unroll
+            for (int w = 0; w < kRows; ++w) {
+                dn[w] = dir_of(on[w], beta);
+                plane[cur ^ 1][ty + w * kRowStep + 1][tz + 1] = dn[w];
+            }
+            if (has_halo) plane[cur ^ 1][hly][hlz] = dir_of(hn, beta);
+#pragma unroll
+            for (int w = 0; w < kRows; ++w) {
+                if (!active[w]) continue;
+                const int ly = ty + w * kRowStep + 1;
+                const unsigned f = fc[w];
+                double s = (double)__builtin_popcount(f & 63u) * dc[w];
+                s -= (f & 1u) ? dm[w] : 0.0;
+                s -= (f & 2u) ? dn[w] : 0.0;
+                s -= (f & 4u) ? plane[cur][ly - 1][tz + 1] : 0.0;
+                s -= (f & 8u) ? plane[cur][ly + 1][tz + 1] : 0.0;
+                s -= (f & 16u) ? plane[cur][ly][tz] : 0.0;
+                s -= (f & 32u) ? plane[cur][ly][tz + 2] : 0.0;
+                const double qc = (f & kPcgUnknown) ? s : 0.0;
+                pnew[c[w]] = dc[w];
+                st.q[c[w]] = qc;
+                acc[0] += dc[w] * qc;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int w = 0; w < kRows; ++w) {
+                dm[w] = dc[w]; dc[w] = dn[w]; fc[w] = on[w].f; on[w] = on2[w]; c[w] += sx;
+            }
+            hn = hn2;
+            cur ^= 1;
+        }
+    }
+    write_partials<1>(acc, st.part_dir, st.dir_blocks);
+}
+
+// x += alpha p, r -= alpha q (alpha = rz / p.q), then partial rz, rr.  Outside the system p = q = r = 0, so the update leaves those cells as they are.
+__global__ void __launch_bounds__(kPB) pcg_update_kernel(PcgState st, int k, const double *__restrict__ pnew) {
+    if (solve_done(st.scal)) return;
+    double pq[1];
+    total_of<1>(st.part_dir, st.dir_blocks, pq);
+    const double alpha = pq[0] != 0.0 ? st.scal[kPcgRZ + (k & 1)] / pq[0] : 0.0;
+    const size_t n = (size_t)st.g.nx * st.g.ny * st.g.nz;
+    const uint8_t *__restrict__ fl = st.flags;
+    double *__restrict__ x = st.x;
+    double *__restrict__ r = st.r;
+    const double *__restrict__ q = st.q;
+    const CellRange cr = block_cells(n, st.nblocks);
+    const size_t stride = cr.step;
+    double acc[2] = {0.0, 0.0};
+    for (size_t c0 = cr.first; c0 < cr.end; c0 += kUpdCells * stride) {
+        double xv[kUpdCells], rv[kUpdCells], id[kUpdCells];
+#pragma unroll
+        for (int u = 0; u < kUpdCells; ++u) {
+            const size_t cu = c0 + u * stride;
+            const size_t c = cu < cr.end ? cu : c0;
+            xv[u] = x[c] + alpha * pnew[c];
+            rv[u] = r[c] - alpha * q[c];
+            id[u] = inv_diag(fl[c]);
+        }
+#pragma unroll
+        for (int u = 0; u < kUpdCells; ++u) {
+            const size_t c = c0 + u * stride;
+            if (c < cr.end) {
+                x[c] = xv[u];
+                r[c] = rv[u];
+                acc[0] += rv[u] * (rv[u] * id[u]);
+                acc[1] += rv[u] * rv[u];
+            }
+        }
+    }
+    write_partials<2>(acc, st.part_update, st.nblocks);
 }
 
 __global__ void project_kernel(PcgGrid g, const double *p, double h, float *u, float *v, float *w,
@@ -179,61 +421,62 @@ __global__ void zero_solid_faces_kernel(PcgGrid g, float *u, float *v, float *w)
     w[idx3(i, j, k, ny, nz + 1)] = 0.0f; w[idx3(i, j, k + 1, ny, nz + 1)] = 0.0f;
 }
 
-__global__ void fill_double_kernel(double *a, double v, size_t n) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) a[i] = v;
-}
-
-__global__ void pressure_in_kernel(PcgGrid g, const float *p_in, double *x) {
-    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long n = (long)g.nx * g.ny * g.nz;
+// x = initial guess on the unknowns (0 on obstacles and on the pinned cell)
+__global__ void pcg_guess_kernel(PcgState st, const float *p_in) {
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t n = (size_t)st.g.nx * st.g.ny * st.g.nz;
     if (c >= n) return;
-    const int k = (int)(c % g.nz), j = (int)((c / g.nz) % g.ny), i = (int)(c / ((long)g.ny * g.nz));
-    x[c] = (fluid(g, i, j, k) && c != g.pin) ? (double)p_in[c] : 0.0;
+    st.x[c] = (p_in && (st.flags[c] & kPcgUnknown)) ? (double)p_in[c] : 0.0;
 }
 
 static inline unsigned blocks(size_t n) { return (unsigned)((n + kPB - 1) / kPB); }
 
+// a multiple of 8 (one share per XCD)
+int pcg_blocks(size_t n) {
+    const size_t b = std::min<size_t>(kPcgMaxBlocks, std::max<size_t>(1, blocks(n)));
+    return (int)((b + 7) / 8 * 8);
+}
+
+// Tiling of the direction kernel: kTileZ x kTileY column tiles, x cut into chunks so that the
+// launch has about kPcgMaxBlocks blocks (rounded up to a multiple of 8).
+void pcg_direction_tiling(int nx, int ny, int nz, PcgState &st) {
+    st.dir_tiles_y = (ny + kTileY - 1) / kTileY;
+    st.dir_tiles_z = (nz + kTileZ - 1) / kTileZ;
+    const long tiles = (long)st.dir_tiles_y * st.dir_tiles_z;
+    const long want = std::max<long>(1, std::min<long>(nx, ((long)kPcgMaxBlocks + tiles - 1) / tiles));
+    st.dir_chunk_x = (int)((nx + want - 1) / want);
+    st.dir_chunks = (nx + st.dir_chunk_x - 1) / st.dir_chunk_x;
+    const long nb = tiles * st.dir_chunks;
+    st.dir_blocks = (int)((nb + 7) / 8 * 8);
+}
+
 hipError_t launch_pcg_setup(const PcgGrid &g, const float *u, const float *v, const float *w, double h, double *b,
-                            double *diag, hipStream_t s) {
+                            uint8_t *flags, hipStream_t s) {
     const size_t n = (size_t)g.nx * g.ny * g.nz;
-    hipLaunchKernelGGL(pcg_setup_kernel, dim3(blocks(n)), dim3(kPB), 0, s, g, u, v, w, h, b, diag);
+    hipLaunchKernelGGL(pcg_setup_kernel, dim3(blocks(n)), dim3(kPB), 0, s, g, u, v, w, h, b, flags);
     return hipGetLastError();
 }
-hipError_t launch_pcg_apply(const PcgGrid &g, const double *diag, const double *x, double *y, hipStream_t s) {
-    const size_t n = (size_t)g.nx * g.ny * g.nz;
-    hipLaunchKernelGGL(pcg_apply_kernel, dim3(blocks(n)), dim3(kPB), 0, s, g, diag, x, y);
+hipError_t launch_pcg_guess(const PcgState &st, const float *p_in, hipStream_t s) {
+    const size_t n = (size_t)st.g.nx * st.g.ny * st.g.nz;
+    hipLaunchKernelGGL(pcg_guess_kernel, dim3(blocks(n)), dim3(kPB), 0, s, st, p_in);
     return hipGetLastError();
 }
-hipError_t launch_pcg_residual(const PcgGrid &g, const double *diag, const double *b, const double *x, double *r,
-                               hipStream_t s) {
-    const size_t n = (size_t)g.nx * g.ny * g.nz;
-    hipLaunchKernelGGL(pcg_residual_kernel, dim3(blocks(n)), dim3(kPB), 0, s, g, diag, b, x, r);
+hipError_t launch_pcg_start(PcgState &st, double tol2, hipStream_t s) {
+    st.it = 0;
+    hipLaunchKernelGGL(pcg_start_kernel, dim3(st.nblocks), dim3(kPB), 0, s, st, tol2);
     return hipGetLastError();
 }
-hipError_t launch_dot(const double *a, const double *b, size_t n, double *partials, int nparts, hipStream_t s) {
-    hipLaunchKernelGGL(dot_kernel, dim3(nparts), dim3(kPB), 0, s, a, b, n, partials);
+hipError_t launch_pcg_iterations(PcgState &st, int count, hipStream_t s) {
+    for (int k = 0; k < count; ++k) {
+        double *pold = st.p[st.it & 1], *pnew = st.p[(st.it + 1) & 1];
+        hipLaunchKernelGGL(pcg_direction_kernel, dim3(st.dir_blocks), dim3(kPB), 0, s, st, st.it, pold, pnew);
+        hipLaunchKernelGGL(pcg_update_kernel, dim3(st.nblocks), dim3(kPB), 0, s, st, st.it, pnew);
+        ++st.it;
+    }
     return hipGetLastError();
 }
-hipError_t launch_finish_dot(const double *partials, int nparts, double *out, hipStream_t s) {
-    hipLaunchKernelGGL(finish_dot_kernel, dim3(1), dim3(kPB), 0, s, partials, nparts, out);
-    return hipGetLastError();
-}
-hipError_t launch_pcg_update_xr(double *x, double *r, const double *p, const double *q, const double *scal, size_t n,
-                                hipStream_t s) {
-    hipLaunchKernelGGL(pcg_update_xr_kernel, dim3(blocks(n)), dim3(kPB), 0, s, x, r, p, q, scal, n);
-    return hipGetLastError();
-}
-hipError_t launch_pcg_precond(const double *r, const double *diag, double *z, size_t n, hipStream_t s) {
-    hipLaunchKernelGGL(pcg_precond_kernel, dim3(blocks(n)), dim3(kPB), 0, s, r, diag, z, n);
-    return hipGetLastError();
-}
-hipError_t launch_pcg_update_p(double *p, const double *z, const double *scal, size_t n, hipStream_t s) {
-    hipLaunchKernelGGL(pcg_update_p_kernel, dim3(blocks(n)), dim3(kPB), 0, s, p, z, scal, n);
-    return hipGetLastError();
-}
-hipError_t launch_pcg_scalars(double *scal, int stage, hipStream_t s) {
-    hipLaunchKernelGGL(pcg_scalars_kernel, dim3(1), dim3(1), 0, s, scal, stage);
+hipError_t launch_pcg_check(const PcgState &st, hipStream_t s) {
+    hipLaunchKernelGGL(pcg_check_kernel, dim3(1), dim3(kPB), 0, s, st, st.it);
     return hipGetLastError();
 }
 hipError_t launch_project(const PcgGrid &g, const double *p, double h, float *u, float *v, float *w,
@@ -245,16 +488,6 @@ hipError_t launch_project(const PcgGrid &g, const double *p, double h, float *u,
 hipError_t launch_zero_solid_faces(const PcgGrid &g, float *u, float *v, float *w, hipStream_t s) {
     const size_t n = (size_t)g.nx * g.ny * g.nz;
     hipLaunchKernelGGL(zero_solid_faces_kernel, dim3(blocks(n)), dim3(kPB), 0, s, g, u, v, w);
-    return hipGetLastError();
-}
-hipError_t launch_fill_double(double *a, double v, size_t n, hipStream_t s) {
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(fill_double_kernel, dim3(blocks(n)), dim3(kPB), 0, s, a, v, n);
-    return hipGetLastError();
-}
-hipError_t launch_pressure_in(const PcgGrid &g, const float *p_in, double *x, hipStream_t s) {
-    const size_t n = (size_t)g.nx * g.ny * g.nz;
-    hipLaunchKernelGGL(pressure_in_kernel, dim3(blocks(n)), dim3(kPB), 0, s, g, p_in, x);
     return hipGetLastError();
 }
 
