@@ -35,7 +35,10 @@ PSI = {"const": 0, "linear": 1, "WENO6": 2}
 EXPORTS = ["fia_default_opts", "fia_ctx_create", "fia_ctx_destroy", "fia_ctx_device_bytes", "fia_last_error",
            "fia_device_count", "fia_move_substances_3d", "fia_advect_flux_pass", "fia_advect_velocity_3d",
            "fia_calc_psi_y", "fia_interpolate", "fia_backtrace_vertices", "fia_split_faces", "fia_cfl_time_step",
-           "fia_pressure_project", "fia_ctx_set_timing", "fia_ctx_get_stats", "fia_ctx_reset_stats"]
+           "fia_pressure_project", "fia_ctx_set_timing", "fia_ctx_get_stats", "fia_ctx_reset_stats",
+           "fia_unet_param_count", "fia_unet_create", "fia_unet_destroy", "fia_unet_set_params",
+           "fia_unet_get_params", "fia_unet_forward", "fia_unet_loss_grad", "fia_unet_train_step", "fia_pressure_rhs",
+           "fia_pressure_project_learned"]
 
 
 class FiaError(RuntimeError):
@@ -58,6 +61,14 @@ class GridC(ctypes.Structure):
     _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int), ("dx", ctypes.c_double),
                 ("density", ctypes.c_void_p), ("u", ctypes.c_void_p), ("v", ctypes.c_void_p),
                 ("w", ctypes.c_void_p), ("on_device", ctypes.c_int)]
+
+
+class LearnedInfo(ctypes.Structure):
+    _fields_ = [("used_network", ctypes.c_int), ("probe_iters", ctypes.c_int), ("guess_rel_residual", ctypes.c_double),
+                ("probe_res_network", ctypes.c_double), ("probe_res_zero", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 class KernelStats(ctypes.Structure):
@@ -106,6 +117,17 @@ def lib():
         L.fia_ctx_set_timing.argtypes = [_vp, _i]
         L.fia_ctx_get_stats.argtypes = [_vp, ctypes.POINTER(KernelStats)]
         L.fia_ctx_reset_stats.argtypes = [_vp]
+        L.fia_unet_param_count.restype = ctypes.c_size_t
+        L.fia_unet_create.argtypes = [_vp, ctypes.c_ulonglong, ctypes.POINTER(_vp)]
+        L.fia_unet_destroy.argtypes = [_vp]
+        L.fia_unet_set_params.argtypes = [_vp, _vp, _vp, ctypes.c_size_t]
+        L.fia_unet_get_params.argtypes = [_vp, _vp, _vp, ctypes.c_size_t]
+        L.fia_unet_forward.argtypes = [_vp, _vp, _i, _i, _i, _vp, _vp, _vp, ctypes.POINTER(_d), _i]
+        L.fia_unet_loss_grad.argtypes = [_vp, _vp, _i, _i, _i, _vp, _vp, ctypes.POINTER(_d), _vp, _i]
+        L.fia_unet_train_step.argtypes = [_vp, _vp, _i, _i, _i, _i, _vp, _vp, ctypes.c_float, ctypes.POINTER(_d)]
+        L.fia_pressure_rhs.argtypes = [_vp, ctypes.POINTER(GridC), _vp, _vp]
+        L.fia_pressure_project_learned.argtypes = [_vp, ctypes.POINTER(GridC), _vp, _vp, _vp, _d, _i, ctypes.POINTER(_i),
+                                                   ctypes.POINTER(_d), ctypes.POINTER(LearnedInfo)]
         _lib = L
     return _lib
 
@@ -115,7 +137,19 @@ def _check(status):
         raise FiaError(status, lib().fia_last_error().decode())
 
 
+def _torch_first():
+    """torch ships its own HIP runtime; when both are loaded, torch's must start first (after
+    libfia.so has opened the device, torch reports no GPU).  No-op without torch."""
+    try:
+        import torch
+    except ImportError:
+        return
+    if torch.version.hip is not None:
+        torch.cuda.is_available()
+
+
 def device_count():
+    _torch_first()
     return lib().fia_device_count()
 
 
@@ -123,6 +157,7 @@ class Context:
     """Owns the device scratch and the HIP stream (fia_ctx)."""
 
     def __init__(self, device=0):
+        _torch_first()
         self._h = _vp()
         _check(lib().fia_ctx_create(device, ctypes.byref(self._h)))
 
@@ -349,3 +384,119 @@ def calc_pressure_gradient_term_3d(ctx, grid, solid=None, rel_tol=1e-6, max_iter
                                     int(max_iters), ctypes.byref(it), ctypes.byref(res))
     _check(st)
     return it.value, res.value
+
+
+# ---- learned warm start of the pressure solve (no counterpart in the reference) --------------
+
+def _solid_ptr(solid, on_device):
+    if solid is None:
+        return None, None
+    if on_device:
+        return solid.data_ptr(), solid
+    keep = np.ascontiguousarray(solid, dtype=np.uint8)
+    return keep.ctypes.data, keep
+
+
+class UNet:
+    """The learned pressure warm start (fia_unet): a 4-level 3-D U-Net, 16/32/48/64 channels,
+    BF16 weights on the matrix cores.  Parameters are one FP32 vector of param_count() values."""
+
+    def __init__(self, ctx, seed=0):
+        self.ctx = ctx
+        self._h = _vp()
+        _check(lib().fia_unet_create(ctx._h, int(seed), ctypes.byref(self._h)))
+
+    @staticmethod
+    def param_count():
+        return int(lib().fia_unet_param_count())
+
+    def close(self):
+        if self._h:
+            lib().fia_unet_destroy(self._h)
+            self._h = _vp()
+
+    def get_params(self):
+        p = np.empty(self.param_count(), dtype=np.float32)
+        _check(lib().fia_unet_get_params(self.ctx._h, self._h, p.ctypes.data, p.size))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _check(lib().fia_unet_set_params(self.ctx._h, self._h, p.ctypes.data, p.size))
+
+    def save(self, path):
+        np.save(path, self.get_params(), allow_pickle=False)
+
+    def load(self, path):
+        self.set_params(np.load(path, allow_pickle=False))
+
+    def forward(self, rhs, solid=None):
+        """Network output y (FP32) and its scale for right-hand side rhs [nx,ny,nz] (FP64):
+        the pressure guess is scale * y (up to a constant)."""
+        nx, ny, nz = rhs.shape
+        pr, on = _ptr(rhs)
+        sp, keep = _solid_ptr(solid, on)
+        if on:
+            import torch
+            y = torch.empty((nx, ny, nz), dtype=torch.float32, device=rhs.device)
+        else:
+            y = np.empty((nx, ny, nz), dtype=np.float32)
+        sc = _d(0)
+        _check(lib().fia_unet_forward(self.ctx._h, self._h, nx, ny, nz, pr, sp, _ptr(y)[0], ctypes.byref(sc), on))
+        return y, sc.value
+
+    def loss_grad(self, rhs, solid=None, with_grad=True):
+        """Residual-norm loss of one sample and (optionally) its FP32 parameter gradient."""
+        nx, ny, nz = rhs.shape
+        pr, on = _ptr(rhs)
+        sp, keep = _solid_ptr(solid, on)
+        g = np.empty(self.param_count(), dtype=np.float32) if with_grad else None
+        loss = _d(0)
+        _check(lib().fia_unet_loss_grad(self.ctx._h, self._h, nx, ny, nz, pr, sp, ctypes.byref(loss),
+                                        g.ctypes.data if g is not None else None, on))
+        return loss.value, g
+
+    def train_step(self, rhs_batch, solid_batch=None, lr=1e-3):
+        """One Adam step on a batch [b, nx, ny, nz] of device tensors; returns the mean loss."""
+        b, nx, ny, nz = rhs_batch.shape
+        pr, on = _ptr(rhs_batch)
+        if not on:
+            raise ValueError("training batches must be device tensors")
+        sp = solid_batch.data_ptr() if solid_batch is not None else None
+        loss = _d(0)
+        _check(lib().fia_unet_train_step(self.ctx._h, self._h, nx, ny, nz, b, pr, sp, float(lr), ctypes.byref(loss)))
+        return loss.value
+
+
+def pressure_rhs(ctx, grid, solid=None):
+    """Right-hand side b = -div(u) dx of the pressure system (as the projection assembles it)."""
+    g = grid._c(with_density=False)
+    sp, keep = _solid_ptr(solid, g.on_device)
+    if g.on_device:
+        import torch
+        rhs = torch.empty((grid.nx, grid.ny, grid.nz), dtype=torch.float64, device=grid.u.device)
+    else:
+        rhs = np.empty((grid.nx, grid.ny, grid.nz), dtype=np.float64)
+    _check(lib().fia_pressure_rhs(ctx._h, ctypes.byref(g), sp, _ptr(rhs)[0]))
+    return rhs
+
+
+def calc_pressure_gradient_term_3d_learned(ctx, grid, net, solid=None, rel_tol=1e-6, max_iters=20000):
+    """calc_pressure_gradient_term_3d with the U-Net warm start (and its fallback to a zero
+    guess).  Returns (iterations, relative residual, info dict)."""
+    g = grid._c(with_density=False)
+    n = grid.nx * grid.ny * grid.nz
+    sp, keep = _solid_ptr(solid, g.on_device)
+    if g.on_device:
+        import torch
+        if grid.pressure is None or not torch.is_tensor(grid.pressure):
+            grid.pressure = torch.zeros((grid.nx, grid.ny, grid.nz), dtype=torch.float32, device=grid.u.device)
+        pp = grid.pressure.data_ptr()
+    else:
+        if grid.pressure is None or not isinstance(grid.pressure, np.ndarray):
+            grid.pressure = np.zeros((grid.nx, grid.ny, grid.nz), dtype=np.float32)
+        pp = grid.pressure.ctypes.data
+    it = _i(0); res = _d(0); info = LearnedInfo()
+    _check(lib().fia_pressure_project_learned(ctx._h, ctypes.byref(g), sp, pp, net._h, float(rel_tol), int(max_iters),
+                                              ctypes.byref(it), ctypes.byref(res), ctypes.byref(info)))
+    return it.value, res.value, info.as_dict()

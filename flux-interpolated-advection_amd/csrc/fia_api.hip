@@ -9,6 +9,7 @@
 
 #include "../../include/fia.h"
 #include "fia_internal.hpp"
+#include "fia_unet.hpp"
 
 using namespace fia;
 
@@ -28,19 +29,6 @@ fia_status fail(fia_status s, const std::string &msg) {
             return fail(FIA_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e));    \
     } while (0)
 
-template <class T> struct DevBuf {
-    T *p = nullptr;
-    size_t n = 0;
-    hipError_t ensure(size_t want) {
-        if (want <= n && p) return hipSuccess;
-        if (p) { hipError_t e = hipFree(p); if (e != hipSuccess) return e; p = nullptr; n = 0; }
-        hipError_t e = hipMalloc((void **)&p, std::max<size_t>(want, 1) * sizeof(T));
-        if (e == hipSuccess) n = want;
-        return e;
-    }
-    void release() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
-    size_t bytes() const { return n * sizeof(T); }
-};
 
 const int kGaussSizes[] = {1, 3, 5, 7, 9, 11, 13, 25, 49};
 
@@ -96,7 +84,7 @@ struct fia_ctx {
     DevBuf<unsigned long long> bounds;     // slot bounds of the families (device side)
     unsigned long long *host_bounds = nullptr;   // pinned: [0] 0, [1] [2] family starts, [3] total
     int *host_flags = nullptr;                   // pinned: twisted count, kernel error
-    double *host_scal = nullptr;                 // pinned: PCG scalars read back while iterating
+    double *host_scal = nullptr;                 // pinned: PCG scalars of two solves read back while iterating
     DevBuf<double> piece_c[3], piece_ay, piece_val;
     DevBuf<float> work[3];
     DevBuf<unsigned> twisted;
@@ -126,6 +114,14 @@ struct fia_ctx {
     struct Span { hipEvent_t a, b; int kind; };
     std::vector<Span> spans;            // recorded, not yet read
     fia_kernel_stats acc{};
+};
+
+struct fia_unet {
+    int device = 0;
+    fia::UNet net;
+    DevBuf<float> y, dy;
+    DevBuf<uint8_t> flags, solid;
+    DevBuf<double> rhs;
 };
 
 namespace {
@@ -558,7 +554,7 @@ fia_status fia_ctx_create(int device, fia_ctx **out) {
     }
     if (hipHostMalloc((void **)&c->host_bounds, 4 * sizeof(unsigned long long)) != hipSuccess ||
         hipHostMalloc((void **)&c->host_flags, 4 * sizeof(int)) != hipSuccess ||
-        hipHostMalloc((void **)&c->host_scal, 16 * sizeof(double)) != hipSuccess) {
+        hipHostMalloc((void **)&c->host_scal, 2 * kPcgScalars * sizeof(double)) != hipSuccess) {
         if (c->host_bounds) (void)hipHostFree(c->host_bounds);
         if (c->host_flags) (void)hipHostFree(c->host_flags);
         (void)hipStreamDestroy(c->stream);
@@ -567,7 +563,7 @@ fia_status fia_ctx_create(int device, fia_ctx **out) {
     }
     memset(c->host_bounds, 0, 4 * sizeof(unsigned long long));
     memset(c->host_flags, 0, 4 * sizeof(int));
-    memset(c->host_scal, 0, 16 * sizeof(double));
+    memset(c->host_scal, 0, 2 * kPcgScalars * sizeof(double));
     if (ensure_common(c) != FIA_OK) {
         (void)hipHostFree(c->host_bounds); (void)hipHostFree(c->host_flags); (void)hipHostFree(c->host_scal);
         (void)hipStreamDestroy(c->stream); delete c; return FIA_ERR_DEVICE;
@@ -923,6 +919,250 @@ fia_status fia_pressure_project(fia_ctx *c, fia_grid3d *g, const uint8_t *solid,
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (iters) *iters = iters_done;
     if (rel_residual) *rel_residual = res;
+    if (!converged) return fail(FIA_ERR_NOT_CONVERGED, "PCG reached max_iters before rel_tol");
+    return FIA_OK;
+}
+
+
+// ---- learned warm start -----------------------------------------------------------------------
+
+size_t fia_unet_param_count(void) { return unet_param_count(); }
+
+fia_status fia_unet_create(fia_ctx *c, unsigned long long seed, fia_unet **out) {
+    if (!c || !out) return fail(FIA_ERR_INVALID_ARG, "ctx or out is NULL");
+    *out = nullptr;
+    HIP_TRY(set_device(c));
+    fia_unet *u = new fia_unet;
+    u->device = c->device;
+    hipError_t e = u->net.init(seed, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        delete u;
+        return fail(FIA_ERR_DEVICE, std::string("U-Net initialisation: ") + hipGetErrorString(e));
+    }
+    *out = u;
+    return FIA_OK;
+}
+
+fia_status fia_unet_destroy(fia_unet *u) {
+    if (!u) return FIA_OK;
+    (void)hipSetDevice(u->device);
+    u->y.release(); u->dy.release(); u->flags.release(); u->solid.release(); u->rhs.release();
+    delete u;
+    return FIA_OK;
+}
+
+fia_status fia_unet_set_params(fia_ctx *c, fia_unet *u, const float *params, size_t n) {
+    if (!c || !u || !params) return fail(FIA_ERR_INVALID_ARG, "NULL argument");
+    if (n != unet_param_count()) return fail(FIA_ERR_INVALID_ARG, "parameter count does not match the network");
+    HIP_TRY(set_device(c));
+    HIP_TRY(u->net.set_params(params, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return FIA_OK;
+}
+
+fia_status fia_unet_get_params(fia_ctx *c, fia_unet *u, float *params, size_t n) {
+    if (!c || !u || !params) return fail(FIA_ERR_INVALID_ARG, "NULL argument");
+    if (n != unet_param_count()) return fail(FIA_ERR_INVALID_ARG, "parameter count does not match the network");
+    HIP_TRY(set_device(c));
+    HIP_TRY(u->net.get_params(params, c->stream));
+    return FIA_OK;
+}
+
+namespace {
+
+fia_status unet_grid(fia_unet *u, int nx, int ny, int nz) {
+    if (nx <= 0 || ny <= 0 || nz <= 0 || nx % 8 || ny % 8 || nz % 8)
+        return fail(FIA_ERR_INVALID_ARG, "network grid sizes must be positive multiples of 8");
+    const size_t n = (size_t)nx * ny * nz;
+    HIP_TRY(u->net.ensure_grid(nx, ny, nz));
+    HIP_TRY(u->y.ensure(n));
+    HIP_TRY(u->dy.ensure(n));
+    HIP_TRY(u->flags.ensure(n));
+    return FIA_OK;
+}
+
+// device copies of host inputs (or the caller's device pointers)
+fia_status unet_inputs(fia_ctx *c, fia_unet *u, size_t n, const double *rhs, const uint8_t *solid, int on_device,
+                       const double *&rd, const uint8_t *&sd) {
+    if (on_device) { rd = rhs; sd = solid; return FIA_OK; }
+    HIP_TRY(u->rhs.ensure(n));
+    HIP_TRY(hipMemcpyAsync(u->rhs.p, rhs, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    rd = u->rhs.p;
+    sd = nullptr;
+    if (solid) {
+        HIP_TRY(u->solid.ensure(n));
+        HIP_TRY(hipMemcpyAsync(u->solid.p, solid, n, hipMemcpyHostToDevice, c->stream));
+        sd = u->solid.p;
+    }
+    return FIA_OK;
+}
+
+}  // namespace
+
+fia_status fia_unet_forward(fia_ctx *c, fia_unet *u, int nx, int ny, int nz, const double *rhs, const uint8_t *solid,
+                            float *y, double *scale, int on_device) {
+    if (!c || !u || !rhs || !y) return fail(FIA_ERR_INVALID_ARG, "NULL argument");
+    HIP_TRY(set_device(c));
+    fia_status st;
+    if ((st = unet_grid(u, nx, ny, nz)) != FIA_OK) return st;
+    const size_t n = (size_t)nx * ny * nz;
+    const double *rd; const uint8_t *sd;
+    if ((st = unet_inputs(c, u, n, rhs, solid, on_device, rd, sd)) != FIA_OK) return st;
+    double sc = 0.0;
+    HIP_TRY(u->net.forward(rd, sd, on_device ? y : u->y.p, &sc, c->stream));
+    if (!on_device) HIP_TRY(hipMemcpyAsync(y, u->y.p, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (scale) *scale = sc;
+    return FIA_OK;
+}
+
+fia_status fia_unet_loss_grad(fia_ctx *c, fia_unet *u, int nx, int ny, int nz, const double *rhs, const uint8_t *solid,
+                              double *loss, float *grad, int on_device) {
+    if (!c || !u || !rhs) return fail(FIA_ERR_INVALID_ARG, "NULL argument");
+    HIP_TRY(set_device(c));
+    fia_status st;
+    if ((st = unet_grid(u, nx, ny, nz)) != FIA_OK) return st;
+    const size_t n = (size_t)nx * ny * nz;
+    const double *rd; const uint8_t *sd;
+    if ((st = unet_inputs(c, u, n, rhs, solid, on_device, rd, sd)) != FIA_OK) return st;
+    HIP_TRY(launch_fluid_flags(sd, nx, ny, nz, u->flags.p, c->stream));
+    HIP_TRY(u->net.forward(rd, sd, u->y.p, nullptr, c->stream));
+    double l = 0.0;
+    HIP_TRY(u->net.loss_residual(u->flags.p, rd, u->y.p, grad ? u->dy.p : nullptr, &l, c->stream));
+    if (grad) {
+        HIP_TRY(u->net.zero_grad(c->stream));
+        HIP_TRY(u->net.backward(u->dy.p, c->stream));
+        HIP_TRY(hipMemcpyAsync(grad, u->net.grad(), unet_param_count() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (loss) *loss = l;
+    return FIA_OK;
+}
+
+fia_status fia_unet_train_step(fia_ctx *c, fia_unet *u, int nx, int ny, int nz, int batch, const double *rhs,
+                               const uint8_t *solid, float lr, double *mean_loss) {
+    if (!c || !u || !rhs || batch <= 0 || !(lr > 0)) return fail(FIA_ERR_INVALID_ARG, "bad argument");
+    HIP_TRY(set_device(c));
+    fia_status st;
+    if ((st = unet_grid(u, nx, ny, nz)) != FIA_OK) return st;
+    const size_t n = (size_t)nx * ny * nz;
+    HIP_TRY(u->net.zero_grad(c->stream));
+    double sum = 0.0;
+    for (int b = 0; b < batch; ++b) {
+        const double *rd = rhs + (size_t)b * n;
+        const uint8_t *sd = solid ? solid + (size_t)b * n : nullptr;
+        HIP_TRY(launch_fluid_flags(sd, nx, ny, nz, u->flags.p, c->stream));
+        HIP_TRY(u->net.forward(rd, sd, u->y.p, nullptr, c->stream));
+        double l = 0.0;
+        HIP_TRY(u->net.loss_residual(u->flags.p, rd, u->y.p, u->dy.p, &l, c->stream));
+        sum += l;
+        HIP_TRY(u->net.backward(u->dy.p, c->stream));
+    }
+    HIP_TRY(u->net.adam_step(lr, 0.9f, 0.999f, 1e-8f, 1.0f / (float)batch, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (mean_loss) *mean_loss = sum / batch;
+    return FIA_OK;
+}
+
+fia_status fia_pressure_rhs(fia_ctx *c, fia_grid3d *g, const uint8_t *solid, double *rhs) {
+    if (!c || !rhs) return fail(FIA_ERR_INVALID_ARG, "NULL argument");
+    fia_status st;
+    if ((st = check_grid(g, false)) != FIA_OK) return st;
+    HIP_TRY(set_device(c));
+    GridView gv;
+    if ((st = upload_grid(c, g, gv, false)) != FIA_OK) return st;
+    // the system is assembled from a copy: obstacle faces are zeroed in the context's mirror only
+    // when the grid lives on the host, so work on a device copy of the velocity either way
+    const size_t nu = (size_t)(g->nx + 1) * g->ny * g->nz, nv = (size_t)g->nx * (g->ny + 1) * g->nz,
+                 nw = (size_t)g->nx * g->ny * (g->nz + 1);
+    if (g->on_device) {
+        HIP_TRY(c->grid[1].ensure(nu)); HIP_TRY(c->grid[2].ensure(nv)); HIP_TRY(c->grid[3].ensure(nw));
+        HIP_TRY(hipMemcpyAsync(c->grid[1].p, gv.u, nu * 4, hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(hipMemcpyAsync(c->grid[2].p, gv.v, nv * 4, hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(hipMemcpyAsync(c->grid[3].p, gv.w, nw * 4, hipMemcpyDeviceToDevice, c->stream));
+        gv.u = c->grid[1].p; gv.v = c->grid[2].p; gv.w = c->grid[3].p;
+    }
+    PcgGrid pg;
+    if ((st = pressure_system(c, g, gv, solid, pg)) != FIA_OK) return st;
+    const size_t n = (size_t)g->nx * g->ny * g->nz;
+    HIP_TRY(hipMemcpyAsync(rhs, c->pcg_b.p, n * sizeof(double),
+                           g->on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return FIA_OK;
+}
+
+fia_status fia_pressure_project_learned(fia_ctx *c, fia_grid3d *g, const uint8_t *solid, float *pressure, fia_unet *u,
+                                        double rel_tol, int max_iters, int *iters, double *rel_residual,
+                                        fia_learned_info *info) {
+    if (!c || !u) return fail(FIA_ERR_INVALID_ARG, "ctx or net is NULL");
+    fia_status st;
+    if ((st = check_grid(g, false)) != FIA_OK) return st;
+    if (!(rel_tol > 0) || max_iters < 0) return fail(FIA_ERR_INVALID_ARG, "bad tolerance or iteration limit");
+    HIP_TRY(set_device(c));
+    if ((st = unet_grid(u, g->nx, g->ny, g->nz)) != FIA_OK) return st;
+    GridView gv;
+    if ((st = upload_grid(c, g, gv, false)) != FIA_OK) return st;
+    const size_t n = (size_t)g->nx * g->ny * g->nz;
+    PcgGrid pg;
+    if ((st = pressure_system(c, g, gv, solid, pg)) != FIA_OK) return st;
+    PcgState sa, sb;
+    if ((st = pcg_bind(c, 0, pg, sa)) != FIA_OK) return st;
+    if ((st = pcg_bind(c, 1, pg, sb)) != FIA_OK) return st;
+    const double tol2 = rel_tol * rel_tol;
+    const int probe = std::min(5, max_iters);
+    // the network's guess
+    HIP_TRY(u->net.forward(c->pcg_b.p, pg.solid, u->y.p, nullptr, c->stream));
+    HIP_TRY(u->net.guess(u->y.p, pg.pin, c->pcg_flags.p, sa.x, c->stream));
+    HIP_TRY(launch_pcg_start(sa, tol2, c->stream));
+    double *ha = c->host_scal, *hb = c->host_scal + kPcgScalars;
+    HIP_TRY(launch_pcg_check(sa, c->stream));
+    HIP_TRY(hipMemcpyAsync(ha, sa.scal, kPcgScalars * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const double bb = ha[kPcgBB] > 0 ? ha[kPcgBB] : 1.0;
+    const double guess_res = sqrt(ha[kPcgRR] / bb);
+    // probe iterations from both starts
+    HIP_TRY(launch_pcg_iterations(sa, probe, c->stream));
+    HIP_TRY(launch_pcg_check(sa, c->stream));
+    HIP_TRY(launch_pcg_guess(sb, nullptr, c->stream));
+    HIP_TRY(launch_pcg_start(sb, tol2, c->stream));
+    HIP_TRY(launch_pcg_iterations(sb, probe, c->stream));
+    HIP_TRY(launch_pcg_check(sb, c->stream));
+    HIP_TRY(hipMemcpyAsync(ha, sa.scal, kPcgScalars * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(hb, sb.scal, kPcgScalars * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const double ra = sqrt(ha[kPcgRR] / bb), rb = sqrt(hb[kPcgRR] / bb);
+    const bool use_net = ra <= rb;
+    PcgState &ps = use_net ? sa : sb;
+    double *hs = use_net ? ha : hb;
+    if ((st = pcg_run(c, ps, max_iters, hs)) != FIA_OK) return st;
+    const int iters_done = (int)hs[kPcgIters];
+    const double res = sqrt(hs[kPcgRR] / bb);
+    const bool converged = hs[kPcgDone] != 0.0;
+    float *pd = nullptr;
+    if (pressure) {
+        HIP_TRY(c->gen[0].ensure(n));
+        pd = g->on_device ? pressure : c->gen[0].p;
+    }
+    HIP_TRY(launch_project(pg, ps.x, g->dx, gv.u, gv.v, gv.w, pd, c->stream));
+    if (!g->on_device) {
+        const size_t nu = (size_t)(g->nx + 1) * g->ny * g->nz, nv = (size_t)g->nx * (g->ny + 1) * g->nz,
+                     nw = (size_t)g->nx * g->ny * (g->nz + 1);
+        HIP_TRY(hipMemcpyAsync(g->u, gv.u, nu * 4, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipMemcpyAsync(g->v, gv.v, nv * 4, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipMemcpyAsync(g->w, gv.w, nw * 4, hipMemcpyDeviceToHost, c->stream));
+        if (pressure) HIP_TRY(hipMemcpyAsync(pressure, pd, n * 4, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (iters) *iters = iters_done;
+    if (rel_residual) *rel_residual = res;
+    if (info) {
+        info->used_network = use_net ? 1 : 0;
+        info->probe_iters = probe;
+        info->guess_rel_residual = guess_res;
+        info->probe_res_network = ra;
+        info->probe_res_zero = rb;
+    }
     if (!converged) return fail(FIA_ERR_NOT_CONVERGED, "PCG reached max_iters before rel_tol");
     return FIA_OK;
 }

@@ -4,7 +4,25 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 namespace fia {
+
+// Device buffer that grows on demand (contents are not kept when it grows).
+template <class T> struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    hipError_t ensure(size_t want) {
+        if (want <= n && p) return hipSuccess;
+        if (p) { hipError_t e = hipFree(p); if (e != hipSuccess) return e; p = nullptr; n = 0; }
+        hipError_t e = hipMalloc((void **)&p, std::max<size_t>(want, 1) * sizeof(T));
+        if (e == hipSuccess) n = want;
+        return e;
+    }
+    void release() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
 
 constexpr int kMaxGauss = 49;
 // work counters are kept as kStatSlots triples (pieces, density points, 1D WENO6 evaluations),

@@ -213,6 +213,54 @@ fia_status fia_ctx_set_timing(fia_ctx *ctx, int enable);
 fia_status fia_ctx_get_stats(fia_ctx *ctx, fia_kernel_stats *out);   /* synchronises */
 fia_status fia_ctx_reset_stats(fia_ctx *ctx);
 
+/* ---- Learned warm start of the pressure solve -----------------------------------------------
+ * No counterpart in the reference, whose solver is amgcl (src/calc_pressure_3d.cpp:141-166): a
+ * 4-level 3-D U-Net (channels 16/32/48/64, 3x3x3 convolutions, BF16 weights and activations on
+ * the matrix cores, FP32 accumulation) maps the right-hand side of the pressure system and the
+ * obstacle mask to an initial guess for the same Jacobi-PCG.  Grid sizes must be multiples of 8.
+ * Parameters are FP32, fia_unet_param_count() of them (layout in fia_unet.hpp). */
+typedef struct fia_unet fia_unet;
+size_t fia_unet_param_count(void);
+/* network on the context's device, He-normal initial weights drawn from seed */
+fia_status fia_unet_create(fia_ctx *ctx, unsigned long long seed, fia_unet **out);
+fia_status fia_unet_destroy(fia_unet *net);
+fia_status fia_unet_set_params(fia_ctx *ctx, fia_unet *net, const float *params, size_t n);   /* host */
+fia_status fia_unet_get_params(fia_ctx *ctx, fia_unet *net, float *params, size_t n);         /* host */
+/* y = network output for right-hand side rhs (nx*ny*nz FP64) and obstacle mask solid (optional),
+ * in units of *scale = sqrt(mean rhs^2 over fluid cells); pointers on the device when on_device */
+fia_status fia_unet_forward(fia_ctx *ctx, fia_unet *net, int nx, int ny, int nz, const double *rhs,
+                            const uint8_t *solid, float *y, double *scale, int on_device);
+/* residual-norm loss |L y - rhs/scale|^2 / |rhs/scale|^2 of one sample (L: Neumann Laplacian of
+ * the fluid cells) and, when grad is not null, its gradient with respect to every parameter
+ * (host array of fia_unet_param_count() floats); no update */
+fia_status fia_unet_loss_grad(fia_ctx *ctx, fia_unet *net, int nx, int ny, int nz, const double *rhs,
+                              const uint8_t *solid, double *loss, float *grad, int on_device);
+/* one Adam step (beta 0.9, 0.999, eps 1e-8) on the mean gradient of `batch` samples stored one
+ * after another in rhs (device, batch*nx*ny*nz FP64) and solid (device, same count, optional);
+ * *mean_loss receives the mean loss before the step */
+fia_status fia_unet_train_step(fia_ctx *ctx, fia_unet *net, int nx, int ny, int nz, int batch,
+                               const double *rhs, const uint8_t *solid, float lr, double *mean_loss);
+
+/* The right-hand side b = -div(u) dx of the pressure system of grid (0 on obstacle cells and on
+ * the pinned cell), as fia_pressure_project assembles it: training data for the network. */
+fia_status fia_pressure_rhs(fia_ctx *ctx, fia_grid3d *grid, const uint8_t *solid, double *rhs);
+
+typedef struct fia_learned_info {
+    int used_network;            /* 1: continued from the network's guess; 0: fell back to zero guess */
+    int probe_iters;             /* iterations run from both starts before choosing                     */
+    double guess_rel_residual;   /* |b - L x0| / |b| of the network's guess                             */
+    double probe_res_network;    /* relative residual after the probe from the network's guess          */
+    double probe_res_zero;       /* ... and from a zero guess                                            */
+} fia_learned_info;
+
+/* fia_pressure_project with the learned warm start: the network's guess and a zero guess each run
+ * probe iterations (5); the solve continues from the network's unless its residual is larger
+ * (then from the zero guess).  Same system, tolerance and outputs as fia_pressure_project; iters
+ * counts the iterations of the solve that was continued. */
+fia_status fia_pressure_project_learned(fia_ctx *ctx, fia_grid3d *grid, const uint8_t *solid, float *pressure,
+                                        fia_unet *net, double rel_tol, int max_iters, int *iters,
+                                        double *rel_residual, fia_learned_info *info);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,146 @@
+"""GPU parity of the learned pressure warm start, through the C ABI.
+
+The HIP network (BF16 weights and activations on the matrix cores, FP32 accumulation) is compared
+with the numpy oracle run with the same BF16 rounding points (oracle/unet.py, bf=True), so the
+differences left come from FP32 summation order and the BF16 roundings it flips: one flipped
+activation at the 2^3 or 4^3 level moves a large part of the output by up to one BF16 unit
+(2^-8), so some inputs agree to 1e-7 and others to ~5e-3.  Tolerances: output relative L2 <= 1e-2
+and no further from the exact FP64 network than the emulating oracle is (x 1.5); loss relative
+<= 1e-3; gradient relative L2 <= 3e-2 with cosine >= 0.999.  The oracle's own gradient is pinned
+by finite differences on the CPU.
+"""
+import numpy as np
+import pytest
+
+import unet
+
+pytestmark = pytest.mark.gpu
+
+
+def sample(n, seed, obstacles=True):
+    rng = np.random.default_rng(seed)
+    rhs = rng.normal(size=(n, n, n))
+    solid = None
+    if obstacles:
+        solid = np.zeros((n, n, n), dtype=np.uint8)
+        solid[2:5, 3:7, 9:12] = 1
+        solid[10:13, 1:4, 2:6] = 1
+        rhs[solid == 1] = 0.0
+    return rhs, solid
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
+
+
+@pytest.fixture(scope="module")
+def net(fia, ctx):
+    u = fia.UNet(ctx, seed=1)
+    yield u
+    u.close()
+
+
+@pytest.mark.parametrize("obstacles", [False, True])
+def test_unet_forward_matches_oracle(net, obstacles):
+    print("case unet forward", obstacles, flush=True)
+    n = 16
+    rhs, solid = sample(n, 2, obstacles)
+    p = unet.init_params(seed=3)
+    net.set_params(p)
+    y, scale = net.forward(rhs, solid)
+    ref, cache = unet.forward(p.astype(np.float64), rhs, solid, bf=True)
+    exact, _ = unet.forward(p.astype(np.float64), rhs, solid, bf=False)
+    assert abs(scale - cache["scale"]) <= 1e-12 * cache["scale"]
+    assert rel_l2(y, ref) <= 1e-2
+    assert rel_l2(y, exact) <= 1.5 * rel_l2(ref, exact)
+
+
+def test_unet_params_round_trip(net):
+    p = unet.init_params(seed=4)
+    net.set_params(p)
+    assert np.array_equal(net.get_params(), p)
+
+
+@pytest.mark.parametrize("obstacles", [False, True])
+def test_unet_gradient_matches_oracle(net, obstacles):
+    print("case unet gradient", obstacles, flush=True)
+    n = 16
+    rhs, solid = sample(n, 5, obstacles)
+    p = unet.init_params(seed=6)
+    net.set_params(p)
+    loss, g = net.loss_grad(rhs, solid)
+    pd = p.astype(np.float64)
+    y, cache = unet.forward(pd, rhs, solid, bf=True)
+    flags = unet.fluid_flags(solid, rhs.shape)
+    lref, dy = unet.loss(y, cache, flags)
+    gref = unet.backward(pd, cache, dy)
+    assert abs(loss - lref) <= 1e-3 * lref
+    assert rel_l2(g, gref) <= 3e-2
+    cos = float(np.dot(g, gref) / (np.linalg.norm(g) * np.linalg.norm(gref)))
+    assert cos >= 0.999
+    # every block of the parameter vector carries gradient where the oracle's does
+    offs, oo, nparam = unet.offsets()
+    for i, (ow, ob) in enumerate(offs):
+        cin, cout, _, creal = unet.CONVS[i]
+        blk = slice(ow, ob + cout)
+        assert rel_l2(g[blk], gref[blk]) <= 1e-1, i
+
+
+def test_unet_training_step_reduces_loss(fia, ctx):
+    import torch
+    print("case unet train", flush=True)
+    n = 16
+    u = fia.UNet(ctx, seed=2)
+    try:
+        batch = [sample(n, 20 + k, obstacles=(k % 2 == 1)) for k in range(2)]
+        rhs = torch.tensor(np.stack([b[0] for b in batch]), dtype=torch.float64, device="cuda:0")
+        solid = torch.tensor(np.stack([b[1] if b[1] is not None else np.zeros((n, n, n), np.uint8) for b in batch]),
+                             dtype=torch.uint8, device="cuda:0")
+        losses = [u.train_step(rhs, solid, lr=1e-3) for _ in range(30)]
+        assert all(np.isfinite(losses))
+        assert losses[-1] < 0.8 * losses[0]
+    finally:
+        u.close()
+
+
+def projection_scene(scenes, n, seed):
+    h = 1.0 / n
+    u, v, w = scenes.curl_noise(n, seed=seed)
+    rng = np.random.default_rng(seed)
+    v = v + 0.3 * rng.normal(size=v.shape).astype(np.float32)
+    v[:, 0] = v[:, -1] = 0
+    solid = scenes.spherical_obstacles(n, count=3, seed=seed)
+    return h, u.astype(np.float32), v.astype(np.float32), w.astype(np.float32), solid
+
+
+@pytest.mark.parametrize("with_solid", [False, True])
+def test_learned_projection_matches_plain(fia, ctx, scenes, net, with_solid):
+    print("case learned projection", with_solid, flush=True)
+    n = 16
+    h, u, v, w, solid = projection_scene(scenes, n, 3)
+    if not with_solid:
+        solid = None
+    net.set_params(unet.init_params(seed=7))
+    g1 = fia.Grid3D(n, n, n, h, None, u.copy(), v.copy(), w.copy())
+    it1, r1 = fia.calc_pressure_gradient_term_3d(ctx, g1, solid=solid, rel_tol=1e-8)
+    g2 = fia.Grid3D(n, n, n, h, None, u.copy(), v.copy(), w.copy())
+    it2, r2, info = fia.calc_pressure_gradient_term_3d_learned(ctx, g2, net, solid=solid, rel_tol=1e-8)
+    assert r1 <= 1e-8 and r2 <= 1e-8
+    assert info["probe_iters"] == 5
+    # the continued solve is the one whose probe residual was smaller
+    assert info["used_network"] == int(info["probe_res_network"] <= info["probe_res_zero"])
+    for a, b in ((g1.u, g2.u), (g1.v, g2.v), (g1.w, g2.w)):
+        assert np.abs(a - b).max() < 1e-5 * max(1.0, np.abs(a).max())
+
+
+def test_pressure_rhs_matches_divergence(fia, ctx, scenes):
+    n = 16
+    h, u, v, w, solid = projection_scene(scenes, n, 4)
+    g = fia.Grid3D(n, n, n, h, None, u.copy(), v.copy(), w.copy())
+    rhs = fia.pressure_rhs(ctx, g)
+    div = scenes.divergence(u.astype(np.float64), v.astype(np.float64), w.astype(np.float64), h)
+    ref = -div * h * h
+    ref.reshape(-1)[-1] = 0.0   # the pinned cell
+    assert np.abs(rhs - ref).max() <= 1e-6 * np.abs(ref).max()
+    # the grid's velocity is not modified
+    assert np.array_equal(g.u, u)
