@@ -10,6 +10,10 @@ psi (Gauss n=1), zero-velocity correction on.  Fields stay resident on the GPU.
   projection  one step = one pressure projection (Jacobi-PCG to 1e-6 relative residual, zero
               initial guess) of the curl-noise velocity plus a buoyancy impulse, 8 spherical
               obstacles as a solid mask
+  warmstart   one step = the pressure projection of one plume step (128^3 unless --n; seeded
+              source, buoyancy, 0-8 obstacles; scenes not used in training) twice on the same
+              velocity: from a zero guess and with the learned U-Net warm start (committed
+              weights), both to 1e-6; value and ms_per_step are the warm-started projection's
 
 Prints one JSON line: value = cell-steps per second / 1e6 over the timed steps; for the
 advection workloads a roofline entry for the WENO6 quadrature kernel measured with HIP events on
@@ -202,19 +206,60 @@ def run_projection(args, fia, scenes, torch):
     }
 
 
+def run_warmstart(args, fia, scenes, torch):
+    learned = importlib.import_module("flux-interpolated-advection_amd.learned")
+    if not os.path.exists(args.weights):
+        print("no trained weights at %s (flux-interpolated-advection_amd/learned.py train)" % args.weights,
+              file=sys.stderr)
+        sys.exit(2)
+    n = args.n
+    ctx = fia.Context(0)
+    net = fia.UNet(ctx, seed=0)
+    net.load(args.weights)
+    # warm-up projections untimed, then the timed ones, all on held-out scene 5000 (one plume run)
+    rows = learned.evaluate(fia, scenes, ctx, net, n, [5000], args.warmup + args.steps, log=lambda s: None)
+    timed = rows[args.warmup:]
+    s = learned.summary(timed)
+    net.close()
+    ctx.close()
+    l_ms, p_ms = s["learned_ms"], s["plain_ms"]
+    gbs = PCG_BYTES_PER_CELL_ITER * n ** 3 * s["learned_iters"] / (l_ms / 1e3) / 1e9
+    return {
+        "metric": METRIC, "value": round(n ** 3 / (l_ms / 1e3) / 1e6, 3), "unit": "Mcell-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(l_ms, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64 solve, bf16 network", "data": "synthetic",
+        "config": {"workload": "pressure projection of a %d^3 buoyant plume step with the learned U-Net warm "
+                               "start (energy-trained, 16/32/48/64 channels) vs from a zero guess, Jacobi-PCG to 1e-6"
+                               % n, "grid": [n, n, n], "weights": os.path.relpath(args.weights, ROOT)},
+        "warm_start": {"plain_ms_per_step": round(p_ms, 3), "learned_ms_per_step": round(l_ms, 3),
+                       "speedup": round(p_ms / l_ms, 4), "plain_iters": s["plain_iters"],
+                       "learned_iters": s["learned_iters"], "network_used": s["network_used"],
+                       "projections": s["projections"], "guess_rel_residual_mean": s["guess_rel_residual_mean"],
+                       "max_rel_residual": s["max_rel_residual"]},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": None,
+                     "kernel": "warm-started projection: PCG iterations at %d B per cell over its whole time "
+                               "(network and probe included)" % PCG_BYTES_PER_CELL_ITER},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--n", type=int, default=None, help="grid edge (256; warmstart: 128)")
     ap.add_argument("--cfl", type=float, default=4.0)
-    ap.add_argument("--workload", choices=("density", "velocity", "projection"), default="density")
+    ap.add_argument("--workload", choices=("density", "velocity", "projection", "warmstart"), default="density")
+    ap.add_argument("--weights", default=os.path.join(ROOT, "flux-interpolated-advection_amd", "weights",
+                                                      "unet_plume128.npy"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=128)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one plain device copy of 1 GiB (a known-size kernel for HBM counter scaling)")
     args = ap.parse_args()
+    if args.n is None:
+        args.n = 128 if args.workload == "warmstart" else 256
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md, sharding)",
               file=sys.stderr)
@@ -231,6 +276,8 @@ def main():
         del a, b
     if args.workload == "projection":
         out = run_projection(args, fia, scenes, torch)
+    elif args.workload == "warmstart":
+        out = run_warmstart(args, fia, scenes, torch)
     else:
         out = run_advection(args, fia, scenes, torch, args.workload == "velocity")
     print(json.dumps(out))

@@ -30,6 +30,7 @@ INTERP = {"linear": 0, "WENO6": 1, "1Dy_linear": 2, "1Dy_WENO6": 3}
 SPLIT = {"xz-CellFaceAligned": 1, "xz-AxisAligned": 2, "x-CellFaceAligned_z-AxisAligned": 3,
          "x-AxisAligned_z-CellFaceAligned": 4}
 PSI = {"const": 0, "linear": 1, "WENO6": 2}
+UNET_LOSS = {"residual": 0, "energy": 1}
 
 # exported symbols of libfia.so (the declarations of include/fia.h)
 EXPORTS = ["fia_default_opts", "fia_ctx_create", "fia_ctx_destroy", "fia_ctx_device_bytes", "fia_last_error",
@@ -38,7 +39,7 @@ EXPORTS = ["fia_default_opts", "fia_ctx_create", "fia_ctx_destroy", "fia_ctx_dev
            "fia_pressure_project", "fia_ctx_set_timing", "fia_ctx_get_stats", "fia_ctx_reset_stats",
            "fia_unet_param_count", "fia_unet_create", "fia_unet_destroy", "fia_unet_set_params",
            "fia_unet_get_params", "fia_unet_forward", "fia_unet_loss_grad", "fia_unet_train_step", "fia_pressure_rhs",
-           "fia_pressure_project_learned"]
+           "fia_pressure_project_learned", "fia_grid_totals", "fia_wall_momentum_loss"]
 
 
 class FiaError(RuntimeError):
@@ -64,8 +65,9 @@ class GridC(ctypes.Structure):
 
 
 class LearnedInfo(ctypes.Structure):
-    _fields_ = [("used_network", ctypes.c_int), ("probe_iters", ctypes.c_int), ("guess_rel_residual", ctypes.c_double),
-                ("probe_res_network", ctypes.c_double), ("probe_res_zero", ctypes.c_double)]
+    _fields_ = [("used_network", ctypes.c_int), ("probe_iters", ctypes.c_int), ("guess_scale", ctypes.c_double),
+                ("guess_rel_residual", ctypes.c_double), ("probe_res_network", ctypes.c_double),
+                ("probe_res_zero", ctypes.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -123,9 +125,13 @@ def lib():
         L.fia_unet_set_params.argtypes = [_vp, _vp, _vp, ctypes.c_size_t]
         L.fia_unet_get_params.argtypes = [_vp, _vp, _vp, ctypes.c_size_t]
         L.fia_unet_forward.argtypes = [_vp, _vp, _i, _i, _i, _vp, _vp, _vp, ctypes.POINTER(_d), _i]
-        L.fia_unet_loss_grad.argtypes = [_vp, _vp, _i, _i, _i, _vp, _vp, ctypes.POINTER(_d), _vp, _i]
-        L.fia_unet_train_step.argtypes = [_vp, _vp, _i, _i, _i, _i, _vp, _vp, ctypes.c_float, ctypes.POINTER(_d)]
+        L.fia_unet_loss_grad.argtypes = [_vp, _vp, _i, _i, _i, _vp, _vp, _i, ctypes.POINTER(_d), ctypes.POINTER(_d), _vp,
+                                         _i]
+        L.fia_unet_train_step.argtypes = [_vp, _vp, _i, _i, _i, _i, _vp, _vp, _i, ctypes.c_float, ctypes.POINTER(_d),
+                                          ctypes.POINTER(_d)]
         L.fia_pressure_rhs.argtypes = [_vp, ctypes.POINTER(GridC), _vp, _vp]
+        L.fia_grid_totals.argtypes = [_vp, ctypes.POINTER(GridC), ctypes.POINTER(_d * 4)]
+        L.fia_wall_momentum_loss.argtypes = [_vp, _i, _i, _i, _d, _vp, _i, ctypes.POINTER(_d * 3)]
         L.fia_pressure_project_learned.argtypes = [_vp, ctypes.POINTER(GridC), _vp, _vp, _vp, _d, _i, ctypes.POINTER(_i),
                                                    ctypes.POINTER(_d), ctypes.POINTER(LearnedInfo)]
         _lib = L
@@ -445,27 +451,30 @@ class UNet:
         _check(lib().fia_unet_forward(self.ctx._h, self._h, nx, ny, nz, pr, sp, _ptr(y)[0], ctypes.byref(sc), on))
         return y, sc.value
 
-    def loss_grad(self, rhs, solid=None, with_grad=True):
-        """Residual-norm loss of one sample and (optionally) its FP32 parameter gradient."""
+    def loss_grad(self, rhs, solid=None, with_grad=True, loss="residual"):
+        """(loss, residual norm |L y - b|^2/|b|^2, FP32 parameter gradient or None) of one sample;
+        loss is "residual" (residual norm) or "energy" (y.L y / 2 - b.y, over |b|^2)."""
         nx, ny, nz = rhs.shape
         pr, on = _ptr(rhs)
         sp, keep = _solid_ptr(solid, on)
         g = np.empty(self.param_count(), dtype=np.float32) if with_grad else None
-        loss = _d(0)
-        _check(lib().fia_unet_loss_grad(self.ctx._h, self._h, nx, ny, nz, pr, sp, ctypes.byref(loss),
-                                        g.ctypes.data if g is not None else None, on))
-        return loss.value, g
+        lv, rv = _d(0), _d(0)
+        _check(lib().fia_unet_loss_grad(self.ctx._h, self._h, nx, ny, nz, pr, sp, UNET_LOSS[loss], ctypes.byref(lv),
+                                        ctypes.byref(rv), g.ctypes.data if g is not None else None, on))
+        return lv.value, rv.value, g
 
-    def train_step(self, rhs_batch, solid_batch=None, lr=1e-3):
-        """One Adam step on a batch [b, nx, ny, nz] of device tensors; returns the mean loss."""
+    def train_step(self, rhs_batch, solid_batch=None, lr=1e-3, loss="residual"):
+        """One Adam step on a batch [b, nx, ny, nz] of device tensors; returns the batch means of
+        the loss and of the residual norm before the step."""
         b, nx, ny, nz = rhs_batch.shape
         pr, on = _ptr(rhs_batch)
         if not on:
             raise ValueError("training batches must be device tensors")
         sp = solid_batch.data_ptr() if solid_batch is not None else None
-        loss = _d(0)
-        _check(lib().fia_unet_train_step(self.ctx._h, self._h, nx, ny, nz, b, pr, sp, float(lr), ctypes.byref(loss)))
-        return loss.value
+        lv, rv = _d(0), _d(0)
+        _check(lib().fia_unet_train_step(self.ctx._h, self._h, nx, ny, nz, b, pr, sp, UNET_LOSS[loss], float(lr),
+                                         ctypes.byref(lv), ctypes.byref(rv)))
+        return lv.value, rv.value
 
 
 def pressure_rhs(ctx, grid, solid=None):
@@ -500,3 +509,23 @@ def calc_pressure_gradient_term_3d_learned(ctx, grid, net, solid=None, rel_tol=1
     _check(lib().fia_pressure_project_learned(ctx._h, ctypes.byref(g), sp, pp, net._h, float(rel_tol), int(max_iters),
                                               ctypes.byref(it), ctypes.byref(res), ctypes.byref(info)))
     return it.value, res.value, info.as_dict()
+
+
+# ---- diagnostics of the reference's output series --------------------------------------------
+
+def grid_totals(ctx, grid):
+    """(total mass, momentum x, y, z): sums times the cell volume (reference src/main.cpp:518-597)."""
+    g = grid._c(with_density=grid.density is not None)
+    out = (_d * 4)()
+    _check(lib().fia_grid_totals(ctx._h, ctypes.byref(g), ctypes.byref(out)))
+    return tuple(out)
+
+
+def wall_momentum_loss(ctx, grid, pressure=None):
+    """Momentum the walls took out through the pressure of one projection (x, y, z), reference
+    src/calc_pressure_3d.cpp:235-266; pressure defaults to grid.pressure."""
+    p = grid.pressure if pressure is None else pressure
+    pp, on = _ptr(p)
+    out = (_d * 3)()
+    _check(lib().fia_wall_momentum_loss(ctx._h, grid.nx, grid.ny, grid.nz, grid.dx, pp, on, ctypes.byref(out)))
+    return tuple(out)

@@ -1018,8 +1018,10 @@ fia_status fia_unet_forward(fia_ctx *c, fia_unet *u, int nx, int ny, int nz, con
 }
 
 fia_status fia_unet_loss_grad(fia_ctx *c, fia_unet *u, int nx, int ny, int nz, const double *rhs, const uint8_t *solid,
-                              double *loss, float *grad, int on_device) {
+                              int loss_mode, double *loss, double *residual, float *grad, int on_device) {
     if (!c || !u || !rhs) return fail(FIA_ERR_INVALID_ARG, "NULL argument");
+    if (loss_mode != FIA_UNET_LOSS_RESIDUAL && loss_mode != FIA_UNET_LOSS_ENERGY)
+        return fail(FIA_ERR_INVALID_ARG, "unknown loss mode");
     HIP_TRY(set_device(c));
     fia_status st;
     if ((st = unet_grid(u, nx, ny, nz)) != FIA_OK) return st;
@@ -1028,8 +1030,8 @@ fia_status fia_unet_loss_grad(fia_ctx *c, fia_unet *u, int nx, int ny, int nz, c
     if ((st = unet_inputs(c, u, n, rhs, solid, on_device, rd, sd)) != FIA_OK) return st;
     HIP_TRY(launch_fluid_flags(sd, nx, ny, nz, u->flags.p, c->stream));
     HIP_TRY(u->net.forward(rd, sd, u->y.p, nullptr, c->stream));
-    double l = 0.0;
-    HIP_TRY(u->net.loss_residual(u->flags.p, rd, u->y.p, grad ? u->dy.p : nullptr, &l, c->stream));
+    double l = 0.0, r = 0.0;
+    HIP_TRY(u->net.loss(loss_mode, u->flags.p, rd, u->y.p, grad ? u->dy.p : nullptr, &l, &r, c->stream));
     if (grad) {
         HIP_TRY(u->net.zero_grad(c->stream));
         HIP_TRY(u->net.backward(u->dy.p, c->stream));
@@ -1037,31 +1039,36 @@ fia_status fia_unet_loss_grad(fia_ctx *c, fia_unet *u, int nx, int ny, int nz, c
     }
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (loss) *loss = l;
+    if (residual) *residual = r;
     return FIA_OK;
 }
 
 fia_status fia_unet_train_step(fia_ctx *c, fia_unet *u, int nx, int ny, int nz, int batch, const double *rhs,
-                               const uint8_t *solid, float lr, double *mean_loss) {
+                               const uint8_t *solid, int loss_mode, float lr, double *mean_loss, double *mean_residual) {
     if (!c || !u || !rhs || batch <= 0 || !(lr > 0)) return fail(FIA_ERR_INVALID_ARG, "bad argument");
+    if (loss_mode != FIA_UNET_LOSS_RESIDUAL && loss_mode != FIA_UNET_LOSS_ENERGY)
+        return fail(FIA_ERR_INVALID_ARG, "unknown loss mode");
     HIP_TRY(set_device(c));
     fia_status st;
     if ((st = unet_grid(u, nx, ny, nz)) != FIA_OK) return st;
     const size_t n = (size_t)nx * ny * nz;
     HIP_TRY(u->net.zero_grad(c->stream));
-    double sum = 0.0;
+    double sum = 0.0, sum_r = 0.0;
     for (int b = 0; b < batch; ++b) {
         const double *rd = rhs + (size_t)b * n;
         const uint8_t *sd = solid ? solid + (size_t)b * n : nullptr;
         HIP_TRY(launch_fluid_flags(sd, nx, ny, nz, u->flags.p, c->stream));
         HIP_TRY(u->net.forward(rd, sd, u->y.p, nullptr, c->stream));
-        double l = 0.0;
-        HIP_TRY(u->net.loss_residual(u->flags.p, rd, u->y.p, u->dy.p, &l, c->stream));
+        double l = 0.0, r = 0.0;
+        HIP_TRY(u->net.loss(loss_mode, u->flags.p, rd, u->y.p, u->dy.p, &l, &r, c->stream));
         sum += l;
+        sum_r += r;
         HIP_TRY(u->net.backward(u->dy.p, c->stream));
     }
     HIP_TRY(u->net.adam_step(lr, 0.9f, 0.999f, 1e-8f, 1.0f / (float)batch, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (mean_loss) *mean_loss = sum / batch;
+    if (mean_residual) *mean_residual = sum_r / batch;
     return FIA_OK;
 }
 
@@ -1114,13 +1121,14 @@ fia_status fia_pressure_project_learned(fia_ctx *c, fia_grid3d *g, const uint8_t
     // the network's guess
     HIP_TRY(u->net.forward(c->pcg_b.p, pg.solid, u->y.p, nullptr, c->stream));
     HIP_TRY(u->net.guess(u->y.p, pg.pin, c->pcg_flags.p, sa.x, c->stream));
+    HIP_TRY(launch_pcg_scale_guess(sa, c->stream));
     HIP_TRY(launch_pcg_start(sa, tol2, c->stream));
     double *ha = c->host_scal, *hb = c->host_scal + kPcgScalars;
     HIP_TRY(launch_pcg_check(sa, c->stream));
     HIP_TRY(hipMemcpyAsync(ha, sa.scal, kPcgScalars * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     const double bb = ha[kPcgBB] > 0 ? ha[kPcgBB] : 1.0;
-    const double guess_res = sqrt(ha[kPcgRR] / bb);
+    const double guess_res = sqrt(ha[kPcgRR] / bb), gamma = ha[kPcgGamma];
     // probe iterations from both starts
     HIP_TRY(launch_pcg_iterations(sa, probe, c->stream));
     HIP_TRY(launch_pcg_check(sa, c->stream));
@@ -1160,10 +1168,56 @@ fia_status fia_pressure_project_learned(fia_ctx *c, fia_grid3d *g, const uint8_t
         info->used_network = use_net ? 1 : 0;
         info->probe_iters = probe;
         info->guess_rel_residual = guess_res;
+        info->guess_scale = gamma;
         info->probe_res_network = ra;
         info->probe_res_zero = rb;
     }
     if (!converged) return fail(FIA_ERR_NOT_CONVERGED, "PCG reached max_iters before rel_tol");
+    return FIA_OK;
+}
+
+// ---- diagnostics of the reference's output series --------------------------------------------
+
+fia_status fia_grid_totals(fia_ctx *c, fia_grid3d *g, double totals[4]) {
+    if (!c || !totals) return fail(FIA_ERR_INVALID_ARG, "NULL argument");
+    fia_status st;
+    if ((st = check_grid(g, false)) != FIA_OK) return st;
+    HIP_TRY(set_device(c));
+    GridView gv;
+    if ((st = upload_grid(c, g, gv, g->density != nullptr)) != FIA_OK) return st;
+    const size_t nc = (size_t)g->nx * g->ny * g->nz;
+    const float *f[4] = {g->density ? gv.rho : nullptr, gv.u, gv.v, gv.w};
+    const size_t n[4] = {nc, (size_t)(g->nx + 1) * g->ny * g->nz, (size_t)g->nx * (g->ny + 1) * g->nz,
+                         (size_t)g->nx * g->ny * (g->nz + 1)};
+    HIP_TRY(c->genD[1].ensure(io_partial_words()));
+    double *out = c->genD[1].p + io_partial_words() - 8;
+    HIP_TRY(launch_field_sums(f, n, c->genD[1].p, out, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->host_scal, out, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const double vol = g->dx * g->dx * g->dx;
+    for (int k = 0; k < 4; ++k) totals[k] = c->host_scal[k] * vol;
+    return FIA_OK;
+}
+
+fia_status fia_wall_momentum_loss(fia_ctx *c, int nx, int ny, int nz, double dx, const float *pressure, int on_device,
+                                  double loss[3]) {
+    if (!c || !pressure || !loss) return fail(FIA_ERR_INVALID_ARG, "NULL argument");
+    if (nx <= 0 || ny <= 0 || nz <= 0 || !(dx > 0)) return fail(FIA_ERR_INVALID_ARG, "bad grid size");
+    HIP_TRY(set_device(c));
+    const size_t n = (size_t)nx * ny * nz;
+    const float *pd = pressure;
+    if (!on_device) {
+        HIP_TRY(c->gen[0].ensure(n));
+        HIP_TRY(hipMemcpyAsync(c->gen[0].p, pressure, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        pd = c->gen[0].p;
+    }
+    HIP_TRY(c->genD[1].ensure(io_partial_words()));
+    double *out = c->genD[1].p + io_partial_words() - 8;
+    HIP_TRY(launch_wall_sums(pd, nx, ny, nz, c->genD[1].p, out, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->host_scal, out, 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const double a = dx * dx;
+    for (int k = 0; k < 3; ++k) loss[k] = a * (c->host_scal[2 * k + 1] - c->host_scal[2 * k]);
     return FIA_OK;
 }
 

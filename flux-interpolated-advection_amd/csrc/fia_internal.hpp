@@ -151,8 +151,8 @@ struct PcgGrid {
 constexpr unsigned kPcgUnknown = 0x80u;
 constexpr int kPcgMaxBlocks = 1024;
 // device scalars of one solve: rz of even and odd iterations, last residual r.r, b.b, tol^2,
-// converged flag, iterations done
-enum { kPcgRZ = 0, kPcgRR = 3, kPcgBB = 4, kPcgTol2 = 5, kPcgDone = 6, kPcgIters = 7, kPcgScalars = 16 };
+// converged flag, iterations done, scale applied to the initial guess
+enum { kPcgRZ = 0, kPcgRR = 3, kPcgBB = 4, kPcgTol2 = 5, kPcgDone = 6, kPcgIters = 7, kPcgGamma = 8, kPcgScalars = 16 };
 // one CG solve in flight: its vectors and device scalars
 struct PcgState {
     PcgGrid g;
@@ -175,11 +175,20 @@ hipError_t launch_pcg_setup(const PcgGrid &g, const float *u, const float *v, co
 hipError_t launch_pcg_guess(const PcgState &st, const float *p_in, hipStream_t s);
 // r = b - L x, scalars; then `count` iterations (each a no-op once converged); the check updates
 // the polled scalars (residual, iterations, converged flag) for the iterations enqueued so far
+// x *= x.b / x.L x (the energy-minimising multiple of the initial guess in x)
+hipError_t launch_pcg_scale_guess(const PcgState &st, hipStream_t s);
 hipError_t launch_pcg_start(PcgState &st, double tol2, hipStream_t s);
 hipError_t launch_pcg_iterations(PcgState &st, int count, hipStream_t s);
 hipError_t launch_pcg_check(const PcgState &st, hipStream_t s);
 hipError_t launch_project(const PcgGrid &g, const double *p, double h, float *u, float *v, float *w,
                           float *pressure_out, hipStream_t s);
 hipError_t launch_zero_solid_faces(const PcgGrid &g, float *u, float *v, float *w, hipStream_t s);
+
+// diagnostics (fia_io.hip): FP64 sums of up to four fields (out[0..3]) and of the pressure over
+// the six boundary layers of cells (out[0..5]); part holds io_partial_words() doubles
+size_t io_partial_words();
+hipError_t launch_field_sums(const float *const fields[4], const size_t sizes[4], double *part, double *out,
+                             hipStream_t s);
+hipError_t launch_wall_sums(const float *p, int nx, int ny, int nz, double *part, double *out, hipStream_t s);
 
 }  // namespace fia

@@ -391,6 +391,36 @@ __global__ void __launch_bounds__(kPB) pcg_update_kernel(PcgState st, int k, con
     write_partials<2>(acc, st.part_update, st.nblocks);
 }
 
+// Energy-optimal scaling of an initial guess: partial sums of x.b and x.(L x) ...
+__global__ void __launch_bounds__(kPB) pcg_energy_kernel(PcgState st) {
+    const size_t n = (size_t)st.g.nx * st.g.ny * st.g.nz;
+    const long sx = (long)st.g.ny * st.g.nz, sy = st.g.nz;
+    const double *x = st.x;
+    double acc[2] = {0.0, 0.0};
+    const CellRange cr = block_cells(n, st.nblocks);
+    for (size_t c = cr.first; c < cr.end; c += cr.step) {
+        const unsigned f = st.flags[c];
+        const double s = stencil(f, c, sx, sy, [&](size_t i) { return x[i]; });
+        if (f & kPcgUnknown) {
+            acc[0] += x[c] * st.b[c];
+            acc[1] += x[c] * s;
+        }
+    }
+    write_partials<2>(acc, st.part_start, st.nblocks);
+}
+
+// ... then x *= gamma = x.b / x.L x, the factor that minimises the energy (the A-norm error) of
+// gamma x; 0 when x.L x is not positive.  Block 0 keeps gamma in the scalars.
+__global__ void __launch_bounds__(kPB) pcg_scale_kernel(PcgState st) {
+    double tot[2];
+    total_of<2>(st.part_start, st.nblocks, tot);
+    const double gamma = tot[1] > 0.0 ? tot[0] / tot[1] : 0.0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) st.scal[kPcgGamma] = gamma;
+    const size_t n = (size_t)st.g.nx * st.g.ny * st.g.nz;
+    const CellRange cr = block_cells(n, st.nblocks);
+    for (size_t c = cr.first; c < cr.end; c += cr.step) st.x[c] *= gamma;
+}
+
 __global__ void project_kernel(PcgGrid g, const double *p, double h, float *u, float *v, float *w,
                                float *pressure_out) {
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -459,6 +489,11 @@ hipError_t launch_pcg_setup(const PcgGrid &g, const float *u, const float *v, co
 hipError_t launch_pcg_guess(const PcgState &st, const float *p_in, hipStream_t s) {
     const size_t n = (size_t)st.g.nx * st.g.ny * st.g.nz;
     hipLaunchKernelGGL(pcg_guess_kernel, dim3(blocks(n)), dim3(kPB), 0, s, st, p_in);
+    return hipGetLastError();
+}
+hipError_t launch_pcg_scale_guess(const PcgState &st, hipStream_t s) {
+    hipLaunchKernelGGL(pcg_energy_kernel, dim3(st.nblocks), dim3(kPB), 0, s, st);
+    hipLaunchKernelGGL(pcg_scale_kernel, dim3(st.nblocks), dim3(kPB), 0, s, st);
     return hipGetLastError();
 }
 hipError_t launch_pcg_start(PcgState &st, double tol2, hipStream_t s) {

@@ -271,8 +271,9 @@ constexpr int kEB = 256;
 static inline unsigned eblocks(size_t n) { return (unsigned)((n + kEB - 1) / kEB); }
 
 constexpr int kRedBlocks = 256;
-// red_ slots: [0, 2 kRedBlocks) partials, then the scale, its inverse, loss terms, loss coefficient
-enum { kRedScale = 2 * kRedBlocks, kRedInvScale, kRedLossNum, kRedLossDen, kRedLossCoef, kRedOut, kRedSlots = kRedOut + 32 };
+// red_ slots: [0, 4 kRedBlocks) partials, then the scale, its inverse, the loss, |b|^2, the loss
+// gradient coefficient, the residual norm ratio |L y - b|^2 / |b|^2
+enum { kRedScale = 4 * kRedBlocks, kRedInvScale, kRedLoss, kRedLossDen, kRedLossCoef, kRedResidual, kRedSlots };
 
 // sum of rhs^2 and count over the fluid cells, per block
 __global__ void __launch_bounds__(kEB) rhs_sumsq_kernel(const double *rhs, const uint8_t *solid, size_t n, double *part) {
@@ -430,14 +431,15 @@ __global__ void __launch_bounds__(kEB) out_bwd_kernel(const float *dy, const uin
 }
 
 // Residual of the network output on the fluid cells: res = L y - rhs / scale (L: the Neumann
-// Laplacian of the fluid cells, flag bits 0-5), and per-block sums of res^2 and (rhs / scale)^2.
+// Laplacian of the fluid cells, flag bits 0-5), and per-block sums of res^2, b^2, y.res, b.y
+// (b = rhs / scale).
 __global__ void __launch_bounds__(kEB) loss_res_kernel(const uint8_t *flags, const float *y, const double *rhs,
                                                       const double *red, int nx, int ny, int nz, float *res, double *part) {
-    __shared__ double sh[2][kEB];
+    __shared__ double sh[4][kEB];
     const size_t n = (size_t)nx * ny * nz;
     const long sx = (long)ny * nz, sy = nz;
     const double inv = red[kRedInvScale];
-    double a0 = 0.0, a1 = 0.0;
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
     for (size_t c = (size_t)blockIdx.x * kEB + threadIdx.x; c < n; c += (size_t)gridDim.x * kEB) {
         const unsigned f = flags[c];
         float r = 0.f;
@@ -451,32 +453,49 @@ __global__ void __launch_bounds__(kEB) loss_res_kernel(const uint8_t *flags, con
             if (f & 32u) s -= y[c + 1];
             const double b = rhs[c] * inv;
             r = (float)(s - b);
-            a0 += (double)r * r;
-            a1 += b * b;
+            a[0] += (double)r * r;
+            a[1] += b * b;
+            a[2] += (double)y[c] * r;
+            a[3] += b * y[c];
         }
         res[c] = r;
     }
-    sh[0][threadIdx.x] = a0; sh[1][threadIdx.x] = a1;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) sh[k][threadIdx.x] = a[k];
     __syncthreads();
     for (int off = kEB / 2; off > 0; off >>= 1) {
-        if ((int)threadIdx.x < off) { sh[0][threadIdx.x] += sh[0][threadIdx.x + off]; sh[1][threadIdx.x] += sh[1][threadIdx.x + off]; }
+        if ((int)threadIdx.x < off)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + off];
         __syncthreads();
     }
-    if (threadIdx.x == 0) { part[blockIdx.x] = sh[0][0]; part[kRedBlocks + blockIdx.x] = sh[1][0]; }
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) part[k * kRedBlocks + blockIdx.x] = sh[k][0];
 }
 
-__global__ void loss_finish_kernel(double *red) {
+// mode 0 (residual norm): loss = |r|^2 / |b|^2, dy = (2 / |b|^2) L r
+// mode 1 (energy): loss = (y.L y / 2 - b.y) / |b|^2 = (y.r - b.y) / (2 |b|^2), dy = r / |b|^2
+__global__ void loss_finish_kernel(double *red, int mode) {
     if (threadIdx.x != 0) return;
-    double a = 0.0, b = 0.0;
-    for (int k = 0; k < kRedBlocks; ++k) { a += red[k]; b += red[kRedBlocks + k]; }
-    red[kRedLossNum] = a;
-    red[kRedLossDen] = b;
-    red[kRedLossCoef] = b > 0.0 ? 2.0 / b : 0.0;
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int k = 0; k < 4; ++k)
+        for (int b = 0; b < kRedBlocks; ++b) a[k] += red[k * kRedBlocks + b];
+    const double den = a[1] > 0.0 ? a[1] : 1.0;
+    red[kRedResidual] = a[0] / den;
+    red[kRedLossDen] = a[1];
+    if (mode == 0) {
+        red[kRedLoss] = a[0] / den;
+        red[kRedLossCoef] = 2.0 / den;
+    } else {
+        red[kRedLoss] = 0.5 * (a[2] - a[3]) / den;
+        red[kRedLossCoef] = 1.0 / den;
+    }
 }
 
-// dy = (2 / |b|^2) L res
+// dy = coef L res (mode 0) or coef res (mode 1)
 __global__ void loss_grad_kernel(const uint8_t *flags, const float *res, const double *red, int nx, int ny, int nz,
-                                 float *dy) {
+                                 int mode, float *dy) {
     const size_t c = (size_t)blockIdx.x * kEB + threadIdx.x;
     const size_t n = (size_t)nx * ny * nz;
     if (c >= n) return;
@@ -484,13 +503,16 @@ __global__ void loss_grad_kernel(const uint8_t *flags, const float *res, const d
     const unsigned f = flags[c];
     float g = 0.f;
     if (f & 63u) {
-        double s = (double)__builtin_popcount(f & 63u) * res[c];
-        if (f & 1u) s -= res[c - sx];
-        if (f & 2u) s -= res[c + sx];
-        if (f & 4u) s -= res[c - sy];
-        if (f & 8u) s -= res[c + sy];
-        if (f & 16u) s -= res[c - 1];
-        if (f & 32u) s -= res[c + 1];
+        double s = res[c];
+        if (mode == 0) {
+            s *= (double)__builtin_popcount(f & 63u);
+            if (f & 1u) s -= res[c - sx];
+            if (f & 2u) s -= res[c + sx];
+            if (f & 4u) s -= res[c - sy];
+            if (f & 8u) s -= res[c + sy];
+            if (f & 16u) s -= res[c - 1];
+            if (f & 32u) s -= res[c + 1];
+        }
         g = (float)(red[kRedLossCoef] * s);
     }
     dy[c] = g;
@@ -732,21 +754,26 @@ hipError_t UNet::forward(const double *rhs, const uint8_t *solid, float *y, doub
     return hipSuccess;
 }
 
-hipError_t UNet::loss_residual(const uint8_t *flags, const double *rhs, const float *y, float *dy, double *loss,
-                               hipStream_t s) {
+hipError_t UNet::loss(int mode, const uint8_t *flags, const double *rhs, const float *y, float *dy, double *loss,
+                      double *residual, hipStream_t s) {
+    if (mode != 0 && mode != 1) return hipErrorInvalidValue;
     const size_t n = vox_[0];
     // the residual goes into the full-resolution gradient buffer of d0a, free at this point
     float *res = gact_[12].p;
     hipLaunchKernelGGL(loss_res_kernel, dim3(kRedBlocks), dim3(kEB), 0, s, flags, y, rhs, red_.p, nx_, ny_, nz_, res, red_.p);
-    hipLaunchKernelGGL(loss_finish_kernel, dim3(1), dim3(64), 0, s, red_.p);
-    if (dy) hipLaunchKernelGGL(loss_grad_kernel, dim3(eblocks(n)), dim3(kEB), 0, s, flags, res, red_.p, nx_, ny_, nz_, dy);
+    hipLaunchKernelGGL(loss_finish_kernel, dim3(1), dim3(64), 0, s, red_.p, mode);
+    if (dy)
+        hipLaunchKernelGGL(loss_grad_kernel, dim3(eblocks(n)), dim3(kEB), 0, s, flags, res, red_.p, nx_, ny_, nz_, mode,
+                           dy);
     hipError_t e;
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    if (loss) {
-        if ((e = hipMemcpyAsync(host_red_ + kRedLossNum, red_.p + kRedLossNum, 2 * sizeof(double), hipMemcpyDeviceToHost, s)) != hipSuccess)
+    if (loss || residual) {
+        if ((e = hipMemcpyAsync(host_red_ + kRedLoss, red_.p + kRedLoss, (kRedResidual - kRedLoss + 1) * sizeof(double),
+                                hipMemcpyDeviceToHost, s)) != hipSuccess)
             return e;
         if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
-        *loss = host_red_[kRedLossDen] > 0 ? host_red_[kRedLossNum] / host_red_[kRedLossDen] : 0.0;
+        if (loss) *loss = host_red_[kRedLoss];
+        if (residual) *residual = host_red_[kRedResidual];
     }
     return hipSuccess;
 }

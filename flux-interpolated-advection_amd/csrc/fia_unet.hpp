@@ -81,11 +81,13 @@ public:
     // network; y (FP32) receives the output in units of scale; *scale the normalisation
     // sqrt(mean rhs^2 over fluid cells), computed on the device and read back.
     hipError_t forward(const double *rhs, const uint8_t *solid, float *y, double *scale, hipStream_t s);
-    // residual-norm loss of the last forward pass against rhs / scale on the fluid cells
-    // (Neumann Laplacian, no pinned cell): loss = |L y - b|^2 / |b|^2, and its gradient dy;
-    // flags: per-cell neighbour flags (bits 0-5) of the fluid cells.
-    hipError_t loss_residual(const uint8_t *flags, const double *rhs, const float *y, float *dy, double *loss,
-                             hipStream_t s);
+    // Loss of the last forward pass against b = rhs / scale on the fluid cells (L: their Neumann
+    // Laplacian, no pinned cell; flags: per-cell neighbour bits 0-5) and its gradient dy:
+    //   mode 0, residual norm: |L y - b|^2 / |b|^2
+    //   mode 1, energy:        (y.L y / 2 - b.y) / |b|^2, the A-norm error of y up to a constant
+    // *residual receives |L y - b|^2 / |b|^2 in both modes; loss, residual, dy may be null.
+    hipError_t loss(int mode, const uint8_t *flags, const double *rhs, const float *y, float *dy, double *loss,
+                    double *residual, hipStream_t s);
     // initial guess of the pressure solve from the last forward pass: x = scale (y - y[pin]) on the
     // unknowns of the pressure system (pcg_flags bit 7), 0 elsewhere
     hipError_t guess(const float *y, long pin, const uint8_t *pcg_flags, double *x, hipStream_t s);

@@ -230,16 +230,21 @@ fia_status fia_unet_get_params(fia_ctx *ctx, fia_unet *net, float *params, size_
  * in units of *scale = sqrt(mean rhs^2 over fluid cells); pointers on the device when on_device */
 fia_status fia_unet_forward(fia_ctx *ctx, fia_unet *net, int nx, int ny, int nz, const double *rhs,
                             const uint8_t *solid, float *y, double *scale, int on_device);
-/* residual-norm loss |L y - rhs/scale|^2 / |rhs/scale|^2 of one sample (L: Neumann Laplacian of
- * the fluid cells) and, when grad is not null, its gradient with respect to every parameter
- * (host array of fia_unet_param_count() floats); no update */
+/* Training losses of the network output y against b = rhs/scale on the fluid cells (L: their
+ * Neumann Laplacian, no pinned cell): the residual norm |L y - b|^2 / |b|^2, or the energy
+ * (y.L y / 2 - b.y) / |b|^2, which differs from half the squared A-norm error of y by a constant. */
+enum { FIA_UNET_LOSS_RESIDUAL = 0, FIA_UNET_LOSS_ENERGY = 1 };
+/* loss of one sample, its residual norm |L y - b|^2 / |b|^2 and, when grad is not null, the loss
+ * gradient with respect to every parameter (host array of fia_unet_param_count() floats); no update */
 fia_status fia_unet_loss_grad(fia_ctx *ctx, fia_unet *net, int nx, int ny, int nz, const double *rhs,
-                              const uint8_t *solid, double *loss, float *grad, int on_device);
+                              const uint8_t *solid, int loss_mode, double *loss, double *residual, float *grad,
+                              int on_device);
 /* one Adam step (beta 0.9, 0.999, eps 1e-8) on the mean gradient of `batch` samples stored one
  * after another in rhs (device, batch*nx*ny*nz FP64) and solid (device, same count, optional);
- * *mean_loss receives the mean loss before the step */
+ * the batch means of loss and residual norm before the step are returned */
 fia_status fia_unet_train_step(fia_ctx *ctx, fia_unet *net, int nx, int ny, int nz, int batch,
-                               const double *rhs, const uint8_t *solid, float lr, double *mean_loss);
+                               const double *rhs, const uint8_t *solid, int loss_mode, float lr, double *mean_loss,
+                               double *mean_residual);
 
 /* The right-hand side b = -div(u) dx of the pressure system of grid (0 on obstacle cells and on
  * the pinned cell), as fia_pressure_project assembles it: training data for the network. */
@@ -248,18 +253,29 @@ fia_status fia_pressure_rhs(fia_ctx *ctx, fia_grid3d *grid, const uint8_t *solid
 typedef struct fia_learned_info {
     int used_network;            /* 1: continued from the network's guess; 0: fell back to zero guess */
     int probe_iters;             /* iterations run from both starts before choosing                     */
-    double guess_rel_residual;   /* |b - L x0| / |b| of the network's guess                             */
+    double guess_scale;          /* energy-optimal factor x0.b / x0.L x0 applied to the network's guess  */
+    double guess_rel_residual;   /* |b - L x0| / |b| of the (scaled) guess                                */
     double probe_res_network;    /* relative residual after the probe from the network's guess          */
     double probe_res_zero;       /* ... and from a zero guess                                            */
 } fia_learned_info;
 
-/* fia_pressure_project with the learned warm start: the network's guess and a zero guess each run
- * probe iterations (5); the solve continues from the network's unless its residual is larger
- * (then from the zero guess).  Same system, tolerance and outputs as fia_pressure_project; iters
+/* fia_pressure_project with the learned warm start: the network's guess, scaled by the factor that
+ * minimises its energy error, and a zero guess each run probe iterations (5); the solve continues
+ * from the network's unless its residual is larger (then from the zero guess).  Same system, tolerance and outputs as fia_pressure_project; iters
  * counts the iterations of the solve that was continued. */
 fia_status fia_pressure_project_learned(fia_ctx *ctx, fia_grid3d *grid, const uint8_t *solid, float *pressure,
                                         fia_unet *net, double rel_tol, int max_iters, int *iters,
                                         double *rel_residual, fia_learned_info *info);
+
+/* Diagnostics written to the reference's *.dat series (src/main.cpp:518-597): totals[0] = total
+ * mass sum(rho) dx^3 (0 when grid->density is NULL), totals[1..3] = total momenta sum(u) dx^3,
+ * sum(v) dx^3, sum(w) dx^3; FP64 sums in a fixed order. */
+fia_status fia_grid_totals(fia_ctx *ctx, fia_grid3d *grid, double totals[4]);
+/* Momentum the walls take out through the pressure of one projection (src/calc_pressure_3d.cpp:
+ * 235-266): loss[0] = dx^2 (sum of p over the cells at x = nx-1 - sum over x = 0), likewise y, z.
+ * The reference adds it to Grid_3D::total_loss_of_momentum_{x,y,z} after every projection. */
+fia_status fia_wall_momentum_loss(fia_ctx *ctx, int nx, int ny, int nz, double dx, const float *pressure,
+                                  int on_device, double loss[3]);
 
 #ifdef __cplusplus
 }

@@ -12,6 +12,7 @@
 #define FIA_SMOKE_SIMULATION_HPP
 
 #include <cstdint>
+#include <fstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -32,6 +33,9 @@ public:
     std::vector<MY_FLOAT_TYPE> velocity_in_voxel_face_x, velocity_in_voxel_face_y, velocity_in_voxel_face_z;
     std::vector<MY_FLOAT_TYPE> pressure;
     std::vector<MY_FLOAT_TYPE> substance_density;
+    // momentum taken out by the walls through the pressure, summed over projections
+    // (reference include/grid_3d.h:52-54, src/calc_pressure_3d.cpp:235-266)
+    MY_FLOAT_TYPE total_loss_of_momentum_x = 0, total_loss_of_momentum_y = 0, total_loss_of_momentum_z = 0;
 
     Grid_3D(int nx, int ny, int nz, MY_FLOAT_TYPE cell_length)
         : Grid_num_x(nx), Grid_num_y(ny), Grid_num_z(nz), min_pos_x(0), min_pos_y(0), min_pos_z(0),
@@ -157,6 +161,33 @@ inline void calc_pressure_gradient_term_3D(Grid_3D &all_grid) {
     double res = 0.0;
     fia_shim::check(fia_pressure_project(fia_shim::context(), &g, nullptr, all_grid.pressure.data(), 0, 1e-6, 100000,
                                          &iters, &res));
+    double loss[3];
+    fia_shim::check(fia_wall_momentum_loss(fia_shim::context(), all_grid.Grid_num_x, all_grid.Grid_num_y,
+                                           all_grid.Grid_num_z, all_grid._cell_length, all_grid.pressure.data(), 0,
+                                           loss));
+    all_grid.total_loss_of_momentum_x += (MY_FLOAT_TYPE)loss[0];
+    all_grid.total_loss_of_momentum_y += (MY_FLOAT_TYPE)loss[1];
+    all_grid.total_loss_of_momentum_z += (MY_FLOAT_TYPE)loss[2];
+}
+
+// reference include/write_substance_density_data.h (src/write_substance_density_data.cpp:11-27):
+// density_data_path/substance_density_<file_number>.dat, "nx ny nz" then one value per line in
+// [x][y][z] order, iostream's default formatting
+inline void write_substance_density_data(Grid_3D &all_grid, int file_number, const std::string density_data_path) {
+    std::ofstream f(density_data_path + "/substance_density_" + std::to_string(file_number) + ".dat", std::ios::out);
+    f << all_grid.Grid_num_x << " " << all_grid.Grid_num_y << " " << all_grid.Grid_num_z << std::endl;
+    for (MY_FLOAT_TYPE v : all_grid.substance_density) f << v << "\n";
+    f.flush();
+}
+
+// Totals written each frame to the reference's *.dat series (src/main.cpp:518-597): total mass and
+// total momenta x, y, z (sums times the cell volume), computed on the GPU in FP64.
+inline void calc_total_mass_and_momentum_3D(Grid_3D &all_grid, double &total_mass, double &momentum_x,
+                                            double &momentum_y, double &momentum_z) {
+    fia_grid3d g = fia_shim::host_grid(all_grid);
+    double t[4];
+    fia_shim::check(fia_grid_totals(fia_shim::context(), &g, t));
+    total_mass = t[0]; momentum_x = t[1]; momentum_y = t[2]; momentum_z = t[3];
 }
 
 // reference include/calc_time_step_length_from_CFL_number.h:33-39

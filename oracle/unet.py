@@ -1,5 +1,5 @@
 """CPU restatement (numpy, FP64) of the learned pressure warm start: the 4-level 3-D U-Net, its
-residual-norm loss and its gradient.  Test infrastructure only: the tests compare the HIP network
+two training losses (residual norm, energy) and its gradient.  Test infrastructure only: the tests compare the HIP network
 against it, and it is never part of the product path.
 
 The reference has no learned component (its solver is amgcl, src/calc_pressure_3d.cpp:141-166),
@@ -9,8 +9,9 @@ vector, each 3x3x3 convolution W[tap][co][ci] (tap = (dx+1)*9 + (dy+1)*3 + (dz+1
 the 1x1 output layer w[16], b last.
 
 Pinning: `backward` is checked against central finite differences of `forward` + `loss` in FP64
-(directional derivatives, tests/test_oracle_unet.py); the loss is checked against the closed form
-(zero for the exact solution of the Neumann system).  With `bf16=True` the forward and backward
+(directional derivatives, tests/test_oracle_unet.py); the losses are checked against closed forms
+(zero residual for the exact solution of the Neumann system; the energy difference equals the
+A-norm of the error).  With `bf16=True` the forward and backward
 round to BF16 where the GPU stores BF16 (weights, input, activations, pre-activation gradients)
 so that the GPU's results can be compared at a tolerance set by FP32 summation order alone.
 """
@@ -157,17 +158,24 @@ def forward(params, rhs, solid=None, bf=False):
     return y, {"acts": acts, "ins": ins, "Ws": Ws, "wo": wo, "scale": scale, "b": b, "bf": bf}
 
 
-def loss(y, cache, flags):
-    """Residual-norm loss |L y - b|^2 / |b|^2 over fluid cells, and dL/dy."""
+def loss(y, cache, flags, mode="residual"):
+    """Training loss and dL/dy on the fluid cells, with b = rhs / scale and r = L y - b:
+    "residual": |r|^2 / |b|^2;  "energy": (y.L y / 2 - b.y) / |b|^2 = (y.r - b.y) / (2 |b|^2)."""
     fl = (flags & 63) != 0
-    r = np.where(fl, laplacian(y, flags) - cache["b"], 0.0)
+    b = np.where(fl, cache["b"], 0.0)
+    r = np.where(fl, laplacian(y, flags) - b, 0.0)
     if cache["bf"]:
         r = f32(r)
-    den = (np.where(fl, cache["b"], 0.0) ** 2).sum()
-    dy = (2.0 / den) * laplacian(r, flags)
+    den = (b ** 2).sum()
+    if mode == "residual":
+        dy = (2.0 / den) * laplacian(r, flags)
+        val = (r ** 2).sum() / den
+    else:
+        dy = np.where(fl, r / den, 0.0)
+        val = 0.5 * ((np.where(fl, y, 0.0) * r).sum() - (b * y).sum()) / den
     if cache["bf"]:
         dy = f32(dy)
-    return (r ** 2).sum() / den, dy
+    return val, dy
 
 
 def backward(params, cache, dy):

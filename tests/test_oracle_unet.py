@@ -7,13 +7,13 @@ import pytest
 import unet
 
 
-def fd_error(p, d, rhs, solid, flags, g):
+def fd_error(p, d, rhs, solid, flags, g, mode="residual"):
     """Smallest relative error of central differences at two step sizes against g . d (a ReLU input
     that changes sign inside the larger step spoils that one; a wrong gradient fails both)."""
     errs = []
     for h in (1e-7, 1e-8):
-        lp = unet.loss(*unet.forward(p + h * d, rhs, solid)[:2], flags)[0]
-        lm = unet.loss(*unet.forward(p - h * d, rhs, solid)[:2], flags)[0]
+        lp = unet.loss(*unet.forward(p + h * d, rhs, solid)[:2], flags, mode)[0]
+        lm = unet.loss(*unet.forward(p - h * d, rhs, solid)[:2], flags, mode)[0]
         fd = (lp - lm) / (2 * h)
         errs.append(abs(fd - g @ d) / max(abs(fd), 1e-12))
     return min(errs)
@@ -30,19 +30,20 @@ def scene(n, seed, obstacles=True):
     return rhs, solid
 
 
+@pytest.mark.parametrize("mode", ["residual", "energy"])
 @pytest.mark.parametrize("obstacles", [False, True])
-def test_gradient_matches_finite_differences(obstacles):
+def test_gradient_matches_finite_differences(obstacles, mode):
     n = 8
     rhs, solid = scene(n, 3, obstacles)
     p = unet.init_params(seed=5).astype(np.float64)
     flags = unet.fluid_flags(solid, rhs.shape)
     y, cache = unet.forward(p, rhs, solid)
-    L0, dy = unet.loss(y, cache, flags)
+    L0, dy = unet.loss(y, cache, flags, mode)
     g = unet.backward(p, cache, dy)
     rng = np.random.default_rng(11)
     for _ in range(3):
         d = rng.normal(size=p.size)
-        assert fd_error(p, d, rhs, solid, flags, g) < 1e-6
+        assert fd_error(p, d, rhs, solid, flags, g, mode) < 1e-6
 
 
 def test_gradient_of_single_blocks():
@@ -76,6 +77,20 @@ def test_loss_is_zero_at_the_solution():
     assert unet.loss(x + 3.0, cache, flags)[0] < 1e-20
     # a wrong answer has the expected loss: y = 0 gives exactly 1
     assert abs(unet.loss(np.zeros_like(x), cache, flags)[0] - 1.0) < 1e-14
+
+
+def test_energy_loss_is_the_a_norm_error():
+    # energy(y) - energy(x) = |y - x|_A^2 / (2 |b|^2) for the solution x of L x = b
+    n = 8
+    rng = np.random.default_rng(2)
+    x = rng.normal(size=(n, n, n))
+    flags = unet.fluid_flags(None, x.shape)
+    b = unet.laplacian(x, flags)
+    cache = {"b": b, "bf": False}
+    y = x + 0.3 * rng.normal(size=x.shape)
+    e = y - x
+    lhs = unet.loss(y, cache, flags, "energy")[0] - unet.loss(x, cache, flags, "energy")[0]
+    assert abs(lhs - (e * unet.laplacian(e, flags)).sum() / (2 * (b ** 2).sum())) < 1e-12
 
 
 def test_laplacian_matches_assembled_matrix():
