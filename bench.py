@@ -10,6 +10,10 @@ psi (Gauss n=1), zero-velocity correction on.  Fields stay resident on the GPU.
   projection  one step = one pressure projection (Jacobi-PCG to 1e-6 relative residual, zero
               initial guess) of the curl-noise velocity plus a buoyancy impulse, 8 spherical
               obstacles as a solid mask
+  scenes      one step = one step of 64 independent plume scenes (edges from {32,48,64,80,96},
+              seeded sources, buoyancy, 0-4 obstacles, CFL 1-3): forcing, velocity advection,
+              projection (U-Net warm start unless --plain) and density advection, on the number
+              of concurrent workers the autotuner picks; value = all cells x steps / time
   warmstart   one step = the pressure projection of one plume step (128^3 unless --n; seeded
               source, buoyancy, 0-8 obstacles; scenes not used in training) twice on the same
               velocity: from a zero guess and with the learned U-Net warm start (committed
@@ -243,6 +247,50 @@ def run_warmstart(args, fia, scenes, torch):
     }
 
 
+def run_scenes(args, fia, torch):
+    batch = importlib.import_module("flux-interpolated-advection_amd.batch")
+    sset = batch.SceneSet(args.scenes, seed=5)
+    candidates = (1, 2, 4, 8)
+    ctxs = [fia.Context(0) for _ in range(max(candidates))]
+    nets = None
+    if not args.plain:
+        nets = []
+        for _ in range(max(candidates)):
+            u = fia.UNet(ctxs[0], seed=0)
+            u.load(args.weights)
+            nets.append(u)
+    od = fia.make_opts(split_method="xz-CellFaceAligned", interpolation_method="WENO6")
+    ov = fia.make_opts(interpolation_method="WENO6")
+    # the autotuner's steps are the warm-up
+    best, tune = batch.autotune_workers(sset, ctxs, od, ov, candidates, nets=nets, log=lambda s: None)
+    for _ in range(args.warmup):
+        sset.step(ctxs[:best], od, ov, nets=nets[:best] if nets else None)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sset.step(ctxs[:best], od, ov, nets=nets[:best] if nets else None)
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    st = sset.stats()
+    for u in nets or []:
+        u.close()
+    for c in ctxs:
+        c.close()
+    return {
+        "metric": METRIC, "value": round(sset.cells * args.steps / sec / 1e6, 3), "unit": "Mcell-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup + len(candidates),
+        "ms_per_step": round(sec / args.steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 advection, f64 solve", "data": "synthetic",
+        "config": {"workload": "%d independent plume scenes, edges %s (%d cells in all), CFL 1-3, 0-4 obstacles; "
+                               "one step = forcing, velocity self-advection, projection to 1e-6 (%s), density "
+                               "advection; WENO6" % (args.scenes, sorted(set(sset.edges)), sset.cells,
+                                                     "plain PCG" if args.plain else "U-Net warm start"),
+                   "scenes": args.scenes, "cells": sset.cells},
+        "scheduler": {"workers": best, "autotune_ms_per_step": {str(k): round(v * 1e3, 1) for k, v in tune.items()}},
+        "pcg": st,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -250,7 +298,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=None, help="grid edge (256; warmstart: 128)")
     ap.add_argument("--cfl", type=float, default=4.0)
-    ap.add_argument("--workload", choices=("density", "velocity", "projection", "warmstart"), default="density")
+    ap.add_argument("--workload", choices=("density", "velocity", "projection", "warmstart", "scenes"),
+                    default="density")
+    ap.add_argument("--scenes", type=int, default=64)
+    ap.add_argument("--plain", action="store_true", help="scenes: plain PCG instead of the warm start")
     ap.add_argument("--weights", default=os.path.join(ROOT, "flux-interpolated-advection_amd", "weights",
                                                       "unet_plume128.npy"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -278,6 +329,8 @@ def main():
         out = run_projection(args, fia, scenes, torch)
     elif args.workload == "warmstart":
         out = run_warmstart(args, fia, scenes, torch)
+    elif args.workload == "scenes":
+        out = run_scenes(args, fia, torch)
     else:
         out = run_advection(args, fia, scenes, torch, args.workload == "velocity")
     print(json.dumps(out))

@@ -39,7 +39,8 @@ EXPORTS = ["fia_default_opts", "fia_ctx_create", "fia_ctx_destroy", "fia_ctx_dev
            "fia_pressure_project", "fia_ctx_set_timing", "fia_ctx_get_stats", "fia_ctx_reset_stats",
            "fia_unet_param_count", "fia_unet_create", "fia_unet_destroy", "fia_unet_set_params",
            "fia_unet_get_params", "fia_unet_forward", "fia_unet_loss_grad", "fia_unet_train_step", "fia_pressure_rhs",
-           "fia_pressure_project_learned", "fia_grid_totals", "fia_wall_momentum_loss"]
+           "fia_pressure_project_learned", "fia_grid_totals", "fia_wall_momentum_loss", "fia_scene_forcing",
+           "fia_scenes_step"]
 
 
 class FiaError(RuntimeError):
@@ -71,6 +72,16 @@ class LearnedInfo(ctypes.Structure):
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class SceneC(ctypes.Structure):
+    """fia_scene (include/fia.h)."""
+    _fields_ = [("grid", GridC), ("solid", ctypes.c_void_p), ("pressure", ctypes.c_void_p),
+                ("source_x", ctypes.c_double), ("source_y", ctypes.c_double), ("source_z", ctypes.c_double),
+                ("source_radius", ctypes.c_double), ("source_value", ctypes.c_double), ("buoyancy", ctypes.c_double),
+                ("cfl", ctypes.c_int), ("movie_fps", ctypes.c_double), ("time", ctypes.c_double), ("frame", ctypes.c_int),
+                ("dt", ctypes.c_double), ("pcg_iters", ctypes.c_int), ("pcg_residual", ctypes.c_double),
+                ("used_network", ctypes.c_int), ("steps", ctypes.c_int), ("status", ctypes.c_int)]
 
 
 class KernelStats(ctypes.Structure):
@@ -132,6 +143,9 @@ def lib():
         L.fia_pressure_rhs.argtypes = [_vp, ctypes.POINTER(GridC), _vp, _vp]
         L.fia_grid_totals.argtypes = [_vp, ctypes.POINTER(GridC), ctypes.POINTER(_d * 4)]
         L.fia_wall_momentum_loss.argtypes = [_vp, _i, _i, _i, _d, _vp, _i, ctypes.POINTER(_d * 3)]
+        L.fia_scene_forcing.argtypes = [_vp, ctypes.POINTER(SceneC), _d]
+        L.fia_scenes_step.argtypes = [ctypes.POINTER(_vp), _i, ctypes.POINTER(SceneC), _i, ctypes.POINTER(AdvectOpts),
+                                      ctypes.POINTER(AdvectOpts), _d, _i, ctypes.POINTER(_vp)]
         L.fia_pressure_project_learned.argtypes = [_vp, ctypes.POINTER(GridC), _vp, _vp, _vp, _d, _i, ctypes.POINTER(_i),
                                                    ctypes.POINTER(_d), ctypes.POINTER(LearnedInfo)]
         _lib = L
@@ -529,3 +543,14 @@ def wall_momentum_loss(ctx, grid, pressure=None):
     out = (_d * 3)()
     _check(lib().fia_wall_momentum_loss(ctx._h, grid.nx, grid.ny, grid.nz, grid.dx, pp, on, ctypes.byref(out)))
     return tuple(out)
+
+
+# ---- many independent scenes (no counterpart in the reference) -------------------------------
+
+def scenes_step(ctxs, scenes, density_opts, velocity_opts, rel_tol=1e-6, max_iters=20000, nets=None):
+    """One step of every scene (a ctypes array of SceneC) on len(ctxs) workers."""
+    w = len(ctxs)
+    ca = (_vp * w)(*[c._h for c in ctxs])
+    na = (_vp * w)(*[n._h for n in nets]) if nets is not None else None
+    _check(lib().fia_scenes_step(ca, w, scenes, len(scenes), ctypes.byref(density_opts), ctypes.byref(velocity_opts),
+                                 float(rel_tol), int(max_iters), na))

@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libfia.so")
-SOURCES = ["fia_kernels.hip", "fia_pcg.hip", "fia_unet.hip", "fia_io.hip", "fia_api.hip"]
+SOURCES = ["fia_kernels.hip", "fia_pcg.hip", "fia_unet.hip", "fia_io.hip", "fia_batch.hip", "fia_api.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("FIA_OFFLOAD_ARCH", "gfx950")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,

@@ -124,6 +124,12 @@ struct fia_unet {
     DevBuf<double> rhs;
 };
 
+namespace fia {
+int ctx_device(const fia_ctx *c) { return c->device; }
+hipStream_t ctx_stream(const fia_ctx *c) { return c->stream; }
+fia_status set_error(fia_status status, const std::string &msg) { return fail(status, msg); }
+}  // namespace fia
+
 namespace {
 
 size_t ctx_bytes(const fia_ctx *c) {

@@ -1,0 +1,153 @@
+// fia_batch.hip - many independent scenes of uneven sizes advanced together (no counterpart in
+// the reference, which runs one grid per process).
+//
+// A step of a scene is the order of the reference's main loop (src/main.cpp:596-740): dt from
+// the CFL number, forcing (a density source and buoyancy on the vertical velocity: the scene's
+// own terms, the reference has none), velocity self-advection, pressure projection, density
+// advection.  Scenes are independent, so they go to worker threads, each owning one library
+// context (one HIP stream and its scratch): small grids are launch- and sync-bound, and the
+// streams of different workers overlap on the GPU.  Scenes are taken largest first from a shared
+// counter (longest-processing-time order), so the step ends with small scenes.
+#include <algorithm>
+#include <atomic>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/fia.h"
+#include "fia_internal.hpp"
+#include "fia_math.hpp"
+
+namespace fia {
+
+// rho = max(rho, value) inside the source sphere (cell centres)
+__global__ void forcing_kernel(int nx, int ny, int nz, double h, float *rho, double cx, double cy, double cz, double r,
+                               double value) {
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t n = (size_t)nx * ny * nz;
+    if (c >= n) return;
+    const int k = (int)(c % nz), j = (int)((c / nz) % ny), i = (int)(c / ((size_t)ny * nz));
+    const double x = (i + 0.5) * h - cx, y = (j + 0.5) * h - cy, z = (k + 0.5) * h - cz;
+    if (r > 0.0 && x * x + y * y + z * z <= r * r) rho[c] = fmaxf(rho[c], (float)value);
+}
+
+// v += dt * beta * (rho averaged to the y face) on interior y faces (y is up)
+__global__ void buoyancy_kernel(int nx, int ny, int nz, const float *rho, float *v, double dt_beta) {
+    const size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t n = (size_t)nx * (ny + 1) * nz;
+    if (f >= n) return;
+    const int k = (int)(f % nz), j = (int)((f / nz) % (ny + 1)), i = (int)(f / ((size_t)(ny + 1) * nz));
+    if (j == 0 || j == ny) return;
+    const double avg = 0.5 * ((double)rho[idx3(i, j, k, ny, nz)] + (double)rho[idx3(i, j - 1, k, ny, nz)]);
+    v[f] = (float)((double)v[f] + dt_beta * avg);
+}
+
+hipError_t launch_forcing(const fia_grid3d &g, double cx, double cy, double cz, double r, double value, double dt_beta,
+                          hipStream_t s) {
+    const size_t n = (size_t)g.nx * g.ny * g.nz, nv = (size_t)g.nx * (g.ny + 1) * g.nz;
+    hipLaunchKernelGGL(forcing_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g.nx, g.ny, g.nz, g.dx,
+                       g.density, cx, cy, cz, r, value);
+    if (dt_beta != 0.0)
+        hipLaunchKernelGGL(buoyancy_kernel, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, s, g.nx, g.ny, g.nz,
+                           g.density, g.v, dt_beta);
+    return hipGetLastError();
+}
+
+}  // namespace fia
+
+namespace {
+
+// one scene step on one context; returns the first failing status
+fia_status scene_step(fia_ctx *c, fia_scene *sc, const fia_advect_opts *od, const fia_advect_opts *ov, double rel_tol,
+                      int max_iters, fia_unet *net) {
+    fia_grid3d *g = &sc->grid;
+    if (!g->on_device || !g->density) return fia::set_error(FIA_ERR_INVALID_ARG, "a scene needs a device grid with density");
+    fia_status st;
+    double dt = 0.0;
+    int write = 0;
+    if ((st = fia_cfl_time_step(c, g, sc->time, sc->frame, sc->cfl, sc->movie_fps, &dt, &write)) != FIA_OK) return st;
+    if ((st = fia_scene_forcing(c, sc, dt)) != FIA_OK) return st;
+    if ((st = fia_advect_velocity_3d(c, g, -1, dt, ov)) != FIA_OK) return st;
+    int iters = 0;
+    double res = 0.0;
+    if (net) {
+        fia_learned_info info;
+        st = fia_pressure_project_learned(c, g, sc->solid, sc->pressure, net, rel_tol, max_iters, &iters, &res, &info);
+        sc->used_network = info.used_network;
+    } else {
+        st = fia_pressure_project(c, g, sc->solid, sc->pressure, 0, rel_tol, max_iters, &iters, &res);
+        sc->used_network = 0;
+    }
+    if (st != FIA_OK) return st;
+    if ((st = fia_move_substances_3d(c, g, sc->frame, dt, od)) != FIA_OK) return st;
+    sc->dt = dt;
+    sc->pcg_iters = iters;
+    sc->pcg_residual = res;
+    sc->time += dt;
+    if (write) sc->frame += 1;
+    sc->steps += 1;
+    return FIA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+fia_status fia_scene_forcing(fia_ctx *c, fia_scene *sc, double dt) {
+    if (!c || !sc) return fia::set_error(FIA_ERR_INVALID_ARG, "ctx or scene is NULL");
+    const fia_grid3d &g = sc->grid;
+    if (!g.on_device || !g.density || !g.v || g.nx <= 0 || g.ny <= 0 || g.nz <= 0)
+        return fia::set_error(FIA_ERR_INVALID_ARG, "a scene needs a device grid with density and velocity");
+    if (hipSetDevice(fia::ctx_device(c)) != hipSuccess) return fia::set_error(FIA_ERR_DEVICE, "hipSetDevice failed");
+    const hipError_t e = fia::launch_forcing(g, sc->source_x, sc->source_y, sc->source_z, sc->source_radius,
+                                             sc->source_value, dt * sc->buoyancy, fia::ctx_stream(c));
+    if (e != hipSuccess) return fia::set_error(FIA_ERR_DEVICE, std::string("forcing: ") + hipGetErrorString(e));
+    return FIA_OK;
+}
+
+fia_status fia_scenes_step(fia_ctx *const *ctxs, int workers, fia_scene *scenes, int n_scenes,
+                           const fia_advect_opts *density_opts, const fia_advect_opts *velocity_opts, double rel_tol,
+                           int max_iters, fia_unet *const *nets) {
+    if (!ctxs || workers <= 0 || (!scenes && n_scenes > 0) || n_scenes < 0 || !density_opts || !velocity_opts)
+        return fia::set_error(FIA_ERR_INVALID_ARG, "bad argument");
+    for (int w = 0; w < workers; ++w)
+        if (!ctxs[w]) return fia::set_error(FIA_ERR_INVALID_ARG, "NULL context");
+    // largest scenes first
+    std::vector<int> order(n_scenes);
+    for (int i = 0; i < n_scenes; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        const fia_grid3d &ga = scenes[a].grid, &gb = scenes[b].grid;
+        return (long long)ga.nx * ga.ny * ga.nz > (long long)gb.nx * gb.ny * gb.nz;
+    });
+    std::atomic<int> next{0};
+    std::vector<fia_status> status(workers, FIA_OK);
+    std::vector<std::string> errors(workers);
+    auto run = [&](int w) {
+        while (true) {
+            const int k = next.fetch_add(1);
+            if (k >= n_scenes) return;
+            fia_scene *sc = &scenes[order[k]];
+            const fia_status st = scene_step(ctxs[w], sc, density_opts, velocity_opts, rel_tol, max_iters,
+                                             nets ? nets[w] : nullptr);
+            sc->status = st;
+            if (st != FIA_OK && status[w] == FIA_OK) {
+                status[w] = st;
+                errors[w] = fia_last_error();
+            }
+        }
+    };
+    const int nthreads = std::min(workers, std::max(n_scenes, 1));
+    if (nthreads == 1) {
+        run(0);
+    } else {
+        std::vector<std::thread> pool;
+        for (int w = 0; w < nthreads; ++w) pool.emplace_back(run, w);
+        for (auto &t : pool) t.join();
+    }
+    // the workers' messages are thread-local to them: hand the first one to the caller's thread
+    for (int w = 0; w < nthreads; ++w)
+        if (status[w] != FIA_OK) return fia::set_error(status[w], errors[w]);
+    return FIA_OK;
+}
+
+}  // extern "C"

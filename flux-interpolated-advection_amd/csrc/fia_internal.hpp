@@ -5,6 +5,9 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <string>
+
+#include "../../include/fia.h"
 
 namespace fia {
 
@@ -183,6 +186,14 @@ hipError_t launch_pcg_check(const PcgState &st, hipStream_t s);
 hipError_t launch_project(const PcgGrid &g, const double *p, double h, float *u, float *v, float *w,
                           float *pressure_out, hipStream_t s);
 hipError_t launch_zero_solid_faces(const PcgGrid &g, float *u, float *v, float *w, hipStream_t s);
+
+// context accessors and the calling thread's error message (fia_api.hip), for other host files
+}  // namespace fia
+struct fia_ctx;
+namespace fia {
+int ctx_device(const fia_ctx *c);
+hipStream_t ctx_stream(const fia_ctx *c);
+fia_status set_error(fia_status status, const std::string &msg);   // returns status
 
 // diagnostics (fia_io.hip): FP64 sums of up to four fields (out[0..3]) and of the pressure over
 // the six boundary layers of cells (out[0..5]); part holds io_partial_words() doubles

@@ -277,6 +277,41 @@ fia_status fia_grid_totals(fia_ctx *ctx, fia_grid3d *grid, double totals[4]);
 fia_status fia_wall_momentum_loss(fia_ctx *ctx, int nx, int ny, int nz, double dx, const float *pressure,
                                   int on_device, double loss[3]);
 
+/* ---- Many independent scenes ------------------------------------------------------------------
+ * No counterpart in the reference, which runs one grid per process.  A scene is a device grid with
+ * its own forcing and clock; one step follows the order of the reference's main loop
+ * (src/main.cpp:596-740): dt from the CFL number, forcing, velocity self-advection (xyz), pressure
+ * projection, density advection. */
+typedef struct fia_scene {
+    fia_grid3d grid;             /* device fields (on_device = 1), density required                   */
+    const uint8_t *solid;        /* device obstacle mask or NULL                                      */
+    float *pressure;             /* device nx*ny*nz floats receiving p, or NULL                       */
+    double source_x, source_y, source_z, source_radius, source_value;   /* density source sphere   */
+    double buoyancy;             /* v += dt * buoyancy * rho (y is up)                                */
+    int cfl;                     /* CFL number (an int, as the reference's)                           */
+    double movie_fps;            /* frame clipping of dt                                              */
+    double time;                 /* in/out: simulated time                                            */
+    int frame;                   /* in/out: movie frame                                               */
+    /* outputs of the last step */
+    double dt;
+    int pcg_iters;
+    double pcg_residual;
+    int used_network;
+    int steps;                   /* steps taken                                                       */
+    fia_status status;
+} fia_scene;
+
+/* The forcing of one scene for a step of length dt: density = max(density, source_value) inside
+ * the source sphere; v += dt * buoyancy * (density averaged to the y face) on interior y faces. */
+fia_status fia_scene_forcing(fia_ctx *ctx, fia_scene *scene, double dt);
+/* One step of every scene.  workers threads, each with its own context (ctxs[w]; one stream and
+ * scratch each) and, for the learned warm start, its own network (nets[w], or nets NULL for plain
+ * PCG), take scenes largest first.  Scenes are independent, so the result does not depend on
+ * workers.  Returns the first failure (every scene's own status is in scene->status). */
+fia_status fia_scenes_step(fia_ctx *const *ctxs, int workers, fia_scene *scenes, int n_scenes,
+                           const fia_advect_opts *density_opts, const fia_advect_opts *velocity_opts,
+                           double rel_tol, int max_iters, fia_unet *const *nets);
+
 #ifdef __cplusplus
 }
 #endif

@@ -66,3 +66,26 @@ def test_cpp_shim_header_compiles():
     r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"), "-x", "c++", "-"],
                        input=src, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_struct_layouts_match_c(fia, tmp_path):
+    # every struct the ctypes mirror declares has the C header's size and field offsets
+    import ctypes
+    pairs = [("fia_grid3d", fia.GridC), ("fia_advect_opts", fia.AdvectOpts), ("fia_kernel_stats", fia.KernelStats),
+             ("fia_learned_info", fia.LearnedInfo), ("fia_scene", fia.SceneC)]
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "fia.h"', 'int main(void) {']
+    for cname, py in pairs:
+        lines.append('printf("%%zu\\n", sizeof(%s));' % cname)
+        for f, _ in py._fields_:
+            lines.append('printf("%%zu\\n", offsetof(%s, %s));' % (cname, f))
+    lines.append('return 0; }')
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    want = []
+    for _, py in pairs:
+        want.append(ctypes.sizeof(py))
+        want += [getattr(py, f).offset for f, _ in py._fields_]
+    assert got == want
