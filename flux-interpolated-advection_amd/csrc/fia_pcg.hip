@@ -461,19 +461,26 @@ __global__ void pcg_guess_kernel(PcgState st, const float *p_in) {
 
 static inline unsigned blocks(size_t n) { return (unsigned)((n + kPB - 1) / kPB); }
 
-// a multiple of 8 (one share per XCD)
+// Blocks of the start and update kernels: about kPcgCellsPerBlock cells each (every block of
+// the next kernel re-sums all partials, so small grids get few blocks), at most kPcgMaxBlocks,
+// a multiple of 8 (one share per XCD).
+constexpr size_t kPcgCellsPerBlock = 2048;
+static size_t target_blocks(size_t n) {
+    return std::min<size_t>(kPcgMaxBlocks, std::max<size_t>(8, (n + kPcgCellsPerBlock - 1) / kPcgCellsPerBlock));
+}
 int pcg_blocks(size_t n) {
-    const size_t b = std::min<size_t>(kPcgMaxBlocks, std::max<size_t>(1, blocks(n)));
+    const size_t b = target_blocks(n);
     return (int)((b + 7) / 8 * 8);
 }
 
 // Tiling of the direction kernel: kTileZ x kTileY column tiles, x cut into chunks so that the
-// launch has about kPcgMaxBlocks blocks (rounded up to a multiple of 8).
+// launch has about as many blocks as target_blocks (rounded up to a multiple of 8).
 void pcg_direction_tiling(int nx, int ny, int nz, PcgState &st) {
     st.dir_tiles_y = (ny + kTileY - 1) / kTileY;
     st.dir_tiles_z = (nz + kTileZ - 1) / kTileZ;
     const long tiles = (long)st.dir_tiles_y * st.dir_tiles_z;
-    const long want = std::max<long>(1, std::min<long>(nx, ((long)kPcgMaxBlocks + tiles - 1) / tiles));
+    const long tb = (long)target_blocks((size_t)nx * ny * nz);
+    const long want = std::max<long>(1, std::min<long>(nx, (tb + tiles - 1) / tiles));
     st.dir_chunk_x = (int)((nx + want - 1) / want);
     st.dir_chunks = (nx + st.dir_chunk_x - 1) / st.dir_chunk_x;
     const long nb = tiles * st.dir_chunks;
