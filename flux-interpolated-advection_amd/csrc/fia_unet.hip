@@ -2,16 +2,17 @@
 // fia_unet.hpp).  No counterpart in the reference.
 //
 // 3x3x3 convolutions are implicit GEMMs on the matrix cores: rows = voxels, columns = output
-// channels, K = 27 taps x input channels, one v_mfma_f32_16x16x16_bf16 per 16 voxels x 16
-// output channels x 16 input channels.  A wave owns 64 voxels (4 row blocks) and all output
-// channels, so every A fragment (4 channels of one voxel, 8 bytes from the channels-last
-// activations) feeds all column blocks.  The data gradient is the same kernel on the flipped,
-// transposed weights.  The weight gradient stages 64-voxel tiles of the pre-activation gradient
+// channels, K = 27 taps x input channels.  A block owns a 4 x 4 x 16 voxel tile; its input plus a
+// one-voxel halo is staged in LDS 32 channels at a time and read by all 27 taps, one
+// v_mfma_f32_16x16x32_bf16 per 16 voxels x 16 output channels x 32 input channels.  A wave owns
+// 64 voxels (4 row blocks) and all output channels, so every A fragment feeds all column blocks.
+// The data gradient is the same kernel on the flipped, transposed weights.  The weight gradient stages 64-voxel tiles of the pre-activation gradient
 // and of the tap-shifted input in LDS and accumulates per tap and voxel chunk; chunks are then
 // summed in a fixed order (no atomics: training is deterministic).
 //
-// MFMA operand layout (16x16x16 BF16, 64 lanes): lane l holds A[l % 16][4 (l / 16) + i] and
-// B[4 (l / 16) + i][l % 16] for i = 0..3, and D[4 (l / 16) + i][l % 16].
+// MFMA operand layouts (64 lanes): 16x16x16 BF16: lane l holds A[l % 16][4 (l / 16) + i] and
+// B[4 (l / 16) + i][l % 16] for i < 4; 16x16x32 BF16: A[l % 16][8 (l / 16) + i] and
+// B[8 (l / 16) + i][l % 16] for i < 8; both: D[4 (l / 16) + i][l % 16] for i < 4.
 #include <math.h>
 #include <string.h>
 
@@ -75,60 +76,122 @@ __device__ __forceinline__ bf16x4 load4(const ConvArgs &a, int x, int y, int z, 
 
 constexpr int kCB = 256;   // threads per convolution block (4 waves x 64 voxels)
 
+// Block tile of the convolution: 4 (x) x 4 (y) x 16 (z) voxels; wave w owns x-plane w of the tile,
+// its 4 row blocks are the tile's 4 y-rows of 16 z-voxels.  The input of the tile plus a one-voxel
+// halo (6 x 6 x 18 voxels) is staged in LDS for 32 input channels at a time and read by all 27
+// taps.  Full 32-channel slices use v_mfma_f32_16x16x32_bf16 (lane l: A[l % 16][8 (l / 16) + i],
+// B[8 (l / 16) + i][l % 16], i < 8); a 16-channel tail uses the 16x16x16 form.
+constexpr int kTX = 4, kTY = 4, kTZ = 16;
+constexpr int kHX = kTX + 2, kHY = kTY + 2, kHZ = kTZ + 2, kHalo = kHX * kHY * kHZ;
+constexpr int kSlice = 32, kSliceStride = kSlice + 8;   // LDS row: 32 channels + padding (bank spread)
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef short i16x8 __attribute__((ext_vector_type(8)));
+
+// eight channels [c, c + 8) of a convolution's input at an in-range voxel (c % 8 == 0: an 8-group
+// never straddles the two sources, whose channel counts are multiples of 16)
+__device__ __forceinline__ i16x8 load8(const ConvArgs &a, int x, int y, int z, int c) {
+    const uint16_t *p;
+    if (c < a.cA)
+        p = a.upA ? a.srcA + vox(x >> 1, y >> 1, z >> 1, a.ny >> 1, a.nz >> 1) * a.cA + c
+                  : a.srcA + vox(x, y, z, a.ny, a.nz) * a.cA + c;
+    else
+        p = a.srcB + vox(x, y, z, a.ny, a.nz) * a.cB + (c - a.cA);
+    return *reinterpret_cast<const i16x8 *>(p);
+}
+
+// Stages input channels [c0, c0 + 8 groups) of the block's halo tile in LDS.
+__device__ __forceinline__ void stage_slice(const ConvArgs &a, uint16_t *tile, int x0, int y0, int z0, int c0,
+                                            int groups) {
+    for (int i = threadIdx.x; i < kHalo * groups; i += kCB) {
+        const int g = i % groups, v = i / groups;
+        const int hz = v % kHZ, hy = (v / kHZ) % kHY, hx = v / (kHZ * kHY);
+        const int x = x0 - 1 + hx, y = y0 - 1 + hy, z = z0 - 1 + hz;
+        i16x8 val = i16x8{0, 0, 0, 0, 0, 0, 0, 0};
+        if (x >= 0 && x < a.nx && y >= 0 && y < a.ny && z >= 0 && z < a.nz) val = load8(a, x, y, z, c0 + 8 * g);
+        *reinterpret_cast<i16x8 *>(&tile[v * kSliceStride + 8 * g]) = val;
+    }
+}
+
+// All 27 taps over one staged slice: K = 32 channels (FULL, 16x16x32) or 16 (16x16x16).
+template <int NB, bool FULL>
+__device__ __forceinline__ void slice_taps(const ConvArgs &a, const uint16_t *tile, int c0, int cin, int wave, int row,
+                                           int grp, f32x4 (&acc)[4][NB]) {
+    for (int t = 0; t < 27; ++t) {
+        const int dx = t / 9, dy = (t / 3) % 3, dz = t % 3;   // halo offsets 0..2
+        const uint16_t *wt = a.w + (size_t)t * a.cout * cin + c0;
+        if (FULL) {
+            bf16x8 A[4];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const int v = ((wave + dx) * kHY + (m + dy)) * kHZ + (row + dz);
+                A[m] = *reinterpret_cast<const bf16x8 *>(&tile[v * kSliceStride + 8 * grp]);
+            }
+#pragma unroll
+            for (int n = 0; n < NB; ++n) {
+                const bf16x8 B = *reinterpret_cast<const bf16x8 *>(wt + (size_t)(n * 16 + row) * cin + 8 * grp);
+#pragma unroll
+                for (int m = 0; m < 4; ++m)
+                    acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[m], B, acc[m][n], 0, 0, 0);
+            }
+        } else {
+            bf16x4 A[4];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const int v = ((wave + dx) * kHY + (m + dy)) * kHZ + (row + dz);
+                A[m] = *reinterpret_cast<const bf16x4 *>(&tile[v * kSliceStride + 4 * grp]);
+            }
+#pragma unroll
+            for (int n = 0; n < NB; ++n) {
+                const bf16x4 B = *reinterpret_cast<const bf16x4 *>(wt + (size_t)(n * 16 + row) * cin + 4 * grp);
+#pragma unroll
+                for (int m = 0; m < 4; ++m)
+                    acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[m], B, acc[m][n], 0, 0, 0);
+            }
+        }
+    }
+}
+
 template <int NB, int MODE>
 __global__ void __launch_bounds__(kCB) conv3_kernel(ConvArgs a) {
+    __shared__ uint16_t tile[kHalo * kSliceStride];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const long nvox = (long)a.nx * a.ny * a.nz;
-    const long vbase = (long)blockIdx.x * kCB + wave * 64;
+    const int row = lane & 15, grp = lane >> 4;
+    const int tx = (a.nx + kTX - 1) / kTX, ty = (a.ny + kTY - 1) / kTY;
+    const int bx = blockIdx.x % tx, by = (blockIdx.x / tx) % ty, bz = blockIdx.x / (tx * ty);
+    const int x0 = bx * kTX, y0 = by * kTY, z0 = bz * kTZ;
     const int cin = a.cA + a.cB;
-    const int row = lane & 15, kq = (lane >> 4) * 4;
-    int vx[4], vy[4], vz[4];
-    bool vin[4];
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-        const long v = vbase + m * 16 + row;
-        vin[m] = v < nvox;
-        const long w = vin[m] ? v : 0;
-        vz[m] = (int)(w % a.nz);
-        vy[m] = (int)((w / a.nz) % a.ny);
-        vx[m] = (int)(w / ((long)a.ny * a.nz));
-    }
     f32x4 acc[4][NB];
 #pragma unroll
     for (int m = 0; m < 4; ++m)
 #pragma unroll
         for (int n = 0; n < NB; ++n) acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int t = 0; t < 27; ++t) {
-        const int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
-        int xs[4], ys[4], zs[4];
-        bool ok[4];
-#pragma unroll
-        for (int m = 0; m < 4; ++m) {
-            xs[m] = vx[m] + dx; ys[m] = vy[m] + dy; zs[m] = vz[m] + dz;
-            ok[m] = vin[m] && xs[m] >= 0 && xs[m] < a.nx && ys[m] >= 0 && ys[m] < a.ny && zs[m] >= 0 && zs[m] < a.nz;
-        }
-        const uint16_t *wt = a.w + (size_t)t * a.cout * cin;
-        for (int c = 0; c < cin; c += 16) {
-            bf16x4 A[4];
-#pragma unroll
-            for (int m = 0; m < 4; ++m) A[m] = ok[m] ? load4(a, xs[m], ys[m], zs[m], c + kq) : bf16x4{0, 0, 0, 0};
-#pragma unroll
-            for (int n = 0; n < NB; ++n) {
-                const bf16x4 B = *reinterpret_cast<const bf16x4 *>(wt + (size_t)(n * 16 + row) * cin + c + kq);
-#pragma unroll
-                for (int m = 0; m < 4; ++m) acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[m], B, acc[m][n], 0, 0, 0);
-            }
-        }
+    const int full = cin / kSlice * kSlice;
+    for (int c0 = 0; c0 < full; c0 += kSlice) {
+        __syncthreads();   // the previous slice's reads are done
+        stage_slice(a, tile, x0, y0, z0, c0, kSlice / 8);
+        __syncthreads();
+        slice_taps<NB, true>(a, tile, c0, cin, wave, row, grp, acc);
     }
+    if (full < cin) {   // a 16-channel tail
+        __syncthreads();
+        stage_slice(a, tile, x0, y0, z0, full, 2);
+        __syncthreads();
+        slice_taps<NB, false>(a, tile, full, cin, wave, row, grp, acc);
+    }
+    // D[i][j] in lane l: j = l % 16 (output channel), i = 4 (l / 16) + r (z-voxel of the row block)
+    const int x = x0 + wave;
 #pragma unroll
-    for (int m = 0; m < 4; ++m)
+    for (int m = 0; m < 4; ++m) {
+        const int y = y0 + m;
 #pragma unroll
         for (int n = 0; n < NB; ++n) {
             const int co = n * 16 + row;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const long v = vbase + m * 16 + kq + r;
-                if (v >= nvox) continue;
+                const int z = z0 + 4 * grp + r;
+                if (x >= a.nx || y >= a.ny || z >= a.nz) continue;
+                const long v = vox(x, y, z, a.ny, a.nz);
                 float val = acc[m][n][r];
                 if (MODE == 0) {
                     val += a.bias[co];
@@ -141,12 +204,13 @@ __global__ void __launch_bounds__(kCB) conv3_kernel(ConvArgs a) {
                 }
             }
         }
+    }
 }
 
 template <int MODE>
 static hipError_t conv3_nb(const ConvArgs &a, hipStream_t s) {
-    const long nvox = (long)a.nx * a.ny * a.nz;
-    const dim3 grid((unsigned)((nvox + kCB - 1) / kCB));
+    const long tiles = (long)((a.nx + kTX - 1) / kTX) * ((a.ny + kTY - 1) / kTY) * ((a.nz + kTZ - 1) / kTZ);
+    const dim3 grid((unsigned)tiles);
     switch (a.cout / 16) {
     case 1: hipLaunchKernelGGL((conv3_kernel<1, MODE>), grid, dim3(kCB), 0, s, a); break;
     case 2: hipLaunchKernelGGL((conv3_kernel<2, MODE>), grid, dim3(kCB), 0, s, a); break;
