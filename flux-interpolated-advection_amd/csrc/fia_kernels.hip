@@ -229,16 +229,25 @@ FIA_HD double piece_flux_weno_integral(const FaceParams &P, const double c[3], d
     const double f = gy - jf;
     const int J = (int)jf;
     const bool zin = bz - 2 >= 0 && bz + 3 <= mz - 1;   // the z rows need no clamping
+    // Stencils wholly inside the field (the common case) take their rows at 32-bit offsets from
+    // the field's base, one add per row; the others are clamped row by row.
+    const bool small = (size_t)mx * (size_t)(my + 1) * (size_t)mz * sizeof(double) < ((size_t)1 << 32);
+    const bool xin = small && zin && bx - 2 >= 0 && bx + 3 <= mx - 1;
     // psi on level J
     double psi;
     {
         const Loader<double> PS{P.psi, mx, my + 1, mz};
         const WenoCoef wz(tz, P.kappa_psi), wx(tx, P.kappa_psi);
+        const bool in = xin && J >= 0 && J <= my;
+        const unsigned sx = (unsigned)(my + 1) * (unsigned)mz;
+        const unsigned o0 = in ? ((unsigned)(bx - 2) * (unsigned)(my + 1) + (unsigned)J) * (unsigned)mz + (unsigned)(bz - 2)
+                               : 0u;
         double xr[6];
 #pragma unroll
         for (int ix = 0; ix < 6; ++ix) {
             double fz[6];
-            load_z6<double>(PS, bx - 2 + ix, J, bz - 2, zin, fz);
+            if (in) load_row6<double>(P.psi, o0 + (unsigned)ix * sx, fz);
+            else load_z6<double>(PS, bx - 2 + ix, J, bz - 2, zin, fz);
             xr[ix] = weno6<double>(fz, wz);
         }
         psi = weno6<double>(xr, wx);
@@ -263,6 +272,10 @@ FIA_HD double piece_flux_weno_integral(const FaceParams &P, const double c[3], d
         }
     }
     const WenoCoef wz(tz, P.kappa_q), wx(tx, P.kappa_q);
+    const bool qin = xin && B - 2 >= 0 && B + 4 <= my - 1;
+    const unsigned qsx = (unsigned)my * (unsigned)mz;
+    const unsigned q0 = qin ? ((unsigned)(bx - 2) * (unsigned)my + (unsigned)(B - 2)) * (unsigned)mz + (unsigned)(bz - 2)
+                            : 0u;
     float yv[2 * NG][6];
 #pragma unroll
     for (int ix = 0; ix < 6; ++ix) {
@@ -271,7 +284,8 @@ FIA_HD double piece_flux_weno_integral(const FaceParams &P, const double c[3], d
         for (int r = 0; r < 7; ++r) {
             if (r == 6 && nh == 1) { zr[6] = 0.0f; continue; }
             float fz[6];
-            load_z6<float>(Q, bx - 2 + ix, B - 2 + r, bz - 2, zin, fz);
+            if (qin) load_row6<float>(P.q, q0 + (unsigned)ix * qsx + (unsigned)r * (unsigned)mz, fz);
+            else load_z6<float>(Q, bx - 2 + ix, B - 2 + r, bz - 2, zin, fz);
             zr[r] = weno6<float>(fz, wz);
         }
 #pragma unroll

@@ -114,6 +114,18 @@ template <class V> FIA_HD void load_z6(const Loader<V> &L, int x, int y, int z0,
     }
 }
 
+// Six consecutive samples at a 32-bit element offset from the field's base (a row that lies
+// inside the field: the caller has checked the bounds, and that the byte offset fits in 32 bits).
+// The address is the uniform base plus a 32-bit byte offset, the form of a global load with a
+// scalar base.
+template <class V> FIA_HD void load_row6(const V *__restrict__ base, unsigned off, V f[6]) {
+    struct Row { V v[6]; } r;
+    const unsigned bytes = off * (unsigned)sizeof(V);
+    __builtin_memcpy(&r, reinterpret_cast<const char *>(base) + bytes, sizeof(r));
+#pragma unroll
+    for (int i = 0; i < 6; ++i) f[i] = r.v[i];
+}
+
 // 3D WENO6, passes z then y then x (reference src/WENO6_interpolation_3d.cpp:7-258).  A pass
 // whose sample coordinate is on a grid node (SX/SY/SZ) is skipped: WENO6 at t = 0 returns f_2
 // exactly (every candidate interpolates node 2 and the weights sum to 1).
