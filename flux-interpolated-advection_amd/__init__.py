@@ -23,7 +23,8 @@ LIB_PATH = os.path.join(HERE, "libfia.so")
 
 FIA_OK = 0
 STATUS = {0: "FIA_OK", 1: "FIA_ERR_INVALID_ARG", 2: "FIA_ERR_UNSUPPORTED", 3: "FIA_ERR_CAPACITY",
-          4: "FIA_ERR_NONFINITE", 5: "FIA_ERR_DEVICE", 6: "FIA_ERR_NO_DEVICE", 7: "FIA_ERR_NOT_CONVERGED"}
+          4: "FIA_ERR_NONFINITE", 5: "FIA_ERR_DEVICE", 6: "FIA_ERR_NO_DEVICE", 7: "FIA_ERR_NOT_CONVERGED",
+          8: "FIA_ERR_CUT"}
 
 # reference strings -> enums (reference src/split_face_3D.cpp:26-43, src/main.cpp:60,79)
 INTERP = {"linear": 0, "WENO6": 1, "1Dy_linear": 2, "1Dy_WENO6": 3}
@@ -33,7 +34,7 @@ PSI = {"const": 0, "linear": 1, "WENO6": 2}
 UNET_LOSS = {"residual": 0, "energy": 1}
 
 # exported symbols of libfia.so (the declarations of include/fia.h)
-EXPORTS = ["fia_default_opts", "fia_ctx_create", "fia_ctx_destroy", "fia_ctx_device_bytes", "fia_last_error",
+EXPORTS = ["fia_default_opts", "fia_ctx_create", "fia_ctx_create_on_stream", "fia_ctx_stream", "fia_ctx_destroy", "fia_ctx_device_bytes", "fia_last_error",
            "fia_device_count", "fia_move_substances_3d", "fia_advect_flux_pass", "fia_advect_velocity_3d",
            "fia_calc_psi_y", "fia_interpolate", "fia_backtrace_vertices", "fia_split_faces", "fia_cfl_time_step",
            "fia_pressure_project", "fia_ctx_set_timing", "fia_ctx_get_stats", "fia_ctx_reset_stats",
@@ -111,6 +112,9 @@ def lib():
         L.fia_default_opts.argtypes = [ctypes.POINTER(AdvectOpts)]
         L.fia_default_opts.restype = None
         L.fia_ctx_create.argtypes = [_i, ctypes.POINTER(_vp)]
+        L.fia_ctx_create_on_stream.argtypes = [_i, _vp, ctypes.POINTER(_vp)]
+        L.fia_ctx_stream.argtypes = [_vp]
+        L.fia_ctx_stream.restype = _vp
         L.fia_ctx_destroy.argtypes = [_vp]
         L.fia_ctx_device_bytes.argtypes = [_vp]
         L.fia_ctx_device_bytes.restype = ctypes.c_size_t
@@ -176,10 +180,19 @@ def device_count():
 class Context:
     """Owns the device scratch and the HIP stream (fia_ctx)."""
 
-    def __init__(self, device=0):
+    def __init__(self, device=0, stream=None):
+        """stream: a hipStream_t handle (int) of the HIP runtime libfia.so links (ROCm's
+        libamdhip64), to enqueue on the caller's stream; torch's streams belong to torch's own
+        HIP runtime and are not valid here.  None: the context creates its own stream."""
         _torch_first()
         self._h = _vp()
-        _check(lib().fia_ctx_create(device, ctypes.byref(self._h)))
+        if stream is None:
+            _check(lib().fia_ctx_create(device, ctypes.byref(self._h)))
+        else:
+            _check(lib().fia_ctx_create_on_stream(device, _vp(stream), ctypes.byref(self._h)))
+
+    def stream(self):
+        return lib().fia_ctx_stream(self._h)
 
     def close(self):
         if self._h:

@@ -74,6 +74,7 @@ struct FluxSet {
 struct fia_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    bool own_stream = true;   // false: the caller's stream (fia_ctx_create_on_stream)
     DevBuf<double> psi;
     DevBuf<float> disp[3];
     FluxSet fsa, fsb;      // face fluxes (and orientation corrections) of two passes
@@ -181,6 +182,9 @@ fia_status check_kernel_errors(fia_ctx *c) {
     HIP_TRY(hipMemcpyAsync(h, c->counters.p, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (h[1] == KE_CAPACITY) return fail(FIA_ERR_CAPACITY, "face cutter capacity exceeded (polygon or stack)");
+    if (h[1] == KE_CUT)
+        return fail(FIA_ERR_CUT, "a cut line crossed a piece's boundary an odd number of times "
+                                 "(the reference's cutter stops the program here: src/facecutter.h:190-203)");
     if (h[1] == KE_NONFINITE) return fail(FIA_ERR_NONFINITE, "non-finite backtraced vertex (NaN/Inf velocity or dt)");
     if (h[1] != 0) return fail(FIA_ERR_DEVICE, "kernel reported error " + std::to_string(h[1]));
     return FIA_OK;
@@ -544,7 +548,8 @@ int fia_device_count(void) {
     return n;
 }
 
-fia_status fia_ctx_create(int device, fia_ctx **out) {
+// A context on a new stream of its own (stream == nullptr) or on the caller's stream.
+static fia_status ctx_create(int device, hipStream_t stream, fia_ctx **out) {
     if (!out) return fail(FIA_ERR_INVALID_ARG, "out is NULL");
     *out = nullptr;
     int n = 0;
@@ -554,16 +559,26 @@ fia_status fia_ctx_create(int device, fia_ctx **out) {
     if (dev >= n) return fail(FIA_ERR_INVALID_ARG, "device index out of range");
     fia_ctx *c = new fia_ctx();
     c->device = dev;
-    if (hipSetDevice(dev) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    c->own_stream = stream == nullptr;
+    if (hipSetDevice(dev) != hipSuccess) {
         delete c;
-        return fail(FIA_ERR_DEVICE, "cannot create the stream");
+        return fail(FIA_ERR_DEVICE, "cannot select the device");
     }
+    if (c->own_stream) {
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+            delete c;
+            return fail(FIA_ERR_DEVICE, "cannot create the stream");
+        }
+    } else {
+        c->stream = stream;
+    }
+    auto drop_stream = [&]() { if (c->own_stream) (void)hipStreamDestroy(c->stream); };
     if (hipHostMalloc((void **)&c->host_bounds, 4 * sizeof(unsigned long long)) != hipSuccess ||
         hipHostMalloc((void **)&c->host_flags, 4 * sizeof(int)) != hipSuccess ||
         hipHostMalloc((void **)&c->host_scal, 2 * kPcgScalars * sizeof(double)) != hipSuccess) {
         if (c->host_bounds) (void)hipHostFree(c->host_bounds);
         if (c->host_flags) (void)hipHostFree(c->host_flags);
-        (void)hipStreamDestroy(c->stream);
+        drop_stream();
         delete c;
         return fail(FIA_ERR_DEVICE, "cannot allocate pinned host memory");
     }
@@ -572,11 +587,20 @@ fia_status fia_ctx_create(int device, fia_ctx **out) {
     memset(c->host_scal, 0, 2 * kPcgScalars * sizeof(double));
     if (ensure_common(c) != FIA_OK) {
         (void)hipHostFree(c->host_bounds); (void)hipHostFree(c->host_flags); (void)hipHostFree(c->host_scal);
-        (void)hipStreamDestroy(c->stream); delete c; return FIA_ERR_DEVICE;
+        drop_stream(); delete c; return FIA_ERR_DEVICE;
     }
     *out = c;
     return FIA_OK;
 }
+
+fia_status fia_ctx_create(int device, fia_ctx **out) { return ctx_create(device, nullptr, out); }
+
+fia_status fia_ctx_create_on_stream(int device, void *stream, fia_ctx **out) {
+    if (!stream) return fail(FIA_ERR_INVALID_ARG, "stream is NULL (use fia_ctx_create for a stream of its own)");
+    return ctx_create(device, (hipStream_t)stream, out);
+}
+
+void *fia_ctx_stream(fia_ctx *c) { return c ? (void *)c->stream : nullptr; }
 
 fia_status fia_ctx_destroy(fia_ctx *c) {
     if (!c) return FIA_OK;
@@ -597,7 +621,7 @@ fia_status fia_ctx_destroy(fia_ctx *c) {
     if (c->host_bounds) (void)hipHostFree(c->host_bounds);
     if (c->host_flags) (void)hipHostFree(c->host_flags);
     if (c->host_scal) (void)hipHostFree(c->host_scal);
-    (void)hipStreamDestroy(c->stream);
+    if (c->own_stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return FIA_OK;
 }

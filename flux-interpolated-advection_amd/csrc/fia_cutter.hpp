@@ -15,6 +15,7 @@
 // colour crosses an iso line more than twice (a twisted face), which is then cut by the exact
 // recursive restatement in fia_cutter_general (fallback kernel).
 #pragma once
+#include "fia_internal.hpp"
 #include "fia_math.hpp"
 
 namespace fia {
@@ -349,9 +350,11 @@ FIA_HD void gen_center(const double (*p)[2], int num, double &cx, double &cy, do
     }
 }
 
-// emit(pts, num, centre_xi, centre_eta) for every piece.
+// emit(pts, num, centre_xi, centre_eta) for every piece.  Returns KE_NONE, KE_CAPACITY (a polygon,
+// the stack or the triangle list overflowed) or KE_CUT (a cut line crossed a piece's boundary an
+// odd number of times; the reference stops the program there, src/facecutter.h:190-203).
 template <class Emit>
-FIA_HD bool general_cut(const double col[4][2], double min_area, Emit &emit) {
+FIA_HD int general_cut(const double col[4][2], double min_area, Emit &emit) {
     GenPoly stack[kGenStack];
     int sp = 0;
     {
@@ -395,7 +398,7 @@ FIA_HD bool general_cut(const double col[4][2], double min_area, Emit &emit) {
                         if (t1 < 1.0) arr1.v[arr1.n++] = add;
                     } else {
                         arr1.v[arr1.n++] = add;
-                        if (sp >= kGenStack) return false;
+                        if (sp >= kGenStack) return KE_CAPACITY;
                         stack[sp++] = arr1;
                         slot = 0;
                         if (t1 < 1.0) arr0.v[arr0.n++] = add;
@@ -405,14 +408,14 @@ FIA_HD bool general_cut(const double col[4][2], double min_area, Emit &emit) {
                     GenPoly &dst = slot == 0 ? arr0 : arr1;
                     dst.v[dst.n++] = cur.v[m];
                 }
-                if (arr0.n >= kGenVerts - 2 || arr1.n >= kGenVerts - 2) return false;
+                if (arr0.n >= kGenVerts - 2 || arr1.n >= kGenVerts - 2) return KE_CAPACITY;
             }
-            if (slot != 0) return false;   // odd number of sign changes: cannot happen on a loop
-            if (sp >= kGenStack) return false;
+            if (slot != 0) return KE_CUT;   // odd number of sign changes: cannot happen on a loop
+            if (sp >= kGenStack) return KE_CAPACITY;
             stack[sp++] = arr0;
         } else if (dim == 0) {
             cur.dim = 1;
-            if (sp >= kGenStack) return false;
+            if (sp >= kGenStack) return KE_CAPACITY;
             stack[sp++] = cur;
         } else if (num >= 3) {
             // subdivide (reference src/facecutter.h:255-324)
@@ -429,7 +432,7 @@ FIA_HD bool general_cut(const double col[4][2], double min_area, Emit &emit) {
                 if (area > min_area) {
                     for (int n = num - 1; n >= 0; --n) {   // pushed in reverse: popped in polygon order
                         const int m = (n + 1 == num) ? 0 : n + 1;
-                        if (tp >= kGenTris) return false;
+                        if (tp >= kGenTris) return KE_CAPACITY;
                         tri[tp][0][0] = cx; tri[tp][0][1] = cy;
                         tri[tp][1][0] = pts[n][0]; tri[tp][1][1] = pts[n][1];
                         tri[tp][2][0] = pts[m][0]; tri[tp][2][1] = pts[m][1];
@@ -457,7 +460,7 @@ FIA_HD bool general_cut(const double col[4][2], double min_area, Emit &emit) {
                     }
                     const int i1 = (imax + 1) % 3, i2 = (imax + 2) % 3;
                     const double mx = 0.5 * (t[imax][0] + t[i1][0]), my = 0.5 * (t[imax][1] + t[i1][1]);
-                    if (tp + 2 > kGenTris) return false;
+                    if (tp + 2 > kGenTris) return KE_CAPACITY;
                     // tri1 = {mid, p1, p2} pushed first so tri0 = {p0, mid, p2} is processed first
                     tri[tp][0][0] = mx; tri[tp][0][1] = my;
                     tri[tp][1][0] = t[i1][0]; tri[tp][1][1] = t[i1][1];
@@ -473,7 +476,7 @@ FIA_HD bool general_cut(const double col[4][2], double min_area, Emit &emit) {
             }
         }
     }
-    return true;
+    return KE_NONE;
 }
 
 }  // namespace fia

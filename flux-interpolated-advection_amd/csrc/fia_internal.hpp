@@ -110,7 +110,7 @@ struct FaceParams {
 };
 
 // errors reported by kernels through FaceParams::error
-enum KernelError { KE_NONE = 0, KE_CAPACITY = 3, KE_NONFINITE = 4 };
+enum KernelError { KE_NONE = 0, KE_CAPACITY = 3, KE_NONFINITE = 4, KE_CUT = 8 };
 
 hipError_t launch_psi(const PsiParams &p, hipStream_t s);
 hipError_t launch_vertices(const VertexParams &p, hipStream_t s);

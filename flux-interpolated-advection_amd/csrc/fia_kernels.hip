@@ -665,7 +665,7 @@ __global__ void __launch_bounds__(64) degenerate_kernel(FaceParams P) {
 
 // Flux of one face with the exact recursive cutter, corners in the given order.
 FIA_HD double general_face_flux(const FaceParams &P, const double v[4][3], const double col[4][2],
-                                                    bool &ok) {
+                                                    int &code) {
     FaceGeom geom;
     geom.init(v);
     double flux = 0.0;
@@ -688,7 +688,7 @@ FIA_HD double general_face_flux(const FaceParams &P, const double v[4][3], const
         int npts = 0;
         if (ay != 0.0) flux += piece_flux(P, c3, ay, npts);
     };
-    ok = general_cut(col, P.min_area, emit);
+    code = general_cut(col, P.min_area, emit);
     return flux;
 }
 
@@ -707,16 +707,17 @@ FIA_HD void general_body(const FaceParams &P, int t) {
     decode_face(P.g, fam, id, i, j, k);
     double v[4][3], col[4][2];
     load_face(P, fam, i, j, k, v, col);
-    bool ok1, ok2;
-    const double lower = general_face_flux(P, v, col, ok1);
+    int code1, code2;
+    const double lower = general_face_flux(P, v, col, code1);
     double vr[4][3], cr[4][2];
     const int order[4] = {0, 3, 2, 1};
     for (int c = 0; c < 4; ++c) {
         for (int a = 0; a < 3; ++a) vr[c][a] = v[order[c]][a];
         cr[c][0] = col[order[c]][0]; cr[c][1] = col[order[c]][1];
     }
-    const double upper = general_face_flux(P, vr, cr, ok2);
-    if (!ok1 || !ok2) report(P.error, KE_CAPACITY);
+    const double upper = general_face_flux(P, vr, cr, code2);
+    if (code1 != KE_NONE) report(P.error, code1);
+    else if (code2 != KE_NONE) report(P.error, code2);
     P.flux[fam][id] = lower;
     P.corr[fam][id] = upper + lower;
 }
@@ -804,9 +805,9 @@ __global__ void __launch_bounds__(64) split_faces_kernel(const double *quads, si
     FaceGeom geom;
     geom.init(v);
     int cnt = 0;
-    bool over = false;
+    int over = KE_NONE;
     auto put = [&](const double r[3], double ax2, double ay2, double az2) {
-        if (cnt >= max_pieces) { over = true; return; }
+        if (cnt >= max_pieces) { over = KE_CAPACITY; return; }
         const size_t o = (id * max_pieces + cnt) * 3;
         cen[o] = geom.p0[0] + r[0]; cen[o + 1] = geom.p0[1] + r[1]; cen[o + 2] = geom.p0[2] + r[2];
         area[o] = 0.5 * ax2; area[o + 1] = 0.5 * ay2; area[o + 2] = 0.5 * az2;
@@ -876,10 +877,11 @@ __global__ void __launch_bounds__(64) split_faces_kernel(const double *quads, si
             geom.rel(cxi, ceta, r);
             put(r, a2[0], a2[1], a2[2]);
         };
-        if (!general_cut(col, min_area, emit)) over = true;
+        const int code = general_cut(col, min_area, emit);
+        if (code != KE_NONE) over = code;
     }
     count[id] = cnt;
-    if (over) atomicCAS(error, 0, (int)KE_CAPACITY);
+    if (over) atomicCAS(error, 0, over);
 }
 
 __global__ void absmax_kernel(const float *a, size_t n, unsigned int *out_bits) {

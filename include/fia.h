@@ -35,7 +35,10 @@ typedef enum fia_status {
     FIA_ERR_NONFINITE = 4,     /* NaN/Inf in an input field or in the time step            */
     FIA_ERR_DEVICE = 5,        /* HIP runtime error                                        */
     FIA_ERR_NO_DEVICE = 6,     /* no GPU visible                                           */
-    FIA_ERR_NOT_CONVERGED = 7  /* pressure solve hit its iteration limit                   */
+    FIA_ERR_NOT_CONVERGED = 7, /* pressure solve hit its iteration limit                   */
+    FIA_ERR_CUT = 8            /* a cut line crossed a piece's boundary an odd number of
+                                  times (the reference's cutter exits there,
+                                  src/facecutter.h:190-203)                                */
 } fia_status;
 
 /* Interpolation methods (the reference's interpolation_method strings). */
@@ -104,6 +107,13 @@ void fia_default_opts(fia_advect_opts *opts);
  * size it saw, and one HIP stream of default priority.  device < 0 picks the current device. */
 fia_status fia_ctx_create(int device, fia_ctx **out);
 fia_status fia_ctx_destroy(fia_ctx *ctx);
+/* A context that enqueues its work on the caller's HIP stream (a hipStream_t of the device),
+ * so that library work orders with the caller's own kernels (torch's current stream, a batch
+ * driver's streams).  Calls still return when their results are complete: the flux path reads
+ * piece-slot counts back between its phases.  The context does not destroy the stream. */
+fia_status fia_ctx_create_on_stream(int device, void *stream, fia_ctx **out);
+/* The stream a context enqueues on (a hipStream_t). */
+void *fia_ctx_stream(fia_ctx *ctx);
 /* Bytes of device memory the context holds now. */
 size_t fia_ctx_device_bytes(const fia_ctx *ctx);
 /* Message of the last error of this thread (never NULL). */

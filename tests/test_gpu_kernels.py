@@ -98,3 +98,32 @@ def test_split_faces_match_oracle(fia, oracle, ctx, split, min_area):
         pc, pa = gc[key(gc)], ga[key(gc)]
         assert np.abs(oc - pc).max() < 1e-9 * 8
         assert np.abs(oa - pa).max() < 1e-12 * 8
+
+
+def test_context_on_callers_stream(fia, ctx, scenes):
+    # a context on a stream the caller created with the HIP runtime libfia.so links gives the same
+    # result as a context with a stream of its own, on that stream
+    import ctypes
+    import numpy as np
+    print("case context on caller stream", flush=True)
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    s = ctypes.c_void_p()
+    assert hip.hipStreamCreate(ctypes.byref(s)) == 0
+    try:
+        c2 = fia.Context(0, stream=s.value)
+        try:
+            assert c2.stream() == s.value
+            n = 16
+            h = 1.0 / n
+            rho = scenes.gaussian_blob(n, centre=(0.5, 0.45, 0.5), sigma=0.12).astype(np.float32)
+            u, v, w = [a.astype(np.float32) for a in scenes.curl_noise(n, seed=2)]
+            dt = scenes.cfl_dt(u, v, w, h, 2.0)
+            g1 = fia.Grid3D(n, n, n, h, rho.copy(), u, v, w)
+            g2 = fia.Grid3D(n, n, n, h, rho.copy(), u, v, w)
+            fia.move_substances_3d(ctx, g1, 0, dt, split_method="xz-CellFaceAligned")
+            fia.move_substances_3d(c2, g2, 0, dt, split_method="xz-CellFaceAligned")
+            assert np.array_equal(g1.density, g2.density)
+        finally:
+            c2.close()
+    finally:
+        hip.hipStreamDestroy(s)
