@@ -74,7 +74,14 @@ def cpu_baseline(scenes, n, cfl, threads):
     O.move_substances(rho.astype(np.float64), u, v, w, h, dt, split="xz-CellFaceAligned", method="WENO6",
                       prec=1, cprec=1, nthreads=threads)
     sec = time.perf_counter() - t0
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), None)
+    except OSError:
+        pass
     return {"value": round(n ** 3 / sec / 1e6, 5), "unit": "Mcell-steps/s", "cores": threads, "kind": "port",
+            "host": {"cpu_model": model, "logical_cpus_visible": os.cpu_count(), "threads_used": threads},
             "sample": "one move_substances_3D step at %d^3 (same workload, CFL %g), FP32 oracle restatement of the "
                       "reference path (per-cell faces, std::thread over x-slabs), %.1f s" % (n, cfl, sec)}
 
