@@ -10,10 +10,12 @@ psi (Gauss n=1), zero-velocity correction on.  Fields stay resident on the GPU.
   projection  one step = one pressure projection (Jacobi-PCG to 1e-6 relative residual, zero
               initial guess) of the curl-noise velocity plus a buoyancy impulse, 8 spherical
               obstacles as a solid mask
-  scenes      one step = one step of 64 independent plume scenes (edges from {32,48,64,80,96},
-              seeded sources, buoyancy, 0-4 obstacles, CFL 1-3): forcing, velocity advection,
+  scenes      one step = one step of --scenes (64) independent plume scenes (edges drawn from
+              --edges, {32,48,64,80,96}; seeded sources, buoyancy, 0-4 obstacles, CFL 1-3):
+              forcing, velocity advection,
               projection (U-Net warm start unless --plain) and density advection, on the number
-              of concurrent workers the autotuner picks; value = all cells x steps / time
+              of concurrent workers the autotuner picks; value = all cells x steps / time;
+              --batched: the pressure solves of all scenes batched (fia_scenes_step_batched)
   warmstart   one step = the pressure projection of one plume step (128^3 unless --n; seeded
               source, buoyancy, 0-8 obstacles; scenes not used in training) twice on the same
               velocity: from a zero guess and with the learned U-Net warm start (committed
@@ -256,7 +258,7 @@ def run_warmstart(args, fia, scenes, torch):
 
 def run_scenes(args, fia, torch):
     batch = importlib.import_module("flux-interpolated-advection_amd.batch")
-    sset = batch.SceneSet(args.scenes, seed=5)
+    sset = batch.SceneSet(args.scenes, seed=5, edges=tuple(int(e) for e in args.edges.split(",")))
     candidates = (1, 2, 4, 8)
     ctxs = [fia.Context(0) for _ in range(max(candidates))]
     nets = None
@@ -269,13 +271,14 @@ def run_scenes(args, fia, torch):
     od = fia.make_opts(split_method="xz-CellFaceAligned", interpolation_method="WENO6")
     ov = fia.make_opts(interpolation_method="WENO6")
     # the autotuner's steps are the warm-up
-    best, tune = batch.autotune_workers(sset, ctxs, od, ov, candidates, nets=nets, log=lambda s: None)
+    best, tune = batch.autotune_workers(sset, ctxs, od, ov, candidates, nets=nets, batched=args.batched,
+                                        log=lambda s: None)
     for _ in range(args.warmup):
-        sset.step(ctxs[:best], od, ov, nets=nets[:best] if nets else None)
+        sset.step(ctxs[:best], od, ov, nets=nets[:best] if nets else None, batched=args.batched)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        sset.step(ctxs[:best], od, ov, nets=nets[:best] if nets else None)
+        sset.step(ctxs[:best], od, ov, nets=nets[:best] if nets else None, batched=args.batched)
     torch.cuda.synchronize()
     sec = time.perf_counter() - t0
     st = sset.stats()
@@ -290,8 +293,9 @@ def run_scenes(args, fia, torch):
         "vs_baseline": None, "dtype": "f32 advection, f64 solve", "data": "synthetic",
         "config": {"workload": "%d independent plume scenes, edges %s (%d cells in all), CFL 1-3, 0-4 obstacles; "
                                "one step = forcing, velocity self-advection, projection to 1e-6 (%s), density "
-                               "advection; WENO6" % (args.scenes, sorted(set(sset.edges)), sset.cells,
-                                                     "plain PCG" if args.plain else "U-Net warm start"),
+                               "advection; WENO6%s" % (args.scenes, sorted(set(sset.edges)), sset.cells,
+                                                       "plain PCG" if args.plain else "U-Net warm start",
+                                                       "; pressure solves batched" if args.batched else ""),
                    "scenes": args.scenes, "cells": sset.cells},
         "scheduler": {"workers": best, "autotune_ms_per_step": {str(k): round(v * 1e3, 1) for k, v in tune.items()}},
         "pcg": st,
@@ -309,6 +313,8 @@ def main():
                     default="density")
     ap.add_argument("--scenes", type=int, default=64)
     ap.add_argument("--plain", action="store_true", help="scenes: plain PCG instead of the warm start")
+    ap.add_argument("--batched", action="store_true", help="scenes: the pressure solves of all scenes batched")
+    ap.add_argument("--edges", default="32,48,64,80,96", help="scenes: grid edges drawn from this list")
     ap.add_argument("--weights", default=os.path.join(ROOT, "flux-interpolated-advection_amd", "weights",
                                                       "unet_plume128.npy"))
     ap.add_argument("--no-cpu-baseline", action="store_true")

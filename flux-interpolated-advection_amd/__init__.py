@@ -41,7 +41,7 @@ EXPORTS = ["fia_default_opts", "fia_ctx_create", "fia_ctx_create_on_stream", "fi
            "fia_unet_param_count", "fia_unet_create", "fia_unet_destroy", "fia_unet_set_params",
            "fia_unet_get_params", "fia_unet_forward", "fia_unet_loss_grad", "fia_unet_train_step", "fia_pressure_rhs",
            "fia_pressure_project_learned", "fia_grid_totals", "fia_wall_momentum_loss", "fia_scene_forcing",
-           "fia_scenes_step"]
+           "fia_scenes_step", "fia_scenes_step_batched"]
 
 
 class FiaError(RuntimeError):
@@ -150,6 +150,7 @@ def lib():
         L.fia_scene_forcing.argtypes = [_vp, ctypes.POINTER(SceneC), _d]
         L.fia_scenes_step.argtypes = [ctypes.POINTER(_vp), _i, ctypes.POINTER(SceneC), _i, ctypes.POINTER(AdvectOpts),
                                       ctypes.POINTER(AdvectOpts), _d, _i, ctypes.POINTER(_vp)]
+        L.fia_scenes_step_batched.argtypes = L.fia_scenes_step.argtypes
         L.fia_pressure_project_learned.argtypes = [_vp, ctypes.POINTER(GridC), _vp, _vp, _vp, _d, _i, ctypes.POINTER(_i),
                                                    ctypes.POINTER(_d), ctypes.POINTER(LearnedInfo)]
         _lib = L
@@ -560,10 +561,13 @@ def wall_momentum_loss(ctx, grid, pressure=None):
 
 # ---- many independent scenes (no counterpart in the reference) -------------------------------
 
-def scenes_step(ctxs, scenes, density_opts, velocity_opts, rel_tol=1e-6, max_iters=20000, nets=None):
-    """One step of every scene (a ctypes array of SceneC) on len(ctxs) workers."""
+def scenes_step(ctxs, scenes, density_opts, velocity_opts, rel_tol=1e-6, max_iters=20000, nets=None,
+                batched=False):
+    """One step of every scene (a ctypes array of SceneC) on len(ctxs) workers; batched: the
+    pressure solves of all scenes run together on ctxs[0] (fia_scenes_step_batched)."""
     w = len(ctxs)
     ca = (_vp * w)(*[c._h for c in ctxs])
     na = (_vp * w)(*[n._h for n in nets]) if nets is not None else None
-    _check(lib().fia_scenes_step(ca, w, scenes, len(scenes), ctypes.byref(density_opts), ctypes.byref(velocity_opts),
-                                 float(rel_tol), int(max_iters), na))
+    fn = lib().fia_scenes_step_batched if batched else lib().fia_scenes_step
+    _check(fn(ca, w, scenes, len(scenes), ctypes.byref(density_opts), ctypes.byref(velocity_opts), float(rel_tol),
+              int(max_iters), na))

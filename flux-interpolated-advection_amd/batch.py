@@ -1,5 +1,6 @@
-"""Many independent smoke scenes of uneven sizes advanced together (fia_scenes_step), with an
-autotuner for the number of concurrent workers.  No counterpart in the reference, which runs one
+"""Many independent smoke scenes of uneven sizes advanced together (fia_scenes_step, or
+fia_scenes_step_batched with the pressure solves of all scenes batched), with an autotuner for the
+number of concurrent workers.  No counterpart in the reference, which runs one
 grid per process.
 
 Each scene is seeded: cubic grid edge drawn from `edges`, a density source sphere (radius 0.06)
@@ -60,8 +61,9 @@ class SceneSet:
             self.edges.append(n)
         self.cells = int(sum(e ** 3 for e in self.edges))
 
-    def step(self, ctxs, density_opts, velocity_opts, nets=None, rel_tol=1e-6):
-        self.fia.scenes_step(ctxs, self.array, density_opts, velocity_opts, rel_tol=rel_tol, nets=nets)
+    def step(self, ctxs, density_opts, velocity_opts, nets=None, rel_tol=1e-6, batched=False):
+        self.fia.scenes_step(ctxs, self.array, density_opts, velocity_opts, rel_tol=rel_tol, nets=nets,
+                             batched=batched)
 
     def stats(self):
         a = self.array
@@ -70,7 +72,7 @@ class SceneSet:
                 "network_used": int(sum(s.used_network for s in a)), "steps": int(min(s.steps for s in a))}
 
 
-def autotune_workers(sset, ctxs, density_opts, velocity_opts, candidates, nets=None, log=print):
+def autotune_workers(sset, ctxs, density_opts, velocity_opts, candidates, nets=None, batched=False, log=print):
     """Times one step of the whole set with each worker count (these steps advance the scenes) and
     returns the fastest count with the timings."""
     import torch
@@ -78,7 +80,7 @@ def autotune_workers(sset, ctxs, density_opts, velocity_opts, candidates, nets=N
     for w in candidates:
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        sset.step(ctxs[:w], density_opts, velocity_opts, nets=nets[:w] if nets else None)
+        sset.step(ctxs[:w], density_opts, velocity_opts, nets=nets[:w] if nets else None, batched=batched)
         torch.cuda.synchronize()
         timings[w] = time.perf_counter() - t0
         log("autotune: %d workers, %.1f ms per step" % (w, timings[w] * 1e3))

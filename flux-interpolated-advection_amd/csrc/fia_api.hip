@@ -109,6 +109,23 @@ struct fia_ctx {
     DevBuf<double> scal;
     DevBuf<unsigned int> bits;
     DevBuf<uint8_t> solid;
+    // pressure systems of scenes solved together (fia_scenes_step_batched)
+    struct SceneSolve {           // one scene's system and its CG solves (two while probing)
+        DevBuf<double> b;
+        DevBuf<uint8_t> flags;
+        PcgBufs slot[2];
+        PcgGrid pg;
+        PcgState st;              // the solve the joint iterations continue
+        int used_network = 0;
+        size_t bytes() const { return b.bytes() + flags.bytes() + slot[0].bytes() + slot[1].bytes(); }
+        void release() { b.release(); flags.release(); slot[0].release(); slot[1].release(); }
+    };
+    std::vector<SceneSolve> solves;
+    DevBuf<PcgState> batch_states;
+    DevBuf<int2> batch_map;
+    DevBuf<double> batch_scal;
+    double *host_batch = nullptr;   // pinned: the scalars of every batched solve
+    size_t host_batch_n = 0;
     // instrumentation
     bool timing = false;
     DevBuf<unsigned long long> stats;   // [0] pieces, [1] density points
@@ -143,6 +160,8 @@ size_t ctx_bytes(const fia_ctx *c) {
     for (int a = 0; a < 4; ++a) b += c->grid[a].bytes();
     for (int a = 0; a < 2; ++a) b += c->genD[a].bytes();
     b += c->pcg_b.bytes() + c->pcg_flags.bytes() + c->pcgs[0].bytes() + c->pcgs[1].bytes();
+    for (const auto &sv : c->solves) b += sv.bytes();
+    b += c->batch_states.bytes() + c->batch_map.bytes() + c->batch_scal.bytes();
     return b;
 }
 
@@ -465,17 +484,16 @@ int velocity_split(int comp, bool one_dy, bool error_pass) {
 
 // The pressure system of a grid: right-hand side and neighbour flags in the context, obstacle
 // faces zeroed, the pinned cell chosen (the last cell, or the last fluid cell with obstacles).
-static fia_status pressure_system(fia_ctx *c, const fia_grid3d *g, const GridView &gv, const uint8_t *solid, PcgGrid &pg) {
+// The pressure system of the grid whose velocity is at gv, with obstacle mask solid_dev (device,
+// or null), into b and flags; pg describes it.  Velocity on faces touching an obstacle is zeroed.
+static fia_status assemble_system(fia_ctx *c, const fia_grid3d *g, const GridView &gv, const uint8_t *solid_dev,
+                                  double *b, uint8_t *flags, PcgGrid &pg) {
     const size_t n = (size_t)g->nx * g->ny * g->nz;
-    pg.nx = g->nx; pg.ny = g->ny; pg.nz = g->nz; pg.solid = nullptr; pg.pin = (long)n - 1;
-    if (solid) {
-        HIP_TRY(c->solid.ensure(n));
-        HIP_TRY(hipMemcpyAsync(c->solid.p, solid, n, g->on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
-                               c->stream));
-        pg.solid = c->solid.p;
+    pg.nx = g->nx; pg.ny = g->ny; pg.nz = g->nz; pg.solid = solid_dev; pg.pin = (long)n - 1;
+    if (solid_dev) {
         // last fluid cell is pinned (the reference pins the last cell, which has no obstacles)
         std::vector<uint8_t> hs(n);
-        HIP_TRY(hipMemcpyAsync(hs.data(), c->solid.p, n, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipMemcpyAsync(hs.data(), solid_dev, n, hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(hipStreamSynchronize(c->stream));
         long pin = -1;
         for (long i = (long)n - 1; i >= 0; --i) if (!hs[i]) { pin = i; break; }
@@ -483,25 +501,75 @@ static fia_status pressure_system(fia_ctx *c, const fia_grid3d *g, const GridVie
         pg.pin = pin;
         HIP_TRY(launch_zero_solid_faces(pg, gv.u, gv.v, gv.w, c->stream));
     }
-    HIP_TRY(c->pcg_b.ensure(n));
-    HIP_TRY(c->pcg_flags.ensure(n));
-    HIP_TRY(launch_pcg_setup(pg, gv.u, gv.v, gv.w, g->dx, c->pcg_b.p, c->pcg_flags.p, c->stream));
+    HIP_TRY(launch_pcg_setup(pg, gv.u, gv.v, gv.w, g->dx, b, flags, c->stream));
     return FIA_OK;
 }
 
-// Solve k's buffers (allocated for the grid of pg) bound into a CG state over the context's system.
-static fia_status pcg_bind(fia_ctx *c, int k, const PcgGrid &pg, PcgState &ps) {
+// The system into the context's own buffers (the mask, host or device, copied first).
+static fia_status pressure_system(fia_ctx *c, const fia_grid3d *g, const GridView &gv, const uint8_t *solid, PcgGrid &pg) {
+    const size_t n = (size_t)g->nx * g->ny * g->nz;
+    if (solid) {
+        HIP_TRY(c->solid.ensure(n));
+        HIP_TRY(hipMemcpyAsync(c->solid.p, solid, n, g->on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                               c->stream));
+    }
+    HIP_TRY(c->pcg_b.ensure(n));
+    HIP_TRY(c->pcg_flags.ensure(n));
+    return assemble_system(c, g, gv, solid ? c->solid.p : nullptr, c->pcg_b.p, c->pcg_flags.p, pg);
+}
+
+// Buffers B (allocated here for the grid of pg) bound into a CG state over the system (b, flags).
+static fia_status bind_solve(fia_ctx::PcgBufs &B, const PcgGrid &pg, const double *b, const uint8_t *flags,
+                             PcgState &ps) {
     const size_t n = (size_t)pg.nx * pg.ny * pg.nz;
-    auto &B = c->pcgs[k];
     const int nb = pcg_blocks(n);
     HIP_TRY(B.x.ensure(n)); HIP_TRY(B.r.ensure(n)); HIP_TRY(B.p0.ensure(n)); HIP_TRY(B.p1.ensure(n));
     pcg_direction_tiling(pg.nx, pg.ny, pg.nz, ps);
     const size_t nd = (size_t)ps.dir_blocks;
     HIP_TRY(B.q.ensure(n)); HIP_TRY(B.part.ensure(5 * (size_t)nb + nd)); HIP_TRY(B.scal.ensure(kPcgScalars));
-    ps.g = pg; ps.flags = c->pcg_flags.p; ps.b = c->pcg_b.p;
+    ps.g = pg; ps.flags = flags; ps.b = b;
     ps.x = B.x.p; ps.r = B.r.p; ps.p[0] = B.p0.p; ps.p[1] = B.p1.p; ps.q = B.q.p;
     ps.scal = B.scal.p; ps.part_start = B.part.p; ps.part_update = B.part.p + 3 * (size_t)nb;
     ps.part_dir = B.part.p + 5 * (size_t)nb; ps.nblocks = nb; ps.it = 0;
+    return FIA_OK;
+}
+
+// Solve k's buffers bound into a CG state over the context's system.
+static fia_status pcg_bind(fia_ctx *c, int k, const PcgGrid &pg, PcgState &ps) {
+    return bind_solve(c->pcgs[k], pg, c->pcg_b.p, c->pcg_flags.p, ps);
+}
+
+// The learned start: sa starts from the network's guess scaled by its energy-optimal factor, sb
+// from zero; both run `probe` iterations and the one with the smaller residual is chosen.  ha and
+// hb (pinned) receive their scalars.
+struct LearnedStart {
+    bool use_net;
+    double guess_res, gamma, ra, rb, bb;
+};
+static fia_status learned_start(fia_ctx *c, fia_unet *u, const PcgGrid &pg, PcgState &sa, PcgState &sb, double tol2,
+                                int probe, double *ha, double *hb, LearnedStart &ls) {
+    HIP_TRY(u->net.forward(sa.b, pg.solid, u->y.p, nullptr, c->stream));
+    HIP_TRY(u->net.guess(u->y.p, pg.pin, sa.flags, sa.x, c->stream));
+    HIP_TRY(launch_pcg_scale_guess(sa, c->stream));
+    HIP_TRY(launch_pcg_start(sa, tol2, c->stream));
+    HIP_TRY(launch_pcg_check(sa, c->stream));
+    HIP_TRY(hipMemcpyAsync(ha, sa.scal, kPcgScalars * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    ls.bb = ha[kPcgBB] > 0 ? ha[kPcgBB] : 1.0;
+    ls.guess_res = sqrt(ha[kPcgRR] / ls.bb);
+    ls.gamma = ha[kPcgGamma];
+    HIP_TRY(launch_pcg_iterations(sa, probe, c->stream));
+    HIP_TRY(launch_pcg_check(sa, c->stream));
+    HIP_TRY(launch_pcg_guess(sb, nullptr, c->stream));
+    HIP_TRY(launch_pcg_start(sb, tol2, c->stream));
+    HIP_TRY(launch_pcg_iterations(sb, probe, c->stream));
+    HIP_TRY(launch_pcg_check(sb, c->stream));
+    HIP_TRY(hipMemcpyAsync(ha, sa.scal, kPcgScalars * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(hb, sb.scal, kPcgScalars * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    ls.ra = sqrt(ha[kPcgRR] / ls.bb);
+    ls.rb = sqrt(hb[kPcgRR] / ls.bb);
+    ls.use_net = ls.ra <= ls.rb;
     return FIA_OK;
 }
 
@@ -519,6 +587,120 @@ static fia_status pcg_run(fia_ctx *c, PcgState &ps, int max_iters, double *hs) {
     }
     return FIA_OK;
 }
+
+// ---- pressure systems of many scenes, solved together ------------------------------------------
+
+namespace {
+fia_status unet_grid(fia_unet *u, int nx, int ny, int nz);
+}
+
+namespace fia {
+
+fia_status scenes_solve_alloc(fia_ctx *c, const fia_scene *scenes, int n) {
+    HIP_TRY(set_device(c));
+    if ((int)c->solves.size() < n) c->solves.resize(n);
+    HIP_TRY(c->batch_states.ensure((size_t)std::max(n, 1)));
+    HIP_TRY(c->batch_scal.ensure((size_t)std::max(n, 1) * kPcgScalars));
+    if (c->host_batch_n < (size_t)n * kPcgScalars) {
+        if (c->host_batch) HIP_TRY(hipHostFree(c->host_batch));
+        c->host_batch = nullptr;
+        c->host_batch_n = 0;
+        HIP_TRY(hipHostMalloc((void **)&c->host_batch, (size_t)std::max(n, 1) * kPcgScalars * sizeof(double)));
+        c->host_batch_n = (size_t)n * kPcgScalars;
+    }
+    return FIA_OK;
+}
+
+// On worker context cw, for scene s (a device grid): its pressure system into main's solve
+// buffers, started as fia_pressure_project (zero guess) or, with a network, as
+// fia_pressure_project_learned (network and zero starts probed, the better one kept).
+fia_status scene_solve_setup(fia_ctx *cw, fia_ctx *main, int s, fia_scene *sc, fia_unet *net, double rel_tol,
+                             int max_iters) {
+    HIP_TRY(set_device(cw));
+    fia_grid3d *g = &sc->grid;
+    fia_ctx::SceneSolve &sv = main->solves[s];
+    const size_t n = (size_t)g->nx * g->ny * g->nz;
+    fia_status st;
+    if (net && (st = unet_grid(net, g->nx, g->ny, g->nz)) != FIA_OK) return st;
+    GridView gv;
+    if ((st = upload_grid(cw, g, gv, false)) != FIA_OK) return st;
+    HIP_TRY(sv.b.ensure(n));
+    HIP_TRY(sv.flags.ensure(n));
+    if ((st = assemble_system(cw, g, gv, sc->solid, sv.b.p, sv.flags.p, sv.pg)) != FIA_OK) return st;
+    const double tol2 = rel_tol * rel_tol;
+    PcgState sa;
+    if ((st = bind_solve(sv.slot[0], sv.pg, sv.b.p, sv.flags.p, sa)) != FIA_OK) return st;
+    if (net) {
+        PcgState sb;
+        if ((st = bind_solve(sv.slot[1], sv.pg, sv.b.p, sv.flags.p, sb)) != FIA_OK) return st;
+        LearnedStart ls;
+        if ((st = learned_start(cw, net, sv.pg, sa, sb, tol2, std::min(5, max_iters), cw->host_scal,
+                                cw->host_scal + kPcgScalars, ls)) != FIA_OK)
+            return st;
+        sv.st = ls.use_net ? sa : sb;
+        sv.used_network = ls.use_net ? 1 : 0;
+    } else {
+        HIP_TRY(launch_pcg_guess(sa, nullptr, cw->stream));
+        HIP_TRY(launch_pcg_start(sa, tol2, cw->stream));
+        sv.st = sa;
+        sv.used_network = 0;
+    }
+    HIP_TRY(hipStreamSynchronize(cw->stream));
+    return FIA_OK;
+}
+
+// On the main context: the started solves of scenes idx[0..m) iterated together (one direction
+// and one update launch per iteration for all of them) until every one has converged or run
+// max_iters iterations, then each scene projected.  Every started solve has run the same number
+// of iterations (0, or the probe).
+fia_status scenes_solve_run(fia_ctx *c, fia_scene *scenes, const int *idx, int m, int max_iters) {
+    HIP_TRY(set_device(c));
+    if (m == 0) return FIA_OK;
+    std::vector<PcgState> states(m);
+    std::vector<int2> map;
+    for (int j = 0; j < m; ++j) {
+        states[j] = c->solves[idx[j]].st;
+        if (states[j].it != states[0].it) return fail(FIA_ERR_INVALID_ARG, "scenes started at different iterations");
+        for (int k = 0; k < states[j].nblocks; ++k) map.push_back(make_int2(j, k));
+    }
+    const int blocks = (int)map.size();
+    for (int j = 0; j < m; ++j)
+        for (int k = 0; k < states[j].dir_blocks; ++k) map.push_back(make_int2(j, k));
+    HIP_TRY(c->batch_map.ensure(map.size()));
+    HIP_TRY(hipMemcpyAsync(c->batch_states.p, states.data(), m * sizeof(PcgState), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->batch_map.p, map.data(), map.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
+    PcgBatch b;
+    b.states = c->batch_states.p; b.map = c->batch_map.p; b.dir_map = c->batch_map.p + blocks;
+    b.n = m; b.blocks = blocks; b.dir_blocks = (int)map.size() - blocks; b.it = states[0].it;
+    double *hs = c->host_batch;
+    const int poll = 32;
+    while (true) {
+        HIP_TRY(launch_pcg_batch_check(b, c->batch_scal.p, c->stream));
+        HIP_TRY(hipMemcpyAsync(hs, c->batch_scal.p, (size_t)m * kPcgScalars * sizeof(double), hipMemcpyDeviceToHost,
+                               c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        bool all = true;
+        for (int j = 0; j < m && all; ++j) all = hs[(size_t)j * kPcgScalars + kPcgDone] != 0.0;
+        if (all || b.it >= max_iters) break;
+        HIP_TRY(launch_pcg_batch_iterations(b, std::min(poll, max_iters - b.it), c->stream));
+    }
+    for (int j = 0; j < m; ++j) {
+        const double *h = hs + (size_t)j * kPcgScalars;
+        fia_scene *sc = &scenes[idx[j]];
+        const fia_ctx::SceneSolve &sv = c->solves[idx[j]];
+        const double bb = h[kPcgBB] > 0 ? h[kPcgBB] : 1.0;
+        sc->pcg_iters = (int)h[kPcgIters];
+        sc->pcg_residual = sqrt(h[kPcgRR] / bb);
+        sc->used_network = sv.used_network;
+        sc->status = h[kPcgDone] != 0.0 ? FIA_OK : FIA_ERR_NOT_CONVERGED;
+        HIP_TRY(launch_project(sv.pg, sv.st.x, sc->grid.dx, sc->grid.u, sc->grid.v, sc->grid.w, sc->pressure,
+                               c->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return FIA_OK;
+}
+
+}  // namespace fia
 
 extern "C" {
 
@@ -615,6 +797,10 @@ fia_status fia_ctx_destroy(fia_ctx *c) {
     for (int a = 0; a < 4; ++a) c->grid[a].release();
     for (int a = 0; a < 2; ++a) c->genD[a].release();
     c->pcg_b.release(); c->pcg_flags.release(); c->pcgs[0].release(); c->pcgs[1].release();
+    for (auto &sv : c->solves) sv.release();
+    c->solves.clear();
+    c->batch_states.release(); c->batch_map.release(); c->batch_scal.release();
+    if (c->host_batch) (void)hipHostFree(c->host_batch);
     c->bounds.release(); c->stats.release();
     for (auto &sp : c->spans) { (void)hipEventDestroy(sp.a); (void)hipEventDestroy(sp.b); }
     c->spans.clear();
@@ -1148,29 +1334,11 @@ fia_status fia_pressure_project_learned(fia_ctx *c, fia_grid3d *g, const uint8_t
     if ((st = pcg_bind(c, 1, pg, sb)) != FIA_OK) return st;
     const double tol2 = rel_tol * rel_tol;
     const int probe = std::min(5, max_iters);
-    // the network's guess
-    HIP_TRY(u->net.forward(c->pcg_b.p, pg.solid, u->y.p, nullptr, c->stream));
-    HIP_TRY(u->net.guess(u->y.p, pg.pin, c->pcg_flags.p, sa.x, c->stream));
-    HIP_TRY(launch_pcg_scale_guess(sa, c->stream));
-    HIP_TRY(launch_pcg_start(sa, tol2, c->stream));
     double *ha = c->host_scal, *hb = c->host_scal + kPcgScalars;
-    HIP_TRY(launch_pcg_check(sa, c->stream));
-    HIP_TRY(hipMemcpyAsync(ha, sa.scal, kPcgScalars * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    const double bb = ha[kPcgBB] > 0 ? ha[kPcgBB] : 1.0;
-    const double guess_res = sqrt(ha[kPcgRR] / bb), gamma = ha[kPcgGamma];
-    // probe iterations from both starts
-    HIP_TRY(launch_pcg_iterations(sa, probe, c->stream));
-    HIP_TRY(launch_pcg_check(sa, c->stream));
-    HIP_TRY(launch_pcg_guess(sb, nullptr, c->stream));
-    HIP_TRY(launch_pcg_start(sb, tol2, c->stream));
-    HIP_TRY(launch_pcg_iterations(sb, probe, c->stream));
-    HIP_TRY(launch_pcg_check(sb, c->stream));
-    HIP_TRY(hipMemcpyAsync(ha, sa.scal, kPcgScalars * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipMemcpyAsync(hb, sb.scal, kPcgScalars * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    const double ra = sqrt(ha[kPcgRR] / bb), rb = sqrt(hb[kPcgRR] / bb);
-    const bool use_net = ra <= rb;
+    LearnedStart ls;
+    if ((st = learned_start(c, u, pg, sa, sb, tol2, probe, ha, hb, ls)) != FIA_OK) return st;
+    const bool use_net = ls.use_net;
+    const double bb = ls.bb;
     PcgState &ps = use_net ? sa : sb;
     double *hs = use_net ? ha : hb;
     if ((st = pcg_run(c, ps, max_iters, hs)) != FIA_OK) return st;
@@ -1197,10 +1365,10 @@ fia_status fia_pressure_project_learned(fia_ctx *c, fia_grid3d *g, const uint8_t
     if (info) {
         info->used_network = use_net ? 1 : 0;
         info->probe_iters = probe;
-        info->guess_rel_residual = guess_res;
-        info->guess_scale = gamma;
-        info->probe_res_network = ra;
-        info->probe_res_zero = rb;
+        info->guess_rel_residual = ls.guess_res;
+        info->guess_scale = ls.gamma;
+        info->probe_res_network = ls.ra;
+        info->probe_res_zero = ls.rb;
     }
     if (!converged) return fail(FIA_ERR_NOT_CONVERGED, "PCG reached max_iters before rel_tol");
     return FIA_OK;

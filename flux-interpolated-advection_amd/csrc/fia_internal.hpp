@@ -183,6 +183,18 @@ hipError_t launch_pcg_scale_guess(const PcgState &st, hipStream_t s);
 hipError_t launch_pcg_start(PcgState &st, double tol2, hipStream_t s);
 hipError_t launch_pcg_iterations(PcgState &st, int count, hipStream_t s);
 hipError_t launch_pcg_check(const PcgState &st, hipStream_t s);
+// Many started solves advanced together (all at the same iteration), one launch per kernel and
+// iteration: their states and, per block of the update and direction launches, (solve, block of
+// that solve), all in device memory.
+struct PcgBatch {
+    const PcgState *states;
+    const int2 *map, *dir_map;   // blocks x (solve, block), dir_blocks x (solve, block)
+    int n, blocks, dir_blocks;
+    int it;
+};
+hipError_t launch_pcg_batch_iterations(PcgBatch &b, int count, hipStream_t s);
+// the convergence test of every solve, then out[s * kPcgScalars ...] = scalars of solve s
+hipError_t launch_pcg_batch_check(const PcgBatch &b, double *out, hipStream_t s);
 hipError_t launch_project(const PcgGrid &g, const double *p, double h, float *u, float *v, float *w,
                           float *pressure_out, hipStream_t s);
 hipError_t launch_zero_solid_faces(const PcgGrid &g, float *u, float *v, float *w, hipStream_t s);
@@ -194,6 +206,12 @@ namespace fia {
 int ctx_device(const fia_ctx *c);
 hipStream_t ctx_stream(const fia_ctx *c);
 fia_status set_error(fia_status status, const std::string &msg);   // returns status
+// batched pressure solves of many scenes (fia_api.hip): buffers on the main context, set-up of
+// one scene on a worker context, then the joint solve and projection on the main context
+fia_status scenes_solve_alloc(fia_ctx *main, const fia_scene *scenes, int n);
+fia_status scene_solve_setup(fia_ctx *worker, fia_ctx *main, int s, fia_scene *scene, fia_unet *net, double rel_tol,
+                             int max_iters);
+fia_status scenes_solve_run(fia_ctx *main, fia_scene *scenes, const int *idx, int m, int max_iters);
 
 // diagnostics (fia_io.hip): FP64 sums of up to four fields (out[0..3]) and of the pressure over
 // the six boundary layers of cells (out[0..5]); part holds io_partial_words() doubles

@@ -81,13 +81,13 @@ __device__ __forceinline__ void block_sum(double (&v)[NV]) {
     __syncthreads();
 }
 
-// Writes this block's sums of v into part[a * nb + block].
+// Writes block blk's sums of v into part[a * nb + blk].
 template <int NV>
-__device__ __forceinline__ void write_partials(double (&v)[NV], double *part, int nb) {
+__device__ __forceinline__ void write_partials(double (&v)[NV], double *part, int nb, int blk) {
     block_sum<NV>(v);
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int a = 0; a < NV; ++a) part[(size_t)a * nb + blockIdx.x] = v[a];
+        for (int a = 0; a < NV; ++a) part[(size_t)a * nb + blk] = v[a];
     }
 }
 
@@ -141,15 +141,15 @@ __device__ __forceinline__ void residual_state(const PcgState &st, int k, double
     }
 }
 
-// Cells of a block.  Workgroups are dispatched to the 8 XCDs round-robin, so block b runs on
-// XCD b % 8; each XCD gets one contiguous eighth of the cells (x-slabs), which its blocks sweep
+// Cells of block blk of a solve.  Workgroups are dispatched to the 8 XCDs round-robin, so block b
+// runs on XCD b % 8 (for a solve launched alone, blk is the block index); each XCD gets one contiguous eighth of the cells (x-slabs), which its blocks sweep
 // together, so the y and x neighbours a block reads were fetched into the same XCD's L2 by its
 // sibling blocks.  nblocks is a multiple of 8.
 struct CellRange {
     size_t first, end, step;
 };
-__device__ __forceinline__ CellRange block_cells(size_t n, int nblocks) {
-    const int xcd = blockIdx.x % 8, slot = blockIdx.x / 8, per = nblocks / 8;
+__device__ __forceinline__ CellRange block_cells(size_t n, int nblocks, int blk) {
+    const int xcd = blk % 8, slot = blk / 8, per = nblocks / 8;
     const size_t chunk = (n + 7) / 8;
     const size_t lo = min(n, (size_t)xcd * chunk), hi = min(n, lo + chunk);
     return {lo + (size_t)slot * kPB + threadIdx.x, hi, (size_t)per * kPB};
@@ -175,12 +175,12 @@ __device__ __forceinline__ double stencil(unsigned f, size_t c, long sx, long sy
 }
 
 // r = b - L x, p_old = 0, and partial rz, rr, bb; block 0 resets the tolerance and the flags
-__global__ void __launch_bounds__(kPB) pcg_start_kernel(PcgState st, double tol2) {
+__device__ __forceinline__ void pcg_start_body(const PcgState &st, int blk, double tol2) {
     const size_t n = (size_t)st.g.nx * st.g.ny * st.g.nz;
     const long sx = (long)st.g.ny * st.g.nz, sy = st.g.nz;
     const double *x = st.x;
     double acc[3] = {0.0, 0.0, 0.0};
-    const CellRange cr = block_cells(n, st.nblocks);
+    const CellRange cr = block_cells(n, st.nblocks, blk);
     for (size_t c = cr.first; c < cr.end; c += cr.step) {
         const unsigned f = st.flags[c];
         const double bc = st.b[c];
@@ -192,8 +192,8 @@ __global__ void __launch_bounds__(kPB) pcg_start_kernel(PcgState st, double tol2
         acc[1] += rv * rv;
         acc[2] += bc * bc;
     }
-    write_partials<3>(acc, st.part_start, st.nblocks);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    write_partials<3>(acc, st.part_start, st.nblocks, blk);
+    if (blk == 0 && threadIdx.x == 0) {
         double *sc = st.scal;
         sc[kPcgTol2] = tol2;
         sc[kPcgDone] = 0.0;
@@ -202,7 +202,7 @@ __global__ void __launch_bounds__(kPB) pcg_start_kernel(PcgState st, double tol2
 }
 
 // Convergence test after k iterations (one block): the state the host polls.
-__global__ void __launch_bounds__(kPB) pcg_check_kernel(PcgState st, int k) {
+__device__ __forceinline__ void pcg_check_body(const PcgState &st, int k) {
     if (solve_done(st.scal)) return;
     double rz, rr, bb;
     residual_state(st, k, rz, rr, bb);
@@ -254,15 +254,16 @@ __device__ __forceinline__ bool halo_cell(int t, int y0, int z0, int &y, int &z,
     return false;
 }
 
-__global__ void __launch_bounds__(kPB) pcg_direction_kernel(PcgState st, int k, const double *__restrict__ pold,
-                                                            double *__restrict__ pnew) {
+__device__ __forceinline__ void pcg_direction_body(const PcgState &st, int blk, int k) {
     static_assert(kTileZ * kTileY == kPB * kRows && 2 * (kTileZ + kTileY) <= kPB, "tile shape");
+    const double *__restrict__ pold = st.p[k & 1];
+    double *__restrict__ pnew = st.p[(k + 1) & 1];
     if (solve_done(st.scal)) return;
     double rz, rr, bb;
     residual_state(st, k, rz, rr, bb);
     const bool done = rr <= st.scal[kPcgTol2] * bb;
     const double beta = k == 0 ? 0.0 : rz / st.scal[kPcgRZ + ((k - 1) & 1)];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (blk == 0 && threadIdx.x == 0) {
         double *sc = st.scal;
         sc[kPcgRZ + (k & 1)] = rz;
         sc[kPcgRR] = rr;
@@ -276,7 +277,7 @@ __global__ void __launch_bounds__(kPB) pcg_direction_kernel(PcgState st, int k, 
     // blocks are dispatched to the 8 XCDs round-robin: consecutive logical tiles go to one XCD,
     // so an XCD sweeps whole y-z slabs of neighbouring tiles
     const int per = st.dir_blocks / 8;
-    const int L = (blockIdx.x % 8) * per + blockIdx.x / 8;
+    const int L = (blk % 8) * per + blk / 8;
     constexpr int kRowStep = kTileY / kRows;
     const int ty = threadIdx.x / kTileZ, tz = threadIdx.x % kTileZ;
     double acc[1] = {0.0};
@@ -350,11 +351,12 @@ __global__ void __launch_bounds__(kPB) pcg_direction_kernel(PcgState st, int k, 
             cur ^= 1;
         }
     }
-    write_partials<1>(acc, st.part_dir, st.dir_blocks);
+    write_partials<1>(acc, st.part_dir, st.dir_blocks, blk);
 }
 
 // x += alpha p, r -= alpha q (alpha = rz / p.q), then partial rz, rr.  Outside the system p = q = r = 0, so the update leaves those cells as they are.
-__global__ void __launch_bounds__(kPB) pcg_update_kernel(PcgState st, int k, const double *__restrict__ pnew) {
+__device__ __forceinline__ void pcg_update_body(const PcgState &st, int blk, int k) {
+    const double *__restrict__ pnew = st.p[(k + 1) & 1];
     if (solve_done(st.scal)) return;
     double pq[1];
     total_of<1>(st.part_dir, st.dir_blocks, pq);
@@ -364,7 +366,7 @@ __global__ void __launch_bounds__(kPB) pcg_update_kernel(PcgState st, int k, con
     double *__restrict__ x = st.x;
     double *__restrict__ r = st.r;
     const double *__restrict__ q = st.q;
-    const CellRange cr = block_cells(n, st.nblocks);
+    const CellRange cr = block_cells(n, st.nblocks, blk);
     const size_t stride = cr.step;
     double acc[2] = {0.0, 0.0};
     for (size_t c0 = cr.first; c0 < cr.end; c0 += kUpdCells * stride) {
@@ -388,16 +390,17 @@ __global__ void __launch_bounds__(kPB) pcg_update_kernel(PcgState st, int k, con
             }
         }
     }
-    write_partials<2>(acc, st.part_update, st.nblocks);
+    write_partials<2>(acc, st.part_update, st.nblocks, blk);
 }
 
 // Energy-optimal scaling of an initial guess: partial sums of x.b and x.(L x) ...
 __global__ void __launch_bounds__(kPB) pcg_energy_kernel(PcgState st) {
+    const int blk = blockIdx.x;
     const size_t n = (size_t)st.g.nx * st.g.ny * st.g.nz;
     const long sx = (long)st.g.ny * st.g.nz, sy = st.g.nz;
     const double *x = st.x;
     double acc[2] = {0.0, 0.0};
-    const CellRange cr = block_cells(n, st.nblocks);
+    const CellRange cr = block_cells(n, st.nblocks, blk);
     for (size_t c = cr.first; c < cr.end; c += cr.step) {
         const unsigned f = st.flags[c];
         const double s = stencil(f, c, sx, sy, [&](size_t i) { return x[i]; });
@@ -406,18 +409,19 @@ __global__ void __launch_bounds__(kPB) pcg_energy_kernel(PcgState st) {
             acc[1] += x[c] * s;
         }
     }
-    write_partials<2>(acc, st.part_start, st.nblocks);
+    write_partials<2>(acc, st.part_start, st.nblocks, blk);
 }
 
 // ... then x *= gamma = x.b / x.L x, the factor that minimises the energy (the A-norm error) of
 // gamma x; 0 when x.L x is not positive.  Block 0 keeps gamma in the scalars.
 __global__ void __launch_bounds__(kPB) pcg_scale_kernel(PcgState st) {
+    const int blk = blockIdx.x;
     double tot[2];
     total_of<2>(st.part_start, st.nblocks, tot);
     const double gamma = tot[1] > 0.0 ? tot[0] / tot[1] : 0.0;
-    if (blockIdx.x == 0 && threadIdx.x == 0) st.scal[kPcgGamma] = gamma;
+    if (blk == 0 && threadIdx.x == 0) st.scal[kPcgGamma] = gamma;
     const size_t n = (size_t)st.g.nx * st.g.ny * st.g.nz;
-    const CellRange cr = block_cells(n, st.nblocks);
+    const CellRange cr = block_cells(n, st.nblocks, blk);
     for (size_t c = cr.first; c < cr.end; c += cr.step) st.x[c] *= gamma;
 }
 
@@ -457,6 +461,38 @@ __global__ void pcg_guess_kernel(PcgState st, const float *p_in) {
     const size_t n = (size_t)st.g.nx * st.g.ny * st.g.nz;
     if (c >= n) return;
     st.x[c] = (p_in && (st.flags[c] & kPcgUnknown)) ? (double)p_in[c] : 0.0;
+}
+
+// One solve per launch ...
+__global__ void __launch_bounds__(kPB) pcg_start_kernel(PcgState st, double tol2) {
+    pcg_start_body(st, blockIdx.x, tol2);
+}
+__global__ void __launch_bounds__(kPB) pcg_direction_kernel(PcgState st, int k) {
+    pcg_direction_body(st, blockIdx.x, k);
+}
+__global__ void __launch_bounds__(kPB) pcg_update_kernel(PcgState st, int k) { pcg_update_body(st, blockIdx.x, k); }
+__global__ void __launch_bounds__(kPB) pcg_check_kernel(PcgState st, int k) { pcg_check_body(st, k); }
+
+// ... or many solves per launch, all at the same iteration: map[b] = (s, j) says that block b of
+// the launch is block j of solve s, so each solve's arithmetic is exactly that of its own launch.
+__global__ void __launch_bounds__(kPB) pcg_direction_batch_kernel(const PcgState *__restrict__ st,
+                                                                  const int2 *__restrict__ map, int k) {
+    const int2 m = map[blockIdx.x];
+    pcg_direction_body(st[m.x], m.y, k);
+}
+__global__ void __launch_bounds__(kPB) pcg_update_batch_kernel(const PcgState *__restrict__ st,
+                                                               const int2 *__restrict__ map, int k) {
+    const int2 m = map[blockIdx.x];
+    pcg_update_body(st[m.x], m.y, k);
+}
+// one block per solve; the solves' polled scalars are then copied to out[s * kPcgScalars ...], so
+// that the host reads all of them with one copy
+__global__ void __launch_bounds__(kPB) pcg_check_batch_kernel(const PcgState *__restrict__ st, int k, double *out) {
+    pcg_check_body(st[blockIdx.x], k);
+    if (threadIdx.x == 0) {   // the thread that wrote the scalars in pcg_check_body
+        const double *sc = st[blockIdx.x].scal;
+        for (int i = 0; i < kPcgScalars; ++i) out[(size_t)blockIdx.x * kPcgScalars + i] = sc[i];
+    }
 }
 
 static inline unsigned blocks(size_t n) { return (unsigned)((n + kPB - 1) / kPB); }
@@ -510,15 +546,28 @@ hipError_t launch_pcg_start(PcgState &st, double tol2, hipStream_t s) {
 }
 hipError_t launch_pcg_iterations(PcgState &st, int count, hipStream_t s) {
     for (int k = 0; k < count; ++k) {
-        double *pold = st.p[st.it & 1], *pnew = st.p[(st.it + 1) & 1];
-        hipLaunchKernelGGL(pcg_direction_kernel, dim3(st.dir_blocks), dim3(kPB), 0, s, st, st.it, pold, pnew);
-        hipLaunchKernelGGL(pcg_update_kernel, dim3(st.nblocks), dim3(kPB), 0, s, st, st.it, pnew);
+        hipLaunchKernelGGL(pcg_direction_kernel, dim3(st.dir_blocks), dim3(kPB), 0, s, st, st.it);
+        hipLaunchKernelGGL(pcg_update_kernel, dim3(st.nblocks), dim3(kPB), 0, s, st, st.it);
         ++st.it;
     }
     return hipGetLastError();
 }
 hipError_t launch_pcg_check(const PcgState &st, hipStream_t s) {
     hipLaunchKernelGGL(pcg_check_kernel, dim3(1), dim3(kPB), 0, s, st, st.it);
+    return hipGetLastError();
+}
+hipError_t launch_pcg_batch_iterations(PcgBatch &b, int count, hipStream_t s) {
+    if (b.n == 0) return hipSuccess;
+    for (int k = 0; k < count; ++k) {
+        hipLaunchKernelGGL(pcg_direction_batch_kernel, dim3(b.dir_blocks), dim3(kPB), 0, s, b.states, b.dir_map, b.it);
+        hipLaunchKernelGGL(pcg_update_batch_kernel, dim3(b.blocks), dim3(kPB), 0, s, b.states, b.map, b.it);
+        ++b.it;
+    }
+    return hipGetLastError();
+}
+hipError_t launch_pcg_batch_check(const PcgBatch &b, double *out, hipStream_t s) {
+    if (b.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(pcg_check_batch_kernel, dim3(b.n), dim3(kPB), 0, s, b.states, b.it, out);
     return hipGetLastError();
 }
 hipError_t launch_project(const PcgGrid &g, const double *p, double h, float *u, float *v, float *w,

@@ -159,3 +159,42 @@ def test_scenes_step_is_independent_of_workers(fia, ctx):
     finally:
         for c in ctxs[1:]:
             c.close()
+
+
+@pytest.mark.parametrize("learned", [False, True])
+def test_batched_scene_step_matches_per_scene_solves(fia, ctx, learned):
+    # the scenes' pressure solves batched into one launch per kernel and iteration give the same
+    # bits as every scene solved on its own: each solve keeps its block decomposition and
+    # reduction order, and the learned start (network guess, probes, choice) is the same
+    import importlib
+    import os
+    batch = importlib.import_module("flux-interpolated-advection_amd.batch")
+    print("case scenes batched vs per-scene solves, learned=%s" % learned, flush=True)
+    od = fia.make_opts(split_method="xz-CellFaceAligned")
+    ov = fia.make_opts(interpolation_method="WENO6")
+    ctxs = [ctx] + [fia.Context(0) for _ in range(2)]
+    nets = None
+    try:
+        if learned:
+            weights = os.path.join(os.path.dirname(batch.__file__), "weights", "unet_plume128.npy")
+            nets = [fia.UNet(c, seed=0) for c in ctxs]
+            for u in nets:
+                u.load(weights)
+        sets = [batch.SceneSet(5, seed=17, edges=(16, 24, 32)) for _ in range(2)]
+        for _ in range(3):
+            sets[0].step(ctxs, od, ov, nets=nets)
+            sets[1].step(ctxs, od, ov, nets=nets, batched=True)
+        for a, b in zip(sets[0].tensors, sets[1].tensors):
+            for k in ("rho", "u", "v", "w", "p"):
+                assert a[k].equal(b[k]), k
+        for a, b in zip(sets[0].array, sets[1].array):
+            assert (a.steps, a.status, a.pcg_iters, a.frame) == (b.steps, b.status, b.pcg_iters, b.frame)
+            assert (a.time, a.dt, a.pcg_residual, a.used_network) == (b.time, b.dt, b.pcg_residual, b.used_network)
+            assert b.steps == 3 and b.status == 0 and b.pcg_residual <= 1e-6
+        if learned:
+            print("network used in %d of 5 scenes" % sum(s.used_network for s in sets[1].array), flush=True)
+    finally:
+        for u in nets or []:
+            u.close()
+        for c in ctxs[1:]:
+            c.close()
