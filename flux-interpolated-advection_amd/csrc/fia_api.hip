@@ -200,7 +200,10 @@ fia_status check_kernel_errors(fia_ctx *c) {
     int *h = c->host_flags;
     HIP_TRY(hipMemcpyAsync(h, c->counters.p, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    if (h[1] == KE_CAPACITY) return fail(FIA_ERR_CAPACITY, "face cutter capacity exceeded (polygon or stack)");
+    if (h[1] == KE_CAPACITY) return fail(FIA_ERR_CAPACITY, "more pieces than max_pieces");
+    if (h[1] == KE_CAP_VERTS) return fail(FIA_ERR_CAPACITY, "face cutter: a piece has more than 30 vertices");
+    if (h[1] == KE_CAP_STACK) return fail(FIA_ERR_CAPACITY, "face cutter: too many pending pieces");
+    if (h[1] == KE_CAP_TRIS) return fail(FIA_ERR_CAPACITY, "face cutter: too many pending triangles (minimal_area)");
     if (h[1] == KE_CUT)
         return fail(FIA_ERR_CUT, "a cut line crossed a piece's boundary an odd number of times "
                                  "(the reference's cutter stops the program here: src/facecutter.h:190-203)");

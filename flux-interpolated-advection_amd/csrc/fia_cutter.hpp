@@ -318,7 +318,10 @@ FIA_HD bool face_degenerate(const double col[4][2]) {
 // subdivide (src/facecutter.h:255-324), in FP64.  Used for twisted faces and minimal_area < 1.
 // Pieces come out in a different order than the reference's depth-first order; the set is the
 // same.  Capacities: kGenVerts vertices per polygon (the reference's MAX_TMP_POINTS), kGenStack
-// pending polygons, kGenTris pending triangles; exceeding one returns false.
+// pending polygons, kGenTris pending triangles; exceeding one returns its KE_CAP_* code.  The
+// pieces below a cut line (which hold no more interior integers along that axis) are cut to the
+// end before the remainder above it, so the stack holds a few remainders however many lines
+// cross the face.
 // ---------------------------------------------------------------------------------------------
 constexpr int kGenVerts = 32;
 constexpr int kGenStack = 8;
@@ -350,9 +353,10 @@ FIA_HD void gen_center(const double (*p)[2], int num, double &cx, double &cy, do
     }
 }
 
-// emit(pts, num, centre_xi, centre_eta) for every piece.  Returns KE_NONE, KE_CAPACITY (a polygon,
-// the stack or the triangle list overflowed) or KE_CUT (a cut line crossed a piece's boundary an
-// odd number of times; the reference stops the program there, src/facecutter.h:190-203).
+// emit(pts, num, centre_xi, centre_eta) for every piece.  Returns KE_NONE, KE_CAP_VERTS,
+// KE_CAP_STACK, KE_CAP_TRIS (a polygon, the stack or the triangle list overflowed) or KE_CUT (a
+// cut line crossed a piece's boundary an odd number of times; the reference stops the program
+// there, src/facecutter.h:190-203).
 template <class Emit>
 FIA_HD int general_cut(const double col[4][2], double min_area, Emit &emit) {
     GenPoly stack[kGenStack];
@@ -376,6 +380,13 @@ FIA_HD int general_cut(const double col[4][2], double min_area, Emit &emit) {
         interior_integers(lo, hi, first, last);
         if (first <= last) {
             const double iso = first;
+            // arr0 is the piece on v[0]'s side, closed last; when that side is above the cut its
+            // stack slot is taken now, under the pieces below the cut
+            int reserved = -1;
+            if (!((dim == 0 ? cur.v[0].cx : cur.v[0].cz) - iso < 0.0)) {
+                if (sp >= kGenStack) return KE_CAP_STACK;
+                reserved = sp++;
+            }
             arr0.n = 0; arr1.n = 0; arr0.dim = dim; arr1.dim = dim;
             int slot = 0;
             arr0.v[arr0.n++] = cur.v[0];
@@ -398,7 +409,7 @@ FIA_HD int general_cut(const double col[4][2], double min_area, Emit &emit) {
                         if (t1 < 1.0) arr1.v[arr1.n++] = add;
                     } else {
                         arr1.v[arr1.n++] = add;
-                        if (sp >= kGenStack) return KE_CAPACITY;
+                        if (sp >= kGenStack) return KE_CAP_STACK;
                         stack[sp++] = arr1;
                         slot = 0;
                         if (t1 < 1.0) arr0.v[arr0.n++] = add;
@@ -408,14 +419,18 @@ FIA_HD int general_cut(const double col[4][2], double min_area, Emit &emit) {
                     GenPoly &dst = slot == 0 ? arr0 : arr1;
                     dst.v[dst.n++] = cur.v[m];
                 }
-                if (arr0.n >= kGenVerts - 2 || arr1.n >= kGenVerts - 2) return KE_CAPACITY;
+                if (arr0.n >= kGenVerts - 2 || arr1.n >= kGenVerts - 2) return KE_CAP_VERTS;
             }
             if (slot != 0) return KE_CUT;   // odd number of sign changes: cannot happen on a loop
-            if (sp >= kGenStack) return KE_CAPACITY;
-            stack[sp++] = arr0;
+            if (reserved >= 0) {
+                stack[reserved] = arr0;
+            } else {
+                if (sp >= kGenStack) return KE_CAP_STACK;
+                stack[sp++] = arr0;
+            }
         } else if (dim == 0) {
             cur.dim = 1;
-            if (sp >= kGenStack) return KE_CAPACITY;
+            if (sp >= kGenStack) return KE_CAP_STACK;
             stack[sp++] = cur;
         } else if (num >= 3) {
             // subdivide (reference src/facecutter.h:255-324)
@@ -432,7 +447,7 @@ FIA_HD int general_cut(const double col[4][2], double min_area, Emit &emit) {
                 if (area > min_area) {
                     for (int n = num - 1; n >= 0; --n) {   // pushed in reverse: popped in polygon order
                         const int m = (n + 1 == num) ? 0 : n + 1;
-                        if (tp >= kGenTris) return KE_CAPACITY;
+                        if (tp >= kGenTris) return KE_CAP_TRIS;
                         tri[tp][0][0] = cx; tri[tp][0][1] = cy;
                         tri[tp][1][0] = pts[n][0]; tri[tp][1][1] = pts[n][1];
                         tri[tp][2][0] = pts[m][0]; tri[tp][2][1] = pts[m][1];
@@ -460,7 +475,7 @@ FIA_HD int general_cut(const double col[4][2], double min_area, Emit &emit) {
                     }
                     const int i1 = (imax + 1) % 3, i2 = (imax + 2) % 3;
                     const double mx = 0.5 * (t[imax][0] + t[i1][0]), my = 0.5 * (t[imax][1] + t[i1][1]);
-                    if (tp + 2 > kGenTris) return KE_CAPACITY;
+                    if (tp + 2 > kGenTris) return KE_CAP_TRIS;
                     // tri1 = {mid, p1, p2} pushed first so tri0 = {p0, mid, p2} is processed first
                     tri[tp][0][0] = mx; tri[tp][0][1] = my;
                     tri[tp][1][0] = t[i1][0]; tri[tp][1][1] = t[i1][1];

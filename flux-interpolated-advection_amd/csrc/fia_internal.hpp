@@ -110,7 +110,11 @@ struct FaceParams {
 };
 
 // errors reported by kernels through FaceParams::error
-enum KernelError { KE_NONE = 0, KE_CAPACITY = 3, KE_NONFINITE = 4, KE_CUT = 8 };
+// KE_CAPACITY: a caller's piece array (fia_split_faces); KE_CAP_*: the general cutter's polygon
+// vertices, stack of pending polygons, list of pending triangles
+enum KernelError {
+    KE_NONE = 0, KE_CAPACITY = 3, KE_NONFINITE = 4, KE_CAP_VERTS = 5, KE_CAP_STACK = 6, KE_CAP_TRIS = 7, KE_CUT = 8
+};
 
 hipError_t launch_psi(const PsiParams &p, hipStream_t s);
 hipError_t launch_vertices(const VertexParams &p, hipStream_t s);

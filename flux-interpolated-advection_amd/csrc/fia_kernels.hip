@@ -4,7 +4,7 @@
 //   psi_kernel       y-prefix psi on the y-faces of G, one thread per (x,z) column, FP64 sums
 //   vertex_kernel    backtraced displacement of every lattice vertex, one thread per vertex
 //   face_kernel      one thread per unique face: FP64 cut into pieces + quadrature -> face flux
-//   general_kernel   the exact recursive cutter for the few twisted faces (small fixed grid)
+//   general_kernel   the exact recursive cutter for the few twisted faces (grid-stride, one wave per SIMD)
 //   gather_kernel    per cell: sum of its six signed face fluxes (deterministic, no atomics)
 // Each interior face is computed once; its flux enters the two neighbouring cells with opposite
 // signs, so nothing is lost or created inside the domain.
@@ -227,7 +227,7 @@ FIA_HD double piece_flux_weno_integral(const FaceParams &P, const double c[3], d
     const double gy = c[1] + ey;
     const double jf = floor(gy);
     const double f = gy - jf;
-    const int J = (int)jf;
+    const int J = floor_index(jf);
     const bool zin = bz - 2 >= 0 && bz + 3 <= mz - 1;   // the z rows need no clamping
     // Stencils wholly inside the field (the common case) take their rows at 32-bit offsets from
     // the field's base, one add per row; the others are clamped row by row.
@@ -588,7 +588,11 @@ FIA_HD void piece_body(const FaceParams &P, unsigned long long q, int &npieces, 
     }
     P.piece_val[q] = val;
 }
-template <int QM> __global__ void __launch_bounds__(kBlock) piece_kernel(FaceParams P, unsigned long long nslots) {
+// The one-point WENO6 integral rule (QM = 1) is held to 128 VGPRs, four waves per SIMD, which its
+// VALU-bound quadrature needs to hide the row loads.
+template <int QM>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(QM == 1 ? 4 : 1)))
+piece_kernel(FaceParams P, unsigned long long nslots) {
     const unsigned long long q = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
     int npieces = 0, npts = 0, nevals = 0;
     if (q < nslots) piece_body<QM>(P, q, npieces, npts, nevals);
@@ -697,8 +701,8 @@ FIA_HD double general_face_flux(const FaceParams &P, const double v[4][3], const
 // (src/backtrace_and_calc_all_quadrature_point_list_for_integrate_density.cpp:77-112); for a
 // twisted face its cutter's pieces depend on that order, so both evaluations are done here: the
 // lower cell's ('+' order v0 v1 v2 v3) goes to flux, the upper cell's (v0 v3 v2 v1) enters
-// through corr = upper + flux.  Small fixed grid: the private stack is large and only a few
-// faces need it.
+// through corr = upper + flux.  A grid-stride loop over the listed faces: the private stack is
+// large and usually only a few faces need it.
 FIA_HD void general_body(const FaceParams &P, int t) {
     const unsigned code = P.twisted[t];
     const int fam = (int)(code >> 30);   // 0, 1 or 2
@@ -1086,7 +1090,9 @@ hipError_t launch_faces_degenerate(const FaceParams &p, hipStream_t s) {
 }
 
 hipError_t launch_faces_general(const FaceParams &p, hipStream_t s) {
-    hipLaunchKernelGGL(general_kernel, dim3(256), dim3(64), 0, s, p);
+    // one wave per block, one wave per SIMD at this kernel's register count: 1024 blocks fill the
+    // 256 CUs (the private stack of every lane of a full chip is about 1 GB of scratch)
+    hipLaunchKernelGGL(general_kernel, dim3(1024), dim3(64), 0, s, p);
     return hipGetLastError();
 }
 

@@ -10,10 +10,11 @@
 // reference's form (src/WENO6_interpolation_3d.cpp:63-123: Newton-form cubic candidates,
 // ideal weights C0 = d4 d5/20, C1 = -d5 d0/10, C2 = d0 d1/20, smoothness indicators /180,
 // alpha = C/(1e-6+S)^2); the smoothness indicators annihilate constants, so they are rewritten
-// as quadratic forms of the differences.  Evaluated that way nothing cancels when the samples
-// are large and close together (the y-prefix psi reaches n*q at the top of a column), so psi
-// can be interpolated with FP32 arithmetic around an FP64 base.  kappa rescales S to the
-// reference's physical units (S_phys = kappa * S) so the 1e-6 epsilon keeps its meaning.
+// in the differences (a common squared first difference plus forms of second differences).
+// Evaluated that way nothing cancels when the samples are large and close together (the
+// y-prefix psi reaches n*q at the top of a column), so psi can be interpolated with FP32
+// arithmetic around an FP64 base.  kappa rescales S to the reference's physical units
+// (S_phys = kappa * S) so the 1e-6 epsilon keeps its meaning.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -36,49 +37,57 @@ FIA_HD float fast_rcp(float x) {
 }
 
 // Coefficients of one WENO6 evaluation that depend only on the fraction t (shared by every
-// evaluation of a pass).  With d_i = t + 2 - i: candidate r is f_2 + g-linear Newton terms with
-// the products a_r = d_r d_{r+1} / 2 and b_r = d_r d_{r+1} d_{r+2} / 6; the ideal weights are
-// C0 = d4 d5 / 20, C1 = -d5 d0 / 10, C2 = d0 d1 / 20 (all positive on [0,1]).
+// evaluation of a pass).  With d_i = t + 2 - i the ideal weights are C0 = d4 d5 / 20,
+// C1 = -d5 d0 / 10, C2 = d0 d1 / 20 (all positive on [0,1]).  Candidate r (the Newton-form cubic
+// on samples r..r+3) equals f_2 + t g_2 + A h_r + B_r D_r with the second and third differences
+// h_k = g_{k+1} - g_k, D_k = h_{k+1} - h_k, A = t (t-1) / 2 and B_0 = t (t-1) (t+4) / 6,
+// B_1 = t (t-1) (t+1) / 6, B_2 = t (t-1) (t-2) / 6.  eps is the reference's 1e-6 in the units of
+// the indicators below (S_phys = kappa S / 180); kappa in (0, 1e3] keeps eps^4 a normal float.
 struct WenoCoef {
-    float t, t1, a0, b0, a1, b1, a2, b2, c0, c1, c2, k;
+    float t, A, B0, B1, B2, c0, c1, c2, eps;
     FIA_HD explicit WenoCoef(float tt, float kappa) {
-        const float d0 = tt + 2.0f, d1 = tt + 1.0f, d2 = tt, d3 = tt - 1.0f, d4 = tt - 2.0f, d5 = tt - 3.0f;
-        t = tt; t1 = d1;
-        a0 = d0 * d1 * 0.5f; b0 = d0 * d1 * d2 * (1.0f / 6.0f);
-        a1 = d1 * d2 * 0.5f; b1 = d1 * d2 * d3 * (1.0f / 6.0f);
-        a2 = d2 * d3 * 0.5f; b2 = d2 * d3 * d4 * (1.0f / 6.0f);
-        c0 = d4 * d5 * (1.0f / 20.0f);
-        c1 = -d5 * d0 * (1.0f / 10.0f);
-        c2 = d0 * d1 * (1.0f / 20.0f);
-        k = kappa * (1.0f / 180.0f);
+        const float p = tt * (tt - 1.0f);
+        t = tt;
+        A = 0.5f * p;
+        B0 = p * (tt + 4.0f) * (1.0f / 6.0f);
+        B1 = p * (tt + 1.0f) * (1.0f / 6.0f);
+        B2 = p * (tt - 2.0f) * (1.0f / 6.0f);
+        c0 = (tt - 2.0f) * (tt - 3.0f) * (1.0f / 20.0f);
+        c1 = (3.0f - tt) * (tt + 2.0f) * (1.0f / 10.0f);
+        c2 = (tt + 2.0f) * (tt + 1.0f) * (1.0f / 20.0f);
+        eps = 1.8e-4f * fast_rcp(kappa);
     }
 };
 
 // delta = WENO6(f)(t) - f_2 from the five first differences g_k = f_{k+1} - f_k.
+//
+// The reference's smoothness indicators (x180), quadratic forms of the four samples of each
+// candidate, are in these differences the common 180 g_2^2 plus a form of two second differences:
+//   S_0 = 180 g_2^2 + 244 h_0^2 - 683 h_0 h_1 + 634 h_1^2
+//   S_1 = 180 g_2^2 + 244 h_1^2 - 293 h_1 h_2 + 244 h_2^2
+//   S_2 = 180 g_2^2 + 634 h_2^2 - 683 h_2 h_3 + 244 h_3^2
+// The weights C_r / (1e-6 + S_phys,r)^2 are normalised, so the factor kappa / 180 of S_phys
+// cancels and e_r = S_r + eps stands in for 1e-6 + S_phys,r.
 FIA_HD float weno6_delta(float g0, float g1, float g2, float g3, float g4, const WenoCoef &w) {
-    // Newton-form cubic candidates on samples {0..3}, {1..4}, {2..5}, relative to f_2
     const float h0 = g1 - g0, h1 = g2 - g1, h2 = g3 - g2, h3 = g4 - g3;
-    const float q0 = (g0 * w.t1 - g1) + h0 * w.a0 + (h1 - h0) * w.b0;
-    const float q1 = g1 * w.t + h1 * w.a1 + (h2 - h1) * w.b1;
-    const float q2 = g2 * w.t + h2 * w.a2 + (h3 - h2) * w.b2;
-    // smoothness indicators (x180): quadratic forms over shared products
-    const float p00 = g0 * g0, p11 = g1 * g1, p22 = g2 * g2, p33 = g3 * g3, p44 = g4 * g4;
-    const float p01 = g0 * g1, p02 = g0 * g2, p12 = g1 * g2, p13 = g1 * g3, p23 = g2 * g3;
-    const float p24 = g2 * g4, p34 = g3 * g4;
-    const float s0 = 244.0f * p00 + 1561.0f * p11 + 814.0f * p22 - 1171.0f * p01 + 683.0f * p02 - 1951.0f * p12;
-    const float s1 = 244.0f * p11 + 961.0f * p22 + 244.0f * p33 - 781.0f * p12 + 293.0f * p13 - 781.0f * p23;
-    const float s2 = 814.0f * p22 + 1561.0f * p33 + 244.0f * p44 - 1951.0f * p23 + 683.0f * p24 - 1171.0f * p34;
-    const float e0 = w.k * s0 + 1e-6f, e1 = w.k * s1 + 1e-6f, e2 = w.k * s2 + 1e-6f;
+    const float D0 = h1 - h0, D1 = h2 - h1, D2 = h3 - h2;
+    const float u0 = w.A * h0 + w.B0 * D0, u1 = w.A * h1 + w.B1 * D1, u2 = w.A * h2 + w.B2 * D2;
+    const float p00 = h0 * h0, p11 = h1 * h1, p22 = h2 * h2, p33 = h3 * h3;
+    const float p01 = h0 * h1, p12 = h1 * h2, p23 = h2 * h3;
+    const float base = 180.0f * (g2 * g2) + w.eps;
+    const float e0 = base + 244.0f * p00 - 683.0f * p01 + 634.0f * p11;
+    const float e1 = base + 244.0f * p11 - 293.0f * p12 + 244.0f * p22;
+    const float e2 = base + 634.0f * p22 - 683.0f * p23 + 244.0f * p33;
     const float E0 = e0 * e0, E1 = e1 * e1, E2 = e2 * e2;
+    const float lin = w.t * g2;
     if (fmaxf(e0, fmaxf(e1, e2)) < 1e9f) {
-        // weights C_r / e_r^2 scaled by E0 E1 E2 (one reciprocal): below e = 1e9 every product
-        // stays under 1e36 and above 1e-36, and the weights are normalised before they meet q
+        // weights C_r / e_r^2 scaled by E0 E1 E2 (one reciprocal): for eps <= e < 1e9 every
+        // product stays between eps^4 and 1e36
         const float a0 = w.c0 * (E1 * E2), a1 = w.c1 * (E0 * E2), a2 = w.c2 * (E0 * E1);
-        const float r = fast_rcp(a0 + a1 + a2);
-        return (a0 * r) * q0 + (a1 * r) * q1 + (a2 * r) * q2;
+        return lin + (a0 * u0 + a1 * u1 + a2 * u2) * fast_rcp(a0 + a1 + a2);
     }
     const float a0 = w.c0 * fast_rcp(E0), a1 = w.c1 * fast_rcp(E1), a2 = w.c2 * fast_rcp(E2);
-    return (a0 * q0 + a1 * q1 + a2 * q2) * fast_rcp(a0 + a1 + a2);
+    return lin + (a0 * u0 + a1 * u1 + a2 * u2) * fast_rcp(a0 + a1 + a2);
 }
 
 // WENO6 of six samples of type V (float, or double for the y-prefix psi).
@@ -199,10 +208,16 @@ template <class V, class L> FIA_HD V weno1dy(const L &ld, int cx, int by, int cz
     return weno6<V>(f, ty, kappa);
 }
 
+// floor(s) as an index, held to |b| <= 2^24 so that b plus a stencil offset never overflows an
+// int (a huge but finite s, such as the centroid of a sliver piece, would otherwise convert to
+// INT_MAX and wrap in the interior tests of the stencil loads).  A stencil that far outside a
+// field clamps every sample to the same edge sample, so the value is that of the exact floor.
+FIA_HD int floor_index(double s) { return (int)fmin(fmax(floor(s), -16777216.0), 16777216.0); }
+
 // floor split of a double sample coordinate into base and fraction
 FIA_HD void split_coord(double s, int &b, float &t) {
     const double f = floor(s);
-    b = (int)f;
+    b = floor_index(s);
     t = (float)(s - f);
 }
 
@@ -219,11 +234,11 @@ FIA_HD V interp_point(const L &ld, int method, double px, double py, double pz, 
     split_coord(py - oy, by, ty);
     if (method == IM_1DY_LINEAR) {
         // column of the containing cell: floor(p - (o - 1/2))
-        const int cx = (int)floor(px - ox + 0.5), cz = (int)floor(pz - oz + 0.5);
+        const int cx = floor_index(px - ox + 0.5), cz = floor_index(pz - oz + 0.5);
         return linear1dy<V>(ld, cx, by, cz, ty);
     }
     if (method == IM_1DY_WENO6) {
-        const int cx = (int)floor(px), cz = (int)floor(pz);
+        const int cx = floor_index(px), cz = floor_index(pz);
         return weno1dy<V>(ld, cx, by, cz, ty, kappa);
     }
     split_coord(px - ox, bx, tx);

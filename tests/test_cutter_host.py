@@ -1,0 +1,72 @@
+"""The library's general face cutter (csrc/fia_cutter.hpp) built for the host and checked against
+the oracle's recursive cutter (a restatement of reference src/facecutter.h:59-324) on twisted
+faces whose corner colours span many cells: the set of pieces must be the same, and no capacity
+of the cutter may run out where the reference's recursion would not."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module")
+def cutter(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("cutter") / "cutter_host.so")
+    cmd = ["/opt/rocm/bin/hipcc", "-x", "c++", "-O2", "-std=c++17", "-fPIC", "-shared", "-D__HIP_PLATFORM_AMD__",
+           "-I/opt/rocm/include", os.path.join(HERE, "cutter_host.cpp"), "-o", out]
+    if not os.path.exists(cmd[0]):
+        pytest.skip("hipcc not found")
+    subprocess.run(cmd, check=True, capture_output=True)
+    L = ctypes.CDLL(out)
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.host_general_cut.argtypes = [dp, ctypes.c_double, ctypes.c_int, dp, dp]
+    L.host_general_cut.restype = ctypes.c_int
+    L.host_split_coord.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                   ctypes.POINTER(ctypes.c_float)]
+    L.host_split_coord.restype = ctypes.c_int
+    return L
+
+
+def _cut(L, col, max_pieces=1024):
+    col = np.ascontiguousarray(col, dtype=np.float64).reshape(8)
+    cen = np.zeros((max_pieces, 2)); area = np.zeros(max_pieces)
+    dp = ctypes.POINTER(ctypes.c_double)
+    n = L.host_general_cut(col.ctypes.data_as(dp), 1.0, max_pieces, cen.ctypes.data_as(dp), area.ctypes.data_as(dp))
+    return n, cen[:max(n, 0)], area[:max(n, 0)]
+
+
+@pytest.mark.parametrize("span", [0.8, 3.0, 8.0])
+def test_general_cutter_matches_oracle_on_stretched_faces(cutter, oracle, span):
+    # random corner colours within +-span cells: most of these quads are twisted (bow ties) and
+    # are crossed by up to 2 span integer lines in each direction
+    rng = np.random.default_rng(int(span * 10))
+    faces = 300 if span > 4 else 600
+    for t in range(faces):
+        col = np.array([10.3, 20.7]) + span * rng.uniform(-1.0, 1.0, size=(4, 2))
+        n, cen, area = _cut(cutter, col)
+        assert n >= 0, (t, n)
+        ref = oracle.cut_face_xieta(col, min_area=1.0, cprec=0, max_pieces=1024)
+        rc = np.array([c for _, c, _ in ref]).reshape(-1, 2)
+        ra = np.array([a for _, _, a in ref])
+        assert n == len(ref), t
+        key = lambda c: np.lexsort((np.round(c[:, 1], 9), np.round(c[:, 0], 9)))
+        assert np.abs(cen[key(cen)] - rc[key(rc)]).max(initial=0.0) < 1e-12, t
+        assert np.abs(area[key(cen)] - ra[key(rc)]).max(initial=0.0) < 1e-12, t
+        assert abs(area.sum() - ra.sum()) < 1e-12
+
+
+def test_split_coord_keeps_far_points_outside_every_field(cutter):
+    # a sliver piece can have a huge but finite centroid; its stencil base must stay far outside
+    # every field without overflowing an int (the interior test of the row loads reads b - 2 and
+    # b + 3), and ordinary coordinates split exactly as floor
+    b, t = ctypes.c_int(), ctypes.c_float()
+    for s in (1e12, -1e12, 2.2e9, -2.2e9, 2.0 ** 31, 1e300, -1e300, float("inf"), float("-inf"), float("nan")):
+        inside = cutter.host_split_coord(s, 1 << 20, ctypes.byref(b), ctypes.byref(t))
+        assert not inside and abs(b.value) <= 1 << 24, s
+    for s in (0.0, 2.5, 13.999, 255.25, -0.5, -3.75):
+        inside = cutter.host_split_coord(s, 256, ctypes.byref(b), ctypes.byref(t))
+        assert b.value == int(np.floor(s)) and abs(t.value - (s - np.floor(s))) < 1e-7
+        assert inside == (2 <= b.value <= 252)
