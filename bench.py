@@ -40,10 +40,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "flux-advection Mcell-steps/s at 256^3; step ms NN-warm-PCG vs PCG at 1e-6 res"
 PEAK_FP32_TFLOPS = 157.3
-# Work the quadrature kernel executes: its 1D WENO6 evaluations, counted on the device, at 100 flop
-# each (sample differences, three Newton candidates, three smoothness indicators over 12 shared
-# products, weights, blend; an FMA counts 2, the reciprocal 1).
-FLOP_PER_WENO6_EXECUTED = 100
+# Work the quadrature kernel executes: its 1D WENO6 evaluations, counted on the device, at 73 flop
+# each (an FMA counts 2, the reciprocal 1): the arithmetic of one evaluation in the compiled kernel
+# (five sample differences, second and third differences, three candidates, the common squared
+# first difference and three indicator forms over 7 products, weights, blend).
+FLOP_PER_WENO6_EXECUTED = 73
 # The reference rule's count: every psi point and density point is one 3D WENO6 interpolation,
 # 43 1D evaluations of 200 flop as written in the reference's weno.h:34-68 (8,600 flop per point).
 FLOP_PER_WENO3D_POINT = 43 * 200
@@ -51,7 +52,7 @@ PEAK_HBM_GBS = 8000.0
 # HBM bytes per launch of piece_kernel<1> at the default workload (256^3, CFL 4), from two
 # rocprofv3 counter passes (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a 1 GiB device copy);
 # recorded in profiles/density_step_256_cfl4_hbm_counters.md
-TRAFFIC_BYTES_256_CFL4 = 11.565e9
+TRAFFIC_BYTES_256_CFL4 = 12.043e9
 # Jacobi-PCG algorithmic traffic per cell and iteration (FP64 vectors, two kernels): direction
 # kernel reads r, p_old (8 + 8) and the flag byte, writes p, q (8 + 8); update kernel reads x, r,
 # p, q (4 x 8) and the flag byte, writes x, r (2 x 8)
