@@ -54,9 +54,9 @@ PEAK_HBM_GBS = 8000.0
 # recorded in profiles/density_step_256_cfl4_hbm_counters.md
 TRAFFIC_BYTES_256_CFL4 = 12.043e9
 # Jacobi-PCG algorithmic traffic per cell and iteration (FP64 vectors, two kernels): direction
-# kernel reads r, p_old (8 + 8) and the flag byte, writes p, q (8 + 8); update kernel reads x, r,
-# p, q (4 x 8) and the flag byte, writes x, r (2 x 8)
-PCG_BYTES_PER_CELL_ITER = 82
+# kernel reads r, p_old (8 + 8) and the flag byte, writes p (8); update kernel reads x, r, p
+# (3 x 8) and the flag byte, writes x, r (2 x 8); q = L p is recomputed, not stored
+PCG_BYTES_PER_CELL_ITER = 66
 
 
 def make_scene(scenes, n, cfl, seed=1):
@@ -221,7 +221,7 @@ def run_projection(args, fia, scenes, torch):
         "config": {"workload": "pressure projection, %d^3, Jacobi-PCG (FP64) to 1e-6 relative residual from a zero "
                                "initial guess, curl-noise velocity + buoyancy impulse, 8 spherical obstacles" % n,
                    "grid": [n, n, n]},
-        "pcg": {"iterations": iters, "rel_residual": res, "warm_start": "none (the U-Net warm start is not built)"},
+        "pcg": {"iterations": iters, "rel_residual": res, "warm_start": "none (zero initial guess; --workload warmstart measures the learned warm start)"},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": None,
                      "kernel": "whole Jacobi-PCG iteration (%d B per cell and iteration)" % PCG_BYTES_PER_CELL_ITER},

@@ -97,14 +97,10 @@ struct fia_ctx {
     DevBuf<int> genI;
     DevBuf<double> pcg_b;         // right-hand side of the pressure system
     DevBuf<uint8_t> pcg_flags;    // per-cell neighbour flags of the pressure system
-    struct PcgBufs {              // one CG solve: x, r, p (two buffers), q, partial sums, scalars
-        DevBuf<double> x, r, p0, p1, q, part, scal;
-        size_t bytes() const {
-            return x.bytes() + r.bytes() + p0.bytes() + p1.bytes() + q.bytes() + part.bytes() + scal.bytes();
-        }
-        void release() {
-            x.release(); r.release(); p0.release(); p1.release(); q.release(); part.release(); scal.release();
-        }
+    struct PcgBufs {              // one CG solve: x, r, p (two buffers), partial sums, scalars
+        DevBuf<double> x, r, p0, p1, part, scal;
+        size_t bytes() const { return x.bytes() + r.bytes() + p0.bytes() + p1.bytes() + part.bytes() + scal.bytes(); }
+        void release() { x.release(); r.release(); p0.release(); p1.release(); part.release(); scal.release(); }
     } pcgs[2];
     DevBuf<double> scal;
     DevBuf<unsigned int> bits;
@@ -525,15 +521,14 @@ static fia_status pressure_system(fia_ctx *c, const fia_grid3d *g, const GridVie
 static fia_status bind_solve(fia_ctx::PcgBufs &B, const PcgGrid &pg, const double *b, const uint8_t *flags,
                              PcgState &ps) {
     const size_t n = (size_t)pg.nx * pg.ny * pg.nz;
-    const int nb = pcg_blocks(n);
-    HIP_TRY(B.x.ensure(n)); HIP_TRY(B.r.ensure(n)); HIP_TRY(B.p0.ensure(n)); HIP_TRY(B.p1.ensure(n));
+    ps.nblocks = pcg_blocks(n);
     pcg_direction_tiling(pg.nx, pg.ny, pg.nz, ps);
-    const size_t nd = (size_t)ps.dir_blocks;
-    HIP_TRY(B.q.ensure(n)); HIP_TRY(B.part.ensure(5 * (size_t)nb + nd)); HIP_TRY(B.scal.ensure(kPcgScalars));
+    HIP_TRY(B.x.ensure(n)); HIP_TRY(B.r.ensure(n)); HIP_TRY(B.p0.ensure(n)); HIP_TRY(B.p1.ensure(n));
+    HIP_TRY(B.part.ensure(pcg_partial_words(ps))); HIP_TRY(B.scal.ensure(kPcgScalars));
     ps.g = pg; ps.flags = flags; ps.b = b;
-    ps.x = B.x.p; ps.r = B.r.p; ps.p[0] = B.p0.p; ps.p[1] = B.p1.p; ps.q = B.q.p;
-    ps.scal = B.scal.p; ps.part_start = B.part.p; ps.part_update = B.part.p + 3 * (size_t)nb;
-    ps.part_dir = B.part.p + 5 * (size_t)nb; ps.nblocks = nb; ps.it = 0;
+    ps.x = B.x.p; ps.r = B.r.p; ps.p[0] = B.p0.p; ps.p[1] = B.p1.p;
+    ps.scal = B.scal.p; ps.part_start = B.part.p; ps.part_dir = B.part.p + 3 * (size_t)ps.nblocks;
+    ps.part_update = ps.part_dir + ps.dir_blocks; ps.it = 0;
     return FIA_OK;
 }
 
@@ -664,17 +659,14 @@ fia_status scenes_solve_run(fia_ctx *c, fia_scene *scenes, const int *idx, int m
     for (int j = 0; j < m; ++j) {
         states[j] = c->solves[idx[j]].st;
         if (states[j].it != states[0].it) return fail(FIA_ERR_INVALID_ARG, "scenes started at different iterations");
-        for (int k = 0; k < states[j].nblocks; ++k) map.push_back(make_int2(j, k));
-    }
-    const int blocks = (int)map.size();
-    for (int j = 0; j < m; ++j)
         for (int k = 0; k < states[j].dir_blocks; ++k) map.push_back(make_int2(j, k));
+    }
     HIP_TRY(c->batch_map.ensure(map.size()));
     HIP_TRY(hipMemcpyAsync(c->batch_states.p, states.data(), m * sizeof(PcgState), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(c->batch_map.p, map.data(), map.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
     PcgBatch b;
-    b.states = c->batch_states.p; b.map = c->batch_map.p; b.dir_map = c->batch_map.p + blocks;
-    b.n = m; b.blocks = blocks; b.dir_blocks = (int)map.size() - blocks; b.it = states[0].it;
+    b.states = c->batch_states.p; b.dir_map = c->batch_map.p;
+    b.n = m; b.dir_blocks = (int)map.size(); b.it = states[0].it;
     double *hs = c->host_batch;
     const int poll = 32;
     while (true) {

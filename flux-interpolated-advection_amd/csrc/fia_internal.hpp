@@ -165,13 +165,13 @@ struct PcgState {
     PcgGrid g;
     const uint8_t *flags;
     const double *b;
-    double *x, *r, *p[2], *q;
+    double *x, *r, *p[2];
     double *scal;         // kPcgScalars doubles
     double *part_start;   // 3 * nblocks partial sums (start kernel: rz, rr, bb)
     double *part_dir;     // dir_blocks (direction kernel: p.q)
-    double *part_update;  // 2 * nblocks (update kernel: rz, rr)
-    int nblocks;          // pcg_blocks(n): start and update kernels
-    int dir_blocks, dir_tiles_y, dir_tiles_z, dir_chunk_x, dir_chunks;   // pcg_direction_tiling
+    double *part_update;  // 2 * dir_blocks (update kernel: rz, rr)
+    int nblocks;          // pcg_blocks(n): start kernel and the guess scaling
+    int dir_blocks, dir_tiles_y, dir_tiles_z, dir_chunk_x, dir_chunks;   // pcg_direction_tiling: both iteration kernels
     int it;             // iterations enqueued since the start (selects the p buffer)
 };
 int pcg_blocks(size_t n);
@@ -188,14 +188,16 @@ hipError_t launch_pcg_start(PcgState &st, double tol2, hipStream_t s);
 hipError_t launch_pcg_iterations(PcgState &st, int count, hipStream_t s);
 hipError_t launch_pcg_check(const PcgState &st, hipStream_t s);
 // Many started solves advanced together (all at the same iteration), one launch per kernel and
-// iteration: their states and, per block of the update and direction launches, (solve, block of
-// that solve), all in device memory.
+// iteration: their states and, per block of the launches, (solve, block of that solve), all in
+// device memory.
 struct PcgBatch {
     const PcgState *states;
-    const int2 *map, *dir_map;   // blocks x (solve, block), dir_blocks x (solve, block)
-    int n, blocks, dir_blocks;
+    const int2 *dir_map;   // dir_blocks x (solve, block)
+    int n, dir_blocks;
     int it;
 };
+// partial-sum words of a solve: 3 nblocks (start) + dir_blocks (p.q) + 2 dir_blocks (rz, rr)
+inline size_t pcg_partial_words(const PcgState &st) { return 3 * (size_t)st.nblocks + 3 * (size_t)st.dir_blocks; }
 hipError_t launch_pcg_batch_iterations(PcgBatch &b, int count, hipStream_t s);
 // the convergence test of every solve, then out[s * kPcgScalars ...] = scalars of solve s
 hipError_t launch_pcg_batch_check(const PcgBatch &b, double *out, hipStream_t s);

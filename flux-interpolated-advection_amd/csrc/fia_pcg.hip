@@ -7,14 +7,16 @@
 // neighbour), b = -div(u) h, the last cell pinned to p = 0.  Obstacle cells (optional mask) are
 // removed from the system and their faces act as walls.
 //
-// One CG iteration is two kernels:
+// One CG iteration is two kernels, both one 2.5-D tiled stencil sweep:
 //   direction: beta from the previous residual, p = z + beta p_old (z = r / diag, not stored),
-//              q = L p (2.5-D tiled stencil), partial p.q;
-//   update:    alpha = rz / p.q, x += alpha p, r -= alpha q, partial r.z and r.r.
+//              q = L p, partial p.q;
+//   update:    alpha = rz / p.q, q = L p again (not stored), x += alpha p, r -= alpha q,
+//              partial r.z and r.r.
 // p alternates between two buffers (a thread reads its neighbours' old p while others write the
 // new one).  The matrix is not stored: one flag byte per cell holds its fluid neighbours.  Per
-// cell and iteration that is 8 (r) + 8 (p old) + 1 (flags) read and 8 + 8 (p, q) written by the
-// direction kernel, 8 x 4 + 1 read and 16 written by the update kernel: 82 bytes.
+// cell and iteration that is 8 (r) + 8 (p old) + 1 (flags) read and 8 (p) written by the
+// direction kernel, 8 x 3 (p, x, r) + 1 read and 16 (x, r) written by the update kernel:
+// 66 bytes.
 //
 // Reductions without fences: every block writes its partial sums with plain stores, and every
 // block of the NEXT kernel sums all partials itself, in block order, before its own work (the
@@ -136,7 +138,7 @@ __device__ __forceinline__ void residual_state(const PcgState &st, int k, double
         rz = tot[0]; rr = tot[1]; bb = tot[2];
     } else {
         double tot[2];
-        total_of<2>(st.part_update, st.nblocks, tot);
+        total_of<2>(st.part_update, st.dir_blocks, tot);
         rz = tot[0]; rr = tot[1]; bb = st.scal[kPcgBB];
     }
 }
@@ -215,35 +217,75 @@ __device__ __forceinline__ void pcg_check_body(const PcgState &st, int k) {
     }
 }
 
-// cells per thread and loop step of the update kernel (independent cells, so their loads overlap)
-constexpr int kUpdCells = 4;
-
-// Iteration k: converged test on the residual of iteration k - 1, beta = rz_k / rz_(k-1),
-// p = z + beta p_old, q = L p, partial p.q.  Outside the system r = p_old = 0, so p = 0 there;
-// q is forced to 0 on cells that are not unknowns (the pinned cell has fluid neighbours).
+// Both kernels of an iteration sweep the grid with one 2.5-D stencil: a block owns a tile of
+// kTileZ x kTileY cell columns (each thread kRows of them, kTileY / kRows rows apart) and marches
+// along x over its chunk of planes.  The value v of every cell (and of a one-cell halo around the
+// tile) is computed once per plane: the x neighbours stay in registers, the y and z neighbours are
+// read from the plane in LDS.  The raw data of the plane after next is loaded one step ahead.
+// For each cell of the tile the operation receives v and q = L v (0 on cells that are not
+// unknowns: the pinned cell has fluid neighbours).
 //
-// 2.5-D stencil: a block owns a tile of kTileZ x kTileY cell columns (each thread kRows of them,
-// kTileY / kRows rows apart) and marches along x over its chunk of planes.  The new direction is
-// computed once per cell: the x neighbours stay in registers, the y and z neighbours are read
-// from the plane in LDS (tile plus a one-cell halo).  The raw values (flags, r, p_old) of the
-// plane after next are loaded one step ahead.
-struct Raw {
+// Iteration k, direction kernel: converged test on the residual of iteration k - 1,
+// beta = rz_k / rz_(k-1), v = p = z + beta p_old (outside the system r = p_old = 0, so p = 0
+// there), written out, and partial p.q.
+// Update kernel: v = p read back, q recomputed with the same arithmetic (the same bits as the
+// direction kernel's q, 16 bytes per cell cheaper than storing and reloading it),
+// x += alpha p, r -= alpha q (alpha = rz / p.q), partial rz, rr.
+struct DirRaw {
     unsigned f;
     double r, p;
 };
-
-__device__ __forceinline__ Raw load_raw(const PcgState &st, const double *pold, int x, int y, int z) {
-    Raw a{0u, 0.0, 0.0};
-    if (x >= 0 && x < st.g.nx && y >= 0 && y < st.g.ny && z >= 0 && z < st.g.nz) {
-        const size_t c = idx3(x, y, z, st.g.ny, st.g.nz);
-        a.f = st.flags[c];
-        a.r = st.r[c];
-        a.p = pold[c];
+struct DirOp {
+    using Raw = DirRaw;
+    const PcgState &st;
+    const double *__restrict__ pold;
+    double *__restrict__ pnew;
+    double beta;
+    __device__ __forceinline__ Raw load(int x, int y, int z, bool) const {
+        Raw a{0u, 0.0, 0.0};
+        if (x >= 0 && x < st.g.nx && y >= 0 && y < st.g.ny && z >= 0 && z < st.g.nz) {
+            const size_t c = idx3(x, y, z, st.g.ny, st.g.nz);
+            a.f = st.flags[c];
+            a.r = st.r[c];
+            a.p = pold[c];
+        }
+        return a;
     }
-    return a;
-}
-
-__device__ __forceinline__ double dir_of(const Raw &a, double beta) { return a.r * inv_diag(a.f) + beta * a.p; }
+    __device__ __forceinline__ double value(const Raw &a) const { return a.r * inv_diag(a.f) + beta * a.p; }
+    __device__ __forceinline__ void cell(size_t c, const Raw &, double v, double q, double *acc) const {
+        pnew[c] = v;
+        acc[0] += v * q;
+    }
+};
+struct UpdRaw {
+    unsigned f;
+    double p, x, r;
+};
+struct UpdOp {
+    using Raw = UpdRaw;
+    const PcgState &st;
+    const double *__restrict__ p;
+    double alpha;
+    // x and r only for the tile's own cells (own), not for the halo
+    __device__ __forceinline__ Raw load(int x, int y, int z, bool own) const {
+        Raw a{0u, 0.0, 0.0, 0.0};
+        if (x >= 0 && x < st.g.nx && y >= 0 && y < st.g.ny && z >= 0 && z < st.g.nz) {
+            const size_t c = idx3(x, y, z, st.g.ny, st.g.nz);
+            a.f = st.flags[c];
+            a.p = p[c];
+            if (own) { a.x = st.x[c]; a.r = st.r[c]; }
+        }
+        return a;
+    }
+    __device__ __forceinline__ double value(const Raw &a) const { return a.p; }
+    __device__ __forceinline__ void cell(size_t c, const Raw &a, double v, double q, double *acc) const {
+        const double rv = a.r - alpha * q;
+        st.x[c] = a.x + alpha * v;
+        st.r[c] = rv;
+        acc[0] += rv * (rv * inv_diag(a.f));
+        acc[1] += rv * rv;
+    }
+};
 
 // the halo cell a thread loads (2 kTileZ y-halo cells, then 2 kTileY z-halo cells), or false
 __device__ __forceinline__ bool halo_cell(int t, int y0, int z0, int &y, int &z, int &ly, int &lz) {
@@ -254,8 +296,84 @@ __device__ __forceinline__ bool halo_cell(int t, int y0, int z0, int &y, int &z,
     return false;
 }
 
-__device__ __forceinline__ void pcg_direction_body(const PcgState &st, int blk, int k) {
+// Tile blk of a solve's sweep (dir_blocks tiles; blocks are dispatched to the 8 XCDs round-robin,
+// so consecutive logical tiles go to one XCD, which sweeps whole y-z slabs of neighbouring tiles).
+template <class Op>
+__device__ __forceinline__ void tile_sweep(const PcgState &st, int blk, const Op &op, double *acc) {
     static_assert(kTileZ * kTileY == kPB * kRows && 2 * (kTileZ + kTileY) <= kPB, "tile shape");
+    using Raw = typename Op::Raw;
+    __shared__ double plane[2][kTileY + 2][kTileZ + 2];
+    const int per = st.dir_blocks / 8;
+    const int L = (blk % 8) * per + blk / 8;
+    constexpr int kRowStep = kTileY / kRows;
+    const int ty = threadIdx.x / kTileZ, tz = threadIdx.x % kTileZ;
+    if (L >= st.dir_tiles_y * st.dir_tiles_z * st.dir_chunks) return;
+    const int y0 = (L % st.dir_tiles_y) * kTileY;
+    const int z0 = ((L / st.dir_tiles_y) % st.dir_tiles_z) * kTileZ;
+    const int xa = (L / (st.dir_tiles_y * st.dir_tiles_z)) * st.dir_chunk_x;
+    const int xb = min(st.g.nx, xa + st.dir_chunk_x);
+    const int z = z0 + tz;
+    int hy = 0, hz = 0, hly = 0, hlz = 0;
+    const bool has_halo = halo_cell(threadIdx.x, y0, z0, hy, hz, hly, hlz);
+    const long sx = (long)st.g.ny * st.g.nz;
+    // prologue: plane xa - 1 (own columns), plane xa (own + halo into LDS), raw plane xa + 1
+    double dm[kRows], dc[kRows];
+    Raw oc[kRows], on[kRows];
+    bool active[kRows];
+    size_t c[kRows];
+#pragma unroll
+    for (int w = 0; w < kRows; ++w) {
+        const int y = y0 + ty + w * kRowStep;
+        active[w] = y < st.g.ny && z < st.g.nz;
+        dm[w] = op.value(op.load(xa - 1, y, z, false));
+        oc[w] = op.load(xa, y, z, true);
+        on[w] = op.load(xa + 1, y, z, true);
+        dc[w] = op.value(oc[w]);
+        plane[0][ty + w * kRowStep + 1][tz + 1] = dc[w];
+        c[w] = idx3(xa, min(y, st.g.ny - 1), min(z, st.g.nz - 1), st.g.ny, st.g.nz);
+    }
+    const Raw hc = has_halo ? op.load(xa, hy, hz, false) : Raw{};
+    Raw hn = has_halo ? op.load(xa + 1, hy, hz, false) : Raw{};
+    if (has_halo) plane[0][hly][hlz] = op.value(hc);
+    int cur = 0;
+    __syncthreads();
+    for (int x = xa; x < xb; ++x) {
+        Raw on2[kRows];
+#pragma unroll
+        for (int w = 0; w < kRows; ++w) on2[w] = op.load(x + 2, y0 + ty + w * kRowStep, z, true);
+        const Raw hn2 = has_halo ? op.load(x + 2, hy, hz, false) : Raw{};
+        double dn[kRows];
+#pragma unroll
+        for (int w = 0; w < kRows; ++w) {
+            dn[w] = op.value(on[w]);
+            plane[cur ^ 1][ty + w * kRowStep + 1][tz + 1] = dn[w];
+        }
+        if (has_halo) plane[cur ^ 1][hly][hlz] = op.value(hn);
+#pragma unroll
+        for (int w = 0; w < kRows; ++w) {
+            if (!active[w]) continue;
+            const int ly = ty + w * kRowStep + 1;
+            const unsigned f = oc[w].f;
+            double s = (double)__builtin_popcount(f & 63u) * dc[w];
+            s -= (f & 1u) ? dm[w] : 0.0;
+            s -= (f & 2u) ? dn[w] : 0.0;
+            s -= (f & 4u) ? plane[cur][ly - 1][tz + 1] : 0.0;
+            s -= (f & 8u) ? plane[cur][ly + 1][tz + 1] : 0.0;
+            s -= (f & 16u) ? plane[cur][ly][tz] : 0.0;
+            s -= (f & 32u) ? plane[cur][ly][tz + 2] : 0.0;
+            op.cell(c[w], oc[w], dc[w], (f & kPcgUnknown) ? s : 0.0, acc);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int w = 0; w < kRows; ++w) {
+            dm[w] = dc[w]; dc[w] = dn[w]; oc[w] = on[w]; on[w] = on2[w]; c[w] += sx;
+        }
+        hn = hn2;
+        cur ^= 1;
+    }
+}
+
+__device__ __forceinline__ void pcg_direction_body(const PcgState &st, int blk, int k) {
     const double *__restrict__ pold = st.p[k & 1];
     double *__restrict__ pnew = st.p[(k + 1) & 1];
     if (solve_done(st.scal)) return;
@@ -272,125 +390,19 @@ __device__ __forceinline__ void pcg_direction_body(const PcgState &st, int blk, 
         if (done) sc[kPcgDone] = 1.0;
     }
     if (done) return;
-
-    __shared__ double plane[2][kTileY + 2][kTileZ + 2];
-    // blocks are dispatched to the 8 XCDs round-robin: consecutive logical tiles go to one XCD,
-    // so an XCD sweeps whole y-z slabs of neighbouring tiles
-    const int per = st.dir_blocks / 8;
-    const int L = (blk % 8) * per + blk / 8;
-    constexpr int kRowStep = kTileY / kRows;
-    const int ty = threadIdx.x / kTileZ, tz = threadIdx.x % kTileZ;
     double acc[1] = {0.0};
-    if (L < st.dir_tiles_y * st.dir_tiles_z * st.dir_chunks) {
-        const int y0 = (L % st.dir_tiles_y) * kTileY;
-        const int z0 = ((L / st.dir_tiles_y) % st.dir_tiles_z) * kTileZ;
-        const int xa = (L / (st.dir_tiles_y * st.dir_tiles_z)) * st.dir_chunk_x;
-        const int xb = min(st.g.nx, xa + st.dir_chunk_x);
-        const int z = z0 + tz;
-        int hy = 0, hz = 0, hly = 0, hlz = 0;
-        const bool has_halo = halo_cell(threadIdx.x, y0, z0, hy, hz, hly, hlz);
-        const long sx = (long)st.g.ny * st.g.nz;
-        // prologue: plane xa - 1 (own columns), plane xa (own + halo into LDS), raw plane xa + 1
-        double dm[kRows], dc[kRows];
-        unsigned fc[kRows];
-        Raw on[kRows];
-        bool active[kRows];
-        size_t c[kRows];
-#pragma unroll
-        for (int w = 0; w < kRows; ++w) {
-            const int y = y0 + ty + w * kRowStep;
-            active[w] = y < st.g.ny && z < st.g.nz;
-            dm[w] = dir_of(load_raw(st, pold, xa - 1, y, z), beta);
-            const Raw oc = load_raw(st, pold, xa, y, z);
-            on[w] = load_raw(st, pold, xa + 1, y, z);
-            fc[w] = oc.f;
-            dc[w] = dir_of(oc, beta);
-            plane[0][ty + w * kRowStep + 1][tz + 1] = dc[w];
-            c[w] = idx3(xa, min(y, st.g.ny - 1), min(z, st.g.nz - 1), st.g.ny, st.g.nz);
-        }
-        const Raw hc = has_halo ? load_raw(st, pold, xa, hy, hz) : Raw{0u, 0.0, 0.0};
-        Raw hn = has_halo ? load_raw(st, pold, xa + 1, hy, hz) : Raw{0u, 0.0, 0.0};
-        if (has_halo) plane[0][hly][hlz] = dir_of(hc, beta);
-        int cur = 0;
-        __syncthreads();
-        for (int x = xa; x < xb; ++x) {
-            Raw on2[kRows];
-#pragma unroll
-            for (int w = 0; w < kRows; ++w) on2[w] = load_raw(st, pold, x + 2, y0 + ty + w * kRowStep, z);
-            const Raw hn2 = has_halo ? load_raw(st, pold, x + 2, hy, hz) : Raw{0u, 0.0, 0.0};
-            double dn[kRows];
-#pragma unroll
-            for (int w = 0; w < kRows; ++w) {
-                dn[w] = dir_of(on[w], beta);
-                plane[cur ^ 1][ty + w * kRowStep + 1][tz + 1] = dn[w];
-            }
-            if (has_halo) plane[cur ^ 1][hly][hlz] = dir_of(hn, beta);
-#pragma unroll
-            for (int w = 0; w < kRows; ++w) {
-                if (!active[w]) continue;
-                const int ly = ty + w * kRowStep + 1;
-                const unsigned f = fc[w];
-                double s = (double)__builtin_popcount(f & 63u) * dc[w];
-                s -= (f & 1u) ? dm[w] : 0.0;
-                s -= (f & 2u) ? dn[w] : 0.0;
-                s -= (f & 4u) ? plane[cur][ly - 1][tz + 1] : 0.0;
-                s -= (f & 8u) ? plane[cur][ly + 1][tz + 1] : 0.0;
-                s -= (f & 16u) ? plane[cur][ly][tz] : 0.0;
-                s -= (f & 32u) ? plane[cur][ly][tz + 2] : 0.0;
-                const double qc = (f & kPcgUnknown) ? s : 0.0;
-                pnew[c[w]] = dc[w];
-                st.q[c[w]] = qc;
-                acc[0] += dc[w] * qc;
-            }
-            __syncthreads();
-#pragma unroll
-            for (int w = 0; w < kRows; ++w) {
-                dm[w] = dc[w]; dc[w] = dn[w]; fc[w] = on[w].f; on[w] = on2[w]; c[w] += sx;
-            }
-            hn = hn2;
-            cur ^= 1;
-        }
-    }
+    tile_sweep(st, blk, DirOp{st, pold, pnew, beta}, acc);
     write_partials<1>(acc, st.part_dir, st.dir_blocks, blk);
 }
 
-// x += alpha p, r -= alpha q (alpha = rz / p.q), then partial rz, rr.  Outside the system p = q = r = 0, so the update leaves those cells as they are.
 __device__ __forceinline__ void pcg_update_body(const PcgState &st, int blk, int k) {
-    const double *__restrict__ pnew = st.p[(k + 1) & 1];
     if (solve_done(st.scal)) return;
     double pq[1];
     total_of<1>(st.part_dir, st.dir_blocks, pq);
     const double alpha = pq[0] != 0.0 ? st.scal[kPcgRZ + (k & 1)] / pq[0] : 0.0;
-    const size_t n = (size_t)st.g.nx * st.g.ny * st.g.nz;
-    const uint8_t *__restrict__ fl = st.flags;
-    double *__restrict__ x = st.x;
-    double *__restrict__ r = st.r;
-    const double *__restrict__ q = st.q;
-    const CellRange cr = block_cells(n, st.nblocks, blk);
-    const size_t stride = cr.step;
     double acc[2] = {0.0, 0.0};
-    for (size_t c0 = cr.first; c0 < cr.end; c0 += kUpdCells * stride) {
-        double xv[kUpdCells], rv[kUpdCells], id[kUpdCells];
-#pragma unroll
-        for (int u = 0; u < kUpdCells; ++u) {
-            const size_t cu = c0 + u * stride;
-            const size_t c = cu < cr.end ? cu : c0;
-            xv[u] = x[c] + alpha * pnew[c];
-            rv[u] = r[c] - alpha * q[c];
-            id[u] = inv_diag(fl[c]);
-        }
-#pragma unroll
-        for (int u = 0; u < kUpdCells; ++u) {
-            const size_t c = c0 + u * stride;
-            if (c < cr.end) {
-                x[c] = xv[u];
-                r[c] = rv[u];
-                acc[0] += rv[u] * (rv[u] * id[u]);
-                acc[1] += rv[u] * rv[u];
-            }
-        }
-    }
-    write_partials<2>(acc, st.part_update, st.nblocks, blk);
+    tile_sweep(st, blk, UpdOp{st, st.p[(k + 1) & 1], alpha}, acc);
+    write_partials<2>(acc, st.part_update, st.dir_blocks, blk);
 }
 
 // Energy-optimal scaling of an initial guess: partial sums of x.b and x.(L x) ...
@@ -547,7 +559,7 @@ hipError_t launch_pcg_start(PcgState &st, double tol2, hipStream_t s) {
 hipError_t launch_pcg_iterations(PcgState &st, int count, hipStream_t s) {
     for (int k = 0; k < count; ++k) {
         hipLaunchKernelGGL(pcg_direction_kernel, dim3(st.dir_blocks), dim3(kPB), 0, s, st, st.it);
-        hipLaunchKernelGGL(pcg_update_kernel, dim3(st.nblocks), dim3(kPB), 0, s, st, st.it);
+        hipLaunchKernelGGL(pcg_update_kernel, dim3(st.dir_blocks), dim3(kPB), 0, s, st, st.it);
         ++st.it;
     }
     return hipGetLastError();
@@ -560,7 +572,7 @@ hipError_t launch_pcg_batch_iterations(PcgBatch &b, int count, hipStream_t s) {
     if (b.n == 0) return hipSuccess;
     for (int k = 0; k < count; ++k) {
         hipLaunchKernelGGL(pcg_direction_batch_kernel, dim3(b.dir_blocks), dim3(kPB), 0, s, b.states, b.dir_map, b.it);
-        hipLaunchKernelGGL(pcg_update_batch_kernel, dim3(b.blocks), dim3(kPB), 0, s, b.states, b.map, b.it);
+        hipLaunchKernelGGL(pcg_update_batch_kernel, dim3(b.dir_blocks), dim3(kPB), 0, s, b.states, b.dir_map, b.it);
         ++b.it;
     }
     return hipGetLastError();
