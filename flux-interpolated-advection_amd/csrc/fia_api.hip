@@ -97,10 +97,14 @@ struct fia_ctx {
     DevBuf<int> genI;
     DevBuf<double> pcg_b;         // right-hand side of the pressure system
     DevBuf<uint8_t> pcg_flags;    // per-cell neighbour flags of the pressure system
-    struct PcgBufs {              // one CG solve: x, r, p (two buffers), partial sums, scalars
-        DevBuf<double> x, r, p0, p1, part, scal;
-        size_t bytes() const { return x.bytes() + r.bytes() + p0.bytes() + p1.bytes() + part.bytes() + scal.bytes(); }
-        void release() { x.release(); r.release(); p0.release(); p1.release(); part.release(); scal.release(); }
+    struct PcgBufs {              // one CG solve: x, r, p (two buffers), q (streaming update), partials, scalars
+        DevBuf<double> x, r, p0, p1, q, part, scal;
+        size_t bytes() const {
+            return x.bytes() + r.bytes() + p0.bytes() + p1.bytes() + q.bytes() + part.bytes() + scal.bytes();
+        }
+        void release() {
+            x.release(); r.release(); p0.release(); p1.release(); q.release(); part.release(); scal.release();
+        }
     } pcgs[2];
     DevBuf<double> scal;
     DevBuf<unsigned int> bits;
@@ -318,7 +322,11 @@ fia_status do_faces(fia_ctx *c, const CompGrid &g, const float *q, bool moving, 
     // slot bounds of the families into pinned host memory
     HIP_TRY(hipMemcpyAsync(c->host_bounds, c->bounds.p, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                            c->stream));
+    // with the twisted and degenerate face counts (complete after the count kernels), which size
+    // the grids of their kernels
+    HIP_TRY(hipMemcpyAsync(c->host_flags, c->counters.p, 3 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    const int n_twisted = c->host_flags[0], n_degen = c->host_flags[2];
     unsigned long long fam_slots[4] = {0, c->host_bounds[1], c->host_bounds[2], c->host_bounds[3]};
     const unsigned long long nslots = fam_slots[3];
     if (!(fam_slots[1] <= fam_slots[2] && fam_slots[2] <= fam_slots[3]))
@@ -332,7 +340,7 @@ fia_status do_faces(fia_ctx *c, const CompGrid &g, const float *q, bool moving, 
     p.piece_ay = c->piece_ay.p;
     p.piece_val = c->piece_val.p;
     HIP_TRY(launch_face_cut(p, fam_slots, c->stream));
-    if (p.seg_cut) HIP_TRY(launch_faces_degenerate(p, c->stream));
+    if (p.seg_cut) HIP_TRY(launch_faces_degenerate(p, n_degen, c->stream));
     // the WENO6 integral quadrature launches are timed on their own (the dominant kernel)
     const bool weno_quad = piece_quadrature_mode(p) != 0;
     int sq = -1;
@@ -340,7 +348,7 @@ fia_status do_faces(fia_ctx *c, const CompGrid &g, const float *q, bool moving, 
     HIP_TRY(launch_pieces(p, nslots, c->stream));
     if (weno_quad) HIP_TRY(span_end(c, sq));
     HIP_TRY(launch_face_sum(p, c->stream));
-    HIP_TRY(launch_faces_general(p, c->stream));
+    HIP_TRY(launch_faces_general(p, n_twisted, c->stream));
     HIP_TRY(span_end(c, sp));
     if (c->timing) { c->acc.face_launches += 1; c->acc.quad_launches += weno_quad ? 1 : 0; }
     return FIA_OK;
@@ -525,8 +533,10 @@ static fia_status bind_solve(fia_ctx::PcgBufs &B, const PcgGrid &pg, const doubl
     pcg_direction_tiling(pg.nx, pg.ny, pg.nz, ps);
     HIP_TRY(B.x.ensure(n)); HIP_TRY(B.r.ensure(n)); HIP_TRY(B.p0.ensure(n)); HIP_TRY(B.p1.ensure(n));
     HIP_TRY(B.part.ensure(pcg_partial_words(ps))); HIP_TRY(B.scal.ensure(kPcgScalars));
+    if (!ps.recompute_q) HIP_TRY(B.q.ensure(n));
     ps.g = pg; ps.flags = flags; ps.b = b;
     ps.x = B.x.p; ps.r = B.r.p; ps.p[0] = B.p0.p; ps.p[1] = B.p1.p;
+    ps.q = ps.recompute_q ? nullptr : B.q.p;
     ps.scal = B.scal.p; ps.part_start = B.part.p; ps.part_dir = B.part.p + 3 * (size_t)ps.nblocks;
     ps.part_update = ps.part_dir + ps.dir_blocks; ps.it = 0;
     return FIA_OK;
@@ -655,18 +665,28 @@ fia_status scenes_solve_run(fia_ctx *c, fia_scene *scenes, const int *idx, int m
     HIP_TRY(set_device(c));
     if (m == 0) return FIA_OK;
     std::vector<PcgState> states(m);
-    std::vector<int2> map;
+    std::vector<int2> map;   // direction blocks of every solve, then update blocks
     for (int j = 0; j < m; ++j) {
         states[j] = c->solves[idx[j]].st;
         if (states[j].it != states[0].it) return fail(FIA_ERR_INVALID_ARG, "scenes started at different iterations");
         for (int k = 0; k < states[j].dir_blocks; ++k) map.push_back(make_int2(j, k));
     }
+    const int dir_blocks = (int)map.size();
+    for (int j = 0; j < m; ++j)
+        if (!states[j].recompute_q)
+            for (int k = 0; k < states[j].upd_blocks; ++k) map.push_back(make_int2(j, k));
+    const int stream_end = (int)map.size();
+    for (int j = 0; j < m; ++j)
+        if (states[j].recompute_q)
+            for (int k = 0; k < states[j].upd_blocks; ++k) map.push_back(make_int2(j, k));
     HIP_TRY(c->batch_map.ensure(map.size()));
     HIP_TRY(hipMemcpyAsync(c->batch_states.p, states.data(), m * sizeof(PcgState), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(c->batch_map.p, map.data(), map.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
     PcgBatch b;
-    b.states = c->batch_states.p; b.dir_map = c->batch_map.p;
-    b.n = m; b.dir_blocks = (int)map.size(); b.it = states[0].it;
+    b.states = c->batch_states.p;
+    b.dir_map = c->batch_map.p; b.stream_map = c->batch_map.p + dir_blocks; b.tile_map = c->batch_map.p + stream_end;
+    b.n = m; b.dir_blocks = dir_blocks; b.stream_blocks = stream_end - dir_blocks;
+    b.tile_blocks = (int)map.size() - stream_end; b.it = states[0].it;
     double *hs = c->host_batch;
     const int poll = 32;
     while (true) {

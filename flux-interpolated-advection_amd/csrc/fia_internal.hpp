@@ -131,9 +131,10 @@ hipError_t launch_face_sum(const FaceParams &p, hipStream_t s);
 size_t scan_temp_words(size_t n);
 hipError_t launch_scan(unsigned long long *tile_tmp, const unsigned *in, unsigned long long *out, size_t n,
                        const FaceParams &p, unsigned long long *bounds, hipStream_t s);
-hipError_t launch_faces_general(const FaceParams &p, hipStream_t s);
+// count: the faces listed for the kernel (its grid; no launch for none)
+hipError_t launch_faces_general(const FaceParams &p, int count, hipStream_t s);
 // degenerate faces of the segment-cutter mode, cut per face into their slots by the pipeline
-hipError_t launch_faces_degenerate(const FaceParams &p, hipStream_t s);
+hipError_t launch_faces_degenerate(const FaceParams &p, int count, hipStream_t s);
 // zero-velocity pass: fluxes of the y faces of undeformed lattice faces (minimal_area >= 1)
 hipError_t launch_zero_velocity_faces(const FaceParams &p, hipStream_t s);
 // out = sum over families of flux differences (scaled); mode 0: out = G1; 1: out = G1 + G2;
@@ -166,12 +167,15 @@ struct PcgState {
     const uint8_t *flags;
     const double *b;
     double *x, *r, *p[2];
+    double *q;            // L p stored by the direction kernel (streaming update), or null (recompute)
     double *scal;         // kPcgScalars doubles
     double *part_start;   // 3 * nblocks partial sums (start kernel: rz, rr, bb)
     double *part_dir;     // dir_blocks (direction kernel: p.q)
-    double *part_update;  // 2 * dir_blocks (update kernel: rz, rr)
-    int nblocks;          // pcg_blocks(n): start kernel and the guess scaling
-    int dir_blocks, dir_tiles_y, dir_tiles_z, dir_chunk_x, dir_chunks;   // pcg_direction_tiling: both iteration kernels
+    double *part_update;  // 2 * upd_blocks (update kernel: rz, rr)
+    int nblocks;          // pcg_blocks(n): start kernel, guess scaling, streaming update
+    int dir_blocks, dir_tiles_y, dir_tiles_z, dir_chunk_x, dir_chunks;   // pcg_direction_tiling
+    bool recompute_q;     // pcg_direction_tiling: the update recomputes q on the direction tiles
+    int upd_blocks;       // blocks of the update kernel: dir_blocks (recompute) or nblocks (streaming)
     int it;             // iterations enqueued since the start (selects the p buffer)
 };
 int pcg_blocks(size_t n);
@@ -192,12 +196,16 @@ hipError_t launch_pcg_check(const PcgState &st, hipStream_t s);
 // device memory.
 struct PcgBatch {
     const PcgState *states;
-    const int2 *dir_map;   // dir_blocks x (solve, block)
-    int n, dir_blocks;
+    const int2 *dir_map;      // dir_blocks entries (solve, block)
+    const int2 *stream_map;   // update blocks of the streaming solves
+    const int2 *tile_map;     // update blocks of the recomputing solves
+    int n, dir_blocks, stream_blocks, tile_blocks;
     int it;
 };
-// partial-sum words of a solve: 3 nblocks (start) + dir_blocks (p.q) + 2 dir_blocks (rz, rr)
-inline size_t pcg_partial_words(const PcgState &st) { return 3 * (size_t)st.nblocks + 3 * (size_t)st.dir_blocks; }
+// partial-sum words of a solve: 3 nblocks (start) + dir_blocks (p.q) + 2 upd_blocks (rz, rr)
+inline size_t pcg_partial_words(const PcgState &st) {
+    return 3 * (size_t)st.nblocks + (size_t)st.dir_blocks + 2 * (size_t)st.upd_blocks;
+}
 hipError_t launch_pcg_batch_iterations(PcgBatch &b, int count, hipStream_t s);
 // the convergence test of every solve, then out[s * kPcgScalars ...] = scalars of solve s
 hipError_t launch_pcg_batch_check(const PcgBatch &b, double *out, hipStream_t s);

@@ -1084,15 +1084,19 @@ hipError_t launch_zero_velocity_faces(const FaceParams &p, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_faces_degenerate(const FaceParams &p, hipStream_t s) {
-    hipLaunchKernelGGL(degenerate_kernel, dim3(1024), dim3(64), 0, s, p);
+// grid-stride kernels over a face list of `count` faces: one 64-lane block per 64 faces, at most
+// 1024 blocks (one wave per SIMD at these kernels' register counts)
+static unsigned list_blocks(int count) { return (unsigned)std::min(1024, (count + 63) / 64); }
+
+hipError_t launch_faces_degenerate(const FaceParams &p, int count, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(degenerate_kernel, dim3(list_blocks(count)), dim3(64), 0, s, p);
     return hipGetLastError();
 }
 
-hipError_t launch_faces_general(const FaceParams &p, hipStream_t s) {
-    // one wave per block, one wave per SIMD at this kernel's register count: 1024 blocks fill the
-    // 256 CUs (the private stack of every lane of a full chip is about 1 GB of scratch)
-    hipLaunchKernelGGL(general_kernel, dim3(1024), dim3(64), 0, s, p);
+hipError_t launch_faces_general(const FaceParams &p, int count, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(general_kernel, dim3(list_blocks(count)), dim3(64), 0, s, p);
     return hipGetLastError();
 }
 

@@ -7,16 +7,20 @@
 // neighbour), b = -div(u) h, the last cell pinned to p = 0.  Obstacle cells (optional mask) are
 // removed from the system and their faces act as walls.
 //
-// One CG iteration is two kernels, both one 2.5-D tiled stencil sweep:
+// One CG iteration is two kernels:
 //   direction: beta from the previous residual, p = z + beta p_old (z = r / diag, not stored),
-//              q = L p, partial p.q;
-//   update:    alpha = rz / p.q, q = L p again (not stored), x += alpha p, r -= alpha q,
-//              partial r.z and r.r.
+//              q = L p (2.5-D tiled stencil sweep), partial p.q;
+//   update:    alpha = rz / p.q, x += alpha p, r -= alpha q, partial r.z and r.r.
+// The update has two forms, chosen per solve from the grid's tiling (pcg_direction_tiling):
+//   streaming: the direction kernel stores q and the update reads it, grid-stride over the cells;
+//   recompute: the update sweeps the direction kernel's tiles and recomputes q = L p from p.
 // p alternates between two buffers (a thread reads its neighbours' old p while others write the
 // new one).  The matrix is not stored: one flag byte per cell holds its fluid neighbours.  Per
-// cell and iteration that is 8 (r) + 8 (p old) + 1 (flags) read and 8 (p) written by the
-// direction kernel, 8 x 3 (p, x, r) + 1 read and 16 (x, r) written by the update kernel:
-// 66 bytes.
+// cell and iteration the direction kernel reads 8 (r) + 8 (p old) + 1 (flags) and writes 8 (p),
+// plus 8 (q) when streaming; the update reads 8 x 3 (p, x, r) + 1, plus 8 (q) when streaming, and
+// writes 16 (x, r): 82 bytes streaming, 66 recomputing.  Recomputing pays where the tiles sweep
+// long x chunks (large grids); on small grids, whose chunks are a few planes, the halo planes
+// cost more than q and the streaming form is faster.
 //
 // Reductions without fences: every block writes its partial sums with plain stores, and every
 // block of the NEXT kernel sums all partials itself, in block order, before its own work (the
@@ -31,6 +35,12 @@ namespace fia {
 
 constexpr int kPB = 256;
 constexpr int kTileZ = 64, kTileY = 16, kRows = 4;   // column tile of the direction kernel, rows per thread
+// The update recomputes q rather than streaming it on grids of at least kRecomputeCells cells
+// whose tiles sweep at least kRecomputeChunk planes.  Measured per iteration, streaming vs
+// recomputing: 64^3 20 vs 24 us (chunks of 2) and 31 us (chunks forced to 4), 96^3 27 vs 33 us,
+// 128^3 43 vs 50 us (chunks of 2) and 42 us (chunks of 4); 256^3 296 vs 254 us (chunks of 16).
+constexpr int kRecomputeChunk = 4;
+constexpr size_t kRecomputeCells = (size_t)1 << 22;
 
 __device__ __forceinline__ bool fluid(const PcgGrid &g, int i, int j, int k) {
     if (i < 0 || j < 0 || k < 0 || i >= g.nx || j >= g.ny || k >= g.nz) return false;
@@ -138,7 +148,7 @@ __device__ __forceinline__ void residual_state(const PcgState &st, int k, double
         rz = tot[0]; rr = tot[1]; bb = tot[2];
     } else {
         double tot[2];
-        total_of<2>(st.part_update, st.dir_blocks, tot);
+        total_of<2>(st.part_update, st.upd_blocks, tot);
         rz = tot[0]; rr = tot[1]; bb = st.scal[kPcgBB];
     }
 }
@@ -254,6 +264,7 @@ struct DirOp {
     __device__ __forceinline__ double value(const Raw &a) const { return a.r * inv_diag(a.f) + beta * a.p; }
     __device__ __forceinline__ void cell(size_t c, const Raw &, double v, double q, double *acc) const {
         pnew[c] = v;
+        if (st.q) st.q[c] = q;
         acc[0] += v * q;
     }
 };
@@ -395,14 +406,53 @@ __device__ __forceinline__ void pcg_direction_body(const PcgState &st, int blk, 
     write_partials<1>(acc, st.part_dir, st.dir_blocks, blk);
 }
 
-__device__ __forceinline__ void pcg_update_body(const PcgState &st, int blk, int k) {
+// cells per thread and loop step of the streaming update (independent cells, so their loads overlap)
+constexpr int kUpdCells = 4;
+
+// Streaming update: x += alpha p, r -= alpha q with the stored q, grid-stride over nblocks blocks.
+// Outside the system p = q = r = 0, so the update leaves those cells as they are.
+__device__ __forceinline__ void update_stream(const PcgState &st, int blk, int k, double alpha, double *acc) {
+    const double *__restrict__ pnew = st.p[(k + 1) & 1];
+    const size_t n = (size_t)st.g.nx * st.g.ny * st.g.nz;
+    const uint8_t *__restrict__ fl = st.flags;
+    double *__restrict__ x = st.x;
+    double *__restrict__ r = st.r;
+    const double *__restrict__ q = st.q;
+    const CellRange cr = block_cells(n, st.nblocks, blk);
+    const size_t stride = cr.step;
+    for (size_t c0 = cr.first; c0 < cr.end; c0 += kUpdCells * stride) {
+        double xv[kUpdCells], rv[kUpdCells], id[kUpdCells];
+#pragma unroll
+        for (int u = 0; u < kUpdCells; ++u) {
+            const size_t cu = c0 + u * stride;
+            const size_t c = cu < cr.end ? cu : c0;
+            xv[u] = x[c] + alpha * pnew[c];
+            rv[u] = r[c] - alpha * q[c];
+            id[u] = inv_diag(fl[c]);
+        }
+#pragma unroll
+        for (int u = 0; u < kUpdCells; ++u) {
+            const size_t c = c0 + u * stride;
+            if (c < cr.end) {
+                x[c] = xv[u];
+                r[c] = rv[u];
+                acc[0] += rv[u] * (rv[u] * id[u]);
+                acc[1] += rv[u] * rv[u];
+            }
+        }
+    }
+}
+
+// Iteration k's update: alpha from the direction kernel's partials, then the solve's form.
+template <bool RECOMPUTE> __device__ __forceinline__ void pcg_update_body(const PcgState &st, int blk, int k) {
     if (solve_done(st.scal)) return;
     double pq[1];
     total_of<1>(st.part_dir, st.dir_blocks, pq);
     const double alpha = pq[0] != 0.0 ? st.scal[kPcgRZ + (k & 1)] / pq[0] : 0.0;
     double acc[2] = {0.0, 0.0};
-    tile_sweep(st, blk, UpdOp{st, st.p[(k + 1) & 1], alpha}, acc);
-    write_partials<2>(acc, st.part_update, st.dir_blocks, blk);
+    if constexpr (RECOMPUTE) tile_sweep(st, blk, UpdOp{st, st.p[(k + 1) & 1], alpha}, acc);
+    else update_stream(st, blk, k, alpha, acc);
+    write_partials<2>(acc, st.part_update, st.upd_blocks, blk);
 }
 
 // Energy-optimal scaling of an initial guess: partial sums of x.b and x.(L x) ...
@@ -482,7 +532,9 @@ __global__ void __launch_bounds__(kPB) pcg_start_kernel(PcgState st, double tol2
 __global__ void __launch_bounds__(kPB) pcg_direction_kernel(PcgState st, int k) {
     pcg_direction_body(st, blockIdx.x, k);
 }
-__global__ void __launch_bounds__(kPB) pcg_update_kernel(PcgState st, int k) { pcg_update_body(st, blockIdx.x, k); }
+template <bool RECOMPUTE> __global__ void __launch_bounds__(kPB) pcg_update_kernel(PcgState st, int k) {
+    pcg_update_body<RECOMPUTE>(st, blockIdx.x, k);
+}
 __global__ void __launch_bounds__(kPB) pcg_check_kernel(PcgState st, int k) { pcg_check_body(st, k); }
 
 // ... or many solves per launch, all at the same iteration: map[b] = (s, j) says that block b of
@@ -492,10 +544,11 @@ __global__ void __launch_bounds__(kPB) pcg_direction_batch_kernel(const PcgState
     const int2 m = map[blockIdx.x];
     pcg_direction_body(st[m.x], m.y, k);
 }
+template <bool RECOMPUTE>
 __global__ void __launch_bounds__(kPB) pcg_update_batch_kernel(const PcgState *__restrict__ st,
                                                                const int2 *__restrict__ map, int k) {
     const int2 m = map[blockIdx.x];
-    pcg_update_body(st[m.x], m.y, k);
+    pcg_update_body<RECOMPUTE>(st[m.x], m.y, k);
 }
 // one block per solve; the solves' polled scalars are then copied to out[s * kPcgScalars ...], so
 // that the host reads all of them with one copy
@@ -533,6 +586,9 @@ void pcg_direction_tiling(int nx, int ny, int nz, PcgState &st) {
     st.dir_chunks = (nx + st.dir_chunk_x - 1) / st.dir_chunk_x;
     const long nb = tiles * st.dir_chunks;
     st.dir_blocks = (int)((nb + 7) / 8 * 8);
+    // the update recomputes q on these tiles on large grids with long chunks
+    st.recompute_q = (size_t)nx * ny * nz >= kRecomputeCells && st.dir_chunk_x >= kRecomputeChunk;
+    st.upd_blocks = st.recompute_q ? st.dir_blocks : pcg_blocks((size_t)nx * ny * nz);
 }
 
 hipError_t launch_pcg_setup(const PcgGrid &g, const float *u, const float *v, const float *w, double h, double *b,
@@ -559,7 +615,8 @@ hipError_t launch_pcg_start(PcgState &st, double tol2, hipStream_t s) {
 hipError_t launch_pcg_iterations(PcgState &st, int count, hipStream_t s) {
     for (int k = 0; k < count; ++k) {
         hipLaunchKernelGGL(pcg_direction_kernel, dim3(st.dir_blocks), dim3(kPB), 0, s, st, st.it);
-        hipLaunchKernelGGL(pcg_update_kernel, dim3(st.dir_blocks), dim3(kPB), 0, s, st, st.it);
+        if (st.recompute_q) hipLaunchKernelGGL(pcg_update_kernel<true>, dim3(st.upd_blocks), dim3(kPB), 0, s, st, st.it);
+        else hipLaunchKernelGGL(pcg_update_kernel<false>, dim3(st.upd_blocks), dim3(kPB), 0, s, st, st.it);
         ++st.it;
     }
     return hipGetLastError();
@@ -572,7 +629,12 @@ hipError_t launch_pcg_batch_iterations(PcgBatch &b, int count, hipStream_t s) {
     if (b.n == 0) return hipSuccess;
     for (int k = 0; k < count; ++k) {
         hipLaunchKernelGGL(pcg_direction_batch_kernel, dim3(b.dir_blocks), dim3(kPB), 0, s, b.states, b.dir_map, b.it);
-        hipLaunchKernelGGL(pcg_update_batch_kernel, dim3(b.dir_blocks), dim3(kPB), 0, s, b.states, b.dir_map, b.it);
+        if (b.stream_blocks)
+            hipLaunchKernelGGL(pcg_update_batch_kernel<false>, dim3(b.stream_blocks), dim3(kPB), 0, s, b.states,
+                               b.stream_map, b.it);
+        if (b.tile_blocks)
+            hipLaunchKernelGGL(pcg_update_batch_kernel<true>, dim3(b.tile_blocks), dim3(kPB), 0, s, b.states, b.tile_map,
+                               b.it);
         ++b.it;
     }
     return hipGetLastError();
