@@ -136,6 +136,30 @@ def test_pressure_projection(fia, oracle, ctx, scenes, with_solid):
     assert np.abs(div).max() * h < 1e-4
 
 
+def test_pressure_projection_large_grid(fia, ctx, scenes):
+    # 176^3 (5.5 M cells, tiles of 6 planes): the update kernel recomputes q = L p instead of
+    # streaming it.  The divergence left after the projection is the solve's residual, so its norm
+    # over the norm of the divergence before the projection must equal the reported relative
+    # residual (to FP32 rounding of the velocities) and lie under the tolerance
+    print("case pcg 176^3 (recomputed q)", flush=True)
+    n = 176; h = 1.0 / n
+    rng = np.random.default_rng(17)
+    u = rng.normal(size=(n + 1, n, n)).astype(np.float32); u[0] = u[-1] = 0
+    v = rng.normal(size=(n, n + 1, n)).astype(np.float32); v[:, 0] = v[:, -1] = 0
+    w = rng.normal(size=(n, n, n + 1)).astype(np.float32); w[:, :, 0] = w[:, :, -1] = 0
+    d0 = scenes.divergence(u.astype(np.float64), v.astype(np.float64), w.astype(np.float64), h) * h
+    grid = fia.Grid3D(n, n, n, h, None, u, v, w)
+    it, res = fia.calc_pressure_gradient_term_3d(ctx, grid, rel_tol=1e-6)
+    print("iterations %d, relative residual %.3e" % (it, res), flush=True)
+    assert res <= 1e-6 and it > 100
+    d1 = scenes.divergence(grid.u.astype(np.float64), grid.v.astype(np.float64), grid.w.astype(np.float64), h) * h
+    d0[-1, -1, -1] = d1[-1, -1, -1] = 0.0   # the pinned cell (it absorbs the total flux, zero here)
+    ratio = np.linalg.norm(d1) / np.linalg.norm(d0)
+    print("divergence norm after / before %.3e" % ratio, flush=True)
+    assert ratio < 1e-6 + 1e-6
+    assert abs(ratio - res) < 0.5 * res + 2e-7
+
+
 def test_scenes_step_is_independent_of_workers(fia, ctx):
     # several uneven scenes stepped by one worker and by three concurrent workers give identical
     # fields: scenes are independent and every kernel is deterministic
