@@ -33,6 +33,22 @@ constexpr int kMaxGauss = 49;
 constexpr int kStatSlots = 1024;
 constexpr int kStatWords = 3;
 
+// n / d for 0 <= n < 2^31 by a multiply and a shift (unsigned division by an invariant divisor,
+// Granlund-Montgomery / Hacker's Delight 10-9): with l = ceil(log2 d), p = 31 + l and
+// m = ceil(2^p / d), 2^p <= m d < 2^p + 2^l, so floor(n m / 2^p) = floor(n / d), and m < 2^32.
+struct FastDiv {
+    uint32_t d = 1, m = 0x80000000u;
+    int p = 31;
+    FastDiv() = default;
+    __host__ __device__ explicit FastDiv(uint32_t dd) : d(dd) {
+        int l = 0;
+        while (((uint64_t)1 << l) < dd) ++l;
+        p = 31 + l;
+        m = (uint32_t)((((uint64_t)1 << p) + dd - 1) / dd);
+    }
+    __host__ __device__ uint32_t div(uint32_t n) const { return (uint32_t)(((uint64_t)n * m) >> p); }
+};
+
 // A component grid G: density (comp = -1, dims n) or one MAC velocity component (comp = 0..2,
 // dims n + e_comp).  Its vertex lattice has dims m + 1 and sits at lattice - e_comp/2 (cells).
 struct CompGrid {
@@ -40,9 +56,14 @@ struct CompGrid {
     int n[3];   // base grid
     int m[3];   // G dims
     int e[3];
+    FastDiv fy[3], fz[3];   // face family f: its y and z extents (face ids decode by two divisions)
     __host__ __device__ void init(int c, int nx, int ny, int nz) {
         comp = c; n[0] = nx; n[1] = ny; n[2] = nz;
         for (int a = 0; a < 3; ++a) { e[a] = (a == c); m[a] = n[a] + e[a]; }
+        for (int f = 0; f < 3; ++f) {
+            fy[f] = FastDiv((uint32_t)(m[1] + (f == 1)));
+            fz[f] = FastDiv((uint32_t)(m[2] + (f == 2)));
+        }
     }
     __host__ __device__ size_t cells() const { return (size_t)m[0] * m[1] * m[2]; }
     __host__ __device__ size_t lattice() const { return (size_t)(m[0] + 1) * (m[1] + 1) * (m[2] + 1); }

@@ -303,11 +303,13 @@ FIA_HD double piece_flux_weno_integral(const FaceParams &P, const double c[3], d
     return ay * (psi + (double)dens);
 }
 
+// face id (< 2^30, checked on the host) -> (i, j, k), id = (i fy + j) fz + k
 FIA_HD void decode_face(const CompGrid &g, int fam, long id, int &i, int &j, int &k) {
-    const int fy = g.m[1] + (fam == 1), fz = g.m[2] + (fam == 2);
-    k = (int)(id % fz);
-    j = (int)((id / fz) % fy);
-    i = (int)(id / ((long)fy * fz));
+    const FastDiv &dz = g.fz[fam], &dy = g.fy[fam];
+    const uint32_t u = (uint32_t)id, q = dz.div(u), r = dy.div(q);
+    k = (int)(u - q * dz.d);
+    j = (int)(q - r * dy.d);
+    i = (int)r;
 }
 
 FIA_HD void report(int *err, int code) { atomic_cas_i32(err, 0, code); }

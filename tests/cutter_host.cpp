@@ -26,3 +26,24 @@ extern "C" int host_split_coord(double s, int m, int *base, float *frac) {
     fia::split_coord(s, *base, *frac);
     return *base - 2 >= 0 && *base + 3 <= m - 1;
 }
+
+// mismatches of fia::FastDiv against hardware division: every divisor up to d_max with every
+// numerator up to n_max, then `random` seeded pairs over the whole range n < 2^31
+extern "C" long host_fastdiv_mismatches(unsigned d_max, unsigned n_max, long random) {
+    long bad = 0;
+    for (unsigned d = 1; d <= d_max; ++d) {
+        const fia::FastDiv f(d);
+        for (unsigned n = 0; n <= n_max; ++n) bad += f.div(n) != n / d;
+        const unsigned top = 0x7fffffffu, last = d * (top / d);
+        bad += f.div(top) != top / d;
+        bad += f.div(last) != last / d;
+        bad += f.div(last - 1) != (last - 1) / d;
+    }
+    unsigned long long x = 88172645463325252ull;
+    for (long t = 0; t < random; ++t) {
+        x ^= x << 13; x ^= x >> 7; x ^= x << 17;
+        const unsigned d = (unsigned)(x % 0x7fffffffu) + 1u, n = (unsigned)(x >> 33) & 0x7fffffffu;
+        bad += fia::FastDiv(d).div(n) != n / d;
+    }
+    return bad;
+}

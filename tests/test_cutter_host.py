@@ -27,6 +27,8 @@ def cutter(tmp_path_factory):
     L.host_split_coord.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                    ctypes.POINTER(ctypes.c_float)]
     L.host_split_coord.restype = ctypes.c_int
+    L.host_fastdiv_mismatches.argtypes = [ctypes.c_uint, ctypes.c_uint, ctypes.c_long]
+    L.host_fastdiv_mismatches.restype = ctypes.c_long
     return L
 
 
@@ -70,3 +72,10 @@ def test_split_coord_keeps_far_points_outside_every_field(cutter):
         inside = cutter.host_split_coord(s, 256, ctypes.byref(b), ctypes.byref(t))
         assert b.value == int(np.floor(s)) and abs(t.value - (s - np.floor(s))) < 1e-7
         assert inside == (2 <= b.value <= 252)
+
+
+def test_fastdiv_matches_integer_division(cutter):
+    # face ids decode with the multiply-shift division of CompGrid (ids < 2^31): every divisor up
+    # to 600 with every numerator up to 70,000 and the largest multiples below 2^31, then a
+    # million random pairs
+    assert cutter.host_fastdiv_mismatches(600, 70000, 1000000) == 0
