@@ -57,6 +57,11 @@ TRAFFIC_BYTES_256_CFL4 = 12.043e9
 # kernel reads r, p_old (8 + 8) and the flag byte, writes p (8); update kernel reads x, r, p
 # (3 x 8) and the flag byte, writes x, r (2 x 8); q = L p is recomputed, not stored
 PCG_BYTES_PER_CELL_ITER = 66
+# one multigrid V-cycle per iteration, per fine cell: the down kernel reads r (8) and the flag
+# byte, writes x (4) and the aggregate's b (0.5); the up kernel reads x (4), r (8), the flag byte
+# and the correction (0.5), writes z (8); the coarser levels add about a seventh of a fine level's
+# 28 bytes; the direction kernel reads z in place of r (no change)
+MG_BYTES_PER_CELL_VCYCLE = 39
 
 
 def make_scene(scenes, n, cfl, seed=1):
@@ -199,6 +204,7 @@ def run_projection(args, fia, scenes, torch):
     u0, v0, w0 = T(u), T(v), T(w)
     solid_d = torch.from_numpy(np.ascontiguousarray(solid, dtype=np.uint8)).to(dev)
     ctx = fia.Context(0)
+    ctx.set_pressure_preconditioner(args.precond)
     grid = fia.Grid3D(n, n, n, h, None, u0.clone(), v0.clone(), w0.clone())
     times, iters, res = [], [], []
     for i in range(args.warmup + args.steps):
@@ -212,19 +218,21 @@ def run_projection(args, fia, scenes, torch):
             times.append(time.perf_counter() - t0); iters.append(it); res.append(rr)
     sec = sum(times)
     it_mean = float(np.mean(iters))
-    gbs = PCG_BYTES_PER_CELL_ITER * n ** 3 * it_mean / (sec / args.steps) / 1e9
+    per_iter = PCG_BYTES_PER_CELL_ITER + (MG_BYTES_PER_CELL_VCYCLE if args.precond == "mg" else 0)
+    gbs = per_iter * n ** 3 * it_mean / (sec / args.steps) / 1e9
     ctx.close()
+    solver = "Jacobi-PCG" if args.precond == "jacobi" else "multigrid-preconditioned CG (V(1,1) per iteration)"
     return {
         "metric": METRIC, "value": round(n ** 3 * args.steps / sec / 1e6, 3), "unit": "Mcell-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(sec / args.steps * 1e3, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "pressure projection, %d^3, Jacobi-PCG (FP64) to 1e-6 relative residual from a zero "
-                               "initial guess, curl-noise velocity + buoyancy impulse, 8 spherical obstacles" % n,
-                   "grid": [n, n, n]},
+        "config": {"workload": "pressure projection, %d^3, %s (FP64 CG) to 1e-6 relative residual from a zero "
+                               "initial guess, curl-noise velocity + buoyancy impulse, 8 spherical obstacles" % (n, solver),
+                   "grid": [n, n, n], "preconditioner": args.precond},
         "pcg": {"iterations": iters, "rel_residual": res, "warm_start": "none (zero initial guess; --workload warmstart measures the learned warm start)"},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": None,
-                     "kernel": "whole Jacobi-PCG iteration (%d B per cell and iteration)" % PCG_BYTES_PER_CELL_ITER},
+                     "kernel": "whole CG iteration (%d B per cell and iteration)" % per_iter},
     }
 
 
@@ -236,6 +244,7 @@ def run_warmstart(args, fia, scenes, torch):
         sys.exit(2)
     n = args.n
     ctx = fia.Context(0)
+    ctx.set_pressure_preconditioner(args.precond)
     net = fia.UNet(ctx, seed=0)
     net.load(args.weights)
     # warm-up projections untimed, then the timed ones, all on held-out scene 5000 (one plume run)
@@ -270,6 +279,8 @@ def run_scenes(args, fia, torch):
     sset = batch.SceneSet(args.scenes, seed=5, edges=tuple(int(e) for e in args.edges.split(",")))
     candidates = (1, 2, 4, 8)
     ctxs = [fia.Context(0) for _ in range(max(candidates))]
+    for c in ctxs:
+        c.set_pressure_preconditioner(args.precond)
     nets = None
     if not args.plain:
         nets = []
@@ -324,6 +335,8 @@ def main():
     ap.add_argument("--plain", action="store_true", help="scenes: plain PCG instead of the warm start")
     ap.add_argument("--batched", action="store_true", help="scenes: the pressure solves of all scenes batched")
     ap.add_argument("--edges", default="32,48,64,80,96", help="scenes: grid edges drawn from this list")
+    ap.add_argument("--precond", choices=("jacobi", "mg"), default="jacobi",
+                    help="projection, warmstart, scenes: preconditioner of the pressure CG")
     ap.add_argument("--weights", default=os.path.join(ROOT, "flux-interpolated-advection_amd", "weights",
                                                       "unet_plume128.npy"))
     ap.add_argument("--no-cpu-baseline", action="store_true")

@@ -41,7 +41,7 @@ EXPORTS = ["fia_default_opts", "fia_ctx_create", "fia_ctx_create_on_stream", "fi
            "fia_unet_param_count", "fia_unet_create", "fia_unet_destroy", "fia_unet_set_params",
            "fia_unet_get_params", "fia_unet_forward", "fia_unet_loss_grad", "fia_unet_train_step", "fia_pressure_rhs",
            "fia_pressure_project_learned", "fia_grid_totals", "fia_wall_momentum_loss", "fia_scene_forcing",
-           "fia_scenes_step", "fia_scenes_step_batched"]
+           "fia_scenes_step", "fia_scenes_step_batched", "fia_ctx_set_pressure_preconditioner"]
 
 
 class FiaError(RuntimeError):
@@ -132,6 +132,7 @@ def lib():
         L.fia_pressure_project.argtypes = [_vp, ctypes.POINTER(GridC), _vp, _vp, _i, _d, _i, ctypes.POINTER(_i),
                                            ctypes.POINTER(_d)]
         L.fia_ctx_set_timing.argtypes = [_vp, _i]
+        L.fia_ctx_set_pressure_preconditioner.argtypes = [_vp, _i]
         L.fia_ctx_get_stats.argtypes = [_vp, ctypes.POINTER(KernelStats)]
         L.fia_ctx_reset_stats.argtypes = [_vp]
         L.fia_unet_param_count.restype = ctypes.c_size_t
@@ -178,6 +179,9 @@ def device_count():
     return lib().fia_device_count()
 
 
+PRECOND = {"jacobi": 0, "mg": 1}
+
+
 class Context:
     """Owns the device scratch and the HIP stream (fia_ctx)."""
 
@@ -194,6 +198,10 @@ class Context:
 
     def stream(self):
         return lib().fia_ctx_stream(self._h)
+
+    def set_pressure_preconditioner(self, kind):
+        """"jacobi" (default) or "mg" (geometric multigrid V-cycle) for later pressure solves."""
+        _check(lib().fia_ctx_set_pressure_preconditioner(self._h, PRECOND[kind]))
 
     def close(self):
         if self._h:

@@ -97,15 +97,33 @@ struct fia_ctx {
     DevBuf<int> genI;
     DevBuf<double> pcg_b;         // right-hand side of the pressure system
     DevBuf<uint8_t> pcg_flags;    // per-cell neighbour flags of the pressure system
-    struct PcgBufs {              // one CG solve: x, r, p (two buffers), q (streaming update), partials, scalars
-        DevBuf<double> x, r, p0, p1, q, part, scal;
+    struct PcgBufs {              // one CG solve: x, r, p (two buffers), q (streaming update), z and r.z
+                                  // partials (multigrid), partials, scalars
+        DevBuf<double> x, r, p0, p1, q, z, part_mg, part, scal;
         size_t bytes() const {
-            return x.bytes() + r.bytes() + p0.bytes() + p1.bytes() + q.bytes() + part.bytes() + scal.bytes();
+            return x.bytes() + r.bytes() + p0.bytes() + p1.bytes() + q.bytes() + z.bytes() + part_mg.bytes() +
+                   part.bytes() + scal.bytes();
         }
         void release() {
-            x.release(); r.release(); p0.release(); p1.release(); q.release(); part.release(); scal.release();
+            x.release(); r.release(); p0.release(); p1.release(); q.release(); z.release(); part_mg.release();
+            part.release(); scal.release();
         }
     } pcgs[2];
+    // multigrid preconditioner: the levels' operators and vectors, and the hierarchy's shape
+    int precond = FIA_PRECOND_JACOBI;
+    struct MgBufs {
+        DevBuf<float4> op[kMgMaxLevels];
+        DevBuf<float> b[kMgMaxLevels], x[kMgMaxLevels], xs[kMgMaxLevels];
+        size_t bytes() const {
+            size_t t = 0;
+            for (int l = 0; l < kMgMaxLevels; ++l) t += op[l].bytes() + b[l].bytes() + x[l].bytes() + xs[l].bytes();
+            return t;
+        }
+        void release() {
+            for (int l = 0; l < kMgMaxLevels; ++l) { op[l].release(); b[l].release(); x[l].release(); xs[l].release(); }
+        }
+    } mgb;
+    MgHier mgh;
     DevBuf<double> scal;
     DevBuf<unsigned int> bits;
     DevBuf<uint8_t> solid;
@@ -145,6 +163,7 @@ struct fia_unet {
 namespace fia {
 int ctx_device(const fia_ctx *c) { return c->device; }
 hipStream_t ctx_stream(const fia_ctx *c) { return c->stream; }
+int ctx_precond(const fia_ctx *c) { return c->precond; }
 fia_status set_error(fia_status status, const std::string &msg) { return fail(status, msg); }
 }  // namespace fia
 
@@ -159,7 +178,7 @@ size_t ctx_bytes(const fia_ctx *c) {
     for (int a = 0; a < 3; ++a) b += c->disp[a].bytes() + c->work[a].bytes() + c->gen[a].bytes();
     for (int a = 0; a < 4; ++a) b += c->grid[a].bytes();
     for (int a = 0; a < 2; ++a) b += c->genD[a].bytes();
-    b += c->pcg_b.bytes() + c->pcg_flags.bytes() + c->pcgs[0].bytes() + c->pcgs[1].bytes();
+    b += c->pcg_b.bytes() + c->pcg_flags.bytes() + c->pcgs[0].bytes() + c->pcgs[1].bytes() + c->mgb.bytes();
     for (const auto &sv : c->solves) b += sv.bytes();
     b += c->batch_states.bytes() + c->batch_map.bytes() + c->batch_scal.bytes();
     return b;
@@ -527,24 +546,60 @@ static fia_status pressure_system(fia_ctx *c, const fia_grid3d *g, const GridVie
 
 // Buffers B (allocated here for the grid of pg) bound into a CG state over the system (b, flags).
 static fia_status bind_solve(fia_ctx::PcgBufs &B, const PcgGrid &pg, const double *b, const uint8_t *flags,
-                             PcgState &ps) {
+                             PcgState &ps, bool mg = false) {
     const size_t n = (size_t)pg.nx * pg.ny * pg.nz;
     ps.nblocks = pcg_blocks(n);
     pcg_direction_tiling(pg.nx, pg.ny, pg.nz, ps);
     HIP_TRY(B.x.ensure(n)); HIP_TRY(B.r.ensure(n)); HIP_TRY(B.p0.ensure(n)); HIP_TRY(B.p1.ensure(n));
     HIP_TRY(B.part.ensure(pcg_partial_words(ps))); HIP_TRY(B.scal.ensure(kPcgScalars));
     if (!ps.recompute_q) HIP_TRY(B.q.ensure(n));
+    if (mg) { HIP_TRY(B.z.ensure(n)); HIP_TRY(B.part_mg.ensure((size_t)ps.dir_blocks)); }
     ps.g = pg; ps.flags = flags; ps.b = b;
     ps.x = B.x.p; ps.r = B.r.p; ps.p[0] = B.p0.p; ps.p[1] = B.p1.p;
     ps.q = ps.recompute_q ? nullptr : B.q.p;
+    ps.z = mg ? B.z.p : nullptr;
+    ps.part_mg = mg ? B.part_mg.p : nullptr;
     ps.scal = B.scal.p; ps.part_start = B.part.p; ps.part_dir = B.part.p + 3 * (size_t)ps.nblocks;
     ps.part_update = ps.part_dir + ps.dir_blocks; ps.it = 0;
     return FIA_OK;
 }
 
+// The multigrid hierarchy of the context's current system (ps's grid and flags) under
+// FIA_PRECOND_MG, built on the context's stream; null under Jacobi.
+static fia_status mg_prepare(fia_ctx *c, const PcgState &ps, const MgHier **out) {
+    *out = nullptr;
+    if (c->precond != FIA_PRECOND_MG) return FIA_OK;
+    MgHier &h = c->mgh;
+    mg_shape(ps.g.nx, ps.g.ny, ps.g.nz, h);
+    const MgLevel &top = h.lv[h.levels - 1];
+    if ((size_t)top.nx * top.ny * top.nz > 256) return fail(FIA_ERR_UNSUPPORTED, "grid too large for the multigrid levels");
+    for (int l = 0; l < h.levels; ++l) {
+        MgLevel &L = h.lv[l];
+        const size_t n = (size_t)L.nx * L.ny * L.nz;
+        HIP_TRY(c->mgb.x[l].ensure(n));
+        HIP_TRY(c->mgb.b[l].ensure(n));
+        L.x = c->mgb.x[l].p;
+        L.b = c->mgb.b[l].p;
+        if (l > 0) {
+            HIP_TRY(c->mgb.op[l].ensure(n)); HIP_TRY(c->mgb.xs[l].ensure(n));
+            L.op = c->mgb.op[l].p; L.xs = c->mgb.xs[l].p;
+        }
+    }
+    HIP_TRY(launch_mg_build(ps, h, c->stream));
+    *out = &h;
+    return FIA_OK;
+}
+
 // Solve k's buffers bound into a CG state over the context's system.
 static fia_status pcg_bind(fia_ctx *c, int k, const PcgGrid &pg, PcgState &ps) {
-    return bind_solve(c->pcgs[k], pg, c->pcg_b.p, c->pcg_flags.p, ps);
+    return bind_solve(c->pcgs[k], pg, c->pcg_b.p, c->pcg_flags.p, ps, c->precond == FIA_PRECOND_MG);
+}
+
+// The start of a solve: r = b - L x and the scalars, then z = M r under multigrid.
+static fia_status pcg_begin(fia_ctx *c, PcgState &ps, double tol2, const MgHier *mg) {
+    HIP_TRY(launch_pcg_start(ps, tol2, c->stream));
+    if (mg) HIP_TRY(launch_mg_vcycle(ps, *mg, c->stream));
+    return FIA_OK;
 }
 
 // The learned start: sa starts from the network's guess scaled by its energy-optimal factor, sb
@@ -555,22 +610,23 @@ struct LearnedStart {
     double guess_res, gamma, ra, rb, bb;
 };
 static fia_status learned_start(fia_ctx *c, fia_unet *u, const PcgGrid &pg, PcgState &sa, PcgState &sb, double tol2,
-                                int probe, double *ha, double *hb, LearnedStart &ls) {
+                                int probe, double *ha, double *hb, LearnedStart &ls, const MgHier *mg) {
     HIP_TRY(u->net.forward(sa.b, pg.solid, u->y.p, nullptr, c->stream));
     HIP_TRY(u->net.guess(u->y.p, pg.pin, sa.flags, sa.x, c->stream));
     HIP_TRY(launch_pcg_scale_guess(sa, c->stream));
-    HIP_TRY(launch_pcg_start(sa, tol2, c->stream));
+    fia_status st;
+    if ((st = pcg_begin(c, sa, tol2, mg)) != FIA_OK) return st;
     HIP_TRY(launch_pcg_check(sa, c->stream));
     HIP_TRY(hipMemcpyAsync(ha, sa.scal, kPcgScalars * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     ls.bb = ha[kPcgBB] > 0 ? ha[kPcgBB] : 1.0;
     ls.guess_res = sqrt(ha[kPcgRR] / ls.bb);
     ls.gamma = ha[kPcgGamma];
-    HIP_TRY(launch_pcg_iterations(sa, probe, c->stream));
+    HIP_TRY(launch_pcg_iterations(sa, probe, c->stream, mg));
     HIP_TRY(launch_pcg_check(sa, c->stream));
     HIP_TRY(launch_pcg_guess(sb, nullptr, c->stream));
-    HIP_TRY(launch_pcg_start(sb, tol2, c->stream));
-    HIP_TRY(launch_pcg_iterations(sb, probe, c->stream));
+    if ((st = pcg_begin(c, sb, tol2, mg)) != FIA_OK) return st;
+    HIP_TRY(launch_pcg_iterations(sb, probe, c->stream, mg));
     HIP_TRY(launch_pcg_check(sb, c->stream));
     HIP_TRY(hipMemcpyAsync(ha, sa.scal, kPcgScalars * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipMemcpyAsync(hb, sb.scal, kPcgScalars * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -584,14 +640,14 @@ static fia_status learned_start(fia_ctx *c, fia_unet *u, const PcgGrid &pg, PcgS
 // Iterates a started solve until it converges or has run max_iters iterations in total; hs
 // (pinned, kPcgScalars doubles) receives its scalars.  The device skips iterations after
 // convergence, so batches of them are enqueued between polls.
-static fia_status pcg_run(fia_ctx *c, PcgState &ps, int max_iters, double *hs) {
+static fia_status pcg_run(fia_ctx *c, PcgState &ps, int max_iters, double *hs, const MgHier *mg) {
     const int poll = 32;
     while (true) {
         HIP_TRY(launch_pcg_check(ps, c->stream));
         HIP_TRY(hipMemcpyAsync(hs, ps.scal, kPcgScalars * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(hipStreamSynchronize(c->stream));
         if (hs[kPcgDone] != 0.0 || ps.it >= max_iters) break;
-        HIP_TRY(launch_pcg_iterations(ps, std::min(poll, max_iters - ps.it), c->stream));
+        HIP_TRY(launch_pcg_iterations(ps, std::min(poll, max_iters - ps.it), c->stream, mg));
     }
     return FIA_OK;
 }
@@ -643,7 +699,7 @@ fia_status scene_solve_setup(fia_ctx *cw, fia_ctx *main, int s, fia_scene *sc, f
         if ((st = bind_solve(sv.slot[1], sv.pg, sv.b.p, sv.flags.p, sb)) != FIA_OK) return st;
         LearnedStart ls;
         if ((st = learned_start(cw, net, sv.pg, sa, sb, tol2, std::min(5, max_iters), cw->host_scal,
-                                cw->host_scal + kPcgScalars, ls)) != FIA_OK)
+                                cw->host_scal + kPcgScalars, ls, nullptr)) != FIA_OK)
             return st;
         sv.st = ls.use_net ? sa : sb;
         sv.used_network = ls.use_net ? 1 : 0;
@@ -811,7 +867,7 @@ fia_status fia_ctx_destroy(fia_ctx *c) {
     for (int a = 0; a < 3; ++a) { c->disp[a].release(); c->work[a].release(); c->gen[a].release(); }
     for (int a = 0; a < 4; ++a) c->grid[a].release();
     for (int a = 0; a < 2; ++a) c->genD[a].release();
-    c->pcg_b.release(); c->pcg_flags.release(); c->pcgs[0].release(); c->pcgs[1].release();
+    c->pcg_b.release(); c->pcg_flags.release(); c->pcgs[0].release(); c->pcgs[1].release(); c->mgb.release();
     for (auto &sv : c->solves) sv.release();
     c->solves.clear();
     c->batch_states.release(); c->batch_map.release(); c->batch_scal.release();
@@ -838,6 +894,13 @@ fia_status fia_ctx_set_timing(fia_ctx *c, int enable) {
         HIP_TRY(hipMemsetAsync(c->stats.p, 0, kStatWords * kStatSlots * sizeof(unsigned long long), c->stream));
         HIP_TRY(hipStreamSynchronize(c->stream));
     }
+    return FIA_OK;
+}
+
+fia_status fia_ctx_set_pressure_preconditioner(fia_ctx *c, fia_precond kind) {
+    if (!c) return fail(FIA_ERR_INVALID_ARG, "ctx is NULL");
+    if (kind != FIA_PRECOND_JACOBI && kind != FIA_PRECOND_MG) return fail(FIA_ERR_INVALID_ARG, "unknown preconditioner");
+    c->precond = (int)kind;
     return FIA_OK;
 }
 
@@ -1130,10 +1193,12 @@ fia_status fia_pressure_project(fia_ctx *c, fia_grid3d *g, const uint8_t *solid,
         if (g->on_device) pd = pressure;
         else if (warm_start) HIP_TRY(hipMemcpyAsync(pd, pressure, n * 4, hipMemcpyHostToDevice, c->stream));
     }
+    const MgHier *mg = nullptr;
+    if ((st = mg_prepare(c, ps, &mg)) != FIA_OK) return st;
     HIP_TRY(launch_pcg_guess(ps, (pressure && warm_start) ? pd : nullptr, c->stream));
-    HIP_TRY(launch_pcg_start(ps, rel_tol * rel_tol, c->stream));
+    if ((st = pcg_begin(c, ps, rel_tol * rel_tol, mg)) != FIA_OK) return st;
     double *hs = c->host_scal;   // pinned
-    if ((st = pcg_run(c, ps, max_iters, hs)) != FIA_OK) return st;
+    if ((st = pcg_run(c, ps, max_iters, hs, mg)) != FIA_OK) return st;
     const int iters_done = (int)hs[kPcgIters];
     const double bb = hs[kPcgBB] > 0 ? hs[kPcgBB] : 1.0;
     const double res = sqrt(hs[kPcgRR] / bb);
@@ -1351,12 +1416,14 @@ fia_status fia_pressure_project_learned(fia_ctx *c, fia_grid3d *g, const uint8_t
     const int probe = std::min(5, max_iters);
     double *ha = c->host_scal, *hb = c->host_scal + kPcgScalars;
     LearnedStart ls;
-    if ((st = learned_start(c, u, pg, sa, sb, tol2, probe, ha, hb, ls)) != FIA_OK) return st;
+    const MgHier *mg = nullptr;
+    if ((st = mg_prepare(c, sa, &mg)) != FIA_OK) return st;
+    if ((st = learned_start(c, u, pg, sa, sb, tol2, probe, ha, hb, ls, mg)) != FIA_OK) return st;
     const bool use_net = ls.use_net;
     const double bb = ls.bb;
     PcgState &ps = use_net ? sa : sb;
     double *hs = use_net ? ha : hb;
-    if ((st = pcg_run(c, ps, max_iters, hs)) != FIA_OK) return st;
+    if ((st = pcg_run(c, ps, max_iters, hs, mg)) != FIA_OK) return st;
     const int iters_done = (int)hs[kPcgIters];
     const double res = sqrt(hs[kPcgRR] / bb);
     const bool converged = hs[kPcgDone] != 0.0;

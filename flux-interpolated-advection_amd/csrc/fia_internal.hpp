@@ -197,10 +197,39 @@ struct PcgState {
     int dir_blocks, dir_tiles_y, dir_tiles_z, dir_chunk_x, dir_chunks;   // pcg_direction_tiling
     bool recompute_q;     // pcg_direction_tiling: the update recomputes q on the direction tiles
     int upd_blocks;       // blocks of the update kernel: dir_blocks (recompute) or nblocks (streaming)
+    // multigrid preconditioner (null for Jacobi): z = M r written by the last V-cycle kernel, with
+    // dir_blocks partial sums of r.z in part_mg
+    double *z;
+    double *part_mg;
     int it;             // iterations enqueued since the start (selects the p buffer)
 };
 int pcg_blocks(size_t n);
 void pcg_direction_tiling(int nx, int ny, int nz, PcgState &st);
+
+// Geometric multigrid preconditioner for the pressure system: level 0 is the system itself (flag
+// bytes), level l + 1 aggregates 2 x 2 x 2 cells of level l with the exact Galerkin operator of
+// piecewise-constant prolongation (coarse face weight = fluid face pairs between the aggregates,
+// diagonal keeping the pinned cell's coupling), stored as float4 {diag, w+x, w+y, w+z}; a cell
+// with diag 0 is not an unknown.  One V(1,1) cycle with damped Jacobi (symmetric, so CG keeps its
+// guarantees) and a fixed number of Jacobi sweeps on the coarsest level.
+constexpr int kMgMaxLevels = 12;
+struct MgLevel {
+    int nx, ny, nz;
+    float4 *op;       // levels >= 1
+    float *b;         // right-hand side (level 0: the residual of the first sweep; it reads r)
+    float *x;         // pre-smoothed iterate (the coarsest level: its solution)
+    float *xs;        // post-smoothed iterate (levels 1 .. L-2)
+};
+struct MgHier {
+    int levels = 0;
+    MgLevel lv[kMgMaxLevels];
+};
+// dims of every level for a grid (coarsest: at most 256 cells or kMgMaxLevels levels)
+void mg_shape(int nx, int ny, int nz, MgHier &h);
+// coarse operators of levels 1 .. L-1 from the solve's flag bytes
+hipError_t launch_mg_build(const PcgState &st, const MgHier &h, hipStream_t s);
+// z = M r and the partial sums of r.z (a no-op once the solve has converged)
+hipError_t launch_mg_vcycle(const PcgState &st, const MgHier &h, hipStream_t s);
 hipError_t launch_pcg_setup(const PcgGrid &g, const float *u, const float *v, const float *w, double h,
                             double *b, uint8_t *flags, hipStream_t s);
 // x = p_in on the unknowns (0 elsewhere); p_in null: x = 0
@@ -210,7 +239,7 @@ hipError_t launch_pcg_guess(const PcgState &st, const float *p_in, hipStream_t s
 // x *= x.b / x.L x (the energy-minimising multiple of the initial guess in x)
 hipError_t launch_pcg_scale_guess(const PcgState &st, hipStream_t s);
 hipError_t launch_pcg_start(PcgState &st, double tol2, hipStream_t s);
-hipError_t launch_pcg_iterations(PcgState &st, int count, hipStream_t s);
+hipError_t launch_pcg_iterations(PcgState &st, int count, hipStream_t s, const MgHier *mg = nullptr);
 hipError_t launch_pcg_check(const PcgState &st, hipStream_t s);
 // Many started solves advanced together (all at the same iteration), one launch per kernel and
 // iteration: their states and, per block of the launches, (solve, block of that solve), all in
@@ -240,6 +269,7 @@ struct fia_ctx;
 namespace fia {
 int ctx_device(const fia_ctx *c);
 hipStream_t ctx_stream(const fia_ctx *c);
+int ctx_precond(const fia_ctx *c);
 fia_status set_error(fia_status status, const std::string &msg);   // returns status
 // batched pressure solves of many scenes (fia_api.hip): buffers on the main context, set-up of
 // one scene on a worker context, then the joint solve and projection on the main context

@@ -151,6 +151,11 @@ __device__ __forceinline__ void residual_state(const PcgState &st, int k, double
         total_of<2>(st.part_update, st.upd_blocks, tot);
         rz = tot[0]; rr = tot[1]; bb = st.scal[kPcgBB];
     }
+    if (st.z) {   // multigrid: r.z of the last V-cycle
+        double t[1];
+        total_of<1>(st.part_mg, st.dir_blocks, t);
+        rz = t[0];
+    }
 }
 
 // Cells of block blk of a solve.  Workgroups are dispatched to the 8 XCDs round-robin, so block b
@@ -186,7 +191,7 @@ __device__ __forceinline__ double stencil(unsigned f, size_t c, long sx, long sy
     return s;
 }
 
-// r = b - L x, p_old = 0, and partial rz, rr, bb; block 0 resets the tolerance and the flags
+// r = b - L x, p_old = 0, and partial rz (Jacobi), rr, bb; block 0 resets the tolerance and the flags
 __device__ __forceinline__ void pcg_start_body(const PcgState &st, int blk, double tol2) {
     const size_t n = (size_t)st.g.nx * st.g.ny * st.g.nz;
     const long sx = (long)st.g.ny * st.g.nz, sy = st.g.nz;
@@ -251,17 +256,20 @@ struct DirOp {
     const double *__restrict__ pold;
     double *__restrict__ pnew;
     double beta;
+    // Jacobi: r (z = r / diag on the fly); multigrid: the V-cycle's z
     __device__ __forceinline__ Raw load(int x, int y, int z, bool) const {
         Raw a{0u, 0.0, 0.0};
         if (x >= 0 && x < st.g.nx && y >= 0 && y < st.g.ny && z >= 0 && z < st.g.nz) {
             const size_t c = idx3(x, y, z, st.g.ny, st.g.nz);
             a.f = st.flags[c];
-            a.r = st.r[c];
+            a.r = st.z ? st.z[c] : st.r[c];
             a.p = pold[c];
         }
         return a;
     }
-    __device__ __forceinline__ double value(const Raw &a) const { return a.r * inv_diag(a.f) + beta * a.p; }
+    __device__ __forceinline__ double value(const Raw &a) const {
+        return (st.z ? a.r : a.r * inv_diag(a.f)) + beta * a.p;
+    }
     __device__ __forceinline__ void cell(size_t c, const Raw &, double v, double q, double *acc) const {
         pnew[c] = v;
         if (st.q) st.q[c] = q;
@@ -525,6 +533,305 @@ __global__ void pcg_guess_kernel(PcgState st, const float *p_in) {
     st.x[c] = (p_in && (st.flags[c] & kPcgUnknown)) ? (double)p_in[c] : 0.0;
 }
 
+// ---- multigrid preconditioner -----------------------------------------------------------------
+// Jacobi damping of the smoother and sweeps on the coarsest level.  The damped Jacobi iteration
+// matrix I - w D^-1 A of a graph Laplacian has spectral radius below 1 for w < 1, which keeps the
+// V-cycle symmetric positive definite.
+constexpr float kMgOmega = 0.8f;
+constexpr int kMgCoarseSweeps = 48;
+constexpr int kMgCoarsestCells = 256;
+
+// A cell of a level: unknown, diagonal and the weights to its six neighbours (-x, +x, -y, +y, -z,
+// +z).  Level 0 reads the flag byte (weight 1 to each fluid neighbour, the diagonal counts them);
+// coarser levels read {diag, w+x, w+y, w+z} and the - weights from the neighbours' + weights.
+struct MgCell {
+    bool unknown;
+    float d;
+    float w[6];
+};
+__device__ __forceinline__ MgCell mg_cell0(const uint8_t *flags, size_t c) {
+    const unsigned f = flags[c];
+    MgCell a;
+    a.unknown = (f & kPcgUnknown) != 0;
+    a.d = (float)__builtin_popcount(f & 63u);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) a.w[q] = (f >> q) & 1u ? 1.0f : 0.0f;
+    return a;
+}
+__device__ __forceinline__ MgCell mg_cell(const MgLevel &L, int i, int j, int k) {
+    const size_t sx = (size_t)L.ny * L.nz, sy = L.nz;
+    const size_t c = idx3(i, j, k, L.ny, L.nz);
+    const float4 o = L.op[c];
+    MgCell a;
+    a.unknown = o.x > 0.0f;
+    a.d = o.x;
+    a.w[0] = i > 0 ? L.op[c - sx].y : 0.0f;
+    a.w[1] = o.y;
+    a.w[2] = j > 0 ? L.op[c - sy].z : 0.0f;
+    a.w[3] = o.z;
+    a.w[4] = k > 0 ? L.op[c - 1].w : 0.0f;
+    a.w[5] = o.w;
+    return a;
+}
+__device__ __forceinline__ bool mg_unknown(const PcgState &st, const MgLevel &L, int l, size_t c) {
+    return l == 0 ? (st.flags[c] & kPcgUnknown) != 0 : L.op[c].x > 0.0f;
+}
+__device__ __forceinline__ MgCell mg_get(const PcgState &st, const MgLevel &L, int l, int i, int j, int k) {
+    return l == 0 ? mg_cell0(st.flags, idx3(i, j, k, L.ny, L.nz)) : mg_cell(L, i, j, k);
+}
+__device__ __forceinline__ void mg_neighbour(int q, int i, int j, int k, int &ni, int &nj, int &nk) {
+    ni = i + (q == 1) - (q == 0);
+    nj = j + (q == 3) - (q == 2);
+    nk = k + (q == 5) - (q == 4);
+}
+
+// Level l + 1's operator, one thread per aggregate: the diagonal sums the children's diagonals
+// less twice the weights of the faces between children; a child's face to an unknown of the next
+// aggregate up an axis adds to that + weight (the - weights are the neighbours' + weights).
+__global__ void __launch_bounds__(kPB) mg_build_kernel(PcgState st, MgLevel F, MgLevel C, int l) {
+    const size_t n = (size_t)C.nx * C.ny * C.nz;
+    const size_t I = (size_t)blockIdx.x * kPB + threadIdx.x;
+    if (I >= n) return;
+    const int K = (int)(I % C.nz), J = (int)((I / C.nz) % C.ny), II = (int)(I / ((size_t)C.ny * C.nz));
+    float d = 0.0f, wp[3] = {0.0f, 0.0f, 0.0f};
+    for (int ci = 0; ci < 2; ++ci)
+        for (int cj = 0; cj < 2; ++cj)
+            for (int ck = 0; ck < 2; ++ck) {
+                const int i = 2 * II + ci, j = 2 * J + cj, k = 2 * K + ck;
+                if (i >= F.nx || j >= F.ny || k >= F.nz) continue;
+                const MgCell a = mg_get(st, F, l, i, j, k);
+                if (!a.unknown) continue;
+                d += a.d;
+#pragma unroll
+                for (int q = 0; q < 6; ++q) {
+                    if (a.w[q] == 0.0f) continue;
+                    int ni, nj, nk;
+                    mg_neighbour(q, i, j, k, ni, nj, nk);
+                    if (ni < 0 || nj < 0 || nk < 0 || ni >= F.nx || nj >= F.ny || nk >= F.nz) continue;
+                    if (!mg_unknown(st, F, l, idx3(ni, nj, nk, F.ny, F.nz))) continue;   // Dirichlet: in d
+                    if ((ni >> 1) == II && (nj >> 1) == J && (nk >> 1) == K) d -= a.w[q];
+                    else if (q & 1) wp[q >> 1] += a.w[q];
+                }
+            }
+    C.op[I] = make_float4(d, wp[0], wp[1], wp[2]);
+}
+
+// b and L x of a cell at level l (level 0: b = r, FP64 in memory)
+__device__ __forceinline__ float mg_b(const PcgState &st, const MgLevel &L, int l, size_t c) {
+    return l == 0 ? (float)st.r[c] : L.b[c];
+}
+
+// Down the V-cycle at level l, one thread per cell of a 2 x 2 x 64 tile (32 aggregates of level
+// l + 1 along z): every unknown cell gets the first smoothing sweep from zero, x = w b / d, and
+// its residual b - L x (the neighbours' x recomputed from their b); each aggregate's right-hand
+// side is the sum of its eight children's residuals, added in a fixed order through LDS.
+constexpr int kMgTileZ = 64;
+__global__ void __launch_bounds__(kPB) mg_down_kernel(PcgState st, MgLevel F, MgLevel C, int l) {
+    if (solve_done(st.scal)) return;
+    __shared__ float res[kPB];
+    const int tz = threadIdx.x % kMgTileZ, ty = (threadIdx.x / kMgTileZ) & 1, tx = threadIdx.x / (2 * kMgTileZ);
+    const int tiles_z = (F.nz + kMgTileZ - 1) / kMgTileZ, tiles_y = (F.ny + 1) / 2;
+    const int bz = blockIdx.x % tiles_z, by = (blockIdx.x / tiles_z) % tiles_y, bx = blockIdx.x / (tiles_z * tiles_y);
+    const int i = 2 * bx + tx, j = 2 * by + ty, k = bz * kMgTileZ + tz;
+    float rc = 0.0f;
+    if (i < F.nx && j < F.ny && k < F.nz) {
+        const size_t c = idx3(i, j, k, F.ny, F.nz);
+        const MgCell a = mg_get(st, F, l, i, j, k);
+        if (a.unknown) {
+            const float bc = mg_b(st, F, l, c);
+            const float xc = kMgOmega * bc / a.d;
+            F.x[c] = xc;
+            float lx = a.d * xc;
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                if (a.w[q] == 0.0f) continue;
+                int ni, nj, nk;
+                mg_neighbour(q, i, j, k, ni, nj, nk);
+                if (ni < 0 || nj < 0 || nk < 0 || ni >= F.nx || nj >= F.ny || nk >= F.nz) continue;
+                const MgCell b = mg_get(st, F, l, ni, nj, nk);
+                if (!b.unknown) continue;
+                lx -= a.w[q] * (kMgOmega * mg_b(st, F, l, idx3(ni, nj, nk, F.ny, F.nz)) / b.d);
+            }
+            rc = bc - lx;
+        } else {
+            F.x[c] = 0.0f;
+        }
+    }
+    res[threadIdx.x] = rc;
+    __syncthreads();
+    if (threadIdx.x < kMgTileZ / 2) {
+        const int az = threadIdx.x;
+        const int I = bx, J = by, K = bz * (kMgTileZ / 2) + az;
+        if (I < C.nx && J < C.ny && K < C.nz) {
+            float sum = 0.0f;
+#pragma unroll
+            for (int cx = 0; cx < 2; ++cx)
+#pragma unroll
+                for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+                    for (int cz = 0; cz < 2; ++cz) sum += res[cx * 2 * kMgTileZ + cy * kMgTileZ + 2 * az + cz];
+            const size_t ci = idx3(I, J, K, C.ny, C.nz);
+            C.b[ci] = C.op[ci].x > 0.0f ? sum : 0.0f;
+        }
+    }
+}
+
+// The coarsest level (at most kMgCoarsestCells cells, one per thread): kMgCoarseSweeps damped
+// Jacobi sweeps from zero, in LDS.
+__global__ void __launch_bounds__(kPB) mg_coarsest_kernel(PcgState st, MgLevel L) {
+    if (solve_done(st.scal)) return;
+    __shared__ float xs[2][kMgCoarsestCells];
+    const int n = L.nx * L.ny * L.nz;
+    const int c = threadIdx.x;
+    int i = 0, j = 0, k = 0;
+    MgCell a{false, 0.0f, {0, 0, 0, 0, 0, 0}};
+    float bc = 0.0f;
+    if (c < n) {
+        k = c % L.nz; j = (c / L.nz) % L.ny; i = c / (L.ny * L.nz);
+        a = mg_cell(L, i, j, k);
+        bc = L.b[c];
+    }
+    xs[0][c] = 0.0f;
+    __syncthreads();
+    int cur = 0;
+    for (int s = 0; s < kMgCoarseSweeps; ++s) {
+        float v = 0.0f;
+        if (c < n && a.unknown) {
+            float lx = a.d * xs[cur][c];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                if (a.w[q] == 0.0f) continue;
+                int ni, nj, nk;
+                mg_neighbour(q, i, j, k, ni, nj, nk);
+                lx -= a.w[q] * xs[cur][(ni * L.ny + nj) * L.nz + nk];
+            }
+            v = xs[cur][c] + kMgOmega * (bc - lx) / a.d;
+        }
+        xs[cur ^ 1][c] = v;
+        __syncthreads();
+        cur ^= 1;
+    }
+    if (c < n) L.x[c] = xs[cur][c];
+}
+
+// Up the V-cycle at level l: x' = x + the aggregate's correction, then one smoothing sweep
+// x'' = x' + w (b - L x') / d (the neighbours' x' recomputed the same way).
+__device__ __forceinline__ float mg_up_cell(const PcgState &st, const MgLevel &F, const float *corr, int cny, int cnz,
+                                            int l, size_t c, float bc) {
+    const int k = (int)(c % F.nz), j = (int)((c / F.nz) % F.ny), i = (int)(c / ((size_t)F.ny * F.nz));
+    const MgCell a = mg_get(st, F, l, i, j, k);
+    if (!a.unknown) return 0.0f;
+    const float xc = F.x[c] + corr[idx3(i >> 1, j >> 1, k >> 1, cny, cnz)];
+    float lx = a.d * xc;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        if (a.w[q] == 0.0f) continue;
+        int ni, nj, nk;
+        mg_neighbour(q, i, j, k, ni, nj, nk);
+        if (ni < 0 || nj < 0 || nk < 0 || ni >= F.nx || nj >= F.ny || nk >= F.nz) continue;
+        const size_t cn = idx3(ni, nj, nk, F.ny, F.nz);
+        if (!mg_unknown(st, F, l, cn)) continue;
+        lx -= a.w[q] * (F.x[cn] + corr[idx3(ni >> 1, nj >> 1, nk >> 1, cny, cnz)]);
+    }
+    return xc + kMgOmega * (bc - lx) / a.d;
+}
+// levels >= 1: one thread per cell, writes xs
+__global__ void __launch_bounds__(kPB) mg_up_kernel(PcgState st, MgLevel F, const float *corr, int cny, int cnz,
+                                                    int l) {
+    if (solve_done(st.scal)) return;
+    const size_t n = (size_t)F.nx * F.ny * F.nz;
+    const size_t c = (size_t)blockIdx.x * kPB + threadIdx.x;
+    if (c < n) F.xs[c] = mg_up_cell(st, F, corr, cny, cnz, l, c, F.b[c]);
+}
+// Level 0 of the V-cycle on the solve's tiles (tile_sweep: the 2.5-D sweep of the CG kernels).
+// Down: v = x = w r / d (the first smoothing sweep from zero), stored, and the residual r - L x
+// into the level's residual buffer (b), summed over the 2 x 2 x 2 aggregates by
+// mg_restrict0_kernel.  Up: v = x + the aggregate's correction, z = v + w (r - L v) / d and
+// partial sums of r.z.
+struct MgDown0Raw {
+    unsigned f;
+    double r;
+};
+struct MgDown0Op {
+    using Raw = MgDown0Raw;
+    const PcgState &st;
+    float *x, *res;
+    __device__ __forceinline__ Raw load(int x0, int y, int z, bool) const {
+        Raw a{0u, 0.0};
+        if (x0 >= 0 && x0 < st.g.nx && y >= 0 && y < st.g.ny && z >= 0 && z < st.g.nz) {
+            const size_t c = idx3(x0, y, z, st.g.ny, st.g.nz);
+            a.f = st.flags[c];
+            a.r = st.r[c];
+        }
+        return a;
+    }
+    __device__ __forceinline__ double value(const Raw &a) const {
+        return (a.f & kPcgUnknown) ? (double)(kMgOmega * (float)a.r / (float)__builtin_popcount(a.f & 63u)) : 0.0;
+    }
+    __device__ __forceinline__ void cell(size_t c, const Raw &a, double v, double q, double *) const {
+        x[c] = (float)v;
+        res[c] = (a.f & kPcgUnknown) ? (float)(a.r - q) : 0.0f;
+    }
+};
+struct MgUp0Raw {
+    unsigned f;
+    double v, r;
+};
+struct MgUp0Op {
+    using Raw = MgUp0Raw;
+    const PcgState &st;
+    const float *x, *corr;
+    int cny, cnz;
+    __device__ __forceinline__ Raw load(int x0, int y, int z, bool own) const {
+        Raw a{0u, 0.0, 0.0};
+        if (x0 >= 0 && x0 < st.g.nx && y >= 0 && y < st.g.ny && z >= 0 && z < st.g.nz) {
+            const size_t c = idx3(x0, y, z, st.g.ny, st.g.nz);
+            // x and the correction exist for every cell: loaded with the flag byte, not after it
+            const float xv = x[c] + corr[idx3(x0 >> 1, y >> 1, z >> 1, cny, cnz)];
+            a.f = st.flags[c];
+            a.v = (a.f & kPcgUnknown) ? (double)xv : 0.0;
+            if (own) a.r = st.r[c];
+        }
+        return a;
+    }
+    __device__ __forceinline__ double value(const Raw &a) const { return a.v; }
+    __device__ __forceinline__ void cell(size_t c, const Raw &a, double v, double q, double *acc) const {
+        double zc = 0.0;
+        if (a.f & kPcgUnknown)
+            zc = (double)((float)v + kMgOmega * (float)(a.r - q) / (float)__builtin_popcount(a.f & 63u));
+        st.z[c] = zc;
+        acc[0] += a.r * zc;
+    }
+};
+__global__ void __launch_bounds__(kPB) mg_down0_kernel(PcgState st, float *x, float *res) {
+    if (solve_done(st.scal)) return;
+    double acc[1] = {0.0};
+    tile_sweep(st, blockIdx.x, MgDown0Op{st, x, res}, acc);
+}
+// b of level 1: the sum of each aggregate's children's residuals (fixed order)
+__global__ void __launch_bounds__(kPB) mg_restrict0_kernel(PcgState st, MgLevel F, MgLevel C) {
+    if (solve_done(st.scal)) return;
+    const size_t n = (size_t)C.nx * C.ny * C.nz;
+    const size_t I = (size_t)blockIdx.x * kPB + threadIdx.x;
+    if (I >= n) return;
+    const int K = (int)(I % C.nz), J = (int)((I / C.nz) % C.ny), II = (int)(I / ((size_t)C.ny * C.nz));
+    float sum = 0.0f;
+    for (int ci = 0; ci < 2; ++ci)
+        for (int cj = 0; cj < 2; ++cj)
+            for (int ck = 0; ck < 2; ++ck) {
+                const int i = 2 * II + ci, j = 2 * J + cj, k = 2 * K + ck;
+                if (i < F.nx && j < F.ny && k < F.nz) sum += F.b[idx3(i, j, k, F.ny, F.nz)];
+            }
+    C.b[I] = C.op[I].x > 0.0f ? sum : 0.0f;
+}
+__global__ void __launch_bounds__(kPB) mg_up0_tile_kernel(PcgState st, const float *x, const float *corr, int cny,
+                                                          int cnz) {
+    if (solve_done(st.scal)) return;
+    double acc[1] = {0.0};
+    tile_sweep(st, blockIdx.x, MgUp0Op{st, x, corr, cny, cnz}, acc);
+    write_partials<1>(acc, st.part_mg, st.dir_blocks, blockIdx.x);
+}
+
 // One solve per launch ...
 __global__ void __launch_bounds__(kPB) pcg_start_kernel(PcgState st, double tol2) {
     pcg_start_body(st, blockIdx.x, tol2);
@@ -612,11 +919,15 @@ hipError_t launch_pcg_start(PcgState &st, double tol2, hipStream_t s) {
     hipLaunchKernelGGL(pcg_start_kernel, dim3(st.nblocks), dim3(kPB), 0, s, st, tol2);
     return hipGetLastError();
 }
-hipError_t launch_pcg_iterations(PcgState &st, int count, hipStream_t s) {
+hipError_t launch_pcg_iterations(PcgState &st, int count, hipStream_t s, const MgHier *mg) {
     for (int k = 0; k < count; ++k) {
         hipLaunchKernelGGL(pcg_direction_kernel, dim3(st.dir_blocks), dim3(kPB), 0, s, st, st.it);
         if (st.recompute_q) hipLaunchKernelGGL(pcg_update_kernel<true>, dim3(st.upd_blocks), dim3(kPB), 0, s, st, st.it);
         else hipLaunchKernelGGL(pcg_update_kernel<false>, dim3(st.upd_blocks), dim3(kPB), 0, s, st, st.it);
+        if (mg) {
+            const hipError_t e = launch_mg_vcycle(st, *mg, s);
+            if (e != hipSuccess) return e;
+        }
         ++st.it;
     }
     return hipGetLastError();
@@ -642,6 +953,50 @@ hipError_t launch_pcg_batch_iterations(PcgBatch &b, int count, hipStream_t s) {
 hipError_t launch_pcg_batch_check(const PcgBatch &b, double *out, hipStream_t s) {
     if (b.n == 0) return hipSuccess;
     hipLaunchKernelGGL(pcg_check_batch_kernel, dim3(b.n), dim3(kPB), 0, s, b.states, b.it, out);
+    return hipGetLastError();
+}
+void mg_shape(int nx, int ny, int nz, MgHier &h) {
+    h.levels = 0;
+    int x = nx, y = ny, z = nz;
+    while (h.levels < kMgMaxLevels) {
+        MgLevel &L = h.lv[h.levels++];
+        L.nx = x; L.ny = y; L.nz = z;
+        L.op = nullptr; L.b = nullptr; L.x = nullptr; L.xs = nullptr;
+        if (h.levels >= 2 && (size_t)x * y * z <= (size_t)kMgCoarsestCells) break;
+        x = (x + 1) / 2; y = (y + 1) / 2; z = (z + 1) / 2;
+    }
+}
+hipError_t launch_mg_build(const PcgState &st, const MgHier &h, hipStream_t s) {
+    for (int l = 0; l + 1 < h.levels; ++l) {
+        const size_t n = (size_t)h.lv[l + 1].nx * h.lv[l + 1].ny * h.lv[l + 1].nz;
+        hipLaunchKernelGGL(mg_build_kernel, dim3(blocks(n)), dim3(kPB), 0, s, st, h.lv[l], h.lv[l + 1], l);
+    }
+    return hipGetLastError();
+}
+hipError_t launch_mg_vcycle(const PcgState &st, const MgHier &h, hipStream_t s) {
+    const int L = h.levels;
+    hipLaunchKernelGGL(mg_down0_kernel, dim3(st.dir_blocks), dim3(kPB), 0, s, st, h.lv[0].x, h.lv[0].b);
+    {
+        const size_t n1 = (size_t)h.lv[1].nx * h.lv[1].ny * h.lv[1].nz;
+        hipLaunchKernelGGL(mg_restrict0_kernel, dim3(blocks(n1)), dim3(kPB), 0, s, st, h.lv[0], h.lv[1]);
+    }
+    for (int l = 1; l + 1 < L; ++l) {
+        const MgLevel &F = h.lv[l];
+        const unsigned tiles = (unsigned)(((F.nz + kMgTileZ - 1) / kMgTileZ) * ((F.ny + 1) / 2) * ((F.nx + 1) / 2));
+        hipLaunchKernelGGL(mg_down_kernel, dim3(tiles), dim3(kPB), 0, s, st, F, h.lv[l + 1], l);
+    }
+    hipLaunchKernelGGL(mg_coarsest_kernel, dim3(1), dim3(kPB), 0, s, st, h.lv[L - 1]);
+    for (int l = L - 2; l >= 0; --l) {
+        const MgLevel &C = h.lv[l + 1];
+        const float *corr = l + 1 == L - 1 ? C.x : C.xs;
+        if (l == 0) {
+            hipLaunchKernelGGL(mg_up0_tile_kernel, dim3(st.dir_blocks), dim3(kPB), 0, s, st, h.lv[0].x, corr, C.ny,
+                               C.nz);
+        } else {
+            const size_t n = (size_t)h.lv[l].nx * h.lv[l].ny * h.lv[l].nz;
+            hipLaunchKernelGGL(mg_up_kernel, dim3(blocks(n)), dim3(kPB), 0, s, st, h.lv[l], corr, C.ny, C.nz, l);
+        }
+    }
     return hipGetLastError();
 }
 hipError_t launch_project(const PcgGrid &g, const double *p, double h, float *u, float *v, float *w,

@@ -66,6 +66,14 @@ typedef enum fia_psi_mode {
     FIA_PSI_WENO6 = 2    /* Gauss-Legendre over 1D-y WENO6 of q on both half cells           */
 } fia_psi_mode;
 
+/* Preconditioner of the pressure solve's conjugate gradients (the reference uses amgcl: CG with a
+ * smoothed-aggregation AMG preconditioner, src/calc_pressure_3d.cpp:141-166). */
+typedef enum fia_precond {
+    FIA_PRECOND_JACOBI = 0,   /* diagonal (default)                                              */
+    FIA_PRECOND_MG = 1        /* geometric multigrid: 2x2x2 aggregation, Galerkin coarse operators, */
+                              /* one V(1,1) cycle with damped Jacobi per CG iteration             */
+} fia_precond;
+
 /* Options of move_substances_3D / advect_velocity_*_flux_advection_3D_std_thread, one field per
  * argument of the reference signature (reference include/move_substances_3d.h:48-67,
  * include/advect_velocity_flux_advection_3d.h:14-124).  Strings are parsed once by the caller
@@ -220,6 +228,10 @@ typedef struct fia_kernel_stats {
 } fia_kernel_stats;
 
 fia_status fia_ctx_set_timing(fia_ctx *ctx, int enable);
+/* The preconditioner of every later pressure solve on this context (fia_pressure_project,
+ * fia_pressure_project_learned, fia_scenes_step).  No counterpart in the reference, whose solver
+ * is fixed; fia_scenes_step_batched requires FIA_PRECOND_JACOBI on ctxs[0]. */
+fia_status fia_ctx_set_pressure_preconditioner(fia_ctx *ctx, fia_precond kind);
 fia_status fia_ctx_get_stats(fia_ctx *ctx, fia_kernel_stats *out);   /* synchronises */
 fia_status fia_ctx_reset_stats(fia_ctx *ctx);
 
