@@ -639,9 +639,11 @@ static fia_status learned_start(fia_ctx *c, fia_unet *u, const PcgGrid &pg, PcgS
 
 // Iterates a started solve until it converges or has run max_iters iterations in total; hs
 // (pinned, kPcgScalars doubles) receives its scalars.  The device skips iterations after
-// convergence, so batches of them are enqueued between polls.
+// convergence, so batches of them are enqueued between polls: 32 for Jacobi (thousands of
+// iterations), 8 under multigrid (tens of iterations of ~19 launches, most of a 32-batch after
+// convergence would be launches that return at once).
 static fia_status pcg_run(fia_ctx *c, PcgState &ps, int max_iters, double *hs, const MgHier *mg) {
-    const int poll = 32;
+    const int poll = mg ? 8 : 32;
     while (true) {
         HIP_TRY(launch_pcg_check(ps, c->stream));
         HIP_TRY(hipMemcpyAsync(hs, ps.scal, kPcgScalars * sizeof(double), hipMemcpyDeviceToHost, c->stream));
