@@ -135,8 +135,15 @@ struct fia_ctx {
         PcgGrid pg;
         PcgState st;              // the solve the joint iterations continue
         int used_network = 0;
-        size_t bytes() const { return b.bytes() + flags.bytes() + slot[0].bytes() + slot[1].bytes(); }
-        void release() { b.release(); flags.release(); slot[0].release(); slot[1].release(); }
+        MgBufs mgb;               // multigrid levels (main context's preconditioner FIA_PRECOND_MG)
+        MgHier mgh;
+        DevBuf<MgLevel> lv_dev;   // mgh.lv in device memory, for the batched V-cycle
+        size_t bytes() const {
+            return b.bytes() + flags.bytes() + slot[0].bytes() + slot[1].bytes() + mgb.bytes() + lv_dev.bytes();
+        }
+        void release() {
+            b.release(); flags.release(); slot[0].release(); slot[1].release(); mgb.release(); lv_dev.release();
+        }
     };
     std::vector<SceneSolve> solves;
     DevBuf<PcgState> batch_states;
@@ -559,8 +566,32 @@ static fia_status bind_solve(fia_ctx::PcgBufs &B, const PcgGrid &pg, const doubl
     ps.q = ps.recompute_q ? nullptr : B.q.p;
     ps.z = mg ? B.z.p : nullptr;
     ps.part_mg = mg ? B.part_mg.p : nullptr;
+    ps.mg_lv = nullptr;
+    ps.mg_levels = 0;
     ps.scal = B.scal.p; ps.part_start = B.part.p; ps.part_dir = B.part.p + 3 * (size_t)ps.nblocks;
     ps.part_update = ps.part_dir + ps.dir_blocks; ps.it = 0;
+    return FIA_OK;
+}
+
+// A multigrid hierarchy for ps's grid in buffers B, its coarse operators built from ps's flags on
+// stream st.
+static fia_status mg_setup(hipStream_t stream, fia_ctx::MgBufs &B, MgHier &h, const PcgState &ps) {
+    mg_shape(ps.g.nx, ps.g.ny, ps.g.nz, h);
+    const MgLevel &top = h.lv[h.levels - 1];
+    if ((size_t)top.nx * top.ny * top.nz > 256) return fail(FIA_ERR_UNSUPPORTED, "grid too large for the multigrid levels");
+    for (int l = 0; l < h.levels; ++l) {
+        MgLevel &L = h.lv[l];
+        const size_t n = (size_t)L.nx * L.ny * L.nz;
+        HIP_TRY(B.x[l].ensure(n));
+        HIP_TRY(B.b[l].ensure(n));
+        L.x = B.x[l].p;
+        L.b = B.b[l].p;
+        if (l > 0) {
+            HIP_TRY(B.op[l].ensure(n)); HIP_TRY(B.xs[l].ensure(n));
+            L.op = B.op[l].p; L.xs = B.xs[l].p;
+        }
+    }
+    HIP_TRY(launch_mg_build(ps, h, stream));
     return FIA_OK;
 }
 
@@ -569,24 +600,9 @@ static fia_status bind_solve(fia_ctx::PcgBufs &B, const PcgGrid &pg, const doubl
 static fia_status mg_prepare(fia_ctx *c, const PcgState &ps, const MgHier **out) {
     *out = nullptr;
     if (c->precond != FIA_PRECOND_MG) return FIA_OK;
-    MgHier &h = c->mgh;
-    mg_shape(ps.g.nx, ps.g.ny, ps.g.nz, h);
-    const MgLevel &top = h.lv[h.levels - 1];
-    if ((size_t)top.nx * top.ny * top.nz > 256) return fail(FIA_ERR_UNSUPPORTED, "grid too large for the multigrid levels");
-    for (int l = 0; l < h.levels; ++l) {
-        MgLevel &L = h.lv[l];
-        const size_t n = (size_t)L.nx * L.ny * L.nz;
-        HIP_TRY(c->mgb.x[l].ensure(n));
-        HIP_TRY(c->mgb.b[l].ensure(n));
-        L.x = c->mgb.x[l].p;
-        L.b = c->mgb.b[l].p;
-        if (l > 0) {
-            HIP_TRY(c->mgb.op[l].ensure(n)); HIP_TRY(c->mgb.xs[l].ensure(n));
-            L.op = c->mgb.op[l].p; L.xs = c->mgb.xs[l].p;
-        }
-    }
-    HIP_TRY(launch_mg_build(ps, h, c->stream));
-    *out = &h;
+    fia_status st;
+    if ((st = mg_setup(c->stream, c->mgb, c->mgh, ps)) != FIA_OK) return st;
+    *out = &c->mgh;
     return FIA_OK;
 }
 
@@ -694,20 +710,32 @@ fia_status scene_solve_setup(fia_ctx *cw, fia_ctx *main, int s, fia_scene *sc, f
     HIP_TRY(sv.flags.ensure(n));
     if ((st = assemble_system(cw, g, gv, sc->solid, sv.b.p, sv.flags.p, sv.pg)) != FIA_OK) return st;
     const double tol2 = rel_tol * rel_tol;
+    const bool use_mg = main->precond == FIA_PRECOND_MG;
     PcgState sa;
-    if ((st = bind_solve(sv.slot[0], sv.pg, sv.b.p, sv.flags.p, sa)) != FIA_OK) return st;
+    if ((st = bind_solve(sv.slot[0], sv.pg, sv.b.p, sv.flags.p, sa, use_mg)) != FIA_OK) return st;
+    const MgHier *mg = nullptr;
+    if (use_mg) {
+        if ((st = mg_setup(cw->stream, sv.mgb, sv.mgh, sa)) != FIA_OK) return st;
+        HIP_TRY(sv.lv_dev.ensure(kMgMaxLevels));
+        HIP_TRY(hipMemcpyAsync(sv.lv_dev.p, sv.mgh.lv, sizeof(sv.mgh.lv), hipMemcpyHostToDevice, cw->stream));
+        sa.mg_lv = sv.lv_dev.p;
+        sa.mg_levels = sv.mgh.levels;
+        mg = &sv.mgh;
+    }
     if (net) {
         PcgState sb;
-        if ((st = bind_solve(sv.slot[1], sv.pg, sv.b.p, sv.flags.p, sb)) != FIA_OK) return st;
+        if ((st = bind_solve(sv.slot[1], sv.pg, sv.b.p, sv.flags.p, sb, use_mg)) != FIA_OK) return st;
+        sb.mg_lv = sa.mg_lv;
+        sb.mg_levels = sa.mg_levels;
         LearnedStart ls;
         if ((st = learned_start(cw, net, sv.pg, sa, sb, tol2, std::min(5, max_iters), cw->host_scal,
-                                cw->host_scal + kPcgScalars, ls, nullptr)) != FIA_OK)
+                                cw->host_scal + kPcgScalars, ls, mg)) != FIA_OK)
             return st;
         sv.st = ls.use_net ? sa : sb;
         sv.used_network = ls.use_net ? 1 : 0;
     } else {
         HIP_TRY(launch_pcg_guess(sa, nullptr, cw->stream));
-        HIP_TRY(launch_pcg_start(sa, tol2, cw->stream));
+        if ((st = pcg_begin(cw, sa, tol2, mg)) != FIA_OK) return st;
         sv.st = sa;
         sv.used_network = 0;
     }
@@ -737,6 +765,40 @@ fia_status scenes_solve_run(fia_ctx *c, fia_scene *scenes, const int *idx, int m
     for (int j = 0; j < m; ++j)
         if (states[j].recompute_q)
             for (int k = 0; k < states[j].upd_blocks; ++k) map.push_back(make_int2(j, k));
+    const int upd_end = (int)map.size();
+    // the batched V-cycle's launches (multigrid): level 1's restriction, then per level the down and
+    // up launches of the solves that have the level below
+    const bool use_mg = m > 0 && states[0].mg_levels > 0;
+    int r0_off = upd_end, down_off[kMgMaxLevels] = {0}, up_off[kMgMaxLevels] = {0};
+    MgBatch mb{};
+    if (use_mg) {
+        for (int j = 0; j < m; ++j) {
+            const MgHier &h = c->solves[idx[j]].mgh;
+            const size_t n1 = (size_t)h.lv[1].nx * h.lv[1].ny * h.lv[1].nz;
+            for (size_t k = 0; k < (n1 + 255) / 256; ++k) map.push_back(make_int2(j, (int)k));
+        }
+        mb.r0_blocks = (int)map.size() - r0_off;
+        for (int l = 1; l < kMgMaxLevels; ++l) {
+            down_off[l] = (int)map.size();
+            for (int j = 0; j < m; ++j) {
+                const MgHier &h = c->solves[idx[j]].mgh;
+                if (l + 1 < h.levels)
+                    for (int k = 0; k < mg_down_tiles(h.lv[l]); ++k) map.push_back(make_int2(j, k));
+            }
+            mb.down_blocks[l] = (int)map.size() - down_off[l];
+        }
+        for (int l = 1; l < kMgMaxLevels; ++l) {
+            up_off[l] = (int)map.size();
+            for (int j = 0; j < m; ++j) {
+                const MgHier &h = c->solves[idx[j]].mgh;
+                if (l + 1 < h.levels) {
+                    const size_t n = (size_t)h.lv[l].nx * h.lv[l].ny * h.lv[l].nz;
+                    for (size_t k = 0; k < (n + 255) / 256; ++k) map.push_back(make_int2(j, (int)k));
+                }
+            }
+            mb.up_blocks[l] = (int)map.size() - up_off[l];
+        }
+    }
     HIP_TRY(c->batch_map.ensure(map.size()));
     HIP_TRY(hipMemcpyAsync(c->batch_states.p, states.data(), m * sizeof(PcgState), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(c->batch_map.p, map.data(), map.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
@@ -744,9 +806,16 @@ fia_status scenes_solve_run(fia_ctx *c, fia_scene *scenes, const int *idx, int m
     b.states = c->batch_states.p;
     b.dir_map = c->batch_map.p; b.stream_map = c->batch_map.p + dir_blocks; b.tile_map = c->batch_map.p + stream_end;
     b.n = m; b.dir_blocks = dir_blocks; b.stream_blocks = stream_end - dir_blocks;
-    b.tile_blocks = (int)map.size() - stream_end; b.it = states[0].it;
+    b.tile_blocks = upd_end - stream_end; b.it = states[0].it;
+    if (use_mg) {
+        mb.r0_map = c->batch_map.p + r0_off;
+        for (int l = 1; l < kMgMaxLevels; ++l) {
+            mb.down_map[l] = c->batch_map.p + down_off[l];
+            mb.up_map[l] = c->batch_map.p + up_off[l];
+        }
+    }
     double *hs = c->host_batch;
-    const int poll = 32;
+    const int poll = use_mg ? 8 : 32;
     while (true) {
         HIP_TRY(launch_pcg_batch_check(b, c->batch_scal.p, c->stream));
         HIP_TRY(hipMemcpyAsync(hs, c->batch_scal.p, (size_t)m * kPcgScalars * sizeof(double), hipMemcpyDeviceToHost,
@@ -755,7 +824,7 @@ fia_status scenes_solve_run(fia_ctx *c, fia_scene *scenes, const int *idx, int m
         bool all = true;
         for (int j = 0; j < m && all; ++j) all = hs[(size_t)j * kPcgScalars + kPcgDone] != 0.0;
         if (all || b.it >= max_iters) break;
-        HIP_TRY(launch_pcg_batch_iterations(b, std::min(poll, max_iters - b.it), c->stream));
+        HIP_TRY(launch_pcg_batch_iterations(b, std::min(poll, max_iters - b.it), c->stream, use_mg ? &mb : nullptr));
     }
     for (int j = 0; j < m; ++j) {
         const double *h = hs + (size_t)j * kPcgScalars;

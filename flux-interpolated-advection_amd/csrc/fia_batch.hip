@@ -190,8 +190,6 @@ fia_status fia_scenes_step_batched(fia_ctx *const *ctxs, int workers, fia_scene 
             return fia::set_error(FIA_ERR_INVALID_ARG, "a scene needs a device grid with density");
     }
     fia_ctx *main = ctxs[0];
-    if (fia::ctx_precond(main) != FIA_PRECOND_JACOBI)
-        return fia::set_error(FIA_ERR_UNSUPPORTED, "batched scene solves use the Jacobi preconditioner");
     if ((st = fia::scenes_solve_alloc(main, scenes, n_scenes)) != FIA_OK) return st;
     const std::vector<int> order = largest_first(scenes, n_scenes);
     std::vector<double> dt(n_scenes, 0.0);

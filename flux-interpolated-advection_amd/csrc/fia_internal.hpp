@@ -182,6 +182,7 @@ constexpr int kPcgMaxBlocks = 1024;
 // device scalars of one solve: rz of even and odd iterations, last residual r.r, b.b, tol^2,
 // converged flag, iterations done, scale applied to the initial guess
 enum { kPcgRZ = 0, kPcgRR = 3, kPcgBB = 4, kPcgTol2 = 5, kPcgDone = 6, kPcgIters = 7, kPcgGamma = 8, kPcgScalars = 16 };
+struct MgLevel;
 // one CG solve in flight: its vectors and device scalars
 struct PcgState {
     PcgGrid g;
@@ -201,6 +202,8 @@ struct PcgState {
     // dir_blocks partial sums of r.z in part_mg
     double *z;
     double *part_mg;
+    const MgLevel *mg_lv;   // the solve's levels in device memory (batched V-cycles), mg_levels of them
+    int mg_levels;
     int it;             // iterations enqueued since the start (selects the p buffer)
 };
 int pcg_blocks(size_t n);
@@ -223,6 +226,21 @@ struct MgLevel {
 struct MgHier {
     int levels = 0;
     MgLevel lv[kMgMaxLevels];
+};
+// the down step of levels >= 1 runs on tiles of 2 x 2 x kMgTileZ cells
+constexpr int kMgTileZ = 64;
+__host__ __device__ inline int mg_down_tiles(const MgLevel &F) {
+    return ((F.nz + kMgTileZ - 1) / kMgTileZ) * ((F.ny + 1) / 2) * ((F.nx + 1) / 2);
+}
+// block maps (solve, block of that solve's launch) of a batched V-cycle: level 1's restriction,
+// and per level l >= 1 the down and up launches of the solves that have level l + 1
+struct MgBatch {
+    const int2 *r0_map;
+    int r0_blocks;
+    const int2 *down_map[kMgMaxLevels];
+    int down_blocks[kMgMaxLevels];
+    const int2 *up_map[kMgMaxLevels];
+    int up_blocks[kMgMaxLevels];
 };
 // dims of every level for a grid (coarsest: at most 256 cells or kMgMaxLevels levels)
 void mg_shape(int nx, int ny, int nz, MgHier &h);
@@ -256,7 +274,8 @@ struct PcgBatch {
 inline size_t pcg_partial_words(const PcgState &st) {
     return 3 * (size_t)st.nblocks + (size_t)st.dir_blocks + 2 * (size_t)st.upd_blocks;
 }
-hipError_t launch_pcg_batch_iterations(PcgBatch &b, int count, hipStream_t s);
+hipError_t launch_pcg_batch_iterations(PcgBatch &b, int count, hipStream_t s, const MgBatch *mb = nullptr);
+hipError_t launch_mg_batch_vcycle(const PcgBatch &b, const MgBatch &mb, hipStream_t s);
 // the convergence test of every solve, then out[s * kPcgScalars ...] = scalars of solve s
 hipError_t launch_pcg_batch_check(const PcgBatch &b, double *out, hipStream_t s);
 hipError_t launch_project(const PcgGrid &g, const double *p, double h, float *u, float *v, float *w,

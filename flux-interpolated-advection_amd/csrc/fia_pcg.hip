@@ -631,13 +631,12 @@ __device__ __forceinline__ float mg_b(const PcgState &st, const MgLevel &L, int 
 // l + 1 along z): every unknown cell gets the first smoothing sweep from zero, x = w b / d, and
 // its residual b - L x (the neighbours' x recomputed from their b); each aggregate's right-hand
 // side is the sum of its eight children's residuals, added in a fixed order through LDS.
-constexpr int kMgTileZ = 64;
-__global__ void __launch_bounds__(kPB) mg_down_kernel(PcgState st, MgLevel F, MgLevel C, int l) {
+__device__ __forceinline__ void mg_down_body(const PcgState &st, const MgLevel &F, const MgLevel &C, int l, int blk) {
     if (solve_done(st.scal)) return;
     __shared__ float res[kPB];
     const int tz = threadIdx.x % kMgTileZ, ty = (threadIdx.x / kMgTileZ) & 1, tx = threadIdx.x / (2 * kMgTileZ);
     const int tiles_z = (F.nz + kMgTileZ - 1) / kMgTileZ, tiles_y = (F.ny + 1) / 2;
-    const int bz = blockIdx.x % tiles_z, by = (blockIdx.x / tiles_z) % tiles_y, bx = blockIdx.x / (tiles_z * tiles_y);
+    const int bz = blk % tiles_z, by = (blk / tiles_z) % tiles_y, bx = blk / (tiles_z * tiles_y);
     const int i = 2 * bx + tx, j = 2 * by + ty, k = bz * kMgTileZ + tz;
     float rc = 0.0f;
     if (i < F.nx && j < F.ny && k < F.nz) {
@@ -681,10 +680,13 @@ __global__ void __launch_bounds__(kPB) mg_down_kernel(PcgState st, MgLevel F, Mg
         }
     }
 }
+__global__ void __launch_bounds__(kPB) mg_down_kernel(PcgState st, MgLevel F, MgLevel C, int l) {
+    mg_down_body(st, F, C, l, blockIdx.x);
+}
 
 // The coarsest level (at most kMgCoarsestCells cells, one per thread): kMgCoarseSweeps damped
 // Jacobi sweeps from zero, in LDS.
-__global__ void __launch_bounds__(kPB) mg_coarsest_kernel(PcgState st, MgLevel L) {
+__device__ __forceinline__ void mg_coarsest_body(const PcgState &st, const MgLevel &L) {
     if (solve_done(st.scal)) return;
     __shared__ float xs[2][kMgCoarsestCells];
     const int n = L.nx * L.ny * L.nz;
@@ -719,6 +721,7 @@ __global__ void __launch_bounds__(kPB) mg_coarsest_kernel(PcgState st, MgLevel L
     }
     if (c < n) L.x[c] = xs[cur][c];
 }
+__global__ void __launch_bounds__(kPB) mg_coarsest_kernel(PcgState st, MgLevel L) { mg_coarsest_body(st, L); }
 
 // Up the V-cycle at level l: x' = x + the aggregate's correction, then one smoothing sweep
 // x'' = x' + w (b - L x') / d (the neighbours' x' recomputed the same way).
@@ -742,12 +745,16 @@ __device__ __forceinline__ float mg_up_cell(const PcgState &st, const MgLevel &F
     return xc + kMgOmega * (bc - lx) / a.d;
 }
 // levels >= 1: one thread per cell, writes xs
-__global__ void __launch_bounds__(kPB) mg_up_kernel(PcgState st, MgLevel F, const float *corr, int cny, int cnz,
-                                                    int l) {
+__device__ __forceinline__ void mg_up_body(const PcgState &st, const MgLevel &F, const float *corr, int cny, int cnz,
+                                           int l, int blk) {
     if (solve_done(st.scal)) return;
     const size_t n = (size_t)F.nx * F.ny * F.nz;
-    const size_t c = (size_t)blockIdx.x * kPB + threadIdx.x;
+    const size_t c = (size_t)blk * kPB + threadIdx.x;
     if (c < n) F.xs[c] = mg_up_cell(st, F, corr, cny, cnz, l, c, F.b[c]);
+}
+__global__ void __launch_bounds__(kPB) mg_up_kernel(PcgState st, MgLevel F, const float *corr, int cny, int cnz,
+                                                    int l) {
+    mg_up_body(st, F, corr, cny, cnz, l, blockIdx.x);
 }
 // Level 0 of the V-cycle on the solve's tiles (tile_sweep: the 2.5-D sweep of the CG kernels).
 // Down: v = x = w r / d (the first smoothing sweep from zero), stored, and the residual r - L x
@@ -809,16 +816,19 @@ struct MgUp0Op {
         acc[0] += a.r * zc;
     }
 };
-__global__ void __launch_bounds__(kPB) mg_down0_kernel(PcgState st, float *x, float *res) {
+__device__ __forceinline__ void mg_down0_body(const PcgState &st, float *x, float *res, int blk) {
     if (solve_done(st.scal)) return;
     double acc[1] = {0.0};
-    tile_sweep(st, blockIdx.x, MgDown0Op{st, x, res}, acc);
+    tile_sweep(st, blk, MgDown0Op{st, x, res}, acc);
+}
+__global__ void __launch_bounds__(kPB) mg_down0_kernel(PcgState st, float *x, float *res) {
+    mg_down0_body(st, x, res, blockIdx.x);
 }
 // b of level 1: the sum of each aggregate's children's residuals (fixed order)
-__global__ void __launch_bounds__(kPB) mg_restrict0_kernel(PcgState st, MgLevel F, MgLevel C) {
+__device__ __forceinline__ void mg_restrict0_body(const PcgState &st, const MgLevel &F, const MgLevel &C, int blk) {
     if (solve_done(st.scal)) return;
     const size_t n = (size_t)C.nx * C.ny * C.nz;
-    const size_t I = (size_t)blockIdx.x * kPB + threadIdx.x;
+    const size_t I = (size_t)blk * kPB + threadIdx.x;
     if (I >= n) return;
     const int K = (int)(I % C.nz), J = (int)((I / C.nz) % C.ny), II = (int)(I / ((size_t)C.ny * C.nz));
     float sum = 0.0f;
@@ -830,12 +840,59 @@ __global__ void __launch_bounds__(kPB) mg_restrict0_kernel(PcgState st, MgLevel 
             }
     C.b[I] = C.op[I].x > 0.0f ? sum : 0.0f;
 }
-__global__ void __launch_bounds__(kPB) mg_up0_tile_kernel(PcgState st, const float *x, const float *corr, int cny,
-                                                          int cnz) {
+__global__ void __launch_bounds__(kPB) mg_restrict0_kernel(PcgState st, MgLevel F, MgLevel C) {
+    mg_restrict0_body(st, F, C, blockIdx.x);
+}
+__device__ __forceinline__ void mg_up0_body(const PcgState &st, const float *x, const float *corr, int cny, int cnz,
+                                            int blk) {
     if (solve_done(st.scal)) return;
     double acc[1] = {0.0};
-    tile_sweep(st, blockIdx.x, MgUp0Op{st, x, corr, cny, cnz}, acc);
-    write_partials<1>(acc, st.part_mg, st.dir_blocks, blockIdx.x);
+    tile_sweep(st, blk, MgUp0Op{st, x, corr, cny, cnz}, acc);
+    write_partials<1>(acc, st.part_mg, st.dir_blocks, blk);
+}
+__global__ void __launch_bounds__(kPB) mg_up0_tile_kernel(PcgState st, const float *x, const float *corr, int cny,
+                                                          int cnz) {
+    mg_up0_body(st, x, corr, cny, cnz, blockIdx.x);
+}
+
+// ... and the V-cycles of many solves per launch (fia_scenes_step_batched): map[b] = (solve s,
+// block of that solve's launch), each solve's levels in st.mg_lv (st.mg_levels of them); a level
+// launch covers the solves that have that level, the coarsest launch is one block per solve.
+__global__ void __launch_bounds__(kPB) mg_down0_batch_kernel(const PcgState *__restrict__ sts,
+                                                             const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    const PcgState &st = sts[m.x];
+    mg_down0_body(st, st.mg_lv[0].x, st.mg_lv[0].b, m.y);
+}
+__global__ void __launch_bounds__(kPB) mg_restrict0_batch_kernel(const PcgState *__restrict__ sts,
+                                                                 const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    const PcgState &st = sts[m.x];
+    mg_restrict0_body(st, st.mg_lv[0], st.mg_lv[1], m.y);
+}
+__global__ void __launch_bounds__(kPB) mg_down_batch_kernel(const PcgState *__restrict__ sts,
+                                                            const int2 *__restrict__ map, int l) {
+    const int2 m = map[blockIdx.x];
+    const PcgState &st = sts[m.x];
+    mg_down_body(st, st.mg_lv[l], st.mg_lv[l + 1], l, m.y);
+}
+__global__ void __launch_bounds__(kPB) mg_coarsest_batch_kernel(const PcgState *__restrict__ sts) {
+    const PcgState &st = sts[blockIdx.x];
+    mg_coarsest_body(st, st.mg_lv[st.mg_levels - 1]);
+}
+__global__ void __launch_bounds__(kPB) mg_up_batch_kernel(const PcgState *__restrict__ sts,
+                                                          const int2 *__restrict__ map, int l) {
+    const int2 m = map[blockIdx.x];
+    const PcgState &st = sts[m.x];
+    const MgLevel &C = st.mg_lv[l + 1];
+    mg_up_body(st, st.mg_lv[l], l + 1 == st.mg_levels - 1 ? C.x : C.xs, C.ny, C.nz, l, m.y);
+}
+__global__ void __launch_bounds__(kPB) mg_up0_batch_kernel(const PcgState *__restrict__ sts,
+                                                           const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    const PcgState &st = sts[m.x];
+    const MgLevel &C = st.mg_lv[1];
+    mg_up0_body(st, st.mg_lv[0].x, st.mg_levels == 2 ? C.x : C.xs, C.ny, C.nz, m.y);
 }
 
 // One solve per launch ...
@@ -942,7 +999,7 @@ hipError_t launch_pcg_check(const PcgState &st, hipStream_t s) {
     hipLaunchKernelGGL(pcg_check_kernel, dim3(1), dim3(kPB), 0, s, st, st.it);
     return hipGetLastError();
 }
-hipError_t launch_pcg_batch_iterations(PcgBatch &b, int count, hipStream_t s) {
+hipError_t launch_pcg_batch_iterations(PcgBatch &b, int count, hipStream_t s, const MgBatch *mb) {
     if (b.n == 0) return hipSuccess;
     for (int k = 0; k < count; ++k) {
         hipLaunchKernelGGL(pcg_direction_batch_kernel, dim3(b.dir_blocks), dim3(kPB), 0, s, b.states, b.dir_map, b.it);
@@ -952,6 +1009,10 @@ hipError_t launch_pcg_batch_iterations(PcgBatch &b, int count, hipStream_t s) {
         if (b.tile_blocks)
             hipLaunchKernelGGL(pcg_update_batch_kernel<true>, dim3(b.tile_blocks), dim3(kPB), 0, s, b.states, b.tile_map,
                                b.it);
+        if (mb) {
+            const hipError_t e = launch_mg_batch_vcycle(b, *mb, s);
+            if (e != hipSuccess) return e;
+        }
         ++b.it;
     }
     return hipGetLastError();
@@ -988,8 +1049,7 @@ hipError_t launch_mg_vcycle(const PcgState &st, const MgHier &h, hipStream_t s) 
     }
     for (int l = 1; l + 1 < L; ++l) {
         const MgLevel &F = h.lv[l];
-        const unsigned tiles = (unsigned)(((F.nz + kMgTileZ - 1) / kMgTileZ) * ((F.ny + 1) / 2) * ((F.nx + 1) / 2));
-        hipLaunchKernelGGL(mg_down_kernel, dim3(tiles), dim3(kPB), 0, s, st, F, h.lv[l + 1], l);
+        hipLaunchKernelGGL(mg_down_kernel, dim3((unsigned)mg_down_tiles(F)), dim3(kPB), 0, s, st, F, h.lv[l + 1], l);
     }
     hipLaunchKernelGGL(mg_coarsest_kernel, dim3(1), dim3(kPB), 0, s, st, h.lv[L - 1]);
     for (int l = L - 2; l >= 0; --l) {
@@ -1003,6 +1063,20 @@ hipError_t launch_mg_vcycle(const PcgState &st, const MgHier &h, hipStream_t s) 
             hipLaunchKernelGGL(mg_up_kernel, dim3(blocks(n)), dim3(kPB), 0, s, st, h.lv[l], corr, C.ny, C.nz, l);
         }
     }
+    return hipGetLastError();
+}
+hipError_t launch_mg_batch_vcycle(const PcgBatch &b, const MgBatch &mb, hipStream_t s) {
+    if (b.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(mg_down0_batch_kernel, dim3(b.dir_blocks), dim3(kPB), 0, s, b.states, b.dir_map);
+    hipLaunchKernelGGL(mg_restrict0_batch_kernel, dim3(mb.r0_blocks), dim3(kPB), 0, s, b.states, mb.r0_map);
+    for (int l = 1; l < kMgMaxLevels; ++l)
+        if (mb.down_blocks[l])
+            hipLaunchKernelGGL(mg_down_batch_kernel, dim3(mb.down_blocks[l]), dim3(kPB), 0, s, b.states, mb.down_map[l], l);
+    hipLaunchKernelGGL(mg_coarsest_batch_kernel, dim3(b.n), dim3(kPB), 0, s, b.states);
+    for (int l = kMgMaxLevels - 1; l >= 1; --l)
+        if (mb.up_blocks[l])
+            hipLaunchKernelGGL(mg_up_batch_kernel, dim3(mb.up_blocks[l]), dim3(kPB), 0, s, b.states, mb.up_map[l], l);
+    hipLaunchKernelGGL(mg_up0_batch_kernel, dim3(b.dir_blocks), dim3(kPB), 0, s, b.states, b.dir_map);
     return hipGetLastError();
 }
 hipError_t launch_project(const PcgGrid &g, const double *p, double h, float *u, float *v, float *w,

@@ -230,7 +230,7 @@ typedef struct fia_kernel_stats {
 fia_status fia_ctx_set_timing(fia_ctx *ctx, int enable);
 /* The preconditioner of every later pressure solve on this context (fia_pressure_project,
  * fia_pressure_project_learned, fia_scenes_step).  No counterpart in the reference, whose solver
- * is fixed; fia_scenes_step_batched requires FIA_PRECOND_JACOBI on ctxs[0]. */
+ * is fixed; fia_scenes_step_batched takes it from ctxs[0]. */
 fia_status fia_ctx_set_pressure_preconditioner(fia_ctx *ctx, fia_precond kind);
 fia_status fia_ctx_get_stats(fia_ctx *ctx, fia_kernel_stats *out);   /* synchronises */
 fia_status fia_ctx_reset_stats(fia_ctx *ctx);

@@ -278,20 +278,23 @@ def test_scenes_step_is_independent_of_workers(fia, ctx):
             c.close()
 
 
+@pytest.mark.parametrize("precond", ["jacobi", "mg"])
 @pytest.mark.parametrize("learned", [False, True])
-def test_batched_scene_step_matches_per_scene_solves(fia, ctx, learned):
+def test_batched_scene_step_matches_per_scene_solves(fia, ctx, learned, precond):
     # the scenes' pressure solves batched into one launch per kernel and iteration give the same
     # bits as every scene solved on its own: each solve keeps its block decomposition and
     # reduction order, and the learned start (network guess, probes, choice) is the same
     import importlib
     import os
     batch = importlib.import_module("flux-interpolated-advection_amd.batch")
-    print("case scenes batched vs per-scene solves, learned=%s" % learned, flush=True)
+    print("case scenes batched vs per-scene solves, learned=%s, %s" % (learned, precond), flush=True)
     od = fia.make_opts(split_method="xz-CellFaceAligned")
     ov = fia.make_opts(interpolation_method="WENO6")
     ctxs = [ctx] + [fia.Context(0) for _ in range(2)]
     nets = None
     try:
+        for c in ctxs:
+            c.set_pressure_preconditioner(precond)
         if learned:
             weights = os.path.join(os.path.dirname(batch.__file__), "weights", "unet_plume128.npy")
             nets = [fia.UNet(c, seed=0) for c in ctxs]
@@ -311,6 +314,7 @@ def test_batched_scene_step_matches_per_scene_solves(fia, ctx, learned):
         if learned:
             print("network used in %d of 5 scenes" % sum(s.used_network for s in sets[1].array), flush=True)
     finally:
+        ctx.set_pressure_preconditioner("jacobi")   # the session's context
         for u in nets or []:
             u.close()
         for c in ctxs[1:]:
