@@ -54,10 +54,15 @@ inline void check(fia_status s) {
     if (s != FIA_OK) throw std::runtime_error(std::string("libfia: ") + fia_last_error());
 }
 
-// One library context per process (device scratch is kept between calls of the same size).
+// One library context per process (device scratch is kept between calls of the same size).  Its
+// pressure solves use the multigrid preconditioner, the counterpart of the reference's AMG
+// (amgcl smoothed aggregation, src/calc_pressure_3d.cpp:141-166).
 inline fia_ctx *context() {
     static fia_ctx *ctx = nullptr;
-    if (!ctx) check(fia_ctx_create(-1, &ctx));
+    if (!ctx) {
+        check(fia_ctx_create(-1, &ctx));
+        check(fia_ctx_set_pressure_preconditioner(ctx, FIA_PRECOND_MG));
+    }
     return ctx;
 }
 
@@ -154,7 +159,8 @@ inline void advect_velocity_xyz_flux_advection_3D_std_thread(
 }
 
 // reference include/calc_pressure_3d.h:9: projection with the reference's Poisson system, solved by
-// Jacobi-PCG to a relative residual of 1e-6 (the reference's amgcl runs with its own defaults)
+// multigrid-preconditioned CG to a relative residual of 1e-6 (the reference's amgcl runs with its
+// own defaults)
 inline void calc_pressure_gradient_term_3D(Grid_3D &all_grid) {
     fia_grid3d g = fia_shim::host_grid(all_grid);
     int iters = 0;
