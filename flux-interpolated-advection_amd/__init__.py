@@ -200,7 +200,7 @@ class Context:
         return lib().fia_ctx_stream(self._h)
 
     def set_pressure_preconditioner(self, kind):
-        """"jacobi" (default) or "mg" (geometric multigrid V-cycle) for later pressure solves."""
+        """"mg" (geometric multigrid V-cycle, the default) or "jacobi" for later pressure solves."""
         _check(lib().fia_ctx_set_pressure_preconditioner(self._h, PRECOND[kind]))
 
     def close(self):
@@ -398,7 +398,8 @@ def calc_time_step_length_from_cfl_number_3d(ctx, grid, now_time, i_movie_frame,
 
 
 def calc_pressure_gradient_term_3d(ctx, grid, solid=None, rel_tol=1e-6, max_iters=20000, warm_start=None):
-    """Projection: Jacobi-PCG (FP64) on the reference's Poisson system, then u -= grad p / dx.
+    """Projection: PCG (FP64; the context's preconditioner, multigrid by default) on the reference's
+    Poisson system, then u -= grad p / dx.
     Returns (iterations, relative residual); grid.pressure receives p."""
     g = grid._c(with_density=False)
     n = grid.nx * grid.ny * grid.nz

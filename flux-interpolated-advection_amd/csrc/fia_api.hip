@@ -110,7 +110,7 @@ struct fia_ctx {
         }
     } pcgs[2];
     // multigrid preconditioner: the levels' operators and vectors, and the hierarchy's shape
-    int precond = FIA_PRECOND_JACOBI;
+    int precond = FIA_PRECOND_MG;
     struct MgBufs {
         DevBuf<float4> op[kMgMaxLevels];
         DevBuf<float> b[kMgMaxLevels], x[kMgMaxLevels], xs[kMgMaxLevels];

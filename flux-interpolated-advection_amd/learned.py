@@ -5,8 +5,8 @@ libfia.so, fia_unet_*).  It has no counterpart in the reference, whose pressure 
 Data: buoyant plume scenes simulated by this library on the GPU.  Each scene has a seeded source
 sphere (radius 0.06, centre in the lower part of the box), buoyancy in [0.5, 2] on the vertical
 (y) velocity and 0-8 spherical obstacles kept clear of the source.  A step adds the source and
-the buoyancy, self-advects the velocity (flux-interpolated, WENO6), projects with Jacobi-PCG to
-1e-6 and advects the density.  Every step after a short warm-up gives one sample: the right-hand
+the buoyancy, self-advects the velocity (flux-interpolated, WENO6), projects with PCG to 1e-6
+(Jacobi-preconditioned by default: `--precond`) and advects the density.  Every step after a short warm-up gives one sample: the right-hand
 side of the pressure system just before the projection, and the obstacle mask.
 
     python flux-interpolated-advection_amd/learned.py train --n 128 --scenes 12 --steps 40 \\
@@ -186,6 +186,8 @@ def summary(rows):
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--precond", choices=("jacobi", "mg"), default="jacobi",
+                    help="preconditioner of the scene and evaluation projections")
     sub = ap.add_subparsers(dest="cmd", required=True)
     t = sub.add_parser("train")
     t.add_argument("--n", type=int, default=128)
@@ -208,6 +210,7 @@ def main(argv=None):
     log = lambda s: print(s, flush=True)
     fia, scenes = _pkg()
     ctx = fia.Context(0)
+    ctx.set_pressure_preconditioner(a.precond)
     net = fia.UNet(ctx, seed=1)
     report = {"n": a.n}
     if a.cmd == "train":

@@ -69,9 +69,9 @@ typedef enum fia_psi_mode {
 /* Preconditioner of the pressure solve's conjugate gradients (the reference uses amgcl: CG with a
  * smoothed-aggregation AMG preconditioner, src/calc_pressure_3d.cpp:141-166). */
 typedef enum fia_precond {
-    FIA_PRECOND_JACOBI = 0,   /* diagonal (default)                                              */
-    FIA_PRECOND_MG = 1        /* geometric multigrid: 2x2x2 aggregation, Galerkin coarse operators, */
-                              /* one V(1,1) cycle with damped Jacobi per CG iteration             */
+    FIA_PRECOND_JACOBI = 0,   /* diagonal                                                        */
+    FIA_PRECOND_MG = 1        /* geometric multigrid (default): 2x2x2 aggregation, Galerkin coarse */
+                              /* operators, one V(1,1) cycle with damped Jacobi per CG iteration  */
 } fia_precond;
 
 /* Options of move_substances_3D / advect_velocity_*_flux_advection_3D_std_thread, one field per
@@ -199,7 +199,8 @@ fia_status fia_cfl_time_step(fia_ctx *ctx, const fia_grid3d *grid, double now_ti
 
 /* Pressure projection: replaces calc_pressure_gradient_term_3D
  * (reference src/calc_pressure_3d.cpp:18-269).  Solves the reference's 7-point Neumann system
- * (last cell pinned to p = 0) with Jacobi-preconditioned CG, FP64 dots and residuals, to
+ * (last cell pinned to p = 0) with preconditioned CG (the context's preconditioner, geometric
+ * multigrid unless fia_ctx_set_pressure_preconditioner chose Jacobi), FP64 dots and residuals, to
  * ||b - A p|| <= rel_tol * ||b||, then u -= grad p / dx on interior faces.  solid (optional,
  * nx*ny*nz bytes, nonzero = obstacle cell) turns faces next to obstacles into Neumann walls and
  * zeroes the velocity on them.  pressure (optional, nx*ny*nz floats) receives p and, when
@@ -229,8 +230,9 @@ typedef struct fia_kernel_stats {
 
 fia_status fia_ctx_set_timing(fia_ctx *ctx, int enable);
 /* The preconditioner of every later pressure solve on this context (fia_pressure_project,
- * fia_pressure_project_learned, fia_scenes_step).  No counterpart in the reference, whose solver
- * is fixed; fia_scenes_step_batched takes it from ctxs[0]. */
+ * fia_pressure_project_learned, fia_scenes_step); a new context uses FIA_PRECOND_MG.  No
+ * counterpart in the reference, whose solver is fixed; fia_scenes_step_batched takes it from
+ * ctxs[0]. */
 fia_status fia_ctx_set_pressure_preconditioner(fia_ctx *ctx, fia_precond kind);
 fia_status fia_ctx_get_stats(fia_ctx *ctx, fia_kernel_stats *out);   /* synchronises */
 fia_status fia_ctx_reset_stats(fia_ctx *ctx);
@@ -239,7 +241,8 @@ fia_status fia_ctx_reset_stats(fia_ctx *ctx);
  * No counterpart in the reference, whose solver is amgcl (src/calc_pressure_3d.cpp:141-166): a
  * 4-level 3-D U-Net (channels 16/32/48/64, 3x3x3 convolutions, BF16 weights and activations on
  * the matrix cores, FP32 accumulation) maps the right-hand side of the pressure system and the
- * obstacle mask to an initial guess for the same Jacobi-PCG.  Grid sizes must be multiples of 8.
+ * obstacle mask to an initial guess for the same PCG (trained against the Jacobi-preconditioned
+ * solve; it also runs under multigrid).  Grid sizes must be multiples of 8.
  * Parameters are FP32, fia_unet_param_count() of them (layout in fia_unet.hpp). */
 typedef struct fia_unet fia_unet;
 size_t fia_unet_param_count(void);

@@ -1,6 +1,7 @@
 """The whole simulation loop on the GPU against the oracle's FP64 run (tests/golden/c1_run.npz,
 written by tests/golden/make_c1_run.py): 32^3 plain method, 10 steps of CFL-1 time step, buoyancy,
-velocity self-advection (WENO6), Jacobi-PCG projection to 1e-8 and density advection (WENO6,
+velocity self-advection (WENO6), projection to 1e-8 (the library's default multigrid-preconditioned
+CG; the oracle solves the same system with its own CG) and density advection (WENO6,
 xz-CellFaceAligned).  The GPU computes the interpolation and quadrature in FP32 and carries the
 FP32 fields from step to step, so its difference from the FP64 run grows with the step count.
 

@@ -149,7 +149,12 @@ def test_pressure_projection_large_grid(fia, ctx, scenes):
     w = rng.normal(size=(n, n, n + 1)).astype(np.float32); w[:, :, 0] = w[:, :, -1] = 0
     d0 = scenes.divergence(u.astype(np.float64), v.astype(np.float64), w.astype(np.float64), h) * h
     grid = fia.Grid3D(n, n, n, h, None, u, v, w)
-    it, res = fia.calc_pressure_gradient_term_3d(ctx, grid, rel_tol=1e-6)
+    cj = fia.Context(0)
+    try:
+        cj.set_pressure_preconditioner("jacobi")
+        it, res = fia.calc_pressure_gradient_term_3d(cj, grid, rel_tol=1e-6)
+    finally:
+        cj.close()
     print("iterations %d, relative residual %.3e" % (it, res), flush=True)
     assert res <= 1e-6 and it > 100
     d1 = scenes.divergence(grid.u.astype(np.float64), grid.v.astype(np.float64), grid.w.astype(np.float64), h) * h
@@ -172,15 +177,17 @@ def test_multigrid_projection_matches_jacobi(fia, ctx, scenes, n, with_solid):
     w = rng.normal(size=(n, n, n + 1)).astype(np.float32); w[:, :, 0] = w[:, :, -1] = 0
     solid = scenes.spherical_obstacles(n, count=3, seed=2) if with_solid else None
     out = {}
-    c2 = fia.Context(0)
+    c1, c2 = fia.Context(0), fia.Context(0)
     try:
+        c1.set_pressure_preconditioner("jacobi")
         c2.set_pressure_preconditioner("mg")
-        for name, c in (("jacobi", ctx), ("mg", c2)):
+        for name, c in (("jacobi", c1), ("mg", c2)):
             grid = fia.Grid3D(n, n, n, h, None, u.copy(), v.copy(), w.copy())
             it, res = fia.calc_pressure_gradient_term_3d(c, grid, solid=solid, rel_tol=1e-9)
             assert res <= 1e-9, name
             out[name] = (grid, it)
     finally:
+        c1.close()
         c2.close()
     (gj, itj), (gm, itm) = out["jacobi"], out["mg"]
     print("iterations jacobi %d, multigrid %d" % (itj, itm), flush=True)
@@ -201,15 +208,17 @@ def test_multigrid_projection_shapes(fia, ctx, scenes, shape):
     v = rng.normal(size=(nx, ny + 1, nz)).astype(np.float32); v[:, 0] = v[:, -1] = 0
     w = rng.normal(size=(nx, ny, nz + 1)).astype(np.float32); w[:, :, 0] = w[:, :, -1] = 0
     res = {}
-    c2 = fia.Context(0)
+    c1, c2 = fia.Context(0), fia.Context(0)
     try:
+        c1.set_pressure_preconditioner("jacobi")
         c2.set_pressure_preconditioner("mg")
-        for name, c in (("jacobi", ctx), ("mg", c2)):
+        for name, c in (("jacobi", c1), ("mg", c2)):
             grid = fia.Grid3D(nx, ny, nz, h, None, u.copy(), v.copy(), w.copy())
             it, rr = fia.calc_pressure_gradient_term_3d(c, grid, rel_tol=1e-9, max_iters=20000)
             assert rr <= 1e-9, name
             res[name] = (grid, it)
     finally:
+        c1.close()
         c2.close()
     (gj, itj), (gm, itm) = res["jacobi"], res["mg"]
     print("iterations jacobi %d, multigrid %d" % (itj, itm), flush=True)
@@ -314,7 +323,7 @@ def test_batched_scene_step_matches_per_scene_solves(fia, ctx, learned, precond)
         if learned:
             print("network used in %d of 5 scenes" % sum(s.used_network for s in sets[1].array), flush=True)
     finally:
-        ctx.set_pressure_preconditioner("jacobi")   # the session's context
+        ctx.set_pressure_preconditioner("mg")   # the session's context: back to the default
         for u in nets or []:
             u.close()
         for c in ctxs[1:]:
