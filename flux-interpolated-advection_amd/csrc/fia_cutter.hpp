@@ -169,15 +169,14 @@ FIA_HD void interior_integers(double lo, double hi, int &first, int &last) {
 // ---------------------------------------------------------------------------------------------
 constexpr double kNoBound = 1e300;
 
-// [lo, hi] &= {u in [0,1]: L <= c0 + u (c1 - c0) <= H}; a constant colour keeps everything or
-// nothing, decided with the half-open rule L <= c < H.
-FIA_HD void clip_band(double c0, double c1, double L, double H, double &lo, double &hi) {
-    const double d = c1 - c0;
+// [lo, hi] &= {u in [0,1]: L <= c0 + u d <= H} for the colour c0 + u d along a segment, with
+// r = 1 / d computed once per segment by the caller (not read when d == 0); a constant colour
+// keeps everything or nothing, decided with the half-open rule L <= c < H.
+FIA_HD void clip_band(double c0, double d, double r, double L, double H, double &lo, double &hi) {
     if (d == 0.0) {
         if (!(c0 >= L && c0 < H)) { lo = 1.0; hi = 0.0; }
         return;
     }
-    const double r = 1.0 / d;
     const double uL = (L - c0) * r, uH = (H - c0) * r;
     lo = fmax(lo, d > 0.0 ? uL : uH);
     hi = fmin(hi, d > 0.0 ? uH : uL);
@@ -214,16 +213,19 @@ struct Chord {
     }
 };
 
-// z-chord crossings of the strip boundary segment a -> b (colour z linear along it)
-FIA_HD void see_z(Chord &lo, Chord &hi, double zl, double zh, double xa, double ea, double za, double xb, double eb,
-                  double zb) {
-    const bool la = za >= zl, lb = zb >= zl, ha = za < zh, hb = zb < zh;
+// z-chord crossings of a part of the strip boundary segment a -> b (colour z linear along it,
+// r = 1 / (zb - za)): membership is decided at the part's end colours z0, z1, and a crossing is
+// placed on the whole segment at u = (level - za) r.
+FIA_HD void see_z(Chord &lo, Chord &hi, double zl, double zh, double z0, double z1, double xa, double ea, double za,
+                  double xb, double eb, double zb, double r) {
+    const bool la = z0 >= zl, lb = z1 >= zl, ha = z0 < zh, hb = z1 < zh;
     if (la == lb && ha == hb) return;
-    const double r = 1.0 / (zb - za);
     lo.see(la, lb, (zl - za) * r, xa, ea, za, xb, eb, zb);
     hi.see(ha, hb, (zh - za) * r, xa, ea, za, xb, eb, zb);
 }
 
+// Each segment's colour differences are inverted once and shared by its crossings and clips:
+// one divide per colour of each square edge and one per x chord.
 template <bool ALL>
 FIA_HD void piece_moments(const double col[4][2], const FaceGeom &g, double xl, double xh, double zl, double zh,
                           SegAcc<ALL> &acc) {
@@ -234,24 +236,23 @@ FIA_HD void piece_moments(const double col[4][2], const FaceGeom &g, double xl, 
         const int j = (i + 1) & 3;
         const double xa = sx[i], ea = se[i], xb = sx[j], eb = se[j];
         const double ca = col[i][0], cb = col[j][0], za = col[i][1], zb = col[j][1];
+        const double dc = cb - ca, dz = zb - za;
+        const double rc = 1.0 / dc, rz = 1.0 / dz;   // inf on a constant colour, then unused
         // crossings of the x cut lines with this edge (the x chords' ends)
         {
             const bool la = ca >= xl, lb = cb >= xl, ha = ca < xh, hb = cb < xh;
             if (la != lb || ha != hb) {
-                const double r = 1.0 / (cb - ca);
-                cxl.see(la, lb, (xl - ca) * r, xa, ea, za, xb, eb, zb);
-                cxh.see(ha, hb, (xh - ca) * r, xa, ea, za, xb, eb, zb);
+                cxl.see(la, lb, (xl - ca) * rc, xa, ea, za, xb, eb, zb);
+                cxh.see(ha, hb, (xh - ca) * rc, xa, ea, za, xb, eb, zb);
             }
         }
         // the edge's part inside the strip: z chord crossings, then its part inside the band
         double lo = 0.0, hi = 1.0;
-        clip_band(ca, cb, xl, xh, lo, hi);
+        clip_band(ca, dc, rc, xl, xh, lo, hi);
         if (lo < hi) {
-            const double x0 = xa + lo * (xb - xa), e0 = ea + lo * (eb - ea), z0 = za + lo * (zb - za);
-            const double x1 = xa + hi * (xb - xa), e1 = ea + hi * (eb - ea), z1 = za + hi * (zb - za);
-            see_z(czl, czh, zl, zh, x0, e0, z0, x1, e1, z1);
+            see_z(czl, czh, zl, zh, za + lo * dz, za + hi * dz, xa, ea, za, xb, eb, zb, rz);
             double lo2 = lo, hi2 = hi;
-            clip_band(za, zb, zl, zh, lo2, hi2);
+            clip_band(za, dz, rz, zl, zh, lo2, hi2);
             if (lo2 < hi2)
                 acc.add(g, xa + lo2 * (xb - xa), ea + lo2 * (eb - ea), xa + hi2 * (xb - xa), ea + hi2 * (eb - ea));
         }
@@ -261,9 +262,10 @@ FIA_HD void piece_moments(const double col[4][2], const FaceGeom &g, double xl, 
     for (int k = 0; k < 2; ++k) {
         const Chord &c = k == 0 ? cxl : cxh;
         if (c.has_exit && c.has_entry) {
-            see_z(czl, czh, zl, zh, c.xe, c.ee, c.ze, c.xn, c.en, c.zn);
+            const double dz = c.zn - c.ze, rz = 1.0 / dz;
+            see_z(czl, czh, zl, zh, c.ze, c.zn, c.xe, c.ee, c.ze, c.xn, c.en, c.zn, rz);
             double lo = 0.0, hi = 1.0;
-            clip_band(c.ze, c.zn, zl, zh, lo, hi);
+            clip_band(c.ze, dz, rz, zl, zh, lo, hi);
             if (lo < hi)
                 acc.add(g, c.xe + lo * (c.xn - c.xe), c.ee + lo * (c.en - c.ee), c.xe + hi * (c.xn - c.xe),
                         c.ee + hi * (c.en - c.ee));
