@@ -90,7 +90,7 @@ struct fia_ctx {
     DevBuf<float> work[3];
     DevBuf<unsigned> twisted;
     DevBuf<unsigned> degen;     // degenerate faces of the segment-cutter mode
-    DevBuf<int> counters;   // [0] twisted count, [1] error
+    DevBuf<int> counters;   // [0] twisted count, [1] error, [2] degenerate count
     DevBuf<float> grid[4];  // device mirrors of host grids: rho, u, v, w
     DevBuf<float> gen[3];   // scratch for test entry points
     DevBuf<double> genD[2];
@@ -357,16 +357,22 @@ fia_status do_faces(fia_ctx *c, const CompGrid &g, const float *q, bool moving, 
     const unsigned long long nslots = fam_slots[3];
     if (!(fam_slots[1] <= fam_slots[2] && fam_slots[2] <= fam_slots[3]))
         return fail(FIA_ERR_DEVICE, "inconsistent piece slot bounds");
-    HIP_TRY(c->slot_face.ensure(nslots));
-    p.slot_face = c->slot_face.p;
+    if (!p.seg_cut) {
+        HIP_TRY(c->slot_face.ensure(nslots));
+        p.slot_face = c->slot_face.p;
+    }
     for (int a = 0; a < 3; ++a) HIP_TRY(c->piece_c[a].ensure(nslots));
     HIP_TRY(c->piece_ay.ensure(nslots));
     HIP_TRY(c->piece_val.ensure(nslots));
     for (int a = 0; a < 3; ++a) p.piece_c[a] = c->piece_c[a].p;
     p.piece_ay = c->piece_ay.p;
     p.piece_val = c->piece_val.p;
-    HIP_TRY(launch_face_cut(p, fam_slots, c->stream));
-    if (p.seg_cut) HIP_TRY(launch_faces_degenerate(p, n_degen, c->stream));
+    if (p.seg_cut) {
+        HIP_TRY(launch_seg_faces(p, c->stream));
+        HIP_TRY(launch_faces_degenerate(p, n_degen, c->stream));
+    } else {
+        HIP_TRY(launch_face_cut(p, fam_slots, c->stream));
+    }
     // the WENO6 integral quadrature launches are timed on their own (the dominant kernel)
     const bool weno_quad = piece_quadrature_mode(p) != 0;
     int sq = -1;

@@ -211,6 +211,18 @@ struct Chord {
         if (ia) { has_exit = true; xe = x; ee = e; ze = z; }
         else { has_entry = true; xn = x; en = e; zn = z; }
     }
+    // the same update written with selects, for chords kept in arrays: a branch per field end
+    // lets the compiler merge the two stores into one through a selected address, which puts
+    // the array in scratch memory
+    FIA_HD void see_sel(bool ia, bool ib, double t, double xa, double ea, double za, double xb, double eb,
+                        double zb) {
+        const bool ex = ia && !ib, nn = ib && !ia;
+        const double x = xa + t * (xb - xa), e = ea + t * (eb - ea), z = za + t * (zb - za);
+        xe = ex ? x : xe; ee = ex ? e : ee; ze = ex ? z : ze;
+        xn = nn ? x : xn; en = nn ? e : en; zn = nn ? z : zn;
+        has_exit = has_exit || ex;
+        has_entry = has_entry || nn;
+    }
 };
 
 // z-chord crossings of a part of the strip boundary segment a -> b (colour z linear along it,
@@ -274,6 +286,103 @@ FIA_HD void piece_moments(const double col[4][2], const FaceGeom &g, double xl, 
     // z chords of the strip
     if (czl.has_exit && czl.has_entry) acc.add(g, czl.xe, czl.ee, czl.xn, czl.en);
     if (czh.has_exit && czh.has_entry) acc.add(g, czh.xe, czh.ee, czh.xn, czh.en);
+}
+
+// Moment terms of one oriented boundary segment a -> b (those SegAcc<false>::add sums), so that a
+// segment bounding two pieces is evaluated once and added to one, subtracted from the other.
+struct SegTerm {
+    double area2, mxi, meta, ay2;
+};
+FIA_HD SegTerm seg_term(const FaceGeom &g, double xa, double ea, double xb, double eb) {
+    const double cr = xa * eb - xb * ea;
+    double ra[3], rb[3];
+    g.rel(xa, ea, ra);
+    g.rel(xb, eb, rb);
+    return SegTerm{cr, cr * (xa + xb), cr * (ea + eb), ra[2] * rb[0] - ra[0] * rb[2]};
+}
+struct PieceAcc {
+    double area2 = 0.0, mxi = 0.0, meta = 0.0, ay2 = 0.0;
+    FIA_HD void add(const SegTerm &t) { area2 += t.area2; mxi += t.mxi; meta += t.meta; ay2 += t.ay2; }
+    FIA_HD void sub(const SegTerm &t) { area2 -= t.area2; mxi -= t.mxi; meta -= t.meta; ay2 -= t.ay2; }
+};
+
+// Every piece of an untwisted, non-degenerate face cut at most once per colour, in one pass: the
+// x level X0 (when xcut; strips s = 0: c_x < X0 and s = 1: c_x >= X0) and the z level Z0 (when
+// zcut; bands m likewise).  NX / NZ (0 or 1) compile the strip / band split in or out; a level
+// that is not cut leaves one strip / band.  The boundary is piece_moments' for each piece, with
+// each segment's terms evaluated once:
+//  - a square edge's part inside strip s and band m belongs to piece (s, m) alone, with the same
+//    end points as piece_moments computes for that piece;
+//  - the x chord (exit -> entry of c_x >= X0) inside band m bounds (1, m), and reversed (0, m);
+//  - the z chord of strip s (exit -> entry of c_z >= Z0) bounds (s, 1), and reversed (s, 0).
+// Strip 0 reuses strip 1's x chord and its z crossing, so the pieces on both sides of a chord
+// share its points (piece_moments recomputes them from the other end, equal up to rounding).
+template <int NX, int NZ>
+FIA_HD void face_pieces(const double col[4][2], const FaceGeom &g, bool xcut, double X0, bool zcut, double Z0,
+                        PieceAcc (&acc)[2][2]) {
+    const double sx[4] = {0.0, 1.0, 1.0, 0.0}, se[4] = {0.0, 0.0, 1.0, 1.0};
+    // strip / band bounds; a split that is not cut leaves strip / band 1 empty
+    const double X = (NX && xcut) ? X0 : kNoBound, Z = (NZ && zcut) ? Z0 : kNoBound;
+    const double xl[2] = {-kNoBound, X}, xh[2] = {X, kNoBound};
+    const double zl[2] = {-kNoBound, Z}, zh[2] = {Z, kNoBound};
+    Chord cx;        // x chord: boundary leaving / entering c_x >= X0
+    Chord cz[2];     // z chord of each strip: its boundary leaving / entering c_z >= Z0
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int j = (i + 1) & 3;
+        const double xa = sx[i], ea = se[i], xb = sx[j], eb = se[j];
+        const double ca = col[i][0], cb = col[j][0], za = col[i][1], zb = col[j][1];
+        const double dc = cb - ca, dz = zb - za;
+        const double rc = 1.0 / dc, rz = 1.0 / dz;   // inf on a constant colour, then unused
+        if (NX) cx.see(ca >= X, cb >= X, (X - ca) * rc, xa, ea, za, xb, eb, zb);
+#pragma unroll
+        for (int s = 0; s <= NX; ++s) {
+            double lo = 0.0, hi = 1.0;
+            clip_band(ca, dc, rc, xl[s], xh[s], lo, hi);
+            if (lo < hi) {
+                if (NZ) cz[s].see_sel(za + lo * dz >= Z, za + hi * dz >= Z, (Z - za) * rz, xa, ea, za, xb, eb, zb);
+#pragma unroll
+                for (int m = 0; m <= NZ; ++m) {
+                    double lo2 = lo, hi2 = hi;
+                    clip_band(za, dz, rz, zl[m], zh[m], lo2, hi2);
+                    if (lo2 < hi2)
+                        acc[s][m].add(seg_term(g, xa + lo2 * (xb - xa), ea + lo2 * (eb - ea), xa + hi2 * (xb - xa),
+                                               ea + hi2 * (eb - ea)));
+                }
+            }
+        }
+    }
+    if (NX && cx.has_exit && cx.has_entry) {
+        const double dz = cx.zn - cx.ze, rz = 1.0 / dz;
+        if (NZ) {
+            // strip 1 runs the chord exit -> entry, strip 0 entry -> exit
+            const bool l0 = cx.ze >= Z, l1 = cx.zn >= Z;
+            const double t = (Z - cx.ze) * rz;
+            cz[1].see_sel(l0, l1, t, cx.xe, cx.ee, cx.ze, cx.xn, cx.en, cx.zn);
+            cz[0].see_sel(l1, l0, t, cx.xe, cx.ee, cx.ze, cx.xn, cx.en, cx.zn);
+        }
+#pragma unroll
+        for (int m = 0; m <= NZ; ++m) {
+            double lo = 0.0, hi = 1.0;
+            clip_band(cx.ze, dz, rz, zl[m], zh[m], lo, hi);
+            if (lo < hi) {
+                const SegTerm t = seg_term(g, cx.xe + lo * (cx.xn - cx.xe), cx.ee + lo * (cx.en - cx.ee),
+                                           cx.xe + hi * (cx.xn - cx.xe), cx.ee + hi * (cx.en - cx.ee));
+                acc[1][m].add(t);
+                acc[0][m].sub(t);
+            }
+        }
+    }
+    if (NZ) {
+#pragma unroll
+        for (int s = 0; s <= NX; ++s) {
+            if (cz[s].has_exit && cz[s].has_entry) {
+                const SegTerm t = seg_term(g, cz[s].xe, cz[s].ee, cz[s].xn, cz[s].en);
+                acc[s][1].add(t);
+                acc[s][0].sub(t);
+            }
+        }
+    }
 }
 
 // True if an integer strictly inside the range of one colour has the corners on alternating

@@ -140,8 +140,11 @@ enum KernelError {
 hipError_t launch_psi(const PsiParams &p, hipStream_t s);
 hipError_t launch_vertices(const VertexParams &p, hipStream_t s);
 hipError_t launch_face_count(const FaceParams &p, hipStream_t s);
-// fam_slots: first slot of each family and the total (slots of family f: [fam_slots[f], fam_slots[f+1]))
+// clipping-pipeline mode: slot owners, then one thread per slot.  fam_slots: first slot of each
+// family and the total (slots of family f: [fam_slots[f], fam_slots[f+1]))
 hipError_t launch_face_cut(const FaceParams &p, const unsigned long long fam_slots[4], hipStream_t s);
+// segment cutter: the face kernels of the three families (pieces into the slots of the scan)
+hipError_t launch_seg_faces(const FaceParams &p, hipStream_t s);
 // 1 or 2: the WENO6 integral quadrature with that many Gauss points per half (piece_kernel<1|2>);
 // 0: the general quadrature (piece_kernel<0>)
 int piece_quadrature_mode(const FaceParams &p);

@@ -1,4 +1,4 @@
-// Host build of the library's general face cutter and coordinate split (flux-interpolated-
+// Host build of the library's face cutters and coordinate split (flux-interpolated-
 // advection_amd/csrc/fia_cutter.hpp and fia_math.hpp, the same sources the GPU kernels compile),
 // for tests/test_cutter_host.py.
 #include "../flux-interpolated-advection_amd/csrc/fia_cutter.hpp"
@@ -46,4 +46,59 @@ extern "C" long host_fastdiv_mismatches(unsigned d_max, unsigned n_max, long ran
         bad += fia::FastDiv(d).div(n) != n / d;
     }
     return bad;
+}
+
+// piece record as the face kernels store it: centre (cells) and y vector area from the moments
+static void piece_record(const fia::FaceGeom &g, double area2, double mxi, double meta, double ay2, double *o) {
+    o[0] = o[1] = o[2] = o[3] = 0.0;
+    if (area2 == 0.0) return;
+    const double inv3a = 1.0 / (3.0 * area2);
+    double r[3];
+    g.rel(mxi * inv3a, meta * inv3a, r);
+    for (int a = 0; a < 3; ++a) o[a] = g.p0[a] + r[a];
+    o[3] = 0.5 * ay2;
+}
+
+// The segment cutter's two forms on one face (corners v12 in cells, colours col8): the one-pass
+// face_pieces (variant 0: x and z splits compiled in, 1: z only, 2: x only) and piece_moments
+// piece by piece.  Writes (centre, y area) per piece, in slot order s * (z bands) + m, from each
+// and returns the piece count, or -1 when the face is outside the one-pass form's scope (twisted,
+// degenerate, more than one cut along a colour, or a cut the variant does not compile in).
+extern "C" int host_face_pieces(const double *v12, const double *col8, int variant, double *one_pass,
+                                double *per_piece) {
+    double v[4][3], col[4][2];
+    for (int c = 0; c < 4; ++c) {
+        for (int a = 0; a < 3; ++a) v[c][a] = v12[3 * c + a];
+        col[c][0] = col8[2 * c]; col[c][1] = col8[2 * c + 1];
+    }
+    if (fia::face_twisted(col) || fia::face_degenerate(col)) return -1;
+    double lx = col[0][0], hx = col[0][0], lz = col[0][1], hz = col[0][1];
+    for (int c = 1; c < 4; ++c) {
+        lx = fmin(lx, col[c][0]); hx = fmax(hx, col[c][0]);
+        lz = fmin(lz, col[c][1]); hz = fmax(hz, col[c][1]);
+    }
+    int x0, x1, z0, z1;
+    fia::interior_integers(lx, hx, x0, x1);
+    fia::interior_integers(lz, hz, z0, z1);
+    const int nxc = x1 >= x0 ? x1 - x0 + 1 : 0, nzc = z1 >= z0 ? z1 - z0 + 1 : 0;
+    if (nxc > 1 || nzc > 1 || (variant == 1 && nxc > 0) || (variant == 2 && nzc > 0)) return -1;
+    fia::FaceGeom g;
+    g.init(v);
+    fia::PieceAcc acc[2][2];
+    if (variant == 0) fia::face_pieces<1, 1>(col, g, nxc == 1, x0, nzc == 1, z0, acc);
+    else if (variant == 1) fia::face_pieces<0, 1>(col, g, false, x0, nzc == 1, z0, acc);
+    else fia::face_pieces<1, 0>(col, g, nxc == 1, x0, false, z0, acc);
+    int k = 0;
+    for (int s = 0; s <= nxc; ++s) {
+        for (int m = 0; m <= nzc; ++m, ++k) {
+            const fia::PieceAcc &a = acc[s][m];
+            piece_record(g, a.area2, a.mxi, a.meta, a.ay2, one_pass + 4 * k);
+            fia::SegAcc<false> b;
+            fia::piece_moments(col, g, s > 0 ? (double)(x0 + s - 1) : -fia::kNoBound,
+                               s < nxc ? (double)(x0 + s) : fia::kNoBound, m > 0 ? (double)(z0 + m - 1) : -fia::kNoBound,
+                               m < nzc ? (double)(z0 + m) : fia::kNoBound, b);
+            piece_record(g, b.area2, b.mxi, b.meta, b.ay2, per_piece + 4 * k);
+        }
+    }
+    return k;
 }

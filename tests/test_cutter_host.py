@@ -1,7 +1,10 @@
-"""The library's general face cutter (csrc/fia_cutter.hpp) built for the host and checked against
-the oracle's recursive cutter (a restatement of reference src/facecutter.h:59-324) on twisted
-faces whose corner colours span many cells: the set of pieces must be the same, and no capacity
-of the cutter may run out where the reference's recursion would not."""
+"""The library's face cutters (csrc/fia_cutter.hpp) built for the host.  The general cutter is
+checked against the oracle's recursive cutter (a restatement of reference src/facecutter.h:59-324)
+on twisted faces whose corner colours span many cells: the set of pieces must be the same, and no
+capacity of the cutter may run out where the reference's recursion would not.  The segment
+cutter's one-pass form (all pieces of a face cut at most once per colour) is checked against its
+piece-by-piece form and against the oracle's split of the same face (reference
+src/split_face_3D.cpp:26-92 with the recursive cutter)."""
 import ctypes
 import os
 import subprocess
@@ -29,6 +32,8 @@ def cutter(tmp_path_factory):
     L.host_split_coord.restype = ctypes.c_int
     L.host_fastdiv_mismatches.argtypes = [ctypes.c_uint, ctypes.c_uint, ctypes.c_long]
     L.host_fastdiv_mismatches.restype = ctypes.c_long
+    L.host_face_pieces.argtypes = [dp, dp, ctypes.c_int, dp, dp]
+    L.host_face_pieces.restype = ctypes.c_int
     return L
 
 
@@ -79,3 +84,51 @@ def test_fastdiv_matches_integer_division(cutter):
     # to 600 with every numerator up to 70,000 and the largest multiples below 2^31, then a
     # million random pairs
     assert cutter.host_fastdiv_mismatches(600, 70000, 1000000) == 0
+
+
+def _backtraced_faces(rng, count, family, strain):
+    """Faces of one family as a backtrace leaves them: the lattice face (cells) with every corner
+    moved by a smooth shift plus a random displacement of size `strain`."""
+    base = {0: [(0, 0, 0), (0, 1, 0), (0, 1, 1), (0, 0, 1)],
+            1: [(0, 0, 0), (1, 0, 0), (1, 0, 1), (0, 0, 1)],
+            2: [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0)]}[family]
+    for _ in range(count):
+        origin = rng.integers(4, 40, size=3).astype(np.float64)
+        shift = rng.uniform(-4.0, 4.0, size=3)
+        quad = origin + np.array(base, dtype=np.float64) + shift + strain * rng.uniform(-1, 1, size=(4, 3))
+        yield quad
+
+
+@pytest.mark.parametrize("family,variant", [(1, 0), (0, 1), (2, 2)])
+def test_one_pass_segment_cutter_matches_per_piece_and_oracle(cutter, oracle, family, variant):
+    # xz-CellFaceAligned colours (x - 1/2, z - 1/2); faces with at most one cut per colour, as the
+    # face kernel of each family takes them (variant: y faces both splits, x faces z, z faces x)
+    rng = np.random.default_rng(40 + family)
+    dp = ctypes.POINTER(ctypes.c_double)
+    checked = 0
+    shapes = {}
+    for quad in _backtraced_faces(rng, 6000, family, 0.35):
+        col = np.ascontiguousarray(quad[:, [0, 2]] - 0.5)
+        v12 = np.ascontiguousarray(quad.reshape(12))
+        one = np.zeros((4, 4)); per = np.zeros((4, 4))
+        n = cutter.host_face_pieces(v12.ctypes.data_as(dp), col.ctypes.data_as(dp), variant,
+                                    one.ctypes.data_as(dp), per.ctypes.data_as(dp))
+        if n < 0:
+            continue
+        checked += 1
+        shapes[n] = shapes.get(n, 0) + 1
+        one, per = one[:n], per[:n]
+        # the two forms: the same segments up to the chord points they share
+        assert np.abs(one - per).max() <= 1e-12, (quad, one, per)
+        # the oracle's split of the face: the same non-empty pieces (centre, y area)
+        cen, an = oracle.split_face(quad, 1.0, "xz-CellFaceAligned")
+        ref = [(c, a[1]) for c, a in zip(cen, an) if abs(a[1]) > 1e-13]
+        got = [(r[:3], r[3]) for r in one if abs(r[3]) > 1e-13]
+        assert len(ref) == len(got), (quad, ref, got)
+        for c, a in got:
+            d = [np.abs(c - rc).max() for rc, _ in ref]
+            k = int(np.argmin(d))
+            assert d[k] <= 1e-10 and abs(a - ref[k][1]) <= 1e-12, (quad, c, a, ref)
+    # the sample covers the cut configurations the kernels see: one piece (no cut), two (one cut)
+    # and, for y faces, four (both cuts)
+    assert checked > 1000 and shapes.get(2, 0) > 100 and (family != 1 or shapes.get(4, 0) > 100)
