@@ -315,6 +315,7 @@ fia_status do_faces(fia_ctx *c, const CompGrid &g, const float *q, bool moving, 
     p.split = s.split; p.interp = s.interp; p.integral = s.integral; p.fam_mask = fam_mask;
     p.general_all = s.min_area < 1.0;
     p.seg_cut = 1;   // boundary segments; degenerate faces take the clipping pipeline
+    p.fuse_value = !p.integral && (p.interp == FIA_INTERP_LINEAR || p.interp == FIA_INTERP_1DY_LINEAR);
     p.kappa_q = 1.0f;
     p.kappa_psi = (float)(h * h);   // psi is stored in units of q*cells; S in physical units
     p.min_area = s.min_area;
@@ -377,7 +378,7 @@ fia_status do_faces(fia_ctx *c, const CompGrid &g, const float *q, bool moving, 
     const bool weno_quad = piece_quadrature_mode(p) != 0;
     int sq = -1;
     if (weno_quad) HIP_TRY(span_begin(c, SK_QUAD, sq));
-    HIP_TRY(launch_pieces(p, nslots, c->stream));
+    if (!p.fuse_value) HIP_TRY(launch_pieces(p, nslots, c->stream));   // fused: the cutters stored the values
     if (weno_quad) HIP_TRY(span_end(c, sq));
     HIP_TRY(launch_face_sum(p, c->stream));
     HIP_TRY(launch_faces_general(p, n_twisted, c->stream));

@@ -115,6 +115,9 @@ struct FaceParams {
     int general_all;         // every face goes through the general cutter (minimal_area < 1)
     int seg_cut;             // pieces from boundary segments (degenerate faces: clipping pipeline);
                              // 0: the streaming clipping pipeline for every face
+    int fuse_value;          // the cutters store piece values, not records (non-integral rule with
+                             // linear or 1Dy_linear psi: one interpolation per piece, no quadrature
+                             // launch)
     float kappa_q, kappa_psi;
     double min_area;
     unsigned long long *stats;   // optional work counters: kStatSlots x kStatWords

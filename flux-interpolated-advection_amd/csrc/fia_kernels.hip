@@ -388,8 +388,20 @@ template <int FAM> __global__ void __launch_bounds__(kBlock) face_expand_kernel(
     face_expand_body<FAM>(P, id);
 }
 
+// Value of a piece under the non-integral rule with linear or 1Dy_linear psi: piece_flux's
+// non-integral branch for those two methods, each with its method fixed at compile time.
+FIA_HD double piece_value_linear(const FaceParams &P, const double c[3], double ay) {
+    const double ex = 0.5 * P.g.e[0], ey = 0.5 * P.g.e[1], ez = 0.5 * P.g.e[2];
+    const Loader<double> PS{P.psi, P.g.m[0], P.g.m[1] + 1, P.g.m[2]};
+    const double psi = P.interp == IM_LINEAR
+                           ? interp_point<double>(PS, IM_LINEAR, c[0], c[1], c[2], 0.5 - ex, -ey, 0.5 - ez, P.kappa_psi)
+                           : interp_point<double>(PS, IM_1DY_LINEAR, c[0], c[1], c[2], 0.5 - ex, -ey, 0.5 - ez,
+                                                  P.kappa_psi);
+    return ay * psi;
+}
+
 // Piece record of slot q from its moments: centre (cells, physical frame) and y vector area; an
-// empty piece stores a zero area.
+// empty piece stores a zero area.  With fuse_value the piece's value goes to piece_val instead.
 FIA_HD void store_piece(const FaceParams &P, const FaceGeom &geom, double area2, double mxi, double meta, double ay2,
                         unsigned long long q) {
     double ay = 0.0, c3[3] = {0.0, 0.0, 0.0};
@@ -400,6 +412,10 @@ FIA_HD void store_piece(const FaceParams &P, const FaceGeom &geom, double area2,
         c3[0] = geom.p0[0] + r[0]; c3[1] = geom.p0[1] + r[1]; c3[2] = geom.p0[2] + r[2];
         ay = 0.5 * ay2;
         if (!std::isfinite(c3[0]) || !std::isfinite(c3[1]) || !std::isfinite(c3[2])) ay = 0.0;   // degenerate sliver
+    }
+    if (P.fuse_value) {
+        P.piece_val[q] = ay != 0.0 ? piece_value_linear(P, c3, ay) : 0.0;
+        return;
     }
     P.piece_c[0][q] = c3[0];
     P.piece_c[1][q] = c3[1];
@@ -758,6 +774,25 @@ FIA_HD void degenerate_body(const FaceParams &P, int t) {
 __global__ void __launch_bounds__(64) degenerate_kernel(FaceParams P) {
     const int count = *P.degen_count;
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < count; t += gridDim.x * blockDim.x) degenerate_body(P, t);
+}
+// With fuse_value (no quadrature launch): the values of the degenerate faces' pieces from the
+// records degenerate_kernel stored.
+FIA_HD void degenerate_value_body(const FaceParams &P, int t) {
+    const unsigned code = P.degen[t];
+    const size_t gid = P.fam_base[code >> 30] + (code & ((1u << 30) - 1u));
+    const unsigned n = P.face_n[gid];
+    const unsigned long long s0 = P.face_start[gid];
+    for (unsigned k = 0; k < n; ++k) {
+        const unsigned long long q = s0 + k;
+        const double ay = P.piece_ay[q];
+        const double c3[3] = {P.piece_c[0][q], P.piece_c[1][q], P.piece_c[2][q]};
+        P.piece_val[q] = ay != 0.0 ? piece_value_linear(P, c3, ay) : 0.0;
+    }
+}
+__global__ void __launch_bounds__(64) degenerate_value_kernel(FaceParams P) {
+    const int count = *P.degen_count;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < count; t += gridDim.x * blockDim.x)
+        degenerate_value_body(P, t);
 }
 
 // Flux of one face with the exact recursive cutter, corners in the given order.
@@ -1185,6 +1220,7 @@ hipError_t launch_seg_faces(const FaceParams &p, hipStream_t s) {
 hipError_t launch_faces_degenerate(const FaceParams &p, int count, hipStream_t s) {
     if (count <= 0) return hipSuccess;
     hipLaunchKernelGGL(degenerate_kernel, dim3(list_blocks(count)), dim3(64), 0, s, p);
+    if (p.fuse_value) hipLaunchKernelGGL(degenerate_value_kernel, dim3(list_blocks(count)), dim3(64), 0, s, p);
     return hipGetLastError();
 }
 
