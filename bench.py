@@ -52,7 +52,7 @@ PEAK_HBM_GBS = 8000.0
 # HBM bytes per launch of piece_kernel<1> at the default workload (256^3, CFL 4), from two
 # rocprofv3 counter passes (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a 1 GiB device copy);
 # recorded in profiles/density_step_256_cfl4_hbm_counters.md
-TRAFFIC_BYTES_256_CFL4 = 12.043e9
+TRAFFIC_BYTES_256_CFL4 = 12.123e9
 # Jacobi-PCG algorithmic traffic per cell and iteration (FP64 vectors, two kernels): direction
 # kernel reads r, p_old (8 + 8) and the flag byte, writes p (8); update kernel reads x, r, p
 # (3 x 8) and the flag byte, writes x, r (2 x 8); q = L p is recomputed, not stored
