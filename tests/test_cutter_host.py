@@ -132,3 +132,33 @@ def test_one_pass_segment_cutter_matches_per_piece_and_oracle(cutter, oracle, fa
     # the sample covers the cut configurations the kernels see: one piece (no cut), two (one cut)
     # and, for y faces, four (both cuts)
     assert checked > 1000 and shapes.get(2, 0) > 100 and (family != 1 or shapes.get(4, 0) > 100)
+
+
+def _cut_counts(lo, hi):
+    """Interior integers of the open colour ranges (lo, hi), element-wise."""
+    first = np.ceil(lo)
+    first = np.where(lo - first == 0, first + 1, first)
+    last = np.floor(hi)
+    last = np.where(hi - last == 0, last - 1, last)
+    return np.maximum(last - first + 1, 0)
+
+
+def test_gpu_step_scene_covers_both_segment_cutter_paths(oracle, scenes):
+    # the 24^3, CFL 4 scene of the GPU step tests (tests/test_gpu_pass.py: h = 1/32, curl noise seed
+    # 1), backtraced with the oracle: every face family has faces the one-pass form takes and faces
+    # with several cuts along a colour, which the face kernel's block cuts piece by piece, so the
+    # GPU parity tests run both paths
+    n, h = 24, 1.0 / 32
+    u, v, w = scenes.curl_noise(n, seed=1)
+    dt = scenes.cfl_dt(u, v, w, h, 4.0)
+    P = oracle.backtrace_vertices(u, v, w, h, dt, method="WENO6") / h
+    corners = {0: [(0, 0, 0), (0, 1, 0), (0, 1, 1), (0, 0, 1)], 1: [(0, 0, 0), (1, 0, 0), (1, 0, 1), (0, 0, 1)],
+               2: [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0)]}
+    one_pass = {0: lambda nx, nz: (nx == 0) & (nz <= 1), 1: lambda nx, nz: (nx <= 1) & (nz <= 1),
+                2: lambda nx, nz: (nz == 0) & (nx <= 1)}
+    for fam, cs in corners.items():
+        ext = [n + (a == fam) for a in range(3)]
+        quad = np.stack([P[c[0]:c[0] + ext[0], c[1]:c[1] + ext[1], c[2]:c[2] + ext[2]] for c in cs])
+        cx, cz = quad[..., 0] - 0.5, quad[..., 2] - 0.5   # xz-CellFaceAligned colours
+        fast = one_pass[fam](_cut_counts(cx.min(0), cx.max(0)), _cut_counts(cz.min(0), cz.max(0)))
+        assert fast.sum() > 0 and (~fast).sum() > 0, (fam, fast.sum(), (~fast).sum())
