@@ -80,7 +80,6 @@ struct fia_ctx {
     FluxSet fsa, fsb;      // face fluxes (and orientation corrections) of two passes
     DevBuf<unsigned> face_n;               // piece slots per face
     DevBuf<unsigned long long> face_start; // exclusive scan of face_n
-    DevBuf<unsigned> slot_face;            // owner face of every piece slot
     DevBuf<unsigned long long> scan_tmp;   // tile sums of the slot scan
     DevBuf<unsigned long long> bounds;     // slot bounds of the families (device side)
     unsigned long long *host_bounds = nullptr;   // pinned: [0] 0, [1] [2] family starts, [3] total
@@ -179,7 +178,7 @@ namespace {
 size_t ctx_bytes(const fia_ctx *c) {
     size_t b = c->psi.bytes() + c->twisted.bytes() + c->degen.bytes() + c->counters.bytes() + c->scal.bytes() + c->bits.bytes() +
                c->solid.bytes() + c->genI.bytes();
-    b += c->fsa.bytes() + c->fsb.bytes() + c->face_n.bytes() + c->face_start.bytes() + c->slot_face.bytes() + c->scan_tmp.bytes() + c->bounds.bytes() +
+    b += c->fsa.bytes() + c->fsb.bytes() + c->face_n.bytes() + c->face_start.bytes() + c->scan_tmp.bytes() + c->bounds.bytes() +
          c->piece_ay.bytes() + c->piece_val.bytes();
     for (int a = 0; a < 3; ++a) b += c->piece_c[a].bytes();
     for (int a = 0; a < 3; ++a) b += c->disp[a].bytes() + c->work[a].bytes() + c->gen[a].bytes();
@@ -314,7 +313,6 @@ fia_status do_faces(fia_ctx *c, const CompGrid &g, const float *q, bool moving, 
     p.degen = c->degen.p; p.degen_count = c->counters.p + 2;
     p.split = s.split; p.interp = s.interp; p.integral = s.integral; p.fam_mask = fam_mask;
     p.general_all = s.min_area < 1.0;
-    p.seg_cut = 1;   // boundary segments; degenerate faces take the clipping pipeline
     p.fuse_value = !p.integral && (p.interp == FIA_INTERP_LINEAR || p.interp == FIA_INTERP_1DY_LINEAR);
     p.kappa_q = 1.0f;
     p.kappa_psi = (float)(h * h);   // psi is stored in units of q*cells; S in physical units
@@ -358,22 +356,15 @@ fia_status do_faces(fia_ctx *c, const CompGrid &g, const float *q, bool moving, 
     const unsigned long long nslots = fam_slots[3];
     if (!(fam_slots[1] <= fam_slots[2] && fam_slots[2] <= fam_slots[3]))
         return fail(FIA_ERR_DEVICE, "inconsistent piece slot bounds");
-    if (!p.seg_cut) {
-        HIP_TRY(c->slot_face.ensure(nslots));
-        p.slot_face = c->slot_face.p;
-    }
     for (int a = 0; a < 3; ++a) HIP_TRY(c->piece_c[a].ensure(nslots));
     HIP_TRY(c->piece_ay.ensure(nslots));
     HIP_TRY(c->piece_val.ensure(nslots));
     for (int a = 0; a < 3; ++a) p.piece_c[a] = c->piece_c[a].p;
     p.piece_ay = c->piece_ay.p;
     p.piece_val = c->piece_val.p;
-    if (p.seg_cut) {
-        HIP_TRY(launch_seg_faces(p, c->stream));
-        HIP_TRY(launch_faces_degenerate(p, n_degen, c->stream));
-    } else {
-        HIP_TRY(launch_face_cut(p, fam_slots, c->stream));
-    }
+    // pieces: one-pass face kernels (segment cutter), then the degenerate faces (clipping pipeline)
+    HIP_TRY(launch_seg_faces(p, c->stream));
+    HIP_TRY(launch_faces_degenerate(p, n_degen, c->stream));
     // the WENO6 integral quadrature launches are timed on their own (the dominant kernel)
     const bool weno_quad = piece_quadrature_mode(p) != 0;
     int sq = -1;
@@ -939,7 +930,7 @@ fia_status fia_ctx_destroy(fia_ctx *c) {
     (void)hipStreamSynchronize(c->stream);
     c->psi.release(); c->twisted.release(); c->degen.release(); c->counters.release(); c->scal.release(); c->bits.release();
     c->solid.release(); c->genI.release();
-    c->fsa.release(); c->fsb.release(); c->face_n.release(); c->face_start.release(); c->slot_face.release(); c->scan_tmp.release();
+    c->fsa.release(); c->fsb.release(); c->face_n.release(); c->face_start.release(); c->scan_tmp.release();
     c->piece_ay.release(); c->piece_val.release();
     for (int a = 0; a < 3; ++a) c->piece_c[a].release();
     for (int a = 0; a < 3; ++a) { c->disp[a].release(); c->work[a].release(); c->gen[a].release(); }

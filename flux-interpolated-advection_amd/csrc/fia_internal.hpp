@@ -113,8 +113,6 @@ struct FaceParams {
     int split, interp, integral;
     int fam_mask;            // families computed (bit 0 x, 1 y, 2 z)
     int general_all;         // every face goes through the general cutter (minimal_area < 1)
-    int seg_cut;             // pieces from boundary segments (degenerate faces: clipping pipeline);
-                             // 0: the streaming clipping pipeline for every face
     int fuse_value;          // the cutters store piece values, not records (non-integral rule with
                              // linear or 1Dy_linear psi: one interpolation per piece, no quadrature
                              // launch)
@@ -125,7 +123,6 @@ struct FaceParams {
     // offsets), piece records (centre in cells, y area) and piece values
     unsigned *face_n;
     unsigned long long *face_start;
-    unsigned *slot_face;     // owner face (global index over the families) of every slot
     size_t fam_base[3];
     double *piece_c[3];
     double *piece_ay;
@@ -143,9 +140,6 @@ enum KernelError {
 hipError_t launch_psi(const PsiParams &p, hipStream_t s);
 hipError_t launch_vertices(const VertexParams &p, hipStream_t s);
 hipError_t launch_face_count(const FaceParams &p, hipStream_t s);
-// clipping-pipeline mode: slot owners, then one thread per slot.  fam_slots: first slot of each
-// family and the total (slots of family f: [fam_slots[f], fam_slots[f+1]))
-hipError_t launch_face_cut(const FaceParams &p, const unsigned long long fam_slots[4], hipStream_t s);
 // segment cutter: the face kernels of the three families (pieces into the slots of the scan)
 hipError_t launch_seg_faces(const FaceParams &p, hipStream_t s);
 // 1 or 2: the WENO6 integral quadrature with that many Gauss points per half (piece_kernel<1|2>);

@@ -348,8 +348,8 @@ template <int FAM> FIA_HD bool seg_one_pass(int nxc, int nzc) {
 
 // Phase 1: slots each face needs, (x strips) x (z strips).  Faces for the general cutter (twisted
 // ones, or all when minimal_area < 1) are queued and get no slots; a non-finite face reports an
-// error and gets none (zero flux).  With the segment cutter, degenerate faces keep their slots and
-// are queued for the clipping pipeline (degenerate_kernel).
+// error and gets none (zero flux).  Degenerate faces keep their slots and are queued for the
+// clipping pipeline (degenerate_kernel).
 template <int FAM> FIA_HD void face_count_body(const FaceParams &P, long id) {
     unsigned n = 0;
     if (P.fam_mask & (1 << FAM)) {
@@ -364,7 +364,7 @@ template <int FAM> FIA_HD void face_count_body(const FaceParams &P, long id) {
             int x0, nxc, z0, nzc;
             colour_ranges(col, x0, nxc, z0, nzc);
             n = (unsigned)((nxc + 1) * (nzc + 1));
-            if (P.seg_cut && face_degenerate(col)) queue_degenerate(P, FAM, id);
+            if (face_degenerate(col)) queue_degenerate(P, FAM, id);
         }
     }
     P.face_n[P.fam_base[FAM] + id] = n;
@@ -375,18 +375,6 @@ template <int FAM> __global__ void __launch_bounds__(kBlock) face_count_kernel(F
     face_count_body<FAM>(P, id);
 }
 
-// Phase 2a: owner of every slot (global face index), from the exclusive scan of phase 1.
-template <int FAM> FIA_HD void face_expand_body(const FaceParams &P, long id) {
-    const size_t gid = P.fam_base[FAM] + id;
-    const unsigned n = P.face_n[gid];
-    const unsigned long long s0 = P.face_start[gid];
-    for (unsigned q = 0; q < n; ++q) P.slot_face[s0 + q] = (unsigned)gid;
-}
-template <int FAM> __global__ void __launch_bounds__(kBlock) face_expand_kernel(FaceParams P) {
-    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= (long)P.g.faces(FAM)) return;
-    face_expand_body<FAM>(P, id);
-}
 
 // Value of a piece under the non-integral rule with linear or 1Dy_linear psi: piece_flux's
 // non-integral branch for those two methods, each with its method fixed at compile time.
@@ -423,43 +411,6 @@ FIA_HD void store_piece(const FaceParams &P, const FaceGeom &geom, double area2,
     P.piece_ay[q] = ay;   // a piece with no y area contributes exactly zero
 }
 
-// Phase 2b (clipping-pipeline mode): one thread per slot [s0, s1) of family FAM cuts its piece
-// (x strip s, z band m of the face, slot = s * (z bands) + m) with the streaming pipeline and
-// stores its record.
-template <int FAM> FIA_HD void slot_cut_body(const FaceParams &P, unsigned long long q) {
-    const unsigned gid = P.slot_face[q];
-    const long id = (long)gid - (long)P.fam_base[FAM];
-    int i, j, k;
-    decode_face(P.g, FAM, id, i, j, k);
-    double v[4][3], col[4][2];
-    load_face(P, FAM, i, j, k, v, col);
-    FaceGeom geom;
-    geom.init(v);
-    int x0, nxc, z0, nzc;
-    colour_ranges(col, x0, nxc, z0, nzc);
-    const int local = (int)(q - P.face_start[gid]);
-    const int s = local / (nzc + 1), m = local - s * (nzc + 1);
-    Pipeline pp;
-    pp.st[0].reset(s > 0, 0, (double)(x0 + s - 1), true);
-    pp.st[1].reset(s < nxc, 0, (double)(x0 + s), false);
-    pp.st[2].reset(m > 0, 1, (double)(z0 + m - 1), true);
-    pp.st[3].reset(m < nzc, 1, (double)(z0 + m), false);
-    pp.acc.reset();
-    const double sq[4][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};
-    for (int c = 0; c < 4; ++c) {
-        CutVert cv;
-        cv.xi = sq[c][0]; cv.eta = sq[c][1]; cv.cx = col[c][0]; cv.cz = col[c][1];
-        pp.push<0>(geom, cv);
-    }
-    pp.close<0>(geom);
-    store_piece(P, geom, pp.acc.n >= 3 ? pp.acc.area2 : 0.0, pp.acc.mxi, pp.acc.meta, pp.acc.ay2, q);
-}
-template <int FAM>
-__global__ void __launch_bounds__(kBlock) slot_cut_kernel(FaceParams P, unsigned long long s0, unsigned long long s1) {
-    const unsigned long long q = s0 + (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= s1) return;
-    slot_cut_body<FAM>(P, q);
-}
 
 // Segment cutter (boundary segments; degenerate faces take the clipping pipeline in
 // degenerate_kernel).  One block of kBlock threads per kBlock faces of family FAM:
@@ -1049,18 +1000,6 @@ hipError_t launch_face_count(const FaceParams &p, hipStream_t s) {
     hipLaunchKernelGGL(face_count_kernel<0>, dim3(grid_for(p.g.faces(0), kBlock)), dim3(kBlock), 0, s, p);
     hipLaunchKernelGGL(face_count_kernel<1>, dim3(grid_for(p.g.faces(1), kBlock)), dim3(kBlock), 0, s, p);
     hipLaunchKernelGGL(face_count_kernel<2>, dim3(grid_for(p.g.faces(2), kBlock)), dim3(kBlock), 0, s, p);
-    return hipGetLastError();
-}
-
-hipError_t launch_face_cut(const FaceParams &p, const unsigned long long fam_slots[4], hipStream_t s) {
-    if (p.fam_mask & 1) hipLaunchKernelGGL(face_expand_kernel<0>, dim3(grid_for(p.g.faces(0), kBlock)), dim3(kBlock), 0, s, p);
-    if (p.fam_mask & 2) hipLaunchKernelGGL(face_expand_kernel<1>, dim3(grid_for(p.g.faces(1), kBlock)), dim3(kBlock), 0, s, p);
-    if (p.fam_mask & 4) hipLaunchKernelGGL(face_expand_kernel<2>, dim3(grid_for(p.g.faces(2), kBlock)), dim3(kBlock), 0, s, p);
-    const unsigned long long n0 = fam_slots[1] - fam_slots[0], n1 = fam_slots[2] - fam_slots[1],
-                             n2 = fam_slots[3] - fam_slots[2];
-    if (n0) hipLaunchKernelGGL(slot_cut_kernel<0>, dim3(grid_for(n0, kBlock)), dim3(kBlock), 0, s, p, fam_slots[0], fam_slots[1]);
-    if (n1) hipLaunchKernelGGL(slot_cut_kernel<1>, dim3(grid_for(n1, kBlock)), dim3(kBlock), 0, s, p, fam_slots[1], fam_slots[2]);
-    if (n2) hipLaunchKernelGGL(slot_cut_kernel<2>, dim3(grid_for(n2, kBlock)), dim3(kBlock), 0, s, p, fam_slots[2], fam_slots[3]);
     return hipGetLastError();
 }
 
