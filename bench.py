@@ -353,8 +353,8 @@ def main():
     if args.precond is None:
         args.precond = "jacobi" if args.workload == "warmstart" else "mg"
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md, sharding)",
-              file=sys.stderr)
+        print("bench.py runs on one GPU only: the library has no multi-GPU path (one GPU's share of a "
+              "sharded grid is the single-GPU workload)", file=sys.stderr)
         sys.exit(2)
 
     import torch

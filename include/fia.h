@@ -116,8 +116,8 @@ void fia_default_opts(fia_advect_opts *opts);
 fia_status fia_ctx_create(int device, fia_ctx **out);
 fia_status fia_ctx_destroy(fia_ctx *ctx);
 /* A context that enqueues its work on the caller's HIP stream (a hipStream_t of the device),
- * so that library work orders with the caller's own kernels (torch's current stream, a batch
- * driver's streams).  Calls still return when their results are complete: the flux path reads
+ * so that library work orders with the caller's own kernels (torch's current stream, the streams
+ * of a caller's scene scheduler).  Calls still return when their results are complete: the flux path reads
  * piece-slot counts back between its phases.  The context does not destroy the stream. */
 fia_status fia_ctx_create_on_stream(int device, void *stream, fia_ctx **out);
 /* The stream a context enqueues on (a hipStream_t). */
