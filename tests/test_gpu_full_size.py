@@ -1,0 +1,60 @@
+"""GPU checks at the benchmark's full size (256^3) through properties that hold at any size.
+
+- Integer translation: with a uniform velocity that moves every vertex by a whole number of cells
+  (dt u / h = (2, 1, 3)), a density step maps the field onto itself shifted by those cells, away
+  from the walls.  It is exact in FP64 (the oracle gives 0 difference at 24^3 for both cases below,
+  zero-velocity correction on): every sample coordinate falls on a node, where WENO6 and linear
+  interpolation return the sample, and the density term along y vanishes.  The cases are WENO6
+  with non-integral psi on uncut faces (one piece per face), and 1Dy_linear with cell-face-aligned
+  colours, whose y faces are cut into 2 x 2 pieces by the one-pass segment cutter and whose piece
+  values the cutter stores itself.  The GPU keeps the fields in FP32, so the bar is 1e-6 relative.
+- Determinism: the benchmark's density step (plume blob, curl-noise velocity, CFL 4, WENO6
+  integral, xz-CellFaceAligned) run twice from the same input gives bit-identical density (face
+  fluxes are summed in a fixed order and gathered without atomics).
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+N = 256
+SHIFT = (2, 1, 3)
+MARGIN = 8
+
+
+@pytest.mark.parametrize("interp,split", [("WENO6", "none"), ("1Dy_linear", "xz-CellFaceAligned")])
+def test_full_size_integer_translation(fia, ctx, interp, split):
+    print("case full-size translation", interp, split, flush=True)
+    n, h = N, 1.0 / N
+    rng = np.random.default_rng(11)
+    rho = rng.uniform(0.5, 1.5, size=(n, n, n)).astype(np.float32)
+    u = np.full((n + 1, n, n), SHIFT[0] * h, np.float32)
+    v = np.full((n, n + 1, n), SHIFT[1] * h, np.float32)
+    w = np.full((n, n, n + 1), SHIFT[2] * h, np.float32)
+    grid = fia.Grid3D(n, n, n, h, rho.copy(), u, v, w)
+    fia.move_substances_3d(ctx, grid, 0, 1.0, split_method=split, interpolation_method=interp, use_integral=False)
+    m, (a, b, c) = MARGIN, SHIFT
+    got = grid.density[m:n - m, m:n - m, m:n - m].astype(np.float64)
+    want = rho[m - a:n - m - a, m - b:n - m - b, m - c:n - m - c].astype(np.float64)
+    err = np.abs(got - want).max() / np.abs(want).max()
+    print("relative L-inf against the shifted field %.3e" % err, flush=True)
+    assert err <= 1e-6
+
+
+def test_full_size_density_step_is_deterministic(fia, ctx, scenes):
+    print("case full-size determinism", flush=True)
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    if root not in sys.path:
+        sys.path.insert(0, root)
+    import bench
+    h, rho, u, v, w, dt = bench.make_scene(scenes, N, 4.0)
+    out = []
+    for _ in range(2):
+        grid = fia.Grid3D(N, N, N, h, rho.copy(), u, v, w)
+        fia.move_substances_3d(ctx, grid, 0, dt, split_method="xz-CellFaceAligned", interpolation_method="WENO6")
+        out.append(grid.density.copy())
+    assert np.isfinite(out[0]).all()
+    assert np.array_equal(out[0], out[1])
