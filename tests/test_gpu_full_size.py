@@ -8,6 +8,8 @@
   with non-integral psi on uncut faces (one piece per face), and 1Dy_linear with cell-face-aligned
   colours, whose y faces are cut into 2 x 2 pieces by the one-pass segment cutter and whose piece
   values the cutter stores itself.  The GPU keeps the fields in FP32, so the bar is 1e-6 relative.
+- Uniform velocity: self-advection of a uniform velocity moving a fraction of cells per step
+  keeps the interior uniform (exact in FP64 up to rounding).
 - Determinism: the benchmark's density step (plume blob, curl-noise velocity, CFL 4, WENO6
   integral, xz-CellFaceAligned) run twice from the same input gives bit-identical density (face
   fluxes are summed in a fixed order and gathered without atomics).
@@ -58,3 +60,24 @@ def test_full_size_density_step_is_deterministic(fia, ctx, scenes):
         out.append(grid.density.copy())
     assert np.isfinite(out[0]).all()
     assert np.array_equal(out[0], out[1])
+
+
+def test_full_size_uniform_velocity_keeps_itself(fia, ctx):
+    # velocity self-advection (WENO6 integral, xz-CellFaceAligned, every component) of a uniform
+    # field moving (1.37, 0.61, 2.29) cells per step: the interior stays uniform (the oracle keeps
+    # it to 5.5e-15 at 24^3); the faces move rigidly and are cut along both colours (2 x 2 pieces
+    # on the y faces of every component grid)
+    print("case full-size uniform velocity", flush=True)
+    n, h = N, 1.0 / N
+    s = (1.37, 0.61, 2.29)
+    c = [np.float32(x * h) for x in s]
+    u = np.full((n + 1, n, n), c[0], np.float32)
+    v = np.full((n, n + 1, n), c[1], np.float32)
+    w = np.full((n, n, n + 1), c[2], np.float32)
+    grid = fia.Grid3D(n, n, n, h, None, u, v, w)
+    fia.advect_velocity_flux_advection_3d(ctx, grid, -1, 1.0, interpolation_method="WENO6")
+    m = MARGIN
+    for name, got, ref in (("u", grid.u, c[0]), ("v", grid.v, c[1]), ("w", grid.w, c[2])):
+        err = float(np.abs(got[m:-m, m:-m, m:-m].astype(np.float64) - float(ref)).max() / abs(float(ref)))
+        print("%s: relative L-inf against the uniform value %.3e" % (name, err), flush=True)
+        assert err <= 1e-5
