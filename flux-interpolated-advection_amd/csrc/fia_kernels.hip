@@ -1059,11 +1059,14 @@ __device__ __forceinline__ unsigned long long block_exclusive_scan(unsigned long
 
 __global__ void __launch_bounds__(kBlock) scan_tile_sums_kernel(const unsigned *in, size_t n,
                                                                 unsigned long long *tile_sum) {
-    const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
+    // the tile read striped over the block (coalesced); the sum does not depend on the order
+    const size_t tile0 = (size_t)blockIdx.x * kScanTile;
     unsigned long long s = 0;
 #pragma unroll
-    for (int i = 0; i < kScanItems; ++i)
-        if (base + i < n) s += in[base + i];
+    for (int i = 0; i < kScanItems; ++i) {
+        const size_t k = tile0 + (size_t)i * kBlock + threadIdx.x;
+        if (k < n) s += in[k];
+    }
     unsigned long long total;
     (void)block_exclusive_scan(s, total);
     if (threadIdx.x == 0) tile_sum[blockIdx.x] = total;
@@ -1087,30 +1090,52 @@ __global__ void __launch_bounds__(kBlock) scan_tile_offsets_kernel(unsigned long
     }
 }
 
+// LDS index of tile element e, padded by one word per kScanItems elements so that both the striped
+// (e = i kBlock + t) and the blocked (e = t kScanItems + i) accesses of a wave spread over the banks
+__device__ __forceinline__ int scan_pad(int e) { return e + e / kScanItems; }
+
+// Each tile is loaded striped over the block (coalesced), transposed in LDS so that every thread
+// scans kScanItems consecutive counts, and its starts stored striped again.
 __global__ void __launch_bounds__(kBlock) scan_tiles_kernel(const unsigned *in, size_t n,
                                                             const unsigned long long *tile_off,
                                                             unsigned long long *out, FaceParams P,
                                                             unsigned long long *bounds) {
-    const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
+    __shared__ unsigned sv[kScanTile + kScanTile / kScanItems];
+    __shared__ unsigned long long so[kScanTile + kScanTile / kScanItems];
+    const size_t tile0 = (size_t)blockIdx.x * kScanTile;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        const int e = i * kBlock + threadIdx.x;
+        const size_t k = tile0 + e;
+        sv[scan_pad(e)] = k < n ? in[k] : 0u;
+    }
+    __syncthreads();
     unsigned v[kScanItems];
     unsigned long long s = 0;
 #pragma unroll
     for (int i = 0; i < kScanItems; ++i) {
-        v[i] = base + i < n ? in[base + i] : 0u;
+        v[i] = sv[scan_pad(threadIdx.x * kScanItems + i)];
         s += v[i];
     }
     unsigned long long total;
     unsigned long long run = tile_off[blockIdx.x] + block_exclusive_scan(s, total);
 #pragma unroll
     for (int i = 0; i < kScanItems; ++i) {
-        const size_t k = base + i;
-        if (k < n) {
-            out[k] = run;
-            if (k == P.fam_base[1]) bounds[1] = run;
-            if (k == P.fam_base[2]) bounds[2] = run;
-            if (k == n - 1) bounds[3] = run + v[i];
-        }
+        so[scan_pad(threadIdx.x * kScanItems + i)] = run;
         run += v[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        const int e = i * kBlock + threadIdx.x;
+        const size_t k = tile0 + e;
+        if (k < n) {
+            const unsigned long long r = so[scan_pad(e)];
+            out[k] = r;
+            if (k == P.fam_base[1]) bounds[1] = r;
+            if (k == P.fam_base[2]) bounds[2] = r;
+            if (k == n - 1) bounds[3] = r + sv[scan_pad(e)];
+        }
     }
 }
 
