@@ -89,9 +89,16 @@ struct Stage {
         const double c = (dim == 0 ? v.cx : v.cz) - iso;
         return upper ? (c >= 0.0) : (c < 0.0);
     }
-    // crossing point on edge a -> b (reference src/facecutter.h:151-161)
+    // crossing point on edge a -> b (reference src/facecutter.h:151-161).  An end point on the
+    // line is the crossing itself, taken exactly: the weights of the reference's form (t1 = -v0 /
+    // (v1 - v0) through a rounded reciprocal) can miss 1 by an ulp there, and the interpolated
+    // other colour of that near-copy of the corner can then round to the far side of a later stage's
+    // line, which folds the clipped loop over itself (the recursive cut of the reference survives
+    // the same near-copy as a zero-area sliver).
     FIA_HD CutVert cross(const CutVert &a, const CutVert &b) const {
         const double v0 = (dim == 0 ? a.cx : a.cz) - iso, v1 = (dim == 0 ? b.cx : b.cz) - iso;
+        if (v1 == 0.0) return b;
+        if (v0 == 0.0) return a;
         const double inv = 1.0 / (v1 - v0);
         const double t0 = v1 * inv, t1 = -v0 * inv;
         CutVert r;

@@ -19,22 +19,20 @@ constexpr int kBlock = 256;
 static inline unsigned grid_for(size_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
 // Every kernel is an index computation around a __host__ __device__ body, so the bodies can also
-// run serially over host arrays (a CPU emulation of the kernels used to check their logic).
-// Their atomics are plain read-modify-writes in that host build.
+// run over host arrays (a CPU emulation of the kernels used to check their logic, on host threads).
+// Their atomics are the compiler's host atomics in that build.
 FIA_HD int atomic_add_i32(int *p, int v) {
 #if defined(__HIP_DEVICE_COMPILE__)
     return atomicAdd(p, v);
 #else
-    const int o = *p;
-    *p = o + v;
-    return o;
+    return __atomic_fetch_add(p, v, __ATOMIC_RELAXED);
 #endif
 }
 FIA_HD void atomic_cas_i32(int *p, int cmp, int v) {
 #if defined(__HIP_DEVICE_COMPILE__)
     atomicCAS(p, cmp, v);
 #else
-    if (*p == cmp) *p = v;
+    __atomic_compare_exchange_n(p, &cmp, v, false, __ATOMIC_RELAXED, __ATOMIC_RELAXED);
 #endif
 }
 

@@ -32,28 +32,33 @@ def sphere_source(n, centre=(0.5, 0.1, 0.5), radius=0.06, value=1.0):
 
 def curl_noise(n, seed=0, kmin=1, kmax=4, amplitude=1.0):
     """Divergence-free MAC velocity: curl of A = sum_k a_k sin(pi k.x)-type modes that vanish on the
-    walls (wavenumbers kmin..kmax per axis, amplitude ~ |k|^-2).  Returns (u, v, w)."""
+    walls (wavenumbers kmin..kmax per axis, amplitude ~ |k|^-2).  n is the cube's edge, or a shape
+    (nx, ny, nz) for a box of cells of edge 1 / max(shape).  Returns (u, v, w)."""
     rng = np.random.default_rng(seed)
-    h = 1.0 / n
-    node = np.arange(n + 1) * h
-    cent = cell_centres(n)
+    shape = (n, n, n) if np.isscalar(n) else tuple(int(a) for a in n)
+    nx, ny, nz = shape
+    h = 1.0 / max(shape)
+    # node and cell-centre coordinates of each axis, scaled to the box edge (x / L in [0, 1]; the
+    # longest edge is 1, written as in the cube so that cubic fields keep their bits)
+    node = [np.arange(m + 1) * h if m == max(shape) else np.arange(m + 1) / m for m in shape]
+    cent = [cell_centres(m) for m in shape]
     # edge-centred potentials: Ax at (i+1/2, j, k), Ay at (i, j+1/2, k), Az at (i, j, k+1/2)
-    Ax = np.zeros((n, n + 1, n + 1)); Ay = np.zeros((n + 1, n, n + 1)); Az = np.zeros((n + 1, n + 1, n))
+    Ax = np.zeros((nx, ny + 1, nz + 1)); Ay = np.zeros((nx + 1, ny, nz + 1)); Az = np.zeros((nx + 1, ny + 1, nz))
     for kx in range(kmin, kmax + 1):
         for ky in range(kmin, kmax + 1):
             for kz in range(kmin, kmax + 1):
                 kk = float(kx * kx + ky * ky + kz * kz)
                 a = rng.normal(size=3) / kk
-                # sin(pi k x) vanishes at the walls x = 0, 1 for every integer k
-                sx_n, sy_n, sz_n = np.sin(np.pi * kx * node), np.sin(np.pi * ky * node), np.sin(np.pi * kz * node)
-                cx_c, cy_c, cz_c = np.cos(np.pi * kx * cent), np.cos(np.pi * ky * cent), np.cos(np.pi * kz * cent)
+                # sin(pi k x / L) vanishes at the walls x = 0, L for every integer k
+                sx_n, sy_n, sz_n = np.sin(np.pi * kx * node[0]), np.sin(np.pi * ky * node[1]), np.sin(np.pi * kz * node[2])
+                cx_c, cy_c, cz_c = np.cos(np.pi * kx * cent[0]), np.cos(np.pi * ky * cent[1]), np.cos(np.pi * kz * cent[2])
                 Ax += a[0] * cx_c[:, None, None] * sy_n[None, :, None] * sz_n[None, None, :]
                 Ay += a[1] * sx_n[:, None, None] * cy_c[None, :, None] * sz_n[None, None, :]
                 Az += a[2] * sx_n[:, None, None] * sy_n[None, :, None] * cz_c[None, None, :]
     # u = curl A on faces
-    u = (Az[:, 1:, :] - Az[:, :-1, :]) / h - (Ay[:, :, 1:] - Ay[:, :, :-1]) / h      # (n+1, n, n)
-    v = (Ax[:, :, 1:] - Ax[:, :, :-1]) / h - (Az[1:, :, :] - Az[:-1, :, :]) / h      # (n, n+1, n)
-    w = (Ay[1:, :, :] - Ay[:-1, :, :]) / h - (Ax[:, 1:, :] - Ax[:, :-1, :]) / h      # (n, n, n+1)
+    u = (Az[:, 1:, :] - Az[:, :-1, :]) / h - (Ay[:, :, 1:] - Ay[:, :, :-1]) / h      # (nx+1, ny, nz)
+    v = (Ax[:, :, 1:] - Ax[:, :, :-1]) / h - (Az[1:, :, :] - Az[:-1, :, :]) / h      # (nx, ny+1, nz)
+    w = (Ay[1:, :, :] - Ay[:-1, :, :]) / h - (Ax[:, 1:, :] - Ax[:, :-1, :]) / h      # (nx, ny, nz+1)
     u[0] = u[-1] = 0.0
     v[:, 0] = v[:, -1] = 0.0
     w[:, :, 0] = w[:, :, -1] = 0.0

@@ -102,3 +102,155 @@ extern "C" int host_face_pieces(const double *v12, const double *col8, int varia
     }
     return k;
 }
+
+// One 1D WENO6 evaluation of the library's difference form (fia_math.hpp weno6) at fraction t
+// from six samples, in FP32 (dbl = 0) or around an FP64 base (dbl = 1, the psi form).
+extern "C" double host_weno6(const double *f6, double t, double kappa, int dbl) {
+    const fia::WenoCoef w((float)t, (float)kappa);
+    if (dbl) return fia::weno6<double>(f6, w);
+    float f[6];
+    for (int i = 0; i < 6; ++i) f[i] = (float)f6[i];
+    return (double)fia::weno6<float>(f, w);
+}
+
+// The clipping pipeline's pieces of one face (the degenerate-face path): every (strip, band) slot
+// in the order s * (z bands) + m, as xi-eta centroid and area (0 for an empty slot).  Returns the
+// slot count, or -1 if the pipeline saw a twisted cut.
+extern "C" int host_pipeline_pieces(const double *col8, double *cen, double *area, int max_pieces) {
+    double col[4][2], v[4][3];
+    const double sq[4][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};
+    for (int c = 0; c < 4; ++c) {
+        col[c][0] = col8[2 * c]; col[c][1] = col8[2 * c + 1];
+        v[c][0] = sq[c][0]; v[c][1] = 0.0; v[c][2] = sq[c][1];
+    }
+    fia::FaceGeom g;
+    g.init(v);
+    double lx = col[0][0], hx = col[0][0], lz = col[0][1], hz = col[0][1];
+    for (int c = 1; c < 4; ++c) {
+        lx = fmin(lx, col[c][0]); hx = fmax(hx, col[c][0]);
+        lz = fmin(lz, col[c][1]); hz = fmax(hz, col[c][1]);
+    }
+    int x0, x1, z0, z1;
+    fia::interior_integers(lx, hx, x0, x1);
+    fia::interior_integers(lz, hz, z0, z1);
+    const int nxc = x1 >= x0 ? x1 - x0 + 1 : 0, nzc = z1 >= z0 ? z1 - z0 + 1 : 0;
+    int k = 0;
+    for (int s = 0; s <= nxc; ++s) {
+        for (int m = 0; m <= nzc; ++m, ++k) {
+            if (k >= max_pieces) return -2;
+            fia::Pipeline pp;
+            pp.st[0].reset(s > 0, 0, (double)(x0 + s - 1), true);
+            pp.st[1].reset(s < nxc, 0, (double)(x0 + s), false);
+            pp.st[2].reset(m > 0, 1, (double)(z0 + m - 1), true);
+            pp.st[3].reset(m < nzc, 1, (double)(z0 + m), false);
+            pp.acc.reset();
+            for (int c = 0; c < 4; ++c) {
+                fia::CutVert cv;
+                cv.xi = sq[c][0]; cv.eta = sq[c][1]; cv.cx = col[c][0]; cv.cz = col[c][1];
+                pp.push<0>(g, cv);
+            }
+            pp.close<0>(g);
+            if (pp.twisted()) return -1;
+            const double a2 = pp.acc.n >= 3 ? pp.acc.area2 : 0.0;
+            area[k] = 0.5 * a2;
+            cen[2 * k] = a2 != 0.0 ? pp.acc.mxi / (3.0 * a2) : 0.0;
+            cen[2 * k + 1] = a2 != 0.0 ? pp.acc.meta / (3.0 * a2) : 0.0;
+        }
+    }
+    return k;
+}
+
+// Piece records of one face by the clipping pipeline, as the degenerate-face kernel stores them:
+// per (strip, band) slot the centre (cells) and y vector area.  Returns the slot count.
+extern "C" int host_pipeline_records(const double *v12, const double *col8, double *rec, int max_pieces) {
+    double v[4][3], col[4][2];
+    for (int c = 0; c < 4; ++c) {
+        for (int a = 0; a < 3; ++a) v[c][a] = v12[3 * c + a];
+        col[c][0] = col8[2 * c]; col[c][1] = col8[2 * c + 1];
+    }
+    fia::FaceGeom geom;
+    geom.init(v);
+    double lx = col[0][0], hx = col[0][0], lz = col[0][1], hz = col[0][1];
+    for (int c = 1; c < 4; ++c) {
+        lx = fmin(lx, col[c][0]); hx = fmax(hx, col[c][0]);
+        lz = fmin(lz, col[c][1]); hz = fmax(hz, col[c][1]);
+    }
+    int x0, x1, z0, z1;
+    fia::interior_integers(lx, hx, x0, x1);
+    fia::interior_integers(lz, hz, z0, z1);
+    const int nxc = x1 >= x0 ? x1 - x0 + 1 : 0, nzc = z1 >= z0 ? z1 - z0 + 1 : 0;
+    int k = 0;
+    const double sq[4][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};
+    for (int s = 0; s <= nxc; ++s) {
+        for (int m = 0; m <= nzc; ++m, ++k) {
+            if (k >= max_pieces) return -2;
+            fia::Pipeline pp;
+            pp.st[0].reset(s > 0, 0, (double)(x0 + s - 1), true);
+            pp.st[1].reset(s < nxc, 0, (double)(x0 + s), false);
+            pp.st[2].reset(m > 0, 1, (double)(z0 + m - 1), true);
+            pp.st[3].reset(m < nzc, 1, (double)(z0 + m), false);
+            pp.acc.reset();
+            for (int c = 0; c < 4; ++c) {
+                fia::CutVert cv;
+                cv.xi = sq[c][0]; cv.eta = sq[c][1]; cv.cx = col[c][0]; cv.cz = col[c][1];
+                pp.push<0>(geom, cv);
+            }
+            pp.close<0>(geom);
+            double ay = 0.0, c3[3] = {0.0, 0.0, 0.0};
+            if (pp.acc.n >= 3 && pp.acc.area2 != 0.0) {
+                const double inv3a = 1.0 / (3.0 * pp.acc.area2);
+                double r[3];
+                geom.rel(pp.acc.mxi * inv3a, pp.acc.meta * inv3a, r);
+                c3[0] = geom.p0[0] + r[0]; c3[1] = geom.p0[1] + r[1]; c3[2] = geom.p0[2] + r[2];
+                ay = 0.5 * pp.acc.ay2;
+            }
+            for (int a = 0; a < 3; ++a) rec[4 * k + a] = c3[a];
+            rec[4 * k + 3] = ay;
+        }
+    }
+    return k;
+}
+
+// Piece records of one face by the boundary-segment cutter piece by piece (piece_moments, the
+// form the face kernels use for faces cut more than once along a colour): per (strip, band) slot
+// the centre (cells) and y vector area.  Returns the slot count, -1 for a twisted or degenerate
+// face (those take other cutters).
+extern "C" int host_segment_records(const double *v12, const double *col8, double *rec, int max_pieces) {
+    double v[4][3], col[4][2];
+    for (int c = 0; c < 4; ++c) {
+        for (int a = 0; a < 3; ++a) v[c][a] = v12[3 * c + a];
+        col[c][0] = col8[2 * c]; col[c][1] = col8[2 * c + 1];
+    }
+    if (fia::face_twisted(col) || fia::face_degenerate(col)) return -1;
+    fia::FaceGeom g;
+    g.init(v);
+    double lx = col[0][0], hx = col[0][0], lz = col[0][1], hz = col[0][1];
+    for (int c = 1; c < 4; ++c) {
+        lx = fmin(lx, col[c][0]); hx = fmax(hx, col[c][0]);
+        lz = fmin(lz, col[c][1]); hz = fmax(hz, col[c][1]);
+    }
+    int x0, x1, z0, z1;
+    fia::interior_integers(lx, hx, x0, x1);
+    fia::interior_integers(lz, hz, z0, z1);
+    const int nxc = x1 >= x0 ? x1 - x0 + 1 : 0, nzc = z1 >= z0 ? z1 - z0 + 1 : 0;
+    int k = 0;
+    for (int s = 0; s <= nxc; ++s) {
+        for (int m = 0; m <= nzc; ++m, ++k) {
+            if (k >= max_pieces) return -2;
+            fia::SegAcc<false> b;
+            fia::piece_moments(col, g, s > 0 ? (double)(x0 + s - 1) : -fia::kNoBound,
+                               s < nxc ? (double)(x0 + s) : fia::kNoBound, m > 0 ? (double)(z0 + m - 1) : -fia::kNoBound,
+                               m < nzc ? (double)(z0 + m) : fia::kNoBound, b);
+            piece_record(g, b.area2, b.mxi, b.meta, b.ay2, rec + 4 * k);
+        }
+    }
+    return k;
+}
+
+// how the face kernels classify a face by its corner colours: bit 0 twisted (recursive cutter),
+// bit 1 degenerate (clipping pipeline)
+extern "C" int host_face_class(const double *col8) {
+    double col[4][2];
+    for (int c = 0; c < 4; ++c) { col[c][0] = col8[2 * c]; col[c][1] = col8[2 * c + 1]; }
+    return (fia::face_twisted(col) ? 1 : 0) | (fia::face_degenerate(col) ? 2 : 0);
+}

@@ -6,35 +6,13 @@ cutter's one-pass form (all pieces of a face cut at most once per colour) is che
 piece-by-piece form and against the oracle's split of the same face (reference
 src/split_face_3D.cpp:26-92 with the recursive cutter)."""
 import ctypes
-import os
-import subprocess
 
 import numpy as np
 import pytest
 
-HERE = os.path.dirname(os.path.abspath(__file__))
-
-
 @pytest.fixture(scope="module")
-def cutter(tmp_path_factory):
-    out = str(tmp_path_factory.mktemp("cutter") / "cutter_host.so")
-    cmd = ["/opt/rocm/bin/hipcc", "-x", "c++", "-O2", "-std=c++17", "-fPIC", "-shared", "-D__HIP_PLATFORM_AMD__",
-           "-I/opt/rocm/include", os.path.join(HERE, "cutter_host.cpp"), "-o", out]
-    if not os.path.exists(cmd[0]):
-        pytest.skip("hipcc not found")
-    subprocess.run(cmd, check=True, capture_output=True)
-    L = ctypes.CDLL(out)
-    dp = ctypes.POINTER(ctypes.c_double)
-    L.host_general_cut.argtypes = [dp, ctypes.c_double, ctypes.c_int, dp, dp]
-    L.host_general_cut.restype = ctypes.c_int
-    L.host_split_coord.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
-                                   ctypes.POINTER(ctypes.c_float)]
-    L.host_split_coord.restype = ctypes.c_int
-    L.host_fastdiv_mismatches.argtypes = [ctypes.c_uint, ctypes.c_uint, ctypes.c_long]
-    L.host_fastdiv_mismatches.restype = ctypes.c_long
-    L.host_face_pieces.argtypes = [dp, dp, ctypes.c_int, dp, dp]
-    L.host_face_pieces.restype = ctypes.c_int
-    return L
+def cutter(cutter_host):
+    return cutter_host.lib
 
 
 def _cut(L, col, max_pieces=1024):
@@ -162,3 +140,52 @@ def test_gpu_step_scene_covers_both_segment_cutter_paths(oracle, scenes):
         cx, cz = quad[..., 0] - 0.5, quad[..., 2] - 0.5   # xz-CellFaceAligned colours
         fast = one_pass[fam](_cut_counts(cx.min(0), cx.max(0)), _cut_counts(cz.min(0), cz.max(0)))
         assert fast.sum() > 0 and (~fast).sum() > 0, (fam, fast.sum(), (~fast).sum())
+
+
+def _nonzero_pieces(pieces):
+    out = sorted((round(float(c[0]), 9), round(float(c[1]), 9), round(float(a), 9)) for c, a in pieces if abs(a) > 1e-12)
+    return np.array(out).reshape(-1, 3)
+
+
+def test_pipeline_on_corner_lying_on_a_cut_line(cutter_host, oracle):
+    # A face of the v-component grid at a wall row of pinned vertices (axis-aligned colours): two
+    # corners sit exactly on the z cut line and one also on the x cut line.  The crossing of the
+    # edge that ends on the x line is that corner; computed through the rounded reciprocal its
+    # weight missed 1 by an ulp, its z colour rounded below the z line, and the pipeline's loop of
+    # piece (x >= 8, z < 2) folded over itself (xi-eta area 0.787 instead of 0.287).
+    col = np.array([[8.0, 2.0], [9.0, 2.0], [8.727837562561035, 1.1455326676368713],
+                    [7.925276763737202, 2.635922431945801]])
+    cen, area = cutter_host.pipeline_pieces(col)
+    got = _nonzero_pieces(zip(cen, area))
+    assert abs(area.sum() - 1.0) < 1e-12
+    for cprec in (0, 1):
+        ref = _nonzero_pieces((c, a) for _, c, a in oracle.cut_face_xieta(col, cprec=cprec))
+        assert got.shape == ref.shape and np.abs(got - ref).max() < 1e-6
+
+
+def test_pipeline_matches_recursive_cut_on_degenerate_faces(cutter_host, oracle):
+    # corners exactly on integer colours (pinned or clamped vertices) next to moved ones: every
+    # face must tile the square, and its pieces must be those of the reference's recursive cut in
+    # one of its two arithmetics (the FP32 form of src/facecutter.h, or FP64)
+    rng = np.random.default_rng(5)
+    bad = checked = 0
+    for t in range(3000):
+        col = np.floor(rng.uniform(0, 4, size=(4, 2))) + 0.5
+        on = rng.choice(4, size=rng.integers(1, 4), replace=False)
+        col[on] = np.round(col[on] + rng.uniform(-0.5, 0.5, size=(len(on), 2)))
+        off = np.setdiff1d(np.arange(4), on)
+        col[off] += rng.uniform(-0.9, 0.9, size=(len(off), 2))
+        twisted, degenerate = cutter_host.face_class(col)
+        if twisted or not degenerate:
+            continue   # the recursive cutter or the segment cutter takes these
+        checked += 1
+        cen, area = cutter_host.pipeline_pieces(col)   # raises if a stage sees a twisted loop
+        assert abs(area.sum() - 1.0) < 1e-9, (t, col.tolist(), area.sum())
+        got = _nonzero_pieces(zip(cen, area))
+        ok = False
+        for cprec in (0, 1):
+            ref = _nonzero_pieces((c, a) for _, c, a in oracle.cut_face_xieta(col, cprec=cprec))
+            ok = ok or (got.shape == ref.shape and np.abs(got - ref).max() < 1e-5)
+        bad += not ok
+    assert checked > 1000
+    assert bad <= 3, bad

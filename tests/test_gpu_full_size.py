@@ -8,6 +8,13 @@
   with non-integral psi on uncut faces (one piece per face), and 1Dy_linear with cell-face-aligned
   colours, whose y faces are cut into 2 x 2 pieces by the one-pass segment cutter and whose piece
   values the cutter stores itself.  The GPU keeps the fields in FP32, so the bar is 1e-6 relative.
+- Fractional translation: a density linear in y moved by a uniform velocity of (1.37, 0.61, 2.29)
+  cells per step with the WENO6 integral rule (the dominant quadrature kernel, one Gauss point per
+  interval) gives the exact average of the shifted profile in the interior: the psi prefix, the
+  psi point on the face level and the midpoint of the residual integral are exact for a linear
+  profile (the oracle holds it to 1e-12 at 32^3, tests/test_oracle_closed_forms.py).  A velocity
+  component linear in y (v = V0 + b y, u and w uniform) likewise gives closed forms: every control
+  volume is moved along x and z and stretched in y.
 - Uniform velocity: self-advection of a uniform velocity moving a fraction of cells per step
   keeps the interior uniform (exact in FP64 up to rounding).
 - Projection: the multigrid-preconditioned solve with obstacles reaches its residual, and the
@@ -46,6 +53,52 @@ def test_full_size_integer_translation(fia, ctx, interp, split):
     err = np.abs(got - want).max() / np.abs(want).max()
     print("relative L-inf against the shifted field %.3e" % err, flush=True)
     assert err <= 1e-6
+
+
+def test_full_size_fractional_translation_is_exact(fia, ctx):
+    print("case full-size fractional translation, WENO6 integral", flush=True)
+    n, h = N, 1.0 / N
+    s = (1.37, 0.61, 2.29)
+    a, b = 0.8, 0.3
+    yc = (np.arange(n) + 0.5) * h
+    rho = np.ascontiguousarray(np.broadcast_to((a + b * yc)[None, :, None], (n, n, n)), dtype=np.float32)
+    u = np.full((n + 1, n, n), s[0] * h, np.float32)
+    v = np.full((n, n + 1, n), s[1] * h, np.float32)
+    w = np.full((n, n, n + 1), s[2] * h, np.float32)
+    grid = fia.Grid3D(n, n, n, h, rho.copy(), u, v, w)
+    fia.move_substances_3d(ctx, grid, 0, 1.0, split_method="xz-CellFaceAligned", interpolation_method="WENO6",
+                           use_integral=True)
+    # the velocity arrays are FP32, so the displacement is s[1] to FP32 rounding of s[1] * h
+    dy = float(np.float32(s[1] * h))
+    want = a + b * (yc - dy)
+    m = MARGIN
+    got = grid.density[m:n - m, m:n - m, m:n - m].astype(np.float64)
+    err = float(np.abs(got - want[None, m:n - m, None]).max() / np.abs(want).max())
+    print("relative L-inf against the shifted linear profile %.3e" % err, flush=True)
+    assert err <= 1e-6
+
+
+def test_full_size_velocity_linear_in_y_is_exact(fia, ctx):
+    print("case full-size velocity linear in y, WENO6 integral", flush=True)
+    n, h = N, 1.0 / N
+    V0, b, U0, W0 = 0.61, 0.5, 1.37, 2.29
+    dt = h
+    u = np.full((n + 1, n, n), U0, np.float32)
+    w = np.full((n, n, n + 1), W0, np.float32)
+    yf = np.arange(n + 1) * h
+    v = np.ascontiguousarray(np.broadcast_to((V0 + b * yf)[None, :, None], (n, n + 1, n)), dtype=np.float32)
+    grid = fia.Grid3D(n, n, n, h, None, u, v, w)
+    fia.advect_velocity_flux_advection_3d(ctx, grid, -1, dt, interpolation_method="WENO6")
+    back = lambda y: y - dt * (V0 + b * y)
+    lo, hi = back((np.arange(n + 1) - 0.5) * h), back((np.arange(n + 1) + 0.5) * h)
+    v_want = (V0 * (hi - lo) + 0.5 * b * (hi ** 2 - lo ** 2)) / h
+    m = MARGIN
+    err_v = float(np.abs(grid.v[m:-m, m:-m, m:-m].astype(np.float64) - v_want[None, m:-m, None]).max() / np.abs(v_want).max())
+    stretch = (back(h) - back(0.0)) / h
+    err_u = float(np.abs(grid.u[m:-m, m:-m, m:-m].astype(np.float64) - U0 * stretch).max() / U0)
+    err_w = float(np.abs(grid.w[m:-m, m:-m, m:-m].astype(np.float64) - W0 * stretch).max() / W0)
+    print("relative L-inf: v %.3e, u %.3e, w %.3e (stretch %.6f)" % (err_v, err_u, err_w, stretch), flush=True)
+    assert max(err_v, err_u, err_w) <= 1e-5
 
 
 def test_full_size_density_step_is_deterministic(fia, ctx, scenes):
