@@ -5,7 +5,18 @@ the unit cube, density = a Gaussian plume blob plus a sphere source, divergence-
 curl-noise velocity (wavenumbers 1-4) scaled so that dt = 4 dx / max|u| (CFL 4), WENO6, integral
 psi (Gauss n=1), zero-velocity correction on.  Fields stay resident on the GPU.
 
-  density     (default) one step = one move_substances_3D (xz-CellFaceAligned): 3 flux passes
+  plume       (default) one step = one step of the simulation loop of the reference's main loop
+              (src/main.cpp:596-740) on that scene with 8 seeded spherical obstacles (radius
+              0.03-0.08) as a solid mask, run natively by fia_scenes_step on one context: dt from
+              CFL 4 (no movie-frame clipping), density source and buoyancy 1.0 on v, velocity
+              self-advection (u, v, w; WENO6 integral, 3 passes each), pressure projection with
+              the obstacles to 1e-6 relative residual from a zero guess by Jacobi-PCG (the plain
+              PCG of the configuration), density advection (3 passes).  The simulation runs on:
+              each timed step continues the last.  Beside it (same line): the same steps under
+              the multigrid-preconditioned CG, the learned warm start against plain PCG at
+              128^3 (step ms of NN-warm-PCG vs PCG at 1e-6), and the mass change of one density
+              step on the final state.
+  density     one step = one move_substances_3D (xz-CellFaceAligned): 3 flux passes
   velocity    one step = advect_velocity_xyz_flux_advection_3D (u, v, w, 3 passes each)
   projection  one step = one pressure projection (Jacobi-PCG to 1e-6 relative residual, zero
               initial guess) of the curl-noise velocity plus a buoyancy impulse, 8 spherical
@@ -21,10 +32,12 @@ psi (Gauss n=1), zero-velocity correction on.  Fields stay resident on the GPU.
               velocity: from a zero guess and with the learned U-Net warm start (committed
               weights), both to 1e-6; value and ms_per_step are the warm-started projection's
 
-Prints one JSON line: value = cell-steps per second / 1e6 over the timed steps; for the
-advection workloads a roofline entry for the WENO6 quadrature kernel measured with HIP events on
-the library's stream, and (density, unless --no-cpu-baseline) a CPU baseline: the oracle's FP32
-restatement of the reference path, multi-threaded, on a bounded 128^3 sample.
+Prints one JSON line: value = cell-steps per second / 1e6 over the timed steps; a roofline entry
+for the dominant kernel measured with HIP events on the library's stream (plume: the CG iteration
+of the pressure solve, HBM; density, velocity: the WENO6 quadrature kernel, FP32 VALU), and
+(plume and density, unless --no-cpu-baseline) a CPU baseline: the oracle's restatement of the
+reference path, multi-threaded, on a bounded sample of the same workload (plume: one whole step at
+--cpu-n, 64^3 by default; density: one density step at 128^3).
 """
 import argparse
 import importlib
@@ -39,7 +52,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "flux-advection Mcell-steps/s at 256^3; step ms NN-warm-PCG vs PCG at 1e-6 res"
-PEAK_FP32_TFLOPS = 157.3
+PEAK_FP32_TFLOPS = 157.3          # published FP32 vector rate (packed FP32 instructions)
+PEAK_FP32_NONPACKED_TFLOPS = 78.6  # one FP32 FMA per lane and cycle: this build emits no packed FP32
 # Work the quadrature kernel executes: its 1D WENO6 evaluations, counted on the device, at 73 flop
 # each (an FMA counts 2, the reciprocal 1): the arithmetic of one evaluation in the compiled kernel
 # (five sample differences, second and third differences, three candidates, the common squared
@@ -97,7 +111,7 @@ def cpu_baseline(scenes, n, cfl, threads):
 def quad_roofline(st, steps, n, cfl, density):
     """Roofline entry of the WENO6 integral quadrature kernel (piece_kernel<1>) from the
     library's HIP-event spans and device work counters of the timed steps.  achieved counts the
-    flops the kernel executes (its 1D WENO6 evaluations x 100); the reference rule's count
+    flops the kernel executes (its 1D WENO6 evaluations x 73); the reference rule's count
     (8,600 flop per 3D point) is reported beside it: the kernel shares the z pass between the
     points of a piece and needs no y pass for psi on a face level, so that count over the
     kernel's time is a throughput of the reference's work, not a fraction of the hardware."""
@@ -108,8 +122,10 @@ def quad_roofline(st, steps, n, cfl, density):
     rule = points * FLOP_PER_WENO3D_POINT / quad_s / 1e12 if quad_s > 0 else 0.0
     traffic = TRAFFIC_BYTES_256_CFL4 if (density and n == 256 and cfl == 4.0) else None
     avg_quad_s = quad_s / qlaunches
-    return {"bound": "mfma", "achieved": round(executed, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+    return {"bound": "valu", "achieved": round(executed, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
             "frac": round(executed / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
+            "peak_nonpacked": PEAK_FP32_NONPACKED_TFLOPS,
+            "frac_nonpacked": round(executed / PEAK_FP32_NONPACKED_TFLOPS, 4),
             "kernel": "piece_kernel<1> (WENO6 integral quadrature)", "compute": "FP32 VALU, no MFMA",
             "avg_launch_ms": round(st["quad_ms"] / qlaunches, 3),
             "weno6_evals_per_launch": st["weno_evals"] / qlaunches, "flop_per_weno6": FLOP_PER_WENO6_EXECUTED,
@@ -187,6 +203,206 @@ def run_advection(args, fia, scenes, torch, velocity):
         out["cpu_baseline"] = cpu_baseline(scenes, args.cpu_n, args.cfl, min(os.cpu_count() or 1, 16))
     ctx.close()
     return out
+
+
+def plume_scene(scenes, n, seed=1):
+    """The 256^3 plume of the measured configuration: blob + source density, curl-noise velocity
+    (max |u| = 1), 8 seeded spherical obstacles kept clear of the source."""
+    h = 1.0 / n
+    source = (0.5, 0.1, 0.5)
+    rho = scenes.gaussian_blob(n, centre=(0.5, 0.3, 0.5), sigma=0.1) + scenes.sphere_source(n, source, 0.06)
+    u, v, w = scenes.curl_noise(n, seed=seed)
+    clear = scenes.sphere_source(n, source, 0.14)
+    solid = scenes.spherical_obstacles(n, count=8, rmin=0.03, rmax=0.08, seed=3, keep_clear=clear)
+    f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)
+    return h, source, f32(rho), f32(u), f32(v), f32(w), np.ascontiguousarray(solid, dtype=np.uint8)
+
+
+class PlumeRun:
+    """One 256^3 plume scene stepped by fia_scenes_step on one context."""
+
+    def __init__(self, fia, torch, n, state, precond, cfl):
+        h, source, rho, u, v, w, solid = state
+        dev = torch.device("cuda:0")
+        T = lambda a: torch.from_numpy(a).to(dev)
+        self.fia, self.torch, self.n = fia, torch, n
+        self.t = {"rho": T(rho), "u": T(u), "v": T(v), "w": T(w), "solid": T(solid)}
+        self.ctx = fia.Context(0)
+        self.ctx.set_pressure_preconditioner(precond)
+        self.scenes = (fia.SceneC * 1)()
+        sc = self.scenes[0]
+        t = self.t
+        sc.grid = fia.GridC(n, n, n, h, t["rho"].data_ptr(), t["u"].data_ptr(), t["v"].data_ptr(), t["w"].data_ptr(), 1)
+        sc.solid = t["solid"].data_ptr()
+        sc.pressure = None
+        sc.source_x, sc.source_y, sc.source_z = source
+        sc.source_radius, sc.source_value, sc.buoyancy = 0.06, 1.0, 1.0
+        sc.cfl = int(cfl)
+        sc.movie_fps = 1e-3   # frames 1000 s apart: dt is the CFL step every time
+        self.od = fia.make_opts(split_method="xz-CellFaceAligned", interpolation_method="WENO6", use_integral=True,
+                                num_gauss_quadrature_point_for_integrate_density=1)
+        self.ov = fia.make_opts(interpolation_method="WENO6", use_integral=True,
+                                num_gauss_quadrature_point_for_integrate_density=1)
+        self.iters, self.res, self.dts = [], [], []
+
+    def step(self):
+        self.fia.scenes_step([self.ctx], self.scenes, self.od, self.ov, rel_tol=1e-6, max_iters=20000)
+        sc = self.scenes[0]
+        self.iters.append(int(sc.pcg_iters)); self.res.append(float(sc.pcg_residual)); self.dts.append(float(sc.dt))
+
+    def timed(self, steps, warmup, log):
+        torch = self.torch
+        for i in range(warmup):
+            self.step()
+            log("warm-up step %d: %d CG iterations" % (i, self.iters[-1]))
+        self.iters, self.res, self.dts = [], [], []
+        torch.cuda.synchronize()
+        self.ctx.set_timing(True)
+        self.ctx.reset_stats()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(steps):
+            self.step()
+            log("timed step %d: %d CG iterations, residual %.2e" % (i, self.iters[-1], self.res[-1]))
+        torch.cuda.synchronize()
+        sec = time.perf_counter() - t0
+        st = self.ctx.stats()
+        self.ctx.set_timing(False)
+        return sec, st
+
+    def grid(self):
+        t = self.t
+        return self.fia.Grid3D(self.n, self.n, self.n, 1.0 / self.n, t["rho"], t["u"], t["v"], t["w"])
+
+    def close(self):
+        self.ctx.close()
+
+
+def pcg_roofline(st, n, precond):
+    """Roofline entry of the CG iteration from the library's HIP-event spans around its launches:
+    algorithmic bytes of one iteration over all cells, divided by the measured time per iteration."""
+    per_cell = PCG_BYTES_PER_CELL_ITER + (MG_BYTES_PER_CELL_VCYCLE if precond == "mg" else 0)
+    iters = max(st["pcg_iters"], 1)
+    ms_iter = st["pcg_ms"] / iters
+    gbs = per_cell * n ** 3 / (ms_iter / 1e3) / 1e9 if ms_iter > 0 else 0.0
+    kernels = ("pcg_direction_kernel + pcg_update_kernel<true> (Jacobi-PCG iteration, FP64 vectors)"
+               if precond == "jacobi" else "multigrid-preconditioned CG iteration (direction, update, V-cycle)")
+    return {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": None, "kernel": kernels,
+            "bytes_per_cell_iteration": per_cell, "avg_iteration_ms": round(ms_iter, 4),
+            "iterations": st["pcg_iters"], "solves": st["pcg_solves"]}
+
+
+def plume_cpu_baseline(scenes, n, cfl, threads):
+    """The oracle's restatement of one whole plume step at n^3 (velocity self-advection, pressure
+    projection with the obstacles, density advection; FP32 fields, per-cell faces as the reference
+    has them, std::thread over x-slabs; the CG of the oracle runs on one thread)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    O.lib()
+    h, source, rho, u, v, w, solid = plume_scene(scenes, n)
+    dt = cfl * h / max(abs(u).max(), abs(v).max(), abs(w).max())
+    t0 = time.perf_counter()
+    ua, va, wa = O.advect_velocity(u, v, w, h, dt, method="WENO6", prec=1, cprec=1, nthreads=threads)
+    ub, vb, wb, p, it, res, rc = O.pressure_project(ua, va, wa, h, solid=solid, tol=1e-6, max_iters=20000)
+    O.move_substances(rho.astype(np.float64), ub, vb, wb, h, dt, split="xz-CellFaceAligned", method="WENO6",
+                      prec=1, cprec=1, nthreads=threads)
+    sec = time.perf_counter() - t0
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), None)
+    except OSError:
+        pass
+    return {"value": round(n ** 3 / sec / 1e6, 5), "unit": "Mcell-steps/s", "cores": threads, "kind": "port",
+            "host": {"cpu_model": model, "logical_cpus_visible": os.cpu_count(), "threads_used": threads},
+            "sample": "one whole plume step at %d^3 (velocity self-advection, CG projection to 1e-6 with the 8 "
+                      "obstacles: %d iterations, density advection; CFL %g), the oracle's FP32 restatement of the "
+                      "reference path (per-cell faces, std::thread over x-slabs), %.1f s" % (n, it, cfl, sec)}
+
+
+def run_plume(args, fia, scenes, torch):
+    n = args.n
+    log = (lambda s: print(s, file=sys.stderr, flush=True)) if args.verbose else (lambda s: None)
+    log("plume %d^3: building the scene" % n)
+    state = plume_scene(scenes, n)
+    run = PlumeRun(fia, torch, n, state, "jacobi", args.cfl)
+    sec, st = run.timed(args.steps, args.warmup, log)
+    mass0 = fia.grid_totals(run.ctx, run.grid())[0]
+    main = {"iters": list(run.iters), "res": list(run.res), "dts": list(run.dts)}
+    # the mass change of one density step on the final state (no source, no forcing)
+    g = run.grid()
+    before = float(g.density.double().sum().item())
+    fia.move_substances_3d(run.ctx, g, 0, run.dts[-1], split_method="xz-CellFaceAligned",
+                           interpolation_method="WENO6")
+    torch.cuda.synchronize()
+    after = float(g.density.double().sum().item())
+    run.close()
+    del run
+    torch.cuda.empty_cache()
+    log("multigrid variant")
+    mg = PlumeRun(fia, torch, n, state, "mg", args.cfl)
+    sec_mg, st_mg = mg.timed(args.steps, args.warmup, log)
+    mg_iters = list(mg.iters)
+    mg.close()
+    del mg
+    torch.cuda.empty_cache()
+    out = {
+        "metric": METRIC, "value": round(n ** 3 * args.steps / sec / 1e6, 3), "unit": "Mcell-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(sec / args.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 advection, f64 solve",
+        "data": "synthetic",
+        "config": {"workload": "plume simulation step, %d^3 (blob + sphere source, curl-noise start, buoyancy 1.0, "
+                               "8 spherical obstacles r 0.03-0.08), CFL %g: velocity self-advection and density "
+                               "advection (WENO6, integral psi, xz-CellFaceAligned density split, zero-velocity "
+                               "correction on), Jacobi-PCG projection to 1e-6 from a zero guess" % (n, args.cfl),
+                   "grid": [n, n, n], "cfl": args.cfl, "preconditioner": "jacobi",
+                   "piece_rule": "the reference's: one psi point per piece plus 1-point Gauss along y for the "
+                                 "density term; the 4x4 per-piece Gauss rule the configuration names is not built"},
+        "roofline": pcg_roofline(st, n, "jacobi"),
+        "pcg": {"iterations": main["iters"], "max_rel_residual": max(main["res"]),
+                "dt": [round(d, 7) for d in main["dts"]]},
+        "kernels_ms_per_step": {"pcg_ms": round(st["pcg_ms"] / args.steps, 3),
+                                **{k: round(st[k] / args.steps, 3)
+                                   for k in ("face_ms", "quad_ms", "vertex_ms", "psi_ms", "gather_ms")}},
+        "roofline_advection": quad_roofline(st, args.steps, n, args.cfl, False),
+        "multigrid": {"ms_per_step": round(sec_mg / args.steps * 1e3, 3),
+                      "value": round(n ** 3 * args.steps / sec_mg / 1e6, 3), "pcg_iterations": mg_iters,
+                      "pcg_ms_per_step": round(st_mg["pcg_ms"] / args.steps, 3),
+                      "roofline": pcg_roofline(st_mg, n, "mg")},
+        "mass": {"total_after_timed_steps": mass0,
+                 "density_step_rel_change": (after - before) / before,
+                 "note": "one density step on the final state, no source: the change is the scheme's wall "
+                         "terms (moving wall faces sample the top-level psi at shifted points; the FP64 oracle "
+                         "shows the same change, tests/test_gpu_c1.py), interior faces cancel exactly"},
+    }
+    if not args.no_warmstart:
+        log("warm start at 128^3")
+        out["warm_start"] = warmstart_summary(args, fia, scenes, torch)
+    if not args.no_cpu_baseline:
+        log("CPU baseline")
+        out["cpu_baseline"] = plume_cpu_baseline(scenes, args.cpu_n, args.cfl, min(os.cpu_count() or 1, 16))
+    return out
+
+
+def warmstart_summary(args, fia, scenes, torch, n=128, steps=4):
+    """NN-warm-PCG vs PCG step ms at 1e-6 on held-out plume scene 5000 (128^3, Jacobi-PCG, the
+    solve the network was trained to start)."""
+    learned = importlib.import_module("flux-interpolated-advection_amd.learned")
+    if not os.path.exists(args.weights):
+        return None
+    ctx = fia.Context(0)
+    ctx.set_pressure_preconditioner("jacobi")
+    net = fia.UNet(ctx, seed=0)
+    net.load(args.weights)
+    rows = learned.evaluate(fia, scenes, ctx, net, n, [5000], 1 + steps, log=lambda s: None)[1:]
+    s = learned.summary(rows)
+    net.close()
+    ctx.close()
+    return {"grid": [n, n, n], "solver": "Jacobi-PCG to 1e-6", "plain_ms_per_step": round(s["plain_ms"], 3),
+            "learned_ms_per_step": round(s["learned_ms"], 3), "speedup": round(s["plain_ms"] / s["learned_ms"], 4),
+            "plain_iters": s["plain_iters"], "learned_iters": s["learned_iters"],
+            "network_used": s["network_used"], "projections": s["projections"]}
 
 
 def run_projection(args, fia, scenes, torch):
@@ -332,8 +548,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=None, help="grid edge (256; warmstart: 128)")
     ap.add_argument("--cfl", type=float, default=4.0)
-    ap.add_argument("--workload", choices=("density", "velocity", "projection", "warmstart", "scenes"),
-                    default="density")
+    ap.add_argument("--workload", choices=("plume", "density", "velocity", "projection", "warmstart", "scenes"),
+                    default="plume")
+    ap.add_argument("--verbose", action="store_true", help="progress lines on stderr")
+    ap.add_argument("--no-warmstart", action="store_true", help="plume: skip the 128^3 warm-start comparison")
     ap.add_argument("--scenes", type=int, default=64)
     ap.add_argument("--plain", action="store_true", help="scenes: plain PCG instead of the warm start")
     ap.add_argument("--batched", action="store_true", help="scenes: the pressure solves of all scenes batched")
@@ -344,7 +562,7 @@ def main():
     ap.add_argument("--weights", default=os.path.join(ROOT, "flux-interpolated-advection_amd", "weights",
                                                       "unet_plume128.npy"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-n", type=int, default=128)
+    ap.add_argument("--cpu-n", type=int, default=None, help="CPU baseline grid (plume: 64, density: 128)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one plain device copy of 1 GiB (a known-size kernel for HBM counter scaling)")
     args = ap.parse_args()
@@ -352,6 +570,8 @@ def main():
         args.n = 128 if args.workload == "warmstart" else 256
     if args.precond is None:
         args.precond = "jacobi" if args.workload == "warmstart" else "mg"
+    if args.cpu_n is None:
+        args.cpu_n = 64 if args.workload == "plume" else 128
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the library has no multi-GPU path (one GPU's share of a "
               "sharded grid is the single-GPU workload)", file=sys.stderr)
@@ -372,6 +592,8 @@ def main():
         out = run_warmstart(args, fia, scenes, torch)
     elif args.workload == "scenes":
         out = run_scenes(args, fia, torch)
+    elif args.workload == "plume":
+        out = run_plume(args, fia, scenes, torch)
     else:
         out = run_advection(args, fia, scenes, torch, args.workload == "velocity")
     print(json.dumps(out))

@@ -192,7 +192,7 @@ size_t ctx_bytes(const fia_ctx *c) {
 
 hipError_t set_device(fia_ctx *c) { return hipSetDevice(c->device); }
 
-enum SpanKind { SK_FACE = 0, SK_VERTEX = 1, SK_PSI = 2, SK_GATHER = 3, SK_QUAD = 4 };
+enum SpanKind { SK_FACE = 0, SK_VERTEX = 1, SK_PSI = 2, SK_GATHER = 3, SK_QUAD = 4, SK_PCG = 5 };
 
 // Opens a timed span on the context's stream (no-op unless timing is enabled); idx receives the
 // span's index for span_end (spans may nest).
@@ -656,14 +656,24 @@ static fia_status learned_start(fia_ctx *c, fia_unet *u, const PcgGrid &pg, PcgS
 // convergence, so batches of them are enqueued between polls: 32 for Jacobi (thousands of
 // iterations), 8 under multigrid (tens of iterations of ~19 launches, most of a 32-batch after
 // convergence would be launches that return at once).
+// With timing on, each batch of iteration launches is a timed span (the polls between batches are
+// host round trips with the stream idle, outside the spans).
 static fia_status pcg_run(fia_ctx *c, PcgState &ps, int max_iters, double *hs, const MgHier *mg) {
     const int poll = mg ? 8 : 32;
+    const int it0 = ps.it;
     while (true) {
         HIP_TRY(launch_pcg_check(ps, c->stream));
         HIP_TRY(hipMemcpyAsync(hs, ps.scal, kPcgScalars * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(hipStreamSynchronize(c->stream));
         if (hs[kPcgDone] != 0.0 || ps.it >= max_iters) break;
+        int sp;
+        HIP_TRY(span_begin(c, SK_PCG, sp));
         HIP_TRY(launch_pcg_iterations(ps, std::min(poll, max_iters - ps.it), c->stream, mg));
+        HIP_TRY(span_end(c, sp));
+    }
+    if (c->timing) {
+        c->acc.pcg_iters += std::max(0LL, (long long)hs[kPcgIters] - (long long)it0);
+        c->acc.pcg_solves += 1;
     }
     return FIA_OK;
 }
@@ -998,6 +1008,7 @@ fia_status fia_ctx_get_stats(fia_ctx *c, fia_kernel_stats *out) {
         else if (sp.kind == SK_VERTEX) c->acc.vertex_ms += ms;
         else if (sp.kind == SK_PSI) c->acc.psi_ms += ms;
         else if (sp.kind == SK_QUAD) c->acc.quad_ms += ms;
+        else if (sp.kind == SK_PCG) c->acc.pcg_ms += ms;
         else c->acc.gather_ms += ms;
         (void)hipEventDestroy(sp.a);
         (void)hipEventDestroy(sp.b);

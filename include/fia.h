@@ -226,6 +226,9 @@ typedef struct fia_kernel_stats {
     long long weno_evals;        /* 1D WENO6 evaluations of the WENO6 integral quadrature kernel   */
     double quad_ms;              /* summed time of the WENO6 integral quadrature launches alone    */
     long long quad_launches;     /* number of those launches                                       */
+    double pcg_ms;               /* summed time of the pressure solves' CG iteration launches       */
+    long long pcg_iters;         /* CG iterations those solves ran                                  */
+    long long pcg_solves;        /* pressure solves                                                 */
 } fia_kernel_stats;
 
 fia_status fia_ctx_set_timing(fia_ctx *ctx, int enable);
