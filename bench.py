@@ -340,13 +340,14 @@ def run_plume(args, fia, scenes, torch):
     run.close()
     del run
     torch.cuda.empty_cache()
-    log("multigrid variant")
-    mg = PlumeRun(fia, torch, n, state, "mg", args.cfl)
-    sec_mg, st_mg = mg.timed(args.steps, args.warmup, log)
-    mg_iters = list(mg.iters)
-    mg.close()
-    del mg
-    torch.cuda.empty_cache()
+    if not args.no_multigrid:
+        log("multigrid variant")
+        mg = PlumeRun(fia, torch, n, state, "mg", args.cfl)
+        sec_mg, st_mg = mg.timed(args.steps, args.warmup, log)
+        mg_iters = list(mg.iters)
+        mg.close()
+        del mg
+        torch.cuda.empty_cache()
     out = {
         "metric": METRIC, "value": round(n ** 3 * args.steps / sec / 1e6, 3), "unit": "Mcell-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(sec / args.steps * 1e3, 3),
@@ -366,10 +367,10 @@ def run_plume(args, fia, scenes, torch):
                                 **{k: round(st[k] / args.steps, 3)
                                    for k in ("face_ms", "quad_ms", "vertex_ms", "psi_ms", "gather_ms")}},
         "roofline_advection": quad_roofline(st, args.steps, n, args.cfl, False),
-        "multigrid": {"ms_per_step": round(sec_mg / args.steps * 1e3, 3),
-                      "value": round(n ** 3 * args.steps / sec_mg / 1e6, 3), "pcg_iterations": mg_iters,
-                      "pcg_ms_per_step": round(st_mg["pcg_ms"] / args.steps, 3),
-                      "roofline": pcg_roofline(st_mg, n, "mg")},
+        "multigrid": None if args.no_multigrid else {
+            "ms_per_step": round(sec_mg / args.steps * 1e3, 3), "value": round(n ** 3 * args.steps / sec_mg / 1e6, 3),
+            "pcg_iterations": mg_iters, "pcg_ms_per_step": round(st_mg["pcg_ms"] / args.steps, 3),
+            "roofline": pcg_roofline(st_mg, n, "mg")},
         "mass": {"total_after_timed_steps": mass0,
                  "density_step_rel_change": (after - before) / before,
                  "note": "one density step on the final state, no source: the change is the scheme's wall "
@@ -552,6 +553,7 @@ def main():
                     default="plume")
     ap.add_argument("--verbose", action="store_true", help="progress lines on stderr")
     ap.add_argument("--no-warmstart", action="store_true", help="plume: skip the 128^3 warm-start comparison")
+    ap.add_argument("--no-multigrid", action="store_true", help="plume: skip the multigrid-preconditioned variant")
     ap.add_argument("--scenes", type=int, default=64)
     ap.add_argument("--plain", action="store_true", help="scenes: plain PCG instead of the warm start")
     ap.add_argument("--batched", action="store_true", help="scenes: the pressure solves of all scenes batched")
