@@ -40,7 +40,7 @@ EXPORTS = ["fia_default_opts", "fia_ctx_create", "fia_ctx_create_on_stream", "fi
            "fia_pressure_project", "fia_ctx_set_timing", "fia_ctx_get_stats", "fia_ctx_reset_stats",
            "fia_unet_param_count", "fia_unet_create", "fia_unet_destroy", "fia_unet_set_params",
            "fia_unet_get_params", "fia_unet_forward", "fia_unet_loss_grad", "fia_unet_train_step", "fia_pressure_rhs",
-           "fia_pressure_project_learned", "fia_grid_totals", "fia_wall_momentum_loss", "fia_scene_forcing",
+           "fia_pressure_project_learned", "fia_grid_totals", "fia_grid_totals_noalloc", "fia_wall_momentum_loss", "fia_scene_forcing",
            "fia_scenes_step", "fia_scenes_step_batched", "fia_ctx_set_pressure_preconditioner"]
 
 
@@ -149,6 +149,7 @@ def lib():
         L.fia_pressure_rhs.argtypes = [_vp, ctypes.POINTER(GridC), _vp, _vp]
         L.fia_grid_totals.argtypes = [_vp, ctypes.POINTER(GridC), ctypes.POINTER(_d * 4)]
         L.fia_wall_momentum_loss.argtypes = [_vp, _i, _i, _i, _d, _vp, _i, ctypes.POINTER(_d * 3)]
+        L.fia_grid_totals_noalloc.argtypes = [_vp, ctypes.POINTER(GridC), ctypes.POINTER(_d * 4)]
         L.fia_scene_forcing.argtypes = [_vp, ctypes.POINTER(SceneC), _d]
         L.fia_scenes_step.argtypes = [ctypes.POINTER(_vp), _i, ctypes.POINTER(SceneC), _i, ctypes.POINTER(AdvectOpts),
                                       ctypes.POINTER(AdvectOpts), _d, _i, ctypes.POINTER(_vp)]
@@ -556,6 +557,15 @@ def grid_totals(ctx, grid):
     g = grid._c(with_density=grid.density is not None)
     out = (_d * 4)()
     _check(lib().fia_grid_totals(ctx._h, ctypes.byref(g), ctypes.byref(out)))
+    return tuple(out)
+
+
+def grid_totals_noalloc(ctx, grid):
+    """grid_totals of a device grid without any allocation (fia_grid_totals_noalloc): the same
+    sums in a fixed device scratch; calls overlapping on one GPU race."""
+    g = grid._c(with_density=grid.density is not None)
+    out = (_d * 4)()
+    _check(lib().fia_grid_totals_noalloc(ctx._h, ctypes.byref(g), ctypes.byref(out)))
     return tuple(out)
 
 

@@ -914,6 +914,11 @@ static fia_status ctx_create(int device, hipStream_t stream, fia_ctx **out) {
         delete c;
         return fail(FIA_ERR_DEVICE, "cannot allocate pinned host memory");
     }
+    if (io_scratch_init(dev) != hipSuccess) {
+        (void)hipHostFree(c->host_bounds); (void)hipHostFree(c->host_flags); (void)hipHostFree(c->host_scal);
+        drop_stream(); delete c;
+        return fail(FIA_ERR_DEVICE, "cannot find the device scratch of the allocation-free entry points");
+    }
     memset(c->host_bounds, 0, 4 * sizeof(unsigned long long));
     memset(c->host_flags, 0, 4 * sizeof(int));
     memset(c->host_scal, 0, 2 * kPcgScalars * sizeof(double));
@@ -1552,6 +1557,30 @@ fia_status fia_grid_totals(fia_ctx *c, fia_grid3d *g, double totals[4]) {
     HIP_TRY(c->genD[1].ensure(io_partial_words()));
     double *out = c->genD[1].p + io_partial_words() - 8;
     HIP_TRY(launch_field_sums(f, n, c->genD[1].p, out, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->host_scal, out, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const double vol = g->dx * g->dx * g->dx;
+    for (int k = 0; k < 4; ++k) totals[k] = c->host_scal[k] * vol;
+    return FIA_OK;
+}
+
+fia_status fia_grid_totals_noalloc(fia_ctx *c, const fia_grid3d *g, double totals[4]) {
+    if (!c || !totals) return fail(FIA_ERR_INVALID_ARG, "NULL argument");
+    fia_status st;
+    if ((st = check_grid(g, false)) != FIA_OK) return st;
+    if (!g->on_device)
+        return fail(FIA_ERR_INVALID_ARG, "fia_grid_totals_noalloc needs device fields (on_device = 1): host fields "
+                                         "would need device copies");
+    HIP_TRY(set_device(c));
+    double *part = io_scratch(c->device);
+    if (!part) return fail(FIA_ERR_DEVICE, "the device scratch of this GPU was not set up");
+    const size_t nc = (size_t)g->nx * g->ny * g->nz;
+    const float *f[4] = {g->density, g->u, g->v, g->w};
+    const size_t n[4] = {nc, (size_t)(g->nx + 1) * g->ny * g->nz, (size_t)g->nx * (g->ny + 1) * g->nz,
+                         (size_t)g->nx * g->ny * (g->nz + 1)};
+    double *out = part + io_partial_words() - 8;
+    // every partial and result is written by the call itself: nothing in the scratch needs clearing
+    HIP_TRY(launch_field_sums(f, n, part, out, c->stream));
     HIP_TRY(hipMemcpyAsync(c->host_scal, out, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     const double vol = g->dx * g->dx * g->dx;

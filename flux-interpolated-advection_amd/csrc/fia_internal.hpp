@@ -300,6 +300,10 @@ fia_status scenes_solve_run(fia_ctx *main, fia_scene *scenes, const int *idx, in
 // diagnostics (fia_io.hip): FP64 sums of up to four fields (out[0..3]) and of the pressure over
 // the six boundary layers of cells (out[0..5]); part holds io_partial_words() doubles
 size_t io_partial_words();
+// the allocation-free totals' fixed device scratch (io_partial_words() doubles): its address on
+// `device` looked up once with that device current (context creation), and that address
+hipError_t io_scratch_init(int device);
+double *io_scratch(int device);
 hipError_t launch_field_sums(const float *const fields[4], const size_t sizes[4], double *part, double *out,
                              hipStream_t s);
 hipError_t launch_wall_sums(const float *p, int nx, int ny, int nz, double *part, double *out, hipStream_t s);

@@ -2,6 +2,8 @@
 // grid (reference src/main.cpp:518-597, written to the *.dat series) and the momentum the walls
 // take out through the pressure at each projection (reference src/calc_pressure_3d.cpp:235-266).
 // FP64 sums over FP32 fields in a fixed order (per-block partials, then one block).
+#include <mutex>
+
 #include "fia_internal.hpp"
 #include "fia_math.hpp"
 
@@ -63,6 +65,34 @@ __global__ void finish_sums_kernel(const double *part, int sets, double *out) {
 }
 
 size_t io_partial_words() { return 6 * (size_t)kIBlocks + 8; }
+
+// Fixed device scratch of the allocation-free totals (fia_grid_totals_noalloc): the partial sums
+// and the four results of one call, io_partial_words() doubles (12,352 bytes), held by every GPU
+// the library runs on until the process ends.  In host code `scratch` only stands for the symbol:
+// kernels and copies use the device address io_scratch() returns for the current device.
+static __device__ __align__(256) double scratch[6 * kIBlocks + 8];
+
+constexpr int kMaxDevices = 64;
+static double *g_scratch_ptr[kMaxDevices];
+static std::mutex g_scratch_mutex;
+
+// Looks up the scratch array's address on `device` (which must be current) once and keeps it.
+hipError_t io_scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = (double *)ptr;
+    return hipSuccess;
+}
+
+double *io_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    return g_scratch_ptr[device];
+}
 
 hipError_t launch_field_sums(const float *const fields[4], const size_t sizes[4], double *part, double *out,
                              hipStream_t s) {

@@ -299,6 +299,16 @@ fia_status fia_pressure_project_learned(fia_ctx *ctx, fia_grid3d *grid, const ui
  * mass sum(rho) dx^3 (0 when grid->density is NULL), totals[1..3] = total momenta sum(u) dx^3,
  * sum(v) dx^3, sum(w) dx^3; FP64 sums in a fixed order. */
 fia_status fia_grid_totals(fia_ctx *ctx, fia_grid3d *grid, double totals[4]);
+
+/* fia_grid_totals without any allocation.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops, hosts
+ * that forbid it).  It computes the same sums with the same kernels (bit-identical totals) into a
+ * fixed scratch of 12,352 bytes that every GPU the library runs on holds until the process ends,
+ * set up when the first context on that GPU is created.  The grid's fields must be on the device
+ * (on_device = 1; FIA_ERR_INVALID_ARG otherwise); any grid size is accepted, the scratch holds a
+ * fixed number of partial sums.  Calls that overlap on one GPU share that scratch and race: do not
+ * run two at once on the same GPU (on any contexts or streams). */
+fia_status fia_grid_totals_noalloc(fia_ctx *ctx, const fia_grid3d *grid, double totals[4]);
 /* Momentum the walls take out through the pressure of one projection (src/calc_pressure_3d.cpp:
  * 235-266): loss[0] = dx^2 (sum of p over the cells at x = nx-1 - sum over x = 0), likewise y, z.
  * The reference adds it to Grid_3D::total_loss_of_momentum_{x,y,z} after every projection. */
