@@ -1,11 +1,17 @@
 // fia_kernels.hip - gfx950 kernels of the flux-interpolated advection step.
 //
 // One flux pass over a component grid G (density, or one staggered velocity component):
-//   psi_kernel       y-prefix psi on the y-faces of G, one thread per (x,z) column, FP64 sums
-//   vertex_kernel    backtraced displacement of every lattice vertex, one thread per vertex
-//   face_kernel      one thread per unique face: FP64 cut into pieces + quadrature -> face flux
-//   general_kernel   the exact recursive cutter for the few twisted faces (grid-stride, one wave per SIMD)
-//   gather_kernel    per cell: sum of its six signed face fluxes (deterministic, no atomics)
+//   psi_kernel        y-prefix psi on the y-faces of G: 64 z-columns of one x plane per block, the
+//                     column's y range in eight segments (one wave each), FP64 sums
+//   vertex_kernel     backtraced displacement of every lattice vertex, one thread per vertex
+//   face_count_kernel slots of every face (x strips x z bands); twisted and degenerate faces listed
+//   scan_*            exclusive scan of the slot counts (three kernels over tiles)
+//   seg_face_kernel   boundary-segment cutter, one thread per face (multi-cut faces piece by piece)
+//   degenerate_kernel clipping pipeline for faces with a corner colour on an interior integer
+//   piece_kernel      quadrature of every piece slot (WENO6 integral rule: piece_kernel<1>)
+//   face_sum_kernel   face flux = ordered sum of its slots
+//   general_kernel    the exact recursive cutter for the few twisted faces (grid-stride)
+//   gather_kernel     per cell: sum of its six signed face fluxes (deterministic, no atomics)
 // Each interior face is computed once; its flux enters the two neighbouring cells with opposite
 // signs, so nothing is lost or created inside the domain.
 #include "fia_cutter.hpp"
@@ -39,43 +45,139 @@ FIA_HD void atomic_cas_i32(int *p, int cmp, int v) {
 // ---------------------------------------------------------------------------------------------
 // psi (reference src/calc_psi_3D.cpp:10-308 density, :311-537 velocity)
 // ---------------------------------------------------------------------------------------------
-FIA_HD void psi_body(const PsiParams &P, long col) {
-    const int mx = P.g.m[0], my = P.g.m[1], mz = P.g.m[2];
-    const int i = (int)(col / mz), k = (int)(col % mz);
-    const Loader<float> Q{P.q, mx, my, mz};
-    double acc = 0.0;
-    P.psi[idx3(i, 0, k, my + 1, mz)] = 0.0;
-    for (int j = 1; j <= my; ++j) {
-        const int jj = j - 1;   // cell below face j
-        float inc;
-        if (!P.integral || P.mode == 0) {
-            inc = Q(i, jj, k);
-        } else if (P.mode == 1) {
-            // trilinear at (x_c, jj + 1/4, z_c) and (x_c, jj + 3/4, z_c): x, z on nodes
-            const float a = 0.25f * Q(i, jj - 1, k) + 0.75f * Q(i, jj, k);   // sample jj - 1/4
-            const float b = 0.75f * Q(i, jj, k) + 0.25f * Q(i, jj + 1, k);   // sample jj + 1/4
-            inc = 0.5f * a + 0.5f * b;
-        } else {
-            // Gauss-Legendre on [jj, jj+1/2] and [jj+1/2, jj+1] over WENO6 in y
-            inc = 0.0f;
-            for (int half = 0; half < 2; ++half) {
-                const double a = jj + 0.5 * half;
-                for (int gq = 0; gq < P.gauss.n; ++gq) {
-                    const double y = 0.25 * P.gauss.x[gq] + 0.25 + a;   // G coords
-                    int by; float ty;
-                    split_coord(y - 0.5, by, ty);
-                    inc += 0.5f * (float)P.gauss.w[gq] * weno1dy<float>(Q, i, by, k, ty, P.kappa);
-                }
+// Increment of psi across cell jj of column (i, k): the integral of q over the cell in y.
+FIA_HD float psi_increment(const PsiParams &P, const Loader<float> &Q, int i, int jj, int k) {
+    if (!P.integral || P.mode == 0) return Q(i, jj, k);
+    if (P.mode == 1) {
+        // trilinear at (x_c, jj + 1/4, z_c) and (x_c, jj + 3/4, z_c): x, z on nodes
+        const float a = 0.25f * Q(i, jj - 1, k) + 0.75f * Q(i, jj, k);   // sample jj - 1/4
+        const float b = 0.75f * Q(i, jj, k) + 0.25f * Q(i, jj + 1, k);   // sample jj + 1/4
+        return 0.5f * a + 0.5f * b;
+    }
+    // Gauss-Legendre on [jj, jj+1/2] and [jj+1/2, jj+1] over WENO6 in y
+    float inc = 0.0f;
+    for (int half = 0; half < 2; ++half) {
+        const double a = jj + 0.5 * half;
+        for (int gq = 0; gq < P.gauss.n; ++gq) {
+            const double y = 0.25 * P.gauss.x[gq] + 0.25 + a;   // G coords
+            int by; float ty;
+            split_coord(y - 0.5, by, ty);
+            inc += 0.5f * (float)P.gauss.w[gq] * weno1dy<float>(Q, i, by, k, ty, P.kappa);
+        }
+    }
+    return inc;
+}
+
+// The y-prefix of a column is summed in kPsiSegs segments of its cells: segment w covers cells
+// [w S, min(my, (w + 1) S)), S = ceil(my / kPsiSegs); psi at the top of a cell of segment w is the
+// sum of the totals of segments 0..w-1 (in that order) plus the running sum inside segment w.
+constexpr int kPsiSegs = 8;
+FIA_HD void psi_segment(int my, int w, int &j0, int &j1) {
+    const int seg = (my + kPsiSegs - 1) / kPsiSegs;
+    j0 = min(my, w * seg);
+    j1 = min(my, j0 + seg);
+}
+
+// The WENO6 integral increments of cells [j0, j1) with the seven rows they read held in registers
+// and slid up the column: every Gauss point of the lower half of cell jj has its stencil base at
+// row jj - 1 and every one of the upper half at row jj (the points lie strictly inside the halves),
+// so the two halves read rows jj-3..jj+2 and jj-2..jj+3, and each cell loads one new row.  The
+// samples, fractions and sums are psi_increment's, in its order.  WRITE: psi above each cell from
+// `off`; returns the running sum (the segment total from off = 0).
+template <bool WRITE>
+FIA_HD double psi_weno_segment(const PsiParams &P, const Loader<float> &Q, int i, int k, int j0, int j1, double off) {
+    const int my = P.g.m[1], mz = P.g.m[2];
+    double *col = P.psi + idx3(i, 0, k, my + 1, mz);
+    float win[7];
+#pragma unroll
+    for (int r = 0; r < 7; ++r) win[r] = Q(i, j0 - 3 + r, k);
+    double acc = off;
+    for (int jj = j0; jj < j1; ++jj) {
+        float inc = 0.0f;
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+            const double a = jj + 0.5 * half;
+            for (int gq = 0; gq < P.gauss.n; ++gq) {
+                const double y = 0.25 * P.gauss.x[gq] + 0.25 + a;   // G coords
+                int by; float ty;
+                split_coord(y - 0.5, by, ty);
+                inc += 0.5f * (float)P.gauss.w[gq] * weno6<float>(&win[half], WenoCoef(ty, P.kappa));
             }
         }
         acc += (double)inc;
-        P.psi[idx3(i, j, k, my + 1, mz)] = acc;
+        if (WRITE) col[(size_t)(jj + 1) * mz] = acc;
+#pragma unroll
+        for (int r = 0; r < 6; ++r) win[r] = win[r + 1];
+        win[6] = Q(i, jj + 4, k);
+    }
+    return acc;
+}
+
+FIA_HD double psi_segment_total(const PsiParams &P, const Loader<float> &Q, int i, int k, int j0, int j1) {
+    if (P.integral && P.mode == 2) return psi_weno_segment<false>(P, Q, i, k, j0, j1, 0.0);
+    double s = 0.0;
+#pragma unroll 8
+    for (int jj = j0; jj < j1; ++jj) s += (double)psi_increment(P, Q, i, jj, k);
+    return s;
+}
+// writes psi on the faces above the cells of [j0, j1) from `off`, the psi of face j0
+FIA_HD void psi_segment_write(const PsiParams &P, const Loader<float> &Q, int i, int k, int j0, int j1, double off) {
+    const int my = P.g.m[1], mz = P.g.m[2];
+    double *col = P.psi + idx3(i, 0, k, my + 1, mz);
+    if (j0 == 0) col[0] = 0.0;
+    if (P.integral && P.mode == 2) {
+        psi_weno_segment<true>(P, Q, i, k, j0, j1, off);
+        return;
+    }
+    double acc = off;
+#pragma unroll 8
+    for (int jj = j0; jj < j1; ++jj) {
+        acc += (double)psi_increment(P, Q, i, jj, k);
+        col[(size_t)(jj + 1) * mz] = acc;
     }
 }
-__global__ void __launch_bounds__(kBlock) psi_kernel(PsiParams P) {
-    const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (col >= (long)P.g.m[0] * P.g.m[2]) return;
-    psi_body(P, col);
+
+// psi of column (i, k) with the segment order of psi_kernel (host replay of the kernel)
+FIA_HD void psi_body(const PsiParams &P, long colid) {
+    const int my = P.g.m[1], mz = P.g.m[2];
+    const int i = (int)(colid / mz), k = (int)(colid % mz);
+    const Loader<float> Q{P.q, P.g.m[0], my, mz};
+    double tot[kPsiSegs];
+    for (int w = 0; w < kPsiSegs; ++w) {
+        int j0, j1;
+        psi_segment(my, w, j0, j1);
+        tot[w] = psi_segment_total(P, Q, i, k, j0, j1);
+    }
+    double off = 0.0;
+    for (int w = 0; w < kPsiSegs; ++w) {
+        int j0, j1;
+        psi_segment(my, w, j0, j1);
+        psi_segment_write(P, Q, i, k, j0, j1, off);
+        off += tot[w];
+    }
+}
+
+// One block per 64 z-columns of one x plane: lanes are consecutive z (coalesced rows of q and
+// psi), the block's kPsiSegs waves take the y segments, so a 256^3 grid runs 8,192 waves instead
+// of one thread per column.  Each wave sums its segment's increments, the totals meet in LDS, and each
+// wave writes its segment's prefix from the sum of the segments below it (the increments are
+// computed again: a second read of q, mostly from L2, instead of holding a whole segment in
+// registers).
+__global__ void __launch_bounds__(64 * kPsiSegs) psi_kernel(PsiParams P) {
+    const int my = P.g.m[1], mz = P.g.m[2];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i = blockIdx.y, k = blockIdx.x * 64 + lane;
+    const bool own = k < mz;
+    const Loader<float> Q{P.q, P.g.m[0], my, mz};
+    int j0, j1;
+    psi_segment(my, w, j0, j1);
+    __shared__ double tot[kPsiSegs][64];
+    tot[w][lane] = own ? psi_segment_total(P, Q, i, k, j0, j1) : 0.0;
+    __syncthreads();
+    if (!own) return;
+    double off = 0.0;
+    for (int b = 0; b < w; ++b) off += tot[b][lane];
+    psi_segment_write(P, Q, i, k, j0, j1, off);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -773,7 +875,7 @@ FIA_HD double general_face_flux(const FaceParams &P, const double v[4][3], const
     return flux;
 }
 
-// Exact recursive cutter for the faces listed by face_kernel.  The reference evaluates every
+// Exact recursive cutter for the faces listed by face_count_kernel.  The reference evaluates every
 // face twice, once per neighbouring cell with the corner order reversed
 // (src/backtrace_and_calc_all_quadrature_point_list_for_integrate_density.cpp:77-112); for a
 // twisted face its cutter's pieces depend on that order, so both evaluations are done here: the
@@ -983,8 +1085,9 @@ __global__ void absmax_kernel(const float *a, size_t n, unsigned int *out_bits) 
 // launchers
 // ---------------------------------------------------------------------------------------------
 hipError_t launch_psi(const PsiParams &p, hipStream_t s) {
-    const size_t cols = (size_t)p.g.m[0] * p.g.m[2];
-    hipLaunchKernelGGL(psi_kernel, dim3(grid_for(cols, kBlock)), dim3(kBlock), 0, s, p);
+    if (p.g.m[0] > 65535) return hipErrorInvalidValue;   // one grid row of blocks per x plane
+    hipLaunchKernelGGL(psi_kernel, dim3((unsigned)((p.g.m[2] + 63) / 64), (unsigned)p.g.m[0]), dim3(64 * kPsiSegs), 0,
+                       s, p);
     return hipGetLastError();
 }
 

@@ -357,10 +357,13 @@ __device__ __forceinline__ void tile_sweep(const PcgState &st, int blk, const Op
     int cur = 0;
     __syncthreads();
     for (int x = xa; x < xb; ++x) {
+        // plane x + 2 is loaded while it is still this chunk's or its +x neighbour plane (the same
+        // for the whole block: no divergence)
+        const bool ahead = x + 2 <= xb;
         Raw on2[kRows];
 #pragma unroll
-        for (int w = 0; w < kRows; ++w) on2[w] = op.load(x + 2, y0 + ty + w * kRowStep, z, true);
-        const Raw hn2 = has_halo ? op.load(x + 2, hy, hz, false) : Raw{};
+        for (int w = 0; w < kRows; ++w) on2[w] = ahead ? op.load(x + 2, y0 + ty + w * kRowStep, z, true) : Raw{};
+        const Raw hn2 = (has_halo && ahead) ? op.load(x + 2, hy, hz, false) : Raw{};
         double dn[kRows];
 #pragma unroll
         for (int w = 0; w < kRows; ++w) {
