@@ -67,10 +67,11 @@ PEAK_HBM_GBS = 8000.0
 # rocprofv3 counter passes (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a 1 GiB device copy);
 # recorded in profiles/density_step_256_cfl4_hbm_counters.md
 TRAFFIC_BYTES_256_CFL4 = 12.123e9
-# Jacobi-PCG algorithmic traffic per cell and iteration (FP64 vectors, two kernels): direction
-# kernel reads r, p_old (8 + 8) and the flag byte, writes p (8); update kernel reads x, r, p
-# (3 x 8) and the flag byte, writes x, r (2 x 8); q = L p is recomputed, not stored
-PCG_BYTES_PER_CELL_ITER = 66
+# Jacobi-PCG algorithmic traffic per cell and iteration (FP64 x and r, FP32 search direction p,
+# two kernels): direction kernel reads r (8), p_old (4) and the flag byte, writes p (4); update
+# kernel reads x, r (2 x 8), p (4) and the flag byte, writes x, r (2 x 8); q = L p is recomputed,
+# not stored
+PCG_BYTES_PER_CELL_ITER = 54
 # one multigrid V-cycle per iteration, per fine cell: the down kernel reads r (8) and the flag
 # byte, writes x (4) and the aggregate's b (0.5); the up kernel reads x (4), r (8), the flag byte
 # and the correction (0.5), writes z (8); the coarser levels add about a seventh of a fine level's

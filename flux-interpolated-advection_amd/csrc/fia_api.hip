@@ -98,7 +98,8 @@ struct fia_ctx {
     DevBuf<uint8_t> pcg_flags;    // per-cell neighbour flags of the pressure system
     struct PcgBufs {              // one CG solve: x, r, p (two buffers), q (streaming update), z and r.z
                                   // partials (multigrid), partials, scalars
-        DevBuf<double> x, r, p0, p1, q, z, part_mg, part, scal;
+        DevBuf<double> x, r, q, z, part_mg, part, scal;
+        DevBuf<float> p0, p1;   // the search direction
         size_t bytes() const {
             return x.bytes() + r.bytes() + p0.bytes() + p1.bytes() + q.bytes() + z.bytes() + part_mg.bytes() +
                    part.bytes() + scal.bytes();

@@ -188,7 +188,8 @@ struct PcgState {
     PcgGrid g;
     const uint8_t *flags;
     const double *b;
-    double *x, *r, *p[2];
+    double *x, *r;
+    float *p[2];          // the search direction (FP32: it only steers the solve; x and r are FP64)
     double *q;            // L p stored by the direction kernel (streaming update), or null (recompute)
     double *scal;         // kPcgScalars doubles
     double *part_start;   // 3 * nblocks partial sums (start kernel: rz, rr, bb)

@@ -15,10 +15,13 @@
 //   streaming: the direction kernel stores q and the update reads it, grid-stride over the cells;
 //   recompute: the update sweeps the direction kernel's tiles and recomputes q = L p from p.
 // p alternates between two buffers (a thread reads its neighbours' old p while others write the
-// new one).  The matrix is not stored: one flag byte per cell holds its fluid neighbours.  Per
-// cell and iteration the direction kernel reads 8 (r) + 8 (p old) + 1 (flags) and writes 8 (p),
-// plus 8 (q) when streaming; the update reads 8 x 3 (p, x, r) + 1, plus 8 (q) when streaming, and
-// writes 16 (x, r): 82 bytes streaming, 66 recomputing.  Recomputing pays where the tiles sweep
+// new one).  x and r are FP64; the search direction p is stored in FP32: the direction kernel
+// rounds p = z + beta p_old before its stencil, so q = L p and p.q are exactly those of the stored
+// p, x += alpha p and r -= alpha q use that same p, and the recurrence keeps r = b - L x.  The
+// matrix is not stored: one flag byte per cell holds its fluid neighbours.  Per cell and iteration
+// the direction kernel reads 8 (r) + 4 (p old) + 1 (flags) and writes 4 (p), plus 8 (q) when
+// streaming; the update reads 4 (p) + 8 x 2 (x, r) + 1, plus 8 (q) when streaming, and writes 16
+// (x, r): 70 bytes streaming, 54 recomputing.  Recomputing pays where the tiles sweep
 // long x chunks (large grids); on small grids, whose chunks are a few planes, the halo planes
 // cost more than q and the streaming form is faster.
 //
@@ -204,7 +207,7 @@ __device__ __forceinline__ void pcg_start_body(const PcgState &st, int blk, doub
         const double s = stencil(f, c, sx, sy, [&](size_t i) { return x[i]; });
         const double rv = (f & kPcgUnknown) ? bc - s : 0.0;
         st.r[c] = rv;
-        st.p[0][c] = 0.0;
+        st.p[0][c] = 0.0f;
         acc[0] += rv * (rv * inv_diag(f));
         acc[1] += rv * rv;
         acc[2] += bc * bc;
@@ -246,19 +249,22 @@ __device__ __forceinline__ void pcg_check_body(const PcgState &st, int k) {
 // Update kernel: v = p read back, q recomputed with the same arithmetic (the same bits as the
 // direction kernel's q, 16 bytes per cell cheaper than storing and reloading it),
 // x += alpha p, r -= alpha q (alpha = rz / p.q), partial rz, rr.
+// Raw records hold the loaded values as stored (FP32 p included): converting at the load would
+// make the sweep wait for the plane it prefetches two steps ahead.
 struct DirRaw {
     unsigned f;
-    double r, p;
+    double r;
+    float p;
 };
 struct DirOp {
     using Raw = DirRaw;
     const PcgState &st;
-    const double *__restrict__ pold;
-    double *__restrict__ pnew;
+    const float *__restrict__ pold;
+    float *__restrict__ pnew;
     double beta;
     // Jacobi: r (z = r / diag on the fly); multigrid: the V-cycle's z
     __device__ __forceinline__ Raw load(int x, int y, int z, bool) const {
-        Raw a{0u, 0.0, 0.0};
+        Raw a{0u, 0.0, 0.0f};
         if (x >= 0 && x < st.g.nx && y >= 0 && y < st.g.ny && z >= 0 && z < st.g.nz) {
             const size_t c = idx3(x, y, z, st.g.ny, st.g.nz);
             a.f = st.flags[c];
@@ -267,27 +273,30 @@ struct DirOp {
         }
         return a;
     }
+    // the direction rounded to FP32 here, before the stencil: q = L p and p.q are those of the
+    // stored p, which the update reads back
     __device__ __forceinline__ double value(const Raw &a) const {
-        return (st.z ? a.r : a.r * inv_diag(a.f)) + beta * a.p;
+        return (double)(float)((st.z ? a.r : a.r * inv_diag(a.f)) + beta * (double)a.p);
     }
     __device__ __forceinline__ void cell(size_t c, const Raw &, double v, double q, double *acc) const {
-        pnew[c] = v;
+        pnew[c] = (float)v;
         if (st.q) st.q[c] = q;
         acc[0] += v * q;
     }
 };
 struct UpdRaw {
     unsigned f;
-    double p, x, r;
+    float p;
+    double x, r;
 };
 struct UpdOp {
     using Raw = UpdRaw;
     const PcgState &st;
-    const double *__restrict__ p;
+    const float *__restrict__ p;
     double alpha;
     // x and r only for the tile's own cells (own), not for the halo
     __device__ __forceinline__ Raw load(int x, int y, int z, bool own) const {
-        Raw a{0u, 0.0, 0.0, 0.0};
+        Raw a{0u, 0.0f, 0.0, 0.0};
         if (x >= 0 && x < st.g.nx && y >= 0 && y < st.g.ny && z >= 0 && z < st.g.nz) {
             const size_t c = idx3(x, y, z, st.g.ny, st.g.nz);
             a.f = st.flags[c];
@@ -296,7 +305,7 @@ struct UpdOp {
         }
         return a;
     }
-    __device__ __forceinline__ double value(const Raw &a) const { return a.p; }
+    __device__ __forceinline__ double value(const Raw &a) const { return (double)a.p; }
     __device__ __forceinline__ void cell(size_t c, const Raw &a, double v, double q, double *acc) const {
         const double rv = a.r - alpha * q;
         st.x[c] = a.x + alpha * v;
@@ -396,8 +405,8 @@ __device__ __forceinline__ void tile_sweep(const PcgState &st, int blk, const Op
 }
 
 __device__ __forceinline__ void pcg_direction_body(const PcgState &st, int blk, int k) {
-    const double *__restrict__ pold = st.p[k & 1];
-    double *__restrict__ pnew = st.p[(k + 1) & 1];
+    const float *__restrict__ pold = st.p[k & 1];
+    float *__restrict__ pnew = st.p[(k + 1) & 1];
     if (solve_done(st.scal)) return;
     double rz, rr, bb;
     residual_state(st, k, rz, rr, bb);
@@ -423,7 +432,7 @@ constexpr int kUpdCells = 4;
 // Streaming update: x += alpha p, r -= alpha q with the stored q, grid-stride over nblocks blocks.
 // Outside the system p = q = r = 0, so the update leaves those cells as they are.
 __device__ __forceinline__ void update_stream(const PcgState &st, int blk, int k, double alpha, double *acc) {
-    const double *__restrict__ pnew = st.p[(k + 1) & 1];
+    const float *__restrict__ pnew = st.p[(k + 1) & 1];
     const size_t n = (size_t)st.g.nx * st.g.ny * st.g.nz;
     const uint8_t *__restrict__ fl = st.flags;
     double *__restrict__ x = st.x;
@@ -437,7 +446,7 @@ __device__ __forceinline__ void update_stream(const PcgState &st, int blk, int k
         for (int u = 0; u < kUpdCells; ++u) {
             const size_t cu = c0 + u * stride;
             const size_t c = cu < cr.end ? cu : c0;
-            xv[u] = x[c] + alpha * pnew[c];
+            xv[u] = x[c] + alpha * (double)pnew[c];
             rv[u] = r[c] - alpha * q[c];
             id[u] = inv_diag(fl[c]);
         }
@@ -791,26 +800,30 @@ struct MgDown0Op {
 };
 struct MgUp0Raw {
     unsigned f;
-    double v, r;
+    float x, corr;
+    double r;
 };
 struct MgUp0Op {
     using Raw = MgUp0Raw;
     const PcgState &st;
     const float *x, *corr;
     int cny, cnz;
+    // the loaded values as stored (x and the correction exist for every cell: loaded with the flag
+    // byte, not after it); combined only where the sweep uses them
     __device__ __forceinline__ Raw load(int x0, int y, int z, bool own) const {
-        Raw a{0u, 0.0, 0.0};
+        Raw a{0u, 0.0f, 0.0f, 0.0};
         if (x0 >= 0 && x0 < st.g.nx && y >= 0 && y < st.g.ny && z >= 0 && z < st.g.nz) {
             const size_t c = idx3(x0, y, z, st.g.ny, st.g.nz);
-            // x and the correction exist for every cell: loaded with the flag byte, not after it
-            const float xv = x[c] + kMgCorrScale * corr[idx3(x0 >> 1, y >> 1, z >> 1, cny, cnz)];
+            a.x = x[c];
+            a.corr = corr[idx3(x0 >> 1, y >> 1, z >> 1, cny, cnz)];
             a.f = st.flags[c];
-            a.v = (a.f & kPcgUnknown) ? (double)xv : 0.0;
             if (own) a.r = st.r[c];
         }
         return a;
     }
-    __device__ __forceinline__ double value(const Raw &a) const { return a.v; }
+    __device__ __forceinline__ double value(const Raw &a) const {
+        return (a.f & kPcgUnknown) ? (double)(a.x + kMgCorrScale * a.corr) : 0.0;
+    }
     __device__ __forceinline__ void cell(size_t c, const Raw &a, double v, double q, double *acc) const {
         double zc = 0.0;
         if (a.f & kPcgUnknown)
