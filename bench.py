@@ -404,7 +404,8 @@ def warmstart_summary(args, fia, scenes, torch, n=128, steps=4):
     return {"grid": [n, n, n], "solver": "Jacobi-PCG to 1e-6", "plain_ms_per_step": round(s["plain_ms"], 3),
             "learned_ms_per_step": round(s["learned_ms"], 3), "speedup": round(s["plain_ms"] / s["learned_ms"], 4),
             "plain_iters": s["plain_iters"], "learned_iters": s["learned_iters"],
-            "network_used": s["network_used"], "projections": s["projections"]}
+            "network_used": s["network_used"], "projections": s["projections"],
+            "guess_rel_residual_mean": s["guess_rel_residual_mean"]}
 
 
 def run_projection(args, fia, scenes, torch):

@@ -159,11 +159,18 @@ FIA_HD void psi_body(const PsiParams &P, long colid) {
 
 // One block per 64 z-columns of one x plane: lanes are consecutive z (coalesced rows of q and
 // psi), the block's kPsiSegs waves take the y segments, so a 256^3 grid runs 8,192 waves instead
-// of one thread per column.  Each wave sums its segment's increments, the totals meet in LDS, and each
-// wave writes its segment's prefix from the sum of the segments below it (the increments are
-// computed again: a second read of q, mostly from L2, instead of holding a whole segment in
-// registers).
-__global__ void __launch_bounds__(64 * kPsiSegs) psi_kernel(PsiParams P) {
+// of one thread per column.  Each wave sums its segment's increments, the totals meet in LDS, and
+// each wave writes its segment's prefix from the sum of the segments below it.  Segments of at
+// most kPsiHeld cells under the cheap rules (const, linear) keep their increments in registers
+// between the two steps (q is read once); otherwise they are computed again (the WENO6 integral
+// rule, whose increments are arithmetic-bound, or long columns).
+constexpr int kPsiHeld = 32;
+// HELD: the increments stay in registers (cheap rules, segments of at most kPsiHeld cells); a
+// template parameter, so the other form keeps its own register budget
+FIA_HD bool psi_held(const PsiParams &P) {
+    return !(P.integral && P.mode == 2) && (P.g.m[1] + kPsiSegs - 1) / kPsiSegs <= kPsiHeld;
+}
+template <bool HELD> __global__ void __launch_bounds__(64 * kPsiSegs) psi_kernel(PsiParams P) {
     const int my = P.g.m[1], mz = P.g.m[2];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int i = blockIdx.y, k = blockIdx.x * 64 + lane;
@@ -172,12 +179,36 @@ __global__ void __launch_bounds__(64 * kPsiSegs) psi_kernel(PsiParams P) {
     int j0, j1;
     psi_segment(my, w, j0, j1);
     __shared__ double tot[kPsiSegs][64];
-    tot[w][lane] = own ? psi_segment_total(P, Q, i, k, j0, j1) : 0.0;
-    __syncthreads();
-    if (!own) return;
-    double off = 0.0;
-    for (int b = 0; b < w; ++b) off += tot[b][lane];
-    psi_segment_write(P, Q, i, k, j0, j1, off);
+    if constexpr (HELD) {
+        float inc[kPsiHeld];
+        double s = 0.0;
+#pragma unroll
+        for (int r = 0; r < kPsiHeld; ++r) {
+            inc[r] = (own && j0 + r < j1) ? psi_increment(P, Q, i, j0 + r, k) : 0.0f;
+            s += (double)inc[r];
+        }
+        tot[w][lane] = s;
+        __syncthreads();
+        if (!own) return;
+        double acc = 0.0;
+        for (int b = 0; b < w; ++b) acc += tot[b][lane];
+        double *col = P.psi + idx3(i, 0, k, my + 1, mz);
+        if (j0 == 0) col[0] = 0.0;
+#pragma unroll
+        for (int r = 0; r < kPsiHeld; ++r) {
+            if (j0 + r < j1) {
+                acc += (double)inc[r];
+                col[(size_t)(j0 + r + 1) * mz] = acc;
+            }
+        }
+    } else {
+        tot[w][lane] = own ? psi_segment_total(P, Q, i, k, j0, j1) : 0.0;
+        __syncthreads();
+        if (!own) return;
+        double off = 0.0;
+        for (int b = 0; b < w; ++b) off += tot[b][lane];
+        psi_segment_write(P, Q, i, k, j0, j1, off);
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1086,8 +1117,9 @@ __global__ void absmax_kernel(const float *a, size_t n, unsigned int *out_bits) 
 // ---------------------------------------------------------------------------------------------
 hipError_t launch_psi(const PsiParams &p, hipStream_t s) {
     if (p.g.m[0] > 65535) return hipErrorInvalidValue;   // one grid row of blocks per x plane
-    hipLaunchKernelGGL(psi_kernel, dim3((unsigned)((p.g.m[2] + 63) / 64), (unsigned)p.g.m[0]), dim3(64 * kPsiSegs), 0,
-                       s, p);
+    const dim3 grid((unsigned)((p.g.m[2] + 63) / 64), (unsigned)p.g.m[0]);
+    if (psi_held(p)) hipLaunchKernelGGL(psi_kernel<true>, grid, dim3(64 * kPsiSegs), 0, s, p);
+    else hipLaunchKernelGGL(psi_kernel<false>, grid, dim3(64 * kPsiSegs), 0, s, p);
     return hipGetLastError();
 }
 
