@@ -91,7 +91,8 @@ class KernelStats(ctypes.Structure):
                 ("vertices", ctypes.c_longlong), ("vertex_ms", ctypes.c_double), ("psi_ms", ctypes.c_double),
                 ("gather_ms", ctypes.c_double), ("gather_cells", ctypes.c_longlong),
                 ("weno_evals", ctypes.c_longlong), ("quad_ms", ctypes.c_double), ("quad_launches", ctypes.c_longlong),
-                ("pcg_ms", ctypes.c_double), ("pcg_iters", ctypes.c_longlong), ("pcg_solves", ctypes.c_longlong)]
+                ("pcg_ms", ctypes.c_double), ("pcg_iters", ctypes.c_longlong), ("pcg_solves", ctypes.c_longlong),
+                ("twisted_faces", ctypes.c_longlong), ("general_pieces", ctypes.c_longlong)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

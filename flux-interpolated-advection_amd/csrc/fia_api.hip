@@ -86,6 +86,9 @@ struct fia_ctx {
     int *host_flags = nullptr;                   // pinned: twisted count, kernel error
     double *host_scal = nullptr;                 // pinned: PCG scalars of two solves read back while iterating
     DevBuf<double> piece_c[3], piece_ay, piece_val;
+    DevBuf<unsigned> gen_n;                 // twisted faces: pieces per face and orientation
+    DevBuf<unsigned long long> gen_start;   // their exclusive scan and total
+    DevBuf<double> gen_c[3], gen_ay, gen_val;   // the twisted faces' piece records and values
     DevBuf<float> work[3];
     DevBuf<unsigned> twisted;
     DevBuf<unsigned> degen;     // degenerate faces of the segment-cutter mode
@@ -181,7 +184,8 @@ size_t ctx_bytes(const fia_ctx *c) {
                c->solid.bytes() + c->genI.bytes();
     b += c->fsa.bytes() + c->fsb.bytes() + c->face_n.bytes() + c->face_start.bytes() + c->scan_tmp.bytes() + c->bounds.bytes() +
          c->piece_ay.bytes() + c->piece_val.bytes();
-    for (int a = 0; a < 3; ++a) b += c->piece_c[a].bytes();
+    for (int a = 0; a < 3; ++a) b += c->piece_c[a].bytes() + c->gen_c[a].bytes();
+    b += c->gen_n.bytes() + c->gen_start.bytes() + c->gen_ay.bytes() + c->gen_val.bytes();
     for (int a = 0; a < 3; ++a) b += c->disp[a].bytes() + c->work[a].bytes() + c->gen[a].bytes();
     for (int a = 0; a < 4; ++a) b += c->grid[a].bytes();
     for (int a = 0; a < 2; ++a) b += c->genD[a].bytes();
@@ -373,7 +377,31 @@ fia_status do_faces(fia_ctx *c, const CompGrid &g, const float *q, bool moving, 
     if (!p.fuse_value) HIP_TRY(launch_pieces(p, nslots, c->stream));   // fused: the cutters stored the values
     if (weno_quad) HIP_TRY(span_end(c, sq));
     HIP_TRY(launch_face_sum(p, c->stream));
-    HIP_TRY(launch_faces_general(p, n_twisted, c->stream));
+    if (n_twisted > 0) {
+        // twisted faces: pieces counted, then stored in records sized from the read-back total
+        HIP_TRY(c->gen_n.ensure(2 * (size_t)n_twisted));
+        HIP_TRY(c->gen_start.ensure(2 * (size_t)n_twisted + 1));
+        p.gen_n = c->gen_n.p;
+        p.gen_start = c->gen_start.p;
+        HIP_TRY(launch_general_count(p, n_twisted, c->stream));
+        HIP_TRY(hipMemcpyAsync(c->host_bounds, c->gen_start.p + 2 * (size_t)n_twisted, sizeof(unsigned long long),
+                               hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipMemcpyAsync(c->host_flags + 1, c->counters.p + 1, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        const unsigned long long npieces = c->host_bounds[0];
+        if (c->host_flags[1] != 0) return check_kernel_errors(c);   // the cutter failed on a face
+        if (npieces > kGenMaxPieces)
+            return fail(FIA_ERR_CAPACITY, "twisted faces: " + std::to_string(npieces) + " pieces in one pass, more than "
+                                          "the general cutter's budget of " + std::to_string(kGenMaxPieces));
+        for (int a = 0; a < 3; ++a) HIP_TRY(c->gen_c[a].ensure(npieces));
+        HIP_TRY(c->gen_ay.ensure(npieces));
+        HIP_TRY(c->gen_val.ensure(npieces));
+        for (int a = 0; a < 3; ++a) p.gen_c[a] = c->gen_c[a].p;
+        p.gen_ay = c->gen_ay.p;
+        p.gen_val = c->gen_val.p;
+        HIP_TRY(launch_general_pieces(p, n_twisted, npieces, c->stream));
+        if (c->timing) { c->acc.twisted_faces += n_twisted; c->acc.general_pieces += (long long)npieces; }
+    }
     HIP_TRY(span_end(c, sp));
     if (c->timing) { c->acc.face_launches += 1; c->acc.quad_launches += weno_quad ? 1 : 0; }
     return FIA_OK;
@@ -948,7 +976,8 @@ fia_status fia_ctx_destroy(fia_ctx *c) {
     c->solid.release(); c->genI.release();
     c->fsa.release(); c->fsb.release(); c->face_n.release(); c->face_start.release(); c->scan_tmp.release();
     c->piece_ay.release(); c->piece_val.release();
-    for (int a = 0; a < 3; ++a) c->piece_c[a].release();
+    for (int a = 0; a < 3; ++a) { c->piece_c[a].release(); c->gen_c[a].release(); }
+    c->gen_n.release(); c->gen_start.release(); c->gen_ay.release(); c->gen_val.release();
     for (int a = 0; a < 3; ++a) { c->disp[a].release(); c->work[a].release(); c->gen[a].release(); }
     for (int a = 0; a < 4; ++a) c->grid[a].release();
     for (int a = 0; a < 2; ++a) c->genD[a].release();

@@ -449,6 +449,13 @@ struct GenPoly {
     int n, dim;
     CutVert v[kGenVerts];
 };
+// copies a polygon's n live vertices only (the arrays live in private scratch memory, where a
+// whole-struct copy would move all kGenVerts entries every time the stack is pushed or popped)
+FIA_HD void gen_copy(GenPoly &d, const GenPoly &s) {
+    d.n = s.n;
+    d.dim = s.dim;
+    for (int i = 0; i < s.n; ++i) d.v[i] = s.v[i];
+}
 
 // centroid and |area| of a xi-eta polygon (reference src/facecutter.h:226-253)
 FIA_HD void gen_center(const double (*p)[2], int num, double &cx, double &cy, double &area) {
@@ -487,7 +494,7 @@ FIA_HD int general_cut(const double col[4][2], double min_area, Emit &emit) {
     }
     GenPoly cur, arr0, arr1;
     while (sp > 0) {
-        cur = stack[--sp];
+        gen_copy(cur, stack[--sp]);
         const int dim = cur.dim, num = cur.n;
         double lo = 1e300, hi = -1e300;
         for (int n = 0; n < num; ++n) {
@@ -528,7 +535,7 @@ FIA_HD int general_cut(const double col[4][2], double min_area, Emit &emit) {
                     } else {
                         arr1.v[arr1.n++] = add;
                         if (sp >= kGenStack) return KE_CAP_STACK;
-                        stack[sp++] = arr1;
+                        gen_copy(stack[sp++], arr1);
                         slot = 0;
                         if (t1 < 1.0) arr0.v[arr0.n++] = add;
                     }
@@ -541,15 +548,15 @@ FIA_HD int general_cut(const double col[4][2], double min_area, Emit &emit) {
             }
             if (slot != 0) return KE_CUT;   // odd number of sign changes: cannot happen on a loop
             if (reserved >= 0) {
-                stack[reserved] = arr0;
+                gen_copy(stack[reserved], arr0);
             } else {
                 if (sp >= kGenStack) return KE_CAP_STACK;
-                stack[sp++] = arr0;
+                gen_copy(stack[sp++], arr0);
             }
         } else if (dim == 0) {
             cur.dim = 1;
             if (sp >= kGenStack) return KE_CAP_STACK;
-            stack[sp++] = cur;
+            gen_copy(stack[sp++], cur);
         } else if (num >= 3) {
             // subdivide (reference src/facecutter.h:255-324)
             double pts[kGenVerts][2];

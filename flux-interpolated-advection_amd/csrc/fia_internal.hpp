@@ -127,8 +127,18 @@ struct FaceParams {
     double *piece_c[3];
     double *piece_ay;
     double *piece_val;
+    // twisted faces (general cutter): pieces per face and orientation (2 per listed face), their
+    // exclusive scan (2 per face, then the total), and the piece records and values
+    unsigned *gen_n;
+    unsigned long long *gen_start;
+    double *gen_c[3];
+    double *gen_ay;
+    double *gen_val;
     GaussTable gauss;
 };
+// the general cutter's piece budget of one flux pass (records of 40 bytes): more pieces than this
+// report FIA_ERR_CAPACITY instead of running
+constexpr unsigned long long kGenMaxPieces = 1ull << 25;
 
 // errors reported by kernels through FaceParams::error
 // KE_CAPACITY: a caller's piece array (fia_split_faces); KE_CAP_*: the general cutter's polygon
@@ -152,8 +162,11 @@ hipError_t launch_face_sum(const FaceParams &p, hipStream_t s);
 size_t scan_temp_words(size_t n);
 hipError_t launch_scan(unsigned long long *tile_tmp, const unsigned *in, unsigned long long *out, size_t n,
                        const FaceParams &p, unsigned long long *bounds, hipStream_t s);
-// count: the faces listed for the kernel (its grid; no launch for none)
-hipError_t launch_faces_general(const FaceParams &p, int count, hipStream_t s);
+// the twisted faces' pieces: count per face and orientation, and their scan (p.gen_n, p.gen_start;
+// gen_start[2 count] = the total), then with the piece records sized from that total: the pieces
+// stored, integrated and summed into the faces' flux and corr (count: the listed faces)
+hipError_t launch_general_count(const FaceParams &p, int count, hipStream_t s);
+hipError_t launch_general_pieces(const FaceParams &p, int count, unsigned long long npieces, hipStream_t s);
 // degenerate faces of the segment-cutter mode, cut per face into their slots by the pipeline
 hipError_t launch_faces_degenerate(const FaceParams &p, int count, hipStream_t s);
 // zero-velocity pass: fluxes of the y faces of undeformed lattice faces (minimal_area >= 1)

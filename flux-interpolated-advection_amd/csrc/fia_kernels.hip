@@ -10,7 +10,8 @@
 //   degenerate_kernel clipping pipeline for faces with a corner colour on an interior integer
 //   piece_kernel      quadrature of every piece slot (WENO6 integral rule: piece_kernel<1>)
 //   face_sum_kernel   face flux = ordered sum of its slots
-//   general_kernel    the exact recursive cutter for the few twisted faces (grid-stride)
+//   general_*         the exact recursive cutter for the few twisted faces: pieces counted and
+//                     stored per face, integrated one thread per piece, summed per face
 //   gather_kernel     per cell: sum of its six signed face fluxes (deterministic, no atomics)
 // Each interior face is computed once; its flux enters the two neighbouring cells with opposite
 // signs, so nothing is lost or created inside the domain.
@@ -877,67 +878,137 @@ __global__ void __launch_bounds__(64) degenerate_value_kernel(FaceParams P) {
         degenerate_value_body(P, t);
 }
 
-// Flux of one face with the exact recursive cutter, corners in the given order.
-FIA_HD double general_face_flux(const FaceParams &P, const double v[4][3], const double col[4][2],
-                                                    int &code) {
-    FaceGeom geom;
-    geom.init(v);
-    double flux = 0.0;
-    auto emit = [&](const double (*pts)[2], int num, double cxi, double ceta) {
-        double ay2 = 0.0;
-        double r0[3], rp[3];
-        geom.rel(pts[0][0], pts[0][1], r0);
-        rp[0] = r0[0]; rp[1] = r0[1]; rp[2] = r0[2];
-        for (int n = 1; n <= num; ++n) {
-            double rn[3];
-            if (n == num) { rn[0] = r0[0]; rn[1] = r0[1]; rn[2] = r0[2]; }
-            else geom.rel(pts[n][0], pts[n][1], rn);
-            ay2 += rp[2] * rn[0] - rp[0] * rn[2];
-            rp[0] = rn[0]; rp[1] = rn[1]; rp[2] = rn[2];
-        }
-        double r[3];
-        geom.rel(cxi, ceta, r);
-        const double c3[3] = {geom.p0[0] + r[0], geom.p0[1] + r[1], geom.p0[2] + r[2]};
-        const double ay = 0.5 * ay2;
-        int npts = 0;
-        if (ay != 0.0) flux += piece_flux(P, c3, ay, npts);
-    };
-    code = general_cut(col, P.min_area, emit);
-    return flux;
-}
-
-// Exact recursive cutter for the faces listed by face_count_kernel.  The reference evaluates every
-// face twice, once per neighbouring cell with the corner order reversed
+// ---------------------------------------------------------------------------------------------
+// Twisted faces: the exact recursive cutter (fia_cutter.hpp general_cut).  The reference evaluates
+// every face twice, once per neighbouring cell with the corner order reversed
 // (src/backtrace_and_calc_all_quadrature_point_list_for_integrate_density.cpp:77-112); for a
-// twisted face its cutter's pieces depend on that order, so both evaluations are done here: the
-// lower cell's ('+' order v0 v1 v2 v3) goes to flux, the upper cell's (v0 v3 v2 v1) enters
-// through corr = upper + flux.  A grid-stride loop over the listed faces: the private stack is
-// large and usually only a few faces need it.
-FIA_HD void general_body(const FaceParams &P, int t) {
+// twisted face its cutter's pieces depend on that order, so both evaluations are done: the lower
+// cell's ('+' order v0 v1 v2 v3, orientation 0) goes to flux, the upper cell's (v0 v3 v2 v1,
+// orientation 1) enters through corr = upper + flux.  In steps, so that no thread integrates a
+// whole face alone:
+//   general_count_kernel  pieces of each orientation of each listed face (t: 2t, 2t + 1)
+//   general_scan_kernel   their exclusive scan (one block) and the total, read back to size
+//                         the piece records
+//   general_emit_kernel   the pieces again, each stored as a record (centre, y area) in its range
+//   general_piece_kernel  the quadrature of every piece, one thread per piece
+//   general_sum_kernel    each orientation's pieces summed in cut order into flux and corr
+// Ranges and order depend only on the faces, so the sums are deterministic.
+// ---------------------------------------------------------------------------------------------
+
+// corners and colours of listed face t in orientation o; returns its family and index
+FIA_HD void general_face(const FaceParams &P, int t, int o, int &fam, long &id, double v[4][3], double col[4][2]) {
     const unsigned code = P.twisted[t];
-    const int fam = (int)(code >> 30);   // 0, 1 or 2
-    const long id = (long)(code & ((1u << 30) - 1u));
+    fam = (int)(code >> 30);   // 0, 1 or 2
+    id = (long)(code & ((1u << 30) - 1u));
     int i, j, k;
     decode_face(P.g, fam, id, i, j, k);
-    double v[4][3], col[4][2];
-    load_face(P, fam, i, j, k, v, col);
-    int code1, code2;
-    const double lower = general_face_flux(P, v, col, code1);
-    double vr[4][3], cr[4][2];
-    const int order[4] = {0, 3, 2, 1};
+    double v0[4][3], c0[4][2];
+    load_face(P, fam, i, j, k, v0, c0);
+    const int order[2][4] = {{0, 1, 2, 3}, {0, 3, 2, 1}};
     for (int c = 0; c < 4; ++c) {
-        for (int a = 0; a < 3; ++a) vr[c][a] = v[order[c]][a];
-        cr[c][0] = col[order[c]][0]; cr[c][1] = col[order[c]][1];
+        for (int a = 0; a < 3; ++a) v[c][a] = v0[order[o][c]][a];
+        col[c][0] = c0[order[o][c]][0]; col[c][1] = c0[order[o][c]][1];
     }
-    const double upper = general_face_flux(P, vr, cr, code2);
-    if (code1 != KE_NONE) report(P.error, code1);
-    else if (code2 != KE_NONE) report(P.error, code2);
-    P.flux[fam][id] = lower;
-    P.corr[fam][id] = upper + lower;
 }
-__global__ void __launch_bounds__(64) general_kernel(FaceParams P) {
+
+// centre (cells, physical frame) and y vector area of a piece the recursive cutter emits
+FIA_HD void general_piece_record(const FaceGeom &geom, const double (*pts)[2], int num, double cxi, double ceta,
+                                 double c3[3], double &ay) {
+    double ay2 = 0.0;
+    double r0[3], rp[3];
+    geom.rel(pts[0][0], pts[0][1], r0);
+    rp[0] = r0[0]; rp[1] = r0[1]; rp[2] = r0[2];
+    for (int n = 1; n <= num; ++n) {
+        double rn[3];
+        if (n == num) { rn[0] = r0[0]; rn[1] = r0[1]; rn[2] = r0[2]; }
+        else geom.rel(pts[n][0], pts[n][1], rn);
+        ay2 += rp[2] * rn[0] - rp[0] * rn[2];
+        rp[0] = rn[0]; rp[1] = rn[1]; rp[2] = rn[2];
+    }
+    double r[3];
+    geom.rel(cxi, ceta, r);
+    c3[0] = geom.p0[0] + r[0]; c3[1] = geom.p0[1] + r[1]; c3[2] = geom.p0[2] + r[2];
+    ay = 0.5 * ay2;
+}
+
+FIA_HD void general_count_body(const FaceParams &P, int t) {
+    for (int o = 0; o < 2; ++o) {
+        int fam;
+        long id;
+        double v[4][3], col[4][2];
+        general_face(P, t, o, fam, id, v, col);
+        unsigned n = 0;
+        auto count = [&](const double (*)[2], int, double, double) { ++n; };
+        const int code = general_cut(col, P.min_area, count);
+        if (code != KE_NONE) { report(P.error, code); n = 0; }
+        P.gen_n[2 * t + o] = n;
+    }
+}
+__global__ void __launch_bounds__(64) general_count_kernel(FaceParams P) {
     const int count = *P.twisted_count;
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < count; t += gridDim.x * blockDim.x) general_body(P, t);
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < count; t += gridDim.x * blockDim.x) general_count_body(P, t);
+}
+
+FIA_HD void general_emit_body(const FaceParams &P, int t) {
+    for (int o = 0; o < 2; ++o) {
+        int fam;
+        long id;
+        double v[4][3], col[4][2];
+        general_face(P, t, o, fam, id, v, col);
+        FaceGeom geom;
+        geom.init(v);
+        const unsigned long long s0 = P.gen_start[2 * t + o];
+        const unsigned cap = P.gen_n[2 * t + o];
+        unsigned k = 0;
+        auto emit = [&](const double (*pts)[2], int num, double cxi, double ceta) {
+            if (k >= cap) return;   // the same cut as the count: never taken
+            double c3[3], ay;
+            general_piece_record(geom, pts, num, cxi, ceta, c3, ay);
+            const unsigned long long q = s0 + k++;
+            P.gen_c[0][q] = c3[0]; P.gen_c[1][q] = c3[1]; P.gen_c[2][q] = c3[2];
+            P.gen_ay[q] = ay;
+        };
+        if (cap > 0) general_cut(col, P.min_area, emit);
+    }
+}
+__global__ void __launch_bounds__(64) general_emit_kernel(FaceParams P) {
+    const int count = *P.twisted_count;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < count; t += gridDim.x * blockDim.x) general_emit_body(P, t);
+}
+
+FIA_HD void general_piece_body(const FaceParams &P, unsigned long long q) {
+    const double ay = P.gen_ay[q];
+    double val = 0.0;
+    if (ay != 0.0) {
+        const double c3[3] = {P.gen_c[0][q], P.gen_c[1][q], P.gen_c[2][q]};
+        int npts = 0;
+        val = piece_flux(P, c3, ay, npts);
+    }
+    P.gen_val[q] = val;
+}
+__global__ void __launch_bounds__(kBlock) general_piece_kernel(FaceParams P, unsigned long long npieces) {
+    const unsigned long long q = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < npieces) general_piece_body(P, q);
+}
+
+FIA_HD void general_sum_body(const FaceParams &P, int t) {
+    double f[2];
+    for (int o = 0; o < 2; ++o) {
+        const unsigned long long s0 = P.gen_start[2 * t + o];
+        const unsigned n = P.gen_n[2 * t + o];
+        double s = 0.0;
+        for (unsigned k = 0; k < n; ++k) s += P.gen_val[s0 + k];
+        f[o] = s;
+    }
+    const unsigned code = P.twisted[t];
+    const int fam = (int)(code >> 30);
+    const long id = (long)(code & ((1u << 30) - 1u));
+    P.flux[fam][id] = f[0];
+    P.corr[fam][id] = f[1] + f[0];
+}
+__global__ void __launch_bounds__(kBlock) general_sum_kernel(FaceParams P) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < *P.twisted_count) general_sum_body(P, t);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1223,6 +1294,24 @@ __global__ void __launch_bounds__(kBlock) scan_tile_offsets_kernel(unsigned long
     }
 }
 
+// one block: gen_start[0..m) = exclusive scan of gen_n, gen_start[m] = total
+__global__ void __launch_bounds__(kBlock) general_scan_kernel(const unsigned *n, unsigned long long *start, int m) {
+    const int per = (m + kBlock - 1) / kBlock;
+    const int b0 = threadIdx.x * per;
+    unsigned long long s = 0;
+    for (int i = 0; i < per; ++i)
+        if (b0 + i < m) s += n[b0 + i];
+    unsigned long long total;
+    unsigned long long run = block_exclusive_scan(s, total);
+    for (int i = 0; i < per; ++i) {
+        if (b0 + i < m) {
+            start[b0 + i] = run;
+            run += n[b0 + i];
+        }
+    }
+    if (threadIdx.x == 0) start[m] = total;
+}
+
 // LDS index of tile element e, padded by one word per kScanItems elements so that both the striped
 // (e = i kBlock + t) and the blocked (e = t kScanItems + i) accesses of a wave spread over the banks
 __device__ __forceinline__ int scan_pad(int e) { return e + e / kScanItems; }
@@ -1321,9 +1410,20 @@ hipError_t launch_faces_degenerate(const FaceParams &p, int count, hipStream_t s
     return hipGetLastError();
 }
 
-hipError_t launch_faces_general(const FaceParams &p, int count, hipStream_t s) {
+hipError_t launch_general_count(const FaceParams &p, int count, hipStream_t s) {
     if (count <= 0) return hipSuccess;
-    hipLaunchKernelGGL(general_kernel, dim3(list_blocks(count)), dim3(64), 0, s, p);
+    hipLaunchKernelGGL(general_count_kernel, dim3(list_blocks(count)), dim3(64), 0, s, p);
+    hipLaunchKernelGGL(general_scan_kernel, dim3(1), dim3(kBlock), 0, s, (const unsigned *)p.gen_n, p.gen_start,
+                       2 * count);
+    return hipGetLastError();
+}
+
+hipError_t launch_general_pieces(const FaceParams &p, int count, unsigned long long npieces, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(general_emit_kernel, dim3(list_blocks(count)), dim3(64), 0, s, p);
+    if (npieces > 0)
+        hipLaunchKernelGGL(general_piece_kernel, dim3(grid_for(npieces, kBlock)), dim3(kBlock), 0, s, p, npieces);
+    hipLaunchKernelGGL(general_sum_kernel, dim3(grid_for((size_t)count, kBlock)), dim3(kBlock), 0, s, p);
     return hipGetLastError();
 }
 

@@ -229,6 +229,8 @@ typedef struct fia_kernel_stats {
     double pcg_ms;               /* summed time of the pressure solves' CG iteration launches       */
     long long pcg_iters;         /* CG iterations those solves ran                                  */
     long long pcg_solves;        /* pressure solves                                                 */
+    long long twisted_faces;     /* faces cut by the recursive (general) cutter, both orientations  */
+    long long general_pieces;    /* the pieces it made of them                                      */
 } fia_kernel_stats;
 
 fia_status fia_ctx_set_timing(fia_ctx *ctx, int enable);

@@ -75,6 +75,9 @@ struct HostCtx {
     HostBuf<double> piece_c[3], piece_ay, piece_val;
     HostBuf<float> work[3];
     HostBuf<unsigned> twisted, degen;
+    HostBuf<unsigned> gen_n;
+    HostBuf<unsigned long long> gen_start;
+    HostBuf<double> gen_c[3], gen_ay, gen_val;
     int counters[4] = {0, 0, 0, 0};
     std::vector<double> stats;   // kStatCols per face pass
     char msg[256] = {0};
@@ -241,24 +244,32 @@ int do_faces(HostCtx &c, const CompGrid &g, const float *q, bool moving, FluxSet
     if (fam_mask & 1) parallel_for((long)g.faces(0), [&](long id) { face_sum_body<0>(p, id); });
     if (fam_mask & 2) parallel_for((long)g.faces(1), [&](long id) { face_sum_body<1>(p, id); });
     if (fam_mask & 4) parallel_for((long)g.faces(2), [&](long id) { face_sum_body<2>(p, id); });
-    parallel_for(n_twisted, [&](long t) { general_body(p, (int)t); });
-    // work of the general cutter: pieces of both orientations of every twisted face
+    // twisted faces: pieces counted per face and orientation, scanned, stored, integrated, summed
     long gen_pieces = 0, gen_max = 0;
-    for (int t = 0; t < n_twisted; ++t) {
-        const unsigned code = c.twisted.p[t];
-        int i, j, k;
-        decode_face(g, (int)(code >> 30), (long)(code & ((1u << 30) - 1u)), i, j, k);
-        double v[4][3], col[4][2];
-        load_face(p, (int)(code >> 30), i, j, k, v, col);
-        long np = 0;
-        auto count = [&](const double (*)[2], int, double, double) { ++np; };
-        general_cut(col, p.min_area, count);
-        double cr[4][2];
-        const int order[4] = {0, 3, 2, 1};
-        for (int cc = 0; cc < 4; ++cc) { cr[cc][0] = col[order[cc]][0]; cr[cc][1] = col[order[cc]][1]; }
-        general_cut(cr, p.min_area, count);
-        gen_pieces += np;
-        gen_max = std::max(gen_max, np);
+    if (n_twisted > 0) {
+        c.gen_n.ensure(2 * (size_t)n_twisted);
+        c.gen_start.ensure(2 * (size_t)n_twisted + 1);
+        p.gen_n = c.gen_n.p;
+        p.gen_start = c.gen_start.p;
+        parallel_for(n_twisted, [&](long t) { general_count_body(p, (int)t); });
+        if (c.counters[1] != 0) return c.counters[1];
+        unsigned long long run2 = 0;
+        for (int t = 0; t < 2 * n_twisted; ++t) {
+            c.gen_start.p[t] = run2;
+            run2 += c.gen_n.p[t];
+            if (t % 2 == 1) gen_max = std::max(gen_max, (long)(c.gen_n.p[t - 1] + c.gen_n.p[t]));
+        }
+        c.gen_start.p[2 * n_twisted] = run2;
+        gen_pieces = (long)run2;
+        for (int a = 0; a < 3; ++a) c.gen_c[a].ensure(run2);
+        c.gen_ay.ensure(run2);
+        c.gen_val.ensure(run2);
+        for (int a = 0; a < 3; ++a) p.gen_c[a] = c.gen_c[a].p;
+        p.gen_ay = c.gen_ay.p;
+        p.gen_val = c.gen_val.p;
+        parallel_for(n_twisted, [&](long t) { general_emit_body(p, (int)t); });
+        parallel_for((long)run2, [&](long q) { general_piece_body(p, (unsigned long long)q); });
+        parallel_for(n_twisted, [&](long t) { general_sum_body(p, (int)t); });
     }
     st[ST_TWISTED] = n_twisted; st[ST_DEGEN] = n_degen; st[ST_SLOTS] = (double)nslots;
     st[ST_GEN_PIECES] = (double)gen_pieces; st[ST_GEN_MAX] = (double)gen_max; st[ST_MULTI_FACES] = (double)multi.load();
