@@ -41,7 +41,8 @@ EXPORTS = ["fia_default_opts", "fia_ctx_create", "fia_ctx_create_on_stream", "fi
            "fia_unet_param_count", "fia_unet_create", "fia_unet_destroy", "fia_unet_set_params",
            "fia_unet_get_params", "fia_unet_forward", "fia_unet_loss_grad", "fia_unet_train_step", "fia_pressure_rhs",
            "fia_pressure_project_learned", "fia_grid_totals", "fia_grid_totals_noalloc", "fia_wall_momentum_loss", "fia_scene_forcing",
-           "fia_scenes_step", "fia_scenes_step_batched", "fia_ctx_set_pressure_preconditioner"]
+           "fia_scenes_step", "fia_scenes_step_batched", "fia_ctx_set_pressure_preconditioner",
+           "fia_ctx_set_graphs"]
 
 
 class FiaError(RuntimeError):
@@ -92,7 +93,8 @@ class KernelStats(ctypes.Structure):
                 ("gather_ms", ctypes.c_double), ("gather_cells", ctypes.c_longlong),
                 ("weno_evals", ctypes.c_longlong), ("quad_ms", ctypes.c_double), ("quad_launches", ctypes.c_longlong),
                 ("pcg_ms", ctypes.c_double), ("pcg_iters", ctypes.c_longlong), ("pcg_solves", ctypes.c_longlong),
-                ("twisted_faces", ctypes.c_longlong), ("general_pieces", ctypes.c_longlong)]
+                ("twisted_faces", ctypes.c_longlong), ("general_pieces", ctypes.c_longlong),
+                ("graph_captures", ctypes.c_longlong), ("graph_replays", ctypes.c_longlong)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -135,6 +137,7 @@ def lib():
                                            ctypes.POINTER(_d)]
         L.fia_ctx_set_timing.argtypes = [_vp, _i]
         L.fia_ctx_set_pressure_preconditioner.argtypes = [_vp, _i]
+        L.fia_ctx_set_graphs.argtypes = [_vp, _i]
         L.fia_ctx_get_stats.argtypes = [_vp, ctypes.POINTER(KernelStats)]
         L.fia_ctx_reset_stats.argtypes = [_vp]
         L.fia_unet_param_count.restype = ctypes.c_size_t
@@ -205,6 +208,11 @@ class Context:
     def set_pressure_preconditioner(self, kind):
         """"mg" (geometric multigrid V-cycle, the default) or "jacobi" for later pressure solves."""
         _check(lib().fia_ctx_set_pressure_preconditioner(self._h, PRECOND[kind]))
+
+    def set_graphs(self, enable):
+        """Replay the pressure solves' CG iteration batches from captured HIP graphs (the default)
+        or launch every kernel directly; the results are bit-identical."""
+        _check(lib().fia_ctx_set_graphs(self._h, 1 if enable else 0))
 
     def close(self):
         if self._h:

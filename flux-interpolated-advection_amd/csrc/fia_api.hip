@@ -154,6 +154,8 @@ struct fia_ctx {
     DevBuf<double> batch_scal;
     double *host_batch = nullptr;   // pinned: the scalars of every batched solve
     size_t host_batch_n = 0;
+    PcgGraphCache graphs;           // captured CG iteration batches (fia_ctx_set_graphs)
+    long long graph_captures0 = 0, graph_replays0 = 0;   // the cache's counters at the last stats reset
     // instrumentation
     bool timing = false;
     DevBuf<unsigned long long> stats;   // [0] pieces, [1] density points
@@ -697,7 +699,7 @@ static fia_status pcg_run(fia_ctx *c, PcgState &ps, int max_iters, double *hs, c
         if (hs[kPcgDone] != 0.0 || ps.it >= max_iters) break;
         int sp;
         HIP_TRY(span_begin(c, SK_PCG, sp));
-        HIP_TRY(launch_pcg_iterations(ps, std::min(poll, max_iters - ps.it), c->stream, mg));
+        HIP_TRY(c->graphs.iterations(ps, std::min(poll, max_iters - ps.it), c->stream, mg));
         HIP_TRY(span_end(c, sp));
     }
     if (c->timing) {
@@ -861,7 +863,7 @@ fia_status scenes_solve_run(fia_ctx *c, fia_scene *scenes, const int *idx, int m
         bool all = true;
         for (int j = 0; j < m && all; ++j) all = hs[(size_t)j * kPcgScalars + kPcgDone] != 0.0;
         if (all || b.it >= max_iters) break;
-        HIP_TRY(launch_pcg_batch_iterations(b, std::min(poll, max_iters - b.it), c->stream, use_mg ? &mb : nullptr));
+        HIP_TRY(c->graphs.batch_iterations(b, std::min(poll, max_iters - b.it), c->stream, use_mg ? &mb : nullptr));
     }
     for (int j = 0; j < m; ++j) {
         const double *h = hs + (size_t)j * kPcgScalars;
@@ -985,6 +987,7 @@ fia_status fia_ctx_destroy(fia_ctx *c) {
     for (auto &sv : c->solves) sv.release();
     c->solves.clear();
     c->batch_states.release(); c->batch_map.release(); c->batch_scal.release();
+    c->graphs.release();
     if (c->host_batch) (void)hipHostFree(c->host_batch);
     c->bounds.release(); c->stats.release();
     for (auto &sp : c->spans) { (void)hipEventDestroy(sp.a); (void)hipEventDestroy(sp.b); }
@@ -1018,6 +1021,15 @@ fia_status fia_ctx_set_pressure_preconditioner(fia_ctx *c, fia_precond kind) {
     return FIA_OK;
 }
 
+fia_status fia_ctx_set_graphs(fia_ctx *c, int enable) {
+    if (!c) return fail(FIA_ERR_INVALID_ARG, "ctx is NULL");
+    HIP_TRY(set_device(c));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->graphs.enabled = enable != 0;
+    if (!enable) c->graphs.release();
+    return FIA_OK;
+}
+
 fia_status fia_ctx_reset_stats(fia_ctx *c) {
     if (!c) return fail(FIA_ERR_INVALID_ARG, "ctx is NULL");
     HIP_TRY(set_device(c));
@@ -1025,6 +1037,8 @@ fia_status fia_ctx_reset_stats(fia_ctx *c) {
     for (auto &sp : c->spans) { (void)hipEventDestroy(sp.a); (void)hipEventDestroy(sp.b); }
     c->spans.clear();
     c->acc = fia_kernel_stats{};
+    c->graph_captures0 = c->graphs.captures;
+    c->graph_replays0 = c->graphs.replays;
     if (c->stats.p) {
         HIP_TRY(hipMemsetAsync(c->stats.p, 0, kStatWords * kStatSlots * sizeof(unsigned long long), c->stream));
         HIP_TRY(hipStreamSynchronize(c->stream));
@@ -1050,6 +1064,8 @@ fia_status fia_ctx_get_stats(fia_ctx *c, fia_kernel_stats *out) {
     }
     c->spans.clear();
     *out = c->acc;
+    out->graph_captures = c->graphs.captures - c->graph_captures0;
+    out->graph_replays = c->graphs.replays - c->graph_replays0;
     if (c->stats.p) {
         std::vector<unsigned long long> h(kStatWords * kStatSlots);
         HIP_TRY(hipMemcpy(h.data(), c->stats.p, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));

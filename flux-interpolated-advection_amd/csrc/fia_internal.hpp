@@ -193,8 +193,13 @@ struct PcgGrid {
 constexpr unsigned kPcgUnknown = 0x80u;
 constexpr int kPcgMaxBlocks = 1024;
 // device scalars of one solve: rz of even and odd iterations, last residual r.r, b.b, tol^2,
-// converged flag, iterations done, scale applied to the initial guess
-enum { kPcgRZ = 0, kPcgRR = 3, kPcgBB = 4, kPcgTol2 = 5, kPcgDone = 6, kPcgIters = 7, kPcgGamma = 8, kPcgScalars = 16 };
+// converged flag, iterations done, scale applied to the initial guess, direction launches that
+// have run (the iteration index as the device counts it: a replayed graph passes the index it was
+// captured with)
+enum {
+    kPcgRZ = 0, kPcgRR = 3, kPcgBB = 4, kPcgTol2 = 5, kPcgDone = 6, kPcgIters = 7, kPcgGamma = 8, kPcgNext = 9,
+    kPcgScalars = 16
+};
 struct MgLevel;
 // one CG solve in flight: its vectors and device scalars
 struct PcgState {
@@ -292,6 +297,44 @@ hipError_t launch_pcg_batch_iterations(PcgBatch &b, int count, hipStream_t s, co
 hipError_t launch_mg_batch_vcycle(const PcgBatch &b, const MgBatch &mb, hipStream_t s);
 // the convergence test of every solve, then out[s * kPcgScalars ...] = scalars of solve s
 hipError_t launch_pcg_batch_check(const PcgBatch &b, double *out, hipStream_t s);
+
+// Batches of CG iterations replayed from captured HIP graphs.  The iteration kernels read the
+// index k only as k & 1 and k == 0 (the counts they report come from kPcgNext), so a batch of
+// `count` iterations starting at k >= 1 launches the same kernels with the same arguments as any
+// other batch of the same solve starting at an index of the same parity: one capture serves every
+// later batch.  Captures are keyed by everything the launches pass (buffers, tiling, multigrid
+// levels, count, parity); the cache keeps kPcgGraphSlots of them and evicts the least recently
+// used.  A batch at k = 0, on the null stream, or with graphs disabled is launched directly.
+constexpr int kPcgGraphSlots = 16;
+class PcgGraphCache {
+  public:
+    PcgGraphCache() = default;
+    PcgGraphCache(const PcgGraphCache &) = delete;
+    PcgGraphCache &operator=(const PcgGraphCache &) = delete;
+    ~PcgGraphCache() { release(); }
+    void release();
+    // st's (or every batched solve's) next `count` iterations; st.it (b.it) advances by count
+    hipError_t iterations(PcgState &st, int count, hipStream_t s, const MgHier *mg);
+    hipError_t batch_iterations(PcgBatch &b, int count, hipStream_t s, const MgBatch *mb);
+    bool enabled = true;
+    long long captures = 0, replays = 0;
+
+  private:
+    struct Entry {
+        hipGraphExec_t exec = nullptr;
+        bool batch = false, has_mg = false;
+        int count = 0, parity = 0;
+        PcgState st{};
+        MgHier mg{};
+        PcgBatch b{};
+        MgBatch mb{};
+        unsigned long long used = 0;
+    };
+    Entry *find(const Entry &key);
+    hipError_t capture_and_launch(Entry &key, hipStream_t s);
+    Entry slots_[kPcgGraphSlots];
+    unsigned long long tick_ = 0;
+};
 hipError_t launch_project(const PcgGrid &g, const double *p, double h, float *u, float *v, float *w,
                           float *pressure_out, hipStream_t s);
 hipError_t launch_zero_solid_faces(const PcgGrid &g, float *u, float *v, float *w, hipStream_t s);

@@ -218,6 +218,7 @@ __device__ __forceinline__ void pcg_start_body(const PcgState &st, int blk, doub
         sc[kPcgTol2] = tol2;
         sc[kPcgDone] = 0.0;
         sc[kPcgIters] = 0.0;
+        sc[kPcgNext] = 0.0;
     }
 }
 
@@ -417,7 +418,8 @@ __device__ __forceinline__ void pcg_direction_body(const PcgState &st, int blk, 
         sc[kPcgRZ + (k & 1)] = rz;
         sc[kPcgRR] = rr;
         if (k == 0) sc[kPcgBB] = bb;   // later kernels read it
-        sc[kPcgIters] = (double)k;
+        sc[kPcgIters] = sc[kPcgNext];  // k, counted on the device
+        sc[kPcgNext] += 1.0;
         if (done) sc[kPcgDone] = 1.0;
     }
     if (done) return;
@@ -1038,6 +1040,133 @@ hipError_t launch_pcg_batch_check(const PcgBatch &b, double *out, hipStream_t s)
     hipLaunchKernelGGL(pcg_check_batch_kernel, dim3(b.n), dim3(kPB), 0, s, b.states, b.it, out);
     return hipGetLastError();
 }
+// ---- graph replay of iteration batches ----------------------------------------------------------
+
+// Field-wise equality of everything the iteration launches pass (the iteration index excepted).
+static bool same_launch(const PcgState &a, const PcgState &b) {
+    return a.g.nx == b.g.nx && a.g.ny == b.g.ny && a.g.nz == b.g.nz && a.g.solid == b.g.solid && a.g.pin == b.g.pin &&
+           a.flags == b.flags && a.b == b.b && a.x == b.x && a.r == b.r && a.p[0] == b.p[0] && a.p[1] == b.p[1] &&
+           a.q == b.q && a.scal == b.scal && a.part_start == b.part_start && a.part_dir == b.part_dir &&
+           a.part_update == b.part_update && a.nblocks == b.nblocks && a.dir_blocks == b.dir_blocks &&
+           a.dir_tiles_y == b.dir_tiles_y && a.dir_tiles_z == b.dir_tiles_z && a.dir_chunk_x == b.dir_chunk_x &&
+           a.dir_chunks == b.dir_chunks && a.recompute_q == b.recompute_q && a.upd_blocks == b.upd_blocks &&
+           a.z == b.z && a.part_mg == b.part_mg && a.mg_lv == b.mg_lv && a.mg_levels == b.mg_levels;
+}
+static bool same_launch(const MgHier &a, const MgHier &b) {
+    if (a.levels != b.levels) return false;
+    for (int l = 0; l < a.levels; ++l) {
+        const MgLevel &x = a.lv[l], &y = b.lv[l];
+        if (x.nx != y.nx || x.ny != y.ny || x.nz != y.nz || x.op != y.op || x.b != y.b || x.x != y.x || x.xs != y.xs)
+            return false;
+    }
+    return true;
+}
+static bool same_launch(const PcgBatch &a, const PcgBatch &b) {
+    return a.states == b.states && a.dir_map == b.dir_map && a.stream_map == b.stream_map && a.tile_map == b.tile_map &&
+           a.n == b.n && a.dir_blocks == b.dir_blocks && a.stream_blocks == b.stream_blocks &&
+           a.tile_blocks == b.tile_blocks;
+}
+static bool same_launch(const MgBatch &a, const MgBatch &b) {
+    if (a.r0_map != b.r0_map || a.r0_blocks != b.r0_blocks) return false;
+    for (int l = 0; l < kMgMaxLevels; ++l)
+        if (a.down_map[l] != b.down_map[l] || a.down_blocks[l] != b.down_blocks[l] || a.up_map[l] != b.up_map[l] ||
+            a.up_blocks[l] != b.up_blocks[l])
+            return false;
+    return true;
+}
+
+void PcgGraphCache::release() {
+    for (Entry &e : slots_) {
+        if (e.exec) (void)hipGraphExecDestroy(e.exec);
+        e = Entry{};
+    }
+}
+
+PcgGraphCache::Entry *PcgGraphCache::find(const Entry &k) {
+    for (Entry &e : slots_) {
+        if (!e.exec || e.batch != k.batch || e.count != k.count || e.parity != k.parity || e.has_mg != k.has_mg) continue;
+        if (k.batch ? same_launch(e.b, k.b) && (!k.has_mg || same_launch(e.mb, k.mb))
+                    : same_launch(e.st, k.st) && (!k.has_mg || same_launch(e.mg, k.mg)))
+            return &e;
+    }
+    return nullptr;
+}
+
+// Captures key's batch on stream s (thread-local capture: other threads' contexts keep launching),
+// stores the executable graph in the least recently used slot and launches it.
+hipError_t PcgGraphCache::capture_and_launch(Entry &key, hipStream_t s) {
+    Entry *slot = &slots_[0];
+    for (Entry &e : slots_)
+        if (!e.exec) { slot = &e; break; } else if (e.used < slot->used) slot = &e;
+    if (slot->exec) (void)hipGraphExecDestroy(slot->exec);
+    *slot = Entry{};
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return e;
+    if (key.batch) {
+        PcgBatch b = key.b;
+        e = launch_pcg_batch_iterations(b, key.count, s, key.has_mg ? &key.mb : nullptr);
+    } else {
+        PcgState st = key.st;
+        e = launch_pcg_iterations(st, key.count, s, key.has_mg ? &key.mg : nullptr);
+    }
+    hipGraph_t graph = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(s, &graph);
+    if (e == hipSuccess) e = e2;
+    hipGraphExec_t exec = nullptr;
+    if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return e;
+    *slot = key;
+    slot->exec = exec;
+    slot->used = ++tick_;
+    ++captures;
+    ++replays;
+    return hipGraphLaunch(exec, s);
+}
+
+hipError_t PcgGraphCache::iterations(PcgState &st, int count, hipStream_t s, const MgHier *mg) {
+    if (!enabled || st.it == 0 || s == nullptr || count <= 0) return launch_pcg_iterations(st, count, s, mg);
+    Entry key;
+    key.batch = false;
+    key.count = count;
+    key.parity = st.it & 1;
+    key.st = st;
+    key.has_mg = mg != nullptr;
+    if (mg) key.mg = *mg;
+    hipError_t e;
+    if (Entry *hit = find(key)) {
+        hit->used = ++tick_;
+        ++replays;
+        e = hipGraphLaunch(hit->exec, s);
+    } else {
+        e = capture_and_launch(key, s);
+    }
+    if (e == hipSuccess) st.it += count;
+    return e;
+}
+
+hipError_t PcgGraphCache::batch_iterations(PcgBatch &b, int count, hipStream_t s, const MgBatch *mb) {
+    if (!enabled || b.it == 0 || s == nullptr || count <= 0 || b.n == 0)
+        return launch_pcg_batch_iterations(b, count, s, mb);
+    Entry key;
+    key.batch = true;
+    key.count = count;
+    key.parity = b.it & 1;
+    key.b = b;
+    key.has_mg = mb != nullptr;
+    if (mb) key.mb = *mb;
+    hipError_t e;
+    if (Entry *hit = find(key)) {
+        hit->used = ++tick_;
+        ++replays;
+        e = hipGraphLaunch(hit->exec, s);
+    } else {
+        e = capture_and_launch(key, s);
+    }
+    if (e == hipSuccess) b.it += count;
+    return e;
+}
+
 void mg_shape(int nx, int ny, int nz, MgHier &h) {
     h.levels = 0;
     int x = nx, y = ny, z = nz;

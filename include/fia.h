@@ -231,6 +231,8 @@ typedef struct fia_kernel_stats {
     long long pcg_solves;        /* pressure solves                                                 */
     long long twisted_faces;     /* faces cut by the recursive (general) cutter, both orientations  */
     long long general_pieces;    /* the pieces it made of them                                      */
+    long long graph_captures;    /* CG iteration batches captured into HIP graphs (fia_ctx_set_graphs) */
+    long long graph_replays;     /* CG iteration batches launched from captured graphs              */
 } fia_kernel_stats;
 
 fia_status fia_ctx_set_timing(fia_ctx *ctx, int enable);
@@ -239,6 +241,12 @@ fia_status fia_ctx_set_timing(fia_ctx *ctx, int enable);
  * counterpart in the reference, whose solver is fixed; fia_scenes_step_batched takes it from
  * ctxs[0]. */
 fia_status fia_ctx_set_pressure_preconditioner(fia_ctx *ctx, fia_precond kind);
+/* Whether the pressure solves of this context replay their CG iteration batches from captured HIP
+ * graphs (on by default; a new capture only when the solve's buffers, grid or multigrid levels
+ * change, up to 16 kept per context).  The results are bit-identical either way.  No counterpart
+ * in the reference (amgcl on the CPU, src/calc_pressure_3d.cpp:141-166).  Synchronises; disabling
+ * frees the captured graphs. */
+fia_status fia_ctx_set_graphs(fia_ctx *ctx, int enable);
 fia_status fia_ctx_get_stats(fia_ctx *ctx, fia_kernel_stats *out);   /* synchronises */
 fia_status fia_ctx_reset_stats(fia_ctx *ctx);
 
