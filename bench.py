@@ -256,6 +256,11 @@ class PlumeRun:
         for i in range(warmup):
             self.step()
             log("warm-up step %d: %d CG iterations" % (i, self.iters[-1]))
+        # launch shapes for this grid, timed on copies of the warmed-up state (fia_ctx_autotune)
+        self.shapes = None
+        if self.dts:
+            self.shapes = self.ctx.autotune(self.grid(), self.od, self.dts[-1], solid=self.t["solid"], reps=2)
+            log("autotuned launch shapes: %s" % self.shapes)
         self.iters, self.res, self.dts = [], [], []
         torch.cuda.synchronize()
         self.ctx.set_timing(True)
@@ -330,7 +335,7 @@ def run_plume(args, fia, scenes, torch):
     run = PlumeRun(fia, torch, n, state, "jacobi", args.cfl)
     sec, st = run.timed(args.steps, args.warmup, log)
     mass0 = fia.grid_totals(run.ctx, run.grid())[0]
-    main = {"iters": list(run.iters), "res": list(run.res), "dts": list(run.dts)}
+    main = {"iters": list(run.iters), "res": list(run.res), "dts": list(run.dts), "shapes": run.shapes}
     # the mass change of one density step on the final state (no source, no forcing)
     g = run.grid()
     before = float(g.density.double().sum().item())
@@ -359,6 +364,7 @@ def run_plume(args, fia, scenes, torch):
                                "advection (WENO6, integral psi, xz-CellFaceAligned density split, zero-velocity "
                                "correction on), Jacobi-PCG projection to 1e-6 from a zero guess" % (n, args.cfl),
                    "grid": [n, n, n], "cfl": args.cfl, "preconditioner": "jacobi",
+                   "launch_shapes": main["shapes"],
                    "piece_rule": "the reference's: one psi point per piece plus 1-point Gauss along y for the "
                                  "density term; the 4x4 per-piece Gauss rule the configuration names is not built"},
         "roofline": pcg_roofline(st, n, "jacobi"),
@@ -512,9 +518,11 @@ def run_scenes(args, fia, torch):
             nets.append(u)
     od = fia.make_opts(split_method="xz-CellFaceAligned", interpolation_method="WENO6")
     ov = fia.make_opts(interpolation_method="WENO6")
-    # the autotuner's steps are the warm-up
+    # the autotuner's steps are the warm-up; the launch shapes are then tuned per grid edge on the
+    # moving scenes (on copies)
     best, tune = batch.autotune_workers(sset, ctxs, od, ov, candidates, nets=nets, batched=args.batched,
                                         log=lambda s: None)
+    shapes = batch.autotune_shapes(sset, ctxs, od, log=lambda s: None)
     for _ in range(args.warmup):
         sset.step(ctxs[:best], od, ov, nets=nets[:best] if nets else None, batched=args.batched)
     torch.cuda.synchronize()
@@ -539,7 +547,8 @@ def run_scenes(args, fia, torch):
                                                        "plain PCG" if args.plain else "U-Net warm start",
                                                        "; pressure solves batched" if args.batched else ""),
                    "scenes": args.scenes, "cells": sset.cells},
-        "scheduler": {"workers": best, "autotune_ms_per_step": {str(k): round(v * 1e3, 1) for k, v in tune.items()}},
+        "scheduler": {"workers": best, "autotune_ms_per_step": {str(k): round(v * 1e3, 1) for k, v in tune.items()},
+                      "launch_shapes": {str(k): v for k, v in sorted(shapes.items())}},
         "pcg": st,
     }
 

@@ -42,7 +42,7 @@ EXPORTS = ["fia_default_opts", "fia_ctx_create", "fia_ctx_create_on_stream", "fi
            "fia_unet_get_params", "fia_unet_forward", "fia_unet_loss_grad", "fia_unet_train_step", "fia_pressure_rhs",
            "fia_pressure_project_learned", "fia_grid_totals", "fia_grid_totals_noalloc", "fia_wall_momentum_loss", "fia_scene_forcing",
            "fia_scenes_step", "fia_scenes_step_batched", "fia_ctx_set_pressure_preconditioner",
-           "fia_ctx_set_graphs"]
+           "fia_ctx_set_graphs", "fia_ctx_set_tuning", "fia_ctx_get_tuning", "fia_ctx_autotune"]
 
 
 class FiaError(RuntimeError):
@@ -100,6 +100,15 @@ class KernelStats(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class TuningC(ctypes.Structure):
+    """fia_tuning: launch shapes of one grid shape (0 / -1 = the built-in choice)."""
+    _fields_ = [("face_block", ctypes.c_int), ("piece_block", ctypes.c_int), ("cell_block", ctypes.c_int),
+                ("pcg_chunk_x", ctypes.c_int), ("pcg_recompute", ctypes.c_int)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 _lib = None
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
@@ -138,6 +147,10 @@ def lib():
         L.fia_ctx_set_timing.argtypes = [_vp, _i]
         L.fia_ctx_set_pressure_preconditioner.argtypes = [_vp, _i]
         L.fia_ctx_set_graphs.argtypes = [_vp, _i]
+        L.fia_ctx_set_tuning.argtypes = [_vp, _i, _i, _i, ctypes.POINTER(TuningC)]
+        L.fia_ctx_get_tuning.argtypes = [_vp, _i, _i, _i, ctypes.POINTER(TuningC)]
+        L.fia_ctx_autotune.argtypes = [_vp, ctypes.POINTER(GridC), _vp, _d, ctypes.POINTER(AdvectOpts), _i,
+                                       ctypes.POINTER(TuningC)]
         L.fia_ctx_get_stats.argtypes = [_vp, ctypes.POINTER(KernelStats)]
         L.fia_ctx_reset_stats.argtypes = [_vp]
         L.fia_unet_param_count.restype = ctypes.c_size_t
@@ -213,6 +226,30 @@ class Context:
         """Replay the pressure solves' CG iteration batches from captured HIP graphs (the default)
         or launch every kernel directly; the results are bit-identical."""
         _check(lib().fia_ctx_set_graphs(self._h, 1 if enable else 0))
+
+    def set_tuning(self, shape, **shapes):
+        """Launch shapes (fia_tuning fields) for grids of shape (nx, ny, nz); shape None sets the
+        default for shapes without their own entry.  Unnamed fields keep their built-in choice."""
+        t = TuningC(0, 0, 0, 0, -1)
+        for k, v in shapes.items():
+            setattr(t, k, int(v))
+        nx, ny, nz = shape if shape is not None else (0, 0, 0)
+        _check(lib().fia_ctx_set_tuning(self._h, nx, ny, nz, ctypes.byref(t)))
+
+    def get_tuning(self, shape):
+        t = TuningC()
+        _check(lib().fia_ctx_get_tuning(self._h, shape[0], shape[1], shape[2], ctypes.byref(t)))
+        return t.as_dict()
+
+    def autotune(self, grid, opts, dt, solid=None, reps=3):
+        """fia_ctx_autotune: times the candidate launch shapes on copies of grid (left unchanged),
+        stores the fastest for its shape and returns them as a dict."""
+        g = grid._c(with_density=True)
+        sp, keep = _solid_ptr(solid, g.on_device)
+        t = TuningC()
+        _check(lib().fia_ctx_autotune(self._h, ctypes.byref(g), sp, float(dt), ctypes.byref(opts), int(reps),
+                                      ctypes.byref(t)))
+        return t.as_dict()
 
     def close(self):
         if self._h:

@@ -1,7 +1,7 @@
 """Many independent smoke scenes of uneven sizes advanced together (fia_scenes_step, or
-fia_scenes_step_batched with the pressure solves of all scenes batched), with an autotuner for the
-number of concurrent workers.  No counterpart in the reference, which runs one
-grid per process.
+fia_scenes_step_batched with the pressure solves of all scenes batched), with autotuners for the
+number of concurrent workers and for the launch shapes of each grid size (fia_ctx_autotune).  No
+counterpart in the reference, which runs one grid per process.
 
 Each scene is seeded: cubic grid edge drawn from `edges`, a density source sphere (radius 0.06)
 in the lower part of the box, buoyancy in [0.5, 2] on the vertical velocity, 0 - max_obstacles
@@ -70,6 +70,28 @@ class SceneSet:
         return {"pcg_iters_mean": float(np.mean([s.pcg_iters for s in a])),
                 "pcg_residual_max": float(max(s.pcg_residual for s in a)),
                 "network_used": int(sum(s.used_network for s in a)), "steps": int(min(s.steps for s in a))}
+
+
+def autotune_shapes(sset, ctxs, density_opts, reps=2, log=print):
+    """fia_ctx_autotune on one scene of each grid edge (on copies: the scenes do not move), with dt
+    from the scene's CFL number; the picked launch shapes go to every context.  Returns
+    {edge: shapes}."""
+    import torch
+    fia, scenes = _pkg()
+    picked = {}
+    for k, n in enumerate(sset.edges):
+        if n in picked:
+            continue
+        t = sset.tensors[k]
+        g = fia.Grid3D(n, n, n, 1.0 / n, t["rho"], t["u"], t["v"], t["w"])
+        vmax = max(float(t[f].abs().max()) for f in ("u", "v", "w"))
+        dt = sset.array[k].cfl * (1.0 / n) / max(vmax, 1e-3)
+        picked[n] = ctxs[0].autotune(g, density_opts, dt, solid=t["solid"], reps=reps)
+        for c in ctxs[1:]:
+            c.set_tuning((n, n, n), **picked[n])
+        log("autotune: %d^3 %s" % (n, picked[n]))
+    torch.cuda.synchronize()
+    return picked
 
 
 def autotune_workers(sset, ctxs, density_opts, velocity_opts, candidates, nets=None, batched=False, log=print):

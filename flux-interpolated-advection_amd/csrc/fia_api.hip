@@ -155,6 +155,13 @@ struct fia_ctx {
     double *host_batch = nullptr;   // pinned: the scalars of every batched solve
     size_t host_batch_n = 0;
     PcgGraphCache graphs;           // captured CG iteration batches (fia_ctx_set_graphs)
+    // launch shapes per cell-grid shape (fia_ctx_set_tuning, fia_ctx_autotune) and for the rest
+    struct ShapeTuning {
+        int nx, ny, nz;
+        fia_tuning t;
+    };
+    std::vector<ShapeTuning> tunings;
+    fia_tuning tuning_default{0, 0, 0, 0, -1};
     long long graph_captures0 = 0, graph_replays0 = 0;   // the cache's counters at the last stats reset
     // instrumentation
     bool timing = false;
@@ -198,6 +205,13 @@ size_t ctx_bytes(const fia_ctx *c) {
 }
 
 hipError_t set_device(fia_ctx *c) { return hipSetDevice(c->device); }
+
+// The launch shapes for grids of nx x ny x nz cells (and their staggered components).
+const fia_tuning &tuning_for(const fia_ctx *c, int nx, int ny, int nz) {
+    for (const auto &s : c->tunings)
+        if (s.nx == nx && s.ny == ny && s.nz == nz) return s.t;
+    return c->tuning_default;
+}
 
 enum SpanKind { SK_FACE = 0, SK_VERTEX = 1, SK_PSI = 2, SK_GATHER = 3, SK_QUAD = 4, SK_PCG = 5 };
 
@@ -325,6 +339,9 @@ fia_status do_faces(fia_ctx *c, const CompGrid &g, const float *q, bool moving, 
     p.kappa_psi = (float)(h * h);   // psi is stored in units of q*cells; S in physical units
     p.min_area = s.min_area;
     p.gauss = make_gauss(s.ng);
+    const fia_tuning &tu = tuning_for(c, g.n[0], g.n[1], g.n[2]);
+    p.face_block = tu.face_block;
+    p.piece_block = tu.piece_block;
     if (c->timing) {
         HIP_TRY(c->stats.ensure(kStatWords * kStatSlots));
         p.stats = c->stats.p;
@@ -418,7 +435,8 @@ hipError_t timed_gather(fia_ctx *c, const CompGrid &g, FluxSet &s1, FluxSet *s2,
     double *f2[3] = {nullptr, nullptr, nullptr}, *c2[3] = {nullptr, nullptr, nullptr};
     if (s2)
         for (int a = 0; a < 3; ++a) { f2[a] = s2->f[a].p; c2[a] = s2->c[a].p; }
-    e = launch_gather(g, f1, c1, s2 ? f2 : nullptr, s2 ? c2 : nullptr, base, out, mode, fm, c->stream);
+    e = launch_gather(g, f1, c1, s2 ? f2 : nullptr, s2 ? c2 : nullptr, base, out, mode, fm, c->stream,
+                      tuning_for(c, g.n[0], g.n[1], g.n[2]).cell_block);
     if (e != hipSuccess) return e;
     if (c->timing) c->acc.gather_cells += (long long)g.cells();
     return span_end(c, sp);
@@ -582,10 +600,10 @@ static fia_status pressure_system(fia_ctx *c, const fia_grid3d *g, const GridVie
 
 // Buffers B (allocated here for the grid of pg) bound into a CG state over the system (b, flags).
 static fia_status bind_solve(fia_ctx::PcgBufs &B, const PcgGrid &pg, const double *b, const uint8_t *flags,
-                             PcgState &ps, bool mg = false) {
+                             PcgState &ps, bool mg, const fia_tuning &tu) {
     const size_t n = (size_t)pg.nx * pg.ny * pg.nz;
     ps.nblocks = pcg_blocks(n);
-    pcg_direction_tiling(pg.nx, pg.ny, pg.nz, ps);
+    pcg_direction_tiling(pg.nx, pg.ny, pg.nz, ps, tu.pcg_chunk_x, tu.pcg_recompute);
     HIP_TRY(B.x.ensure(n)); HIP_TRY(B.r.ensure(n)); HIP_TRY(B.p0.ensure(n)); HIP_TRY(B.p1.ensure(n));
     HIP_TRY(B.part.ensure(pcg_partial_words(ps))); HIP_TRY(B.scal.ensure(kPcgScalars));
     if (!ps.recompute_q) HIP_TRY(B.q.ensure(n));
@@ -637,7 +655,8 @@ static fia_status mg_prepare(fia_ctx *c, const PcgState &ps, const MgHier **out)
 
 // Solve k's buffers bound into a CG state over the context's system.
 static fia_status pcg_bind(fia_ctx *c, int k, const PcgGrid &pg, PcgState &ps) {
-    return bind_solve(c->pcgs[k], pg, c->pcg_b.p, c->pcg_flags.p, ps, c->precond == FIA_PRECOND_MG);
+    return bind_solve(c->pcgs[k], pg, c->pcg_b.p, c->pcg_flags.p, ps, c->precond == FIA_PRECOND_MG,
+                      tuning_for(c, pg.nx, pg.ny, pg.nz));
 }
 
 // The start of a solve: r = b - L x and the scalars, then z = M r under multigrid.
@@ -751,7 +770,8 @@ fia_status scene_solve_setup(fia_ctx *cw, fia_ctx *main, int s, fia_scene *sc, f
     const double tol2 = rel_tol * rel_tol;
     const bool use_mg = main->precond == FIA_PRECOND_MG;
     PcgState sa;
-    if ((st = bind_solve(sv.slot[0], sv.pg, sv.b.p, sv.flags.p, sa, use_mg)) != FIA_OK) return st;
+    const fia_tuning &tu = tuning_for(main, sv.pg.nx, sv.pg.ny, sv.pg.nz);
+    if ((st = bind_solve(sv.slot[0], sv.pg, sv.b.p, sv.flags.p, sa, use_mg, tu)) != FIA_OK) return st;
     const MgHier *mg = nullptr;
     if (use_mg) {
         if ((st = mg_setup(cw->stream, sv.mgb, sv.mgh, sa)) != FIA_OK) return st;
@@ -763,7 +783,7 @@ fia_status scene_solve_setup(fia_ctx *cw, fia_ctx *main, int s, fia_scene *sc, f
     }
     if (net) {
         PcgState sb;
-        if ((st = bind_solve(sv.slot[1], sv.pg, sv.b.p, sv.flags.p, sb, use_mg)) != FIA_OK) return st;
+        if ((st = bind_solve(sv.slot[1], sv.pg, sv.b.p, sv.flags.p, sb, use_mg, tu)) != FIA_OK) return st;
         sb.mg_lv = sa.mg_lv;
         sb.mg_levels = sa.mg_levels;
         LearnedStart ls;
@@ -1027,6 +1047,199 @@ fia_status fia_ctx_set_graphs(fia_ctx *c, int enable) {
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->graphs.enabled = enable != 0;
     if (!enable) c->graphs.release();
+    return FIA_OK;
+}
+
+static bool tuning_valid(const fia_tuning *t) {
+    auto blk = [](int b) { return b == 0 || b == 64 || b == 128 || b == 256; };
+    return blk(t->face_block) && blk(t->piece_block) && blk(t->cell_block) && t->pcg_chunk_x >= 0 &&
+           t->pcg_recompute >= -1 && t->pcg_recompute <= 1;
+}
+
+fia_status fia_ctx_set_tuning(fia_ctx *c, int nx, int ny, int nz, const fia_tuning *t) {
+    if (!c || !t) return fail(FIA_ERR_INVALID_ARG, "NULL argument");
+    if (!tuning_valid(t)) return fail(FIA_ERR_INVALID_ARG, "block sizes must be 0, 64, 128 or 256, the chunk >= 0 "
+                                                           "and pcg_recompute -1, 0 or 1");
+    if (nx == 0 && ny == 0 && nz == 0) {
+        c->tuning_default = *t;
+        return FIA_OK;
+    }
+    if (nx <= 0 || ny <= 0 || nz <= 0) return fail(FIA_ERR_INVALID_ARG, "grid shape must be positive (or all zero)");
+    for (auto &s : c->tunings)
+        if (s.nx == nx && s.ny == ny && s.nz == nz) {
+            s.t = *t;
+            return FIA_OK;
+        }
+    if (c->tunings.size() >= 64) return fail(FIA_ERR_CAPACITY, "64 grid shapes already have their own launch shapes");
+    c->tunings.push_back({nx, ny, nz, *t});
+    return FIA_OK;
+}
+
+fia_status fia_ctx_get_tuning(fia_ctx *c, int nx, int ny, int nz, fia_tuning *t) {
+    if (!c || !t) return fail(FIA_ERR_INVALID_ARG, "NULL argument");
+    *t = tuning_for(c, nx, ny, nz);
+    return FIA_OK;
+}
+
+namespace {
+// The autotuner's copies of a grid and its timing events, released on every path.
+struct TuneScratch {
+    DevBuf<float> src[4], work[4];
+    DevBuf<uint8_t> solid;
+    hipEvent_t a = nullptr, b = nullptr;
+    ~TuneScratch() {
+        for (int k = 0; k < 4; ++k) { src[k].release(); work[k].release(); }
+        solid.release();
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+int effective_block(int b) { return (b == 64 || b == 128) ? b : 256; }
+}  // namespace
+
+// Coordinate descent over the launch shapes of g's grid shape, timed on copies of g: each block
+// size in turn (faces, pieces, cells) over {64, 128, 256} by whole density steps, then the CG
+// chunking by iterations of g's pressure system.  A candidate replaces the current choice only
+// when it is at least 2% faster (timer noise does not move the choice back and forth).
+fia_status fia_ctx_autotune(fia_ctx *c, const fia_grid3d *g, const uint8_t *solid, double dt,
+                            const fia_advect_opts *o, int reps, fia_tuning *out) {
+    if (!c) return fail(FIA_ERR_INVALID_ARG, "ctx is NULL");
+    fia_status st;
+    if ((st = check_grid(g, true)) != FIA_OK) return st;
+    if ((st = check_opts(o)) != FIA_OK) return st;
+    if (!(dt > 0.0) || !isfinite(dt)) return fail(FIA_ERR_INVALID_ARG, "dt must be positive and finite");
+    if (reps < 1 || reps > 50) return fail(FIA_ERR_INVALID_ARG, "reps must be in 1..50");
+    HIP_TRY(set_device(c));
+    const size_t nc = (size_t)g->nx * g->ny * g->nz;
+    const size_t len[4] = {nc, (size_t)(g->nx + 1) * g->ny * g->nz, (size_t)g->nx * (g->ny + 1) * g->nz,
+                           (size_t)g->nx * g->ny * (g->nz + 1)};
+    const float *fields[4] = {g->density, g->u, g->v, g->w};
+    const hipMemcpyKind kind = g->on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    TuneScratch ts;
+    for (int k = 0; k < 4; ++k) {
+        HIP_TRY(ts.src[k].ensure(len[k]));
+        HIP_TRY(ts.work[k].ensure(len[k]));
+        HIP_TRY(hipMemcpyAsync(ts.src[k].p, fields[k], len[k] * 4, kind, c->stream));
+    }
+    if (solid) {
+        HIP_TRY(ts.solid.ensure(nc));
+        HIP_TRY(hipMemcpyAsync(ts.solid.p, solid, nc, kind, c->stream));
+    }
+    HIP_TRY(hipEventCreate(&ts.a));
+    HIP_TRY(hipEventCreate(&ts.b));
+    fia_grid3d wg = *g;
+    wg.on_device = 1;
+    wg.density = ts.work[0].p; wg.u = ts.work[1].p; wg.v = ts.work[2].p; wg.w = ts.work[3].p;
+    auto restore = [&]() -> hipError_t {
+        for (int k = 0; k < 4; ++k) {
+            const hipError_t e = hipMemcpyAsync(ts.work[k].p, ts.src[k].p, len[k] * 4, hipMemcpyDeviceToDevice, c->stream);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    };
+    auto elapsed = [&](double &ms) -> hipError_t {
+        hipError_t e = hipEventSynchronize(ts.b);
+        float f = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&f, ts.a, ts.b);
+        ms = f;
+        return e;
+    };
+    const bool timing = c->timing;
+    c->timing = false;
+    struct RestoreTiming {
+        fia_ctx *c;
+        bool on;
+        ~RestoreTiming() { c->timing = on; }
+    } keep{c, timing};
+    fia_tuning best = tuning_for(c, g->nx, g->ny, g->nz);
+    if ((st = fia_ctx_set_tuning(c, g->nx, g->ny, g->nz, &best)) != FIA_OK) return st;
+    // one density step on fresh copies: the fastest of reps timed runs after an untimed one
+    auto time_step = [&](double &ms) -> fia_status {
+        ms = INFINITY;
+        for (int r = 0; r <= reps; ++r) {
+            HIP_TRY(restore());
+            HIP_TRY(hipEventRecord(ts.a, c->stream));
+            fia_status s2 = fia_move_substances_3d(c, &wg, 0, dt, o);
+            if (s2 != FIA_OK) return s2;
+            HIP_TRY(hipEventRecord(ts.b, c->stream));
+            double t;
+            HIP_TRY(elapsed(t));
+            if (r > 0) ms = std::min(ms, t);
+        }
+        return FIA_OK;
+    };
+    double best_ms;
+    if ((st = time_step(best_ms)) != FIA_OK) return st;
+    int fia_tuning::*const knobs[3] = {&fia_tuning::face_block, &fia_tuning::piece_block, &fia_tuning::cell_block};
+    for (int k = 0; k < 3; ++k) {
+        for (int b : {64, 128, 256}) {
+            if (effective_block(best.*knobs[k]) == b) continue;
+            fia_tuning t = best;
+            t.*knobs[k] = b;
+            if ((st = fia_ctx_set_tuning(c, g->nx, g->ny, g->nz, &t)) != FIA_OK) return st;
+            double ms;
+            if ((st = time_step(ms)) != FIA_OK) return st;
+            if (ms < 0.98 * best_ms) { best = t; best_ms = ms; }
+        }
+        if ((st = fia_ctx_set_tuning(c, g->nx, g->ny, g->nz, &best)) != FIA_OK) return st;
+    }
+    // the CG sweep: iterations of g's pressure system (never converging: tolerance 0) under the
+    // context's preconditioner, per chunk length and q policy
+    HIP_TRY(restore());
+    GridView gv{wg.density, wg.u, wg.v, wg.w};
+    PcgGrid pg;
+    if ((st = pressure_system(c, &wg, gv, solid ? ts.solid.p : nullptr, pg)) != FIA_OK) return st;
+    const bool use_mg = c->precond == FIA_PRECOND_MG;
+    const int iters = use_mg ? 8 : 24;
+    auto time_cg = [&](const fia_tuning &t, double &ms) -> fia_status {
+        PcgState ps;
+        fia_status s2;
+        if ((s2 = bind_solve(c->pcgs[0], pg, c->pcg_b.p, c->pcg_flags.p, ps, use_mg, t)) != FIA_OK) return s2;
+        const MgHier *mg = nullptr;
+        if ((s2 = mg_prepare(c, ps, &mg)) != FIA_OK) return s2;
+        HIP_TRY(launch_pcg_guess(ps, nullptr, c->stream));
+        if ((s2 = pcg_begin(c, ps, 0.0, mg)) != FIA_OK) return s2;
+        HIP_TRY(launch_pcg_iterations(ps, 2, c->stream, mg));
+        ms = INFINITY;
+        for (int r = 0; r < reps; ++r) {
+            HIP_TRY(hipEventRecord(ts.a, c->stream));
+            HIP_TRY(launch_pcg_iterations(ps, iters, c->stream, mg));
+            HIP_TRY(hipEventRecord(ts.b, c->stream));
+            double t2;
+            HIP_TRY(elapsed(t2));
+            ms = std::min(ms, t2 / iters);
+        }
+        return FIA_OK;
+    };
+    double best_cg;
+    if ((st = time_cg(best, best_cg)) != FIA_OK) return st;
+    std::vector<std::pair<int, bool>> tried;
+    {
+        PcgState ps;
+        pcg_direction_tiling(pg.nx, pg.ny, pg.nz, ps, best.pcg_chunk_x, best.pcg_recompute);
+        tried.push_back({ps.dir_chunk_x, ps.recompute_q});
+    }
+    for (int chunk : {2, 4, 8, 16, 32, 64, 128}) {
+        if (chunk > pg.nx) break;
+        for (int rq : {0, 1}) {
+            fia_tuning t = best;
+            t.pcg_chunk_x = chunk;
+            t.pcg_recompute = rq;
+            PcgState ps;
+            pcg_direction_tiling(pg.nx, pg.ny, pg.nz, ps, chunk, rq);
+            if (ps.dir_blocks > 4 * kPcgMaxBlocks) continue;   // every block re-sums the partials
+            bool seen = false;
+            for (const auto &p : tried) seen = seen || (p.first == ps.dir_chunk_x && p.second == ps.recompute_q);
+            if (seen) continue;
+            tried.push_back({ps.dir_chunk_x, ps.recompute_q});
+            double ms;
+            if ((st = time_cg(t, ms)) != FIA_OK) return st;
+            if (ms < 0.98 * best_cg) { best = t; best_cg = ms; }
+        }
+    }
+    if ((st = fia_ctx_set_tuning(c, g->nx, g->ny, g->nz, &best)) != FIA_OK) return st;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (out) *out = best;
     return FIA_OK;
 }
 

@@ -135,6 +135,10 @@ struct FaceParams {
     double *gen_ay;
     double *gen_val;
     GaussTable gauss;
+    // threads per block of the face kernels (count, cutter, sum, zero velocity) and of the
+    // quadrature kernel: 64, 128 or 256, anything else 256.  Each face and piece is one thread's
+    // work, so the results do not depend on them.
+    int face_block, piece_block;
 };
 // the general cutter's piece budget of one flux pass (records of 40 bytes): more pieces than this
 // report FIA_ERR_CAPACITY instead of running
@@ -173,8 +177,10 @@ hipError_t launch_faces_degenerate(const FaceParams &p, int count, hipStream_t s
 hipError_t launch_zero_velocity_faces(const FaceParams &p, hipStream_t s);
 // out = sum over families of flux differences (scaled); mode 0: out = G1; 1: out = G1 + G2;
 // 2: out = base - G1 (y family only).
+// block: cells per block (64, 128 or 256, anything else 256)
 hipError_t launch_gather(const CompGrid &g, double *const f1[3], double *const c1[3], double *const f2[3],
-                         double *const c2[3], const float *base, float *out, int mode, int fam_mask, hipStream_t s);
+                         double *const c2[3], const float *base, float *out, int mode, int fam_mask, hipStream_t s,
+                         int block = 0);
 hipError_t launch_interpolate(const float *field, int mx, int my, int mz, double ox, double oy, double oz,
                               int method, float kappa, const float *pts_cells, size_t n, float *out,
                               hipStream_t s);
@@ -226,7 +232,9 @@ struct PcgState {
     int it;             // iterations enqueued since the start (selects the p buffer)
 };
 int pcg_blocks(size_t n);
-void pcg_direction_tiling(int nx, int ny, int nz, PcgState &st);
+// chunk_x > 0: the x planes one tile sweeps (else about kPcgCellsPerBlock cells per block);
+// recompute 0 / 1: the update streams q / recomputes it (else chosen by grid size and chunk)
+void pcg_direction_tiling(int nx, int ny, int nz, PcgState &st, int chunk_x = 0, int recompute = -1);
 
 // Geometric multigrid preconditioner for the pressure system: level 0 is the system itself (flag
 // bytes), level l + 1 aggregates 2 x 2 x 2 cells of level l with the exact Galerkin operator of

@@ -24,6 +24,8 @@ namespace fia {
 constexpr int kBlock = 256;
 
 static inline unsigned grid_for(size_t n, int block) { return (unsigned)((n + block - 1) / block); }
+// a tunable block size: 64, 128 or 256 threads (the kernels' launch bound), anything else 256
+static inline int block_or_default(int b) { return (b == 64 || b == 128) ? b : kBlock; }
 
 // Every kernel is an index computation around a __host__ __device__ body, so the bodies can also
 // run over host arrays (a CPU emulation of the kernels used to check their logic, on host threads).
@@ -1201,9 +1203,10 @@ hipError_t launch_vertices(const VertexParams &p, hipStream_t s) {
 
 hipError_t launch_face_count(const FaceParams &p, hipStream_t s) {
     // every family writes its entries so the scan sees no stale counts
-    hipLaunchKernelGGL(face_count_kernel<0>, dim3(grid_for(p.g.faces(0), kBlock)), dim3(kBlock), 0, s, p);
-    hipLaunchKernelGGL(face_count_kernel<1>, dim3(grid_for(p.g.faces(1), kBlock)), dim3(kBlock), 0, s, p);
-    hipLaunchKernelGGL(face_count_kernel<2>, dim3(grid_for(p.g.faces(2), kBlock)), dim3(kBlock), 0, s, p);
+    const int b = block_or_default(p.face_block);
+    hipLaunchKernelGGL(face_count_kernel<0>, dim3(grid_for(p.g.faces(0), b)), dim3(b), 0, s, p);
+    hipLaunchKernelGGL(face_count_kernel<1>, dim3(grid_for(p.g.faces(1), b)), dim3(b), 0, s, p);
+    hipLaunchKernelGGL(face_count_kernel<2>, dim3(grid_for(p.g.faces(2), b)), dim3(b), 0, s, p);
     return hipGetLastError();
 }
 
@@ -1214,18 +1217,20 @@ int piece_quadrature_mode(const FaceParams &p) {
 
 hipError_t launch_pieces(const FaceParams &p, unsigned long long nslots, hipStream_t s) {
     if (nslots == 0) return hipSuccess;
-    const dim3 grid(grid_for(nslots, kBlock));
+    const int b = block_or_default(p.piece_block);
+    const dim3 grid(grid_for(nslots, b));
     const int qm = piece_quadrature_mode(p);
-    if (qm == 1) hipLaunchKernelGGL(piece_kernel<1>, grid, dim3(kBlock), 0, s, p, nslots);
-    else if (qm == 2) hipLaunchKernelGGL(piece_kernel<2>, grid, dim3(kBlock), 0, s, p, nslots);
-    else hipLaunchKernelGGL(piece_kernel<0>, grid, dim3(kBlock), 0, s, p, nslots);
+    if (qm == 1) hipLaunchKernelGGL(piece_kernel<1>, grid, dim3(b), 0, s, p, nslots);
+    else if (qm == 2) hipLaunchKernelGGL(piece_kernel<2>, grid, dim3(b), 0, s, p, nslots);
+    else hipLaunchKernelGGL(piece_kernel<0>, grid, dim3(b), 0, s, p, nslots);
     return hipGetLastError();
 }
 
 hipError_t launch_face_sum(const FaceParams &p, hipStream_t s) {
-    if (p.fam_mask & 1) hipLaunchKernelGGL(face_sum_kernel<0>, dim3(grid_for(p.g.faces(0), kBlock)), dim3(kBlock), 0, s, p);
-    if (p.fam_mask & 2) hipLaunchKernelGGL(face_sum_kernel<1>, dim3(grid_for(p.g.faces(1), kBlock)), dim3(kBlock), 0, s, p);
-    if (p.fam_mask & 4) hipLaunchKernelGGL(face_sum_kernel<2>, dim3(grid_for(p.g.faces(2), kBlock)), dim3(kBlock), 0, s, p);
+    const int b = block_or_default(p.face_block);
+    if (p.fam_mask & 1) hipLaunchKernelGGL(face_sum_kernel<0>, dim3(grid_for(p.g.faces(0), b)), dim3(b), 0, s, p);
+    if (p.fam_mask & 2) hipLaunchKernelGGL(face_sum_kernel<1>, dim3(grid_for(p.g.faces(1), b)), dim3(b), 0, s, p);
+    if (p.fam_mask & 4) hipLaunchKernelGGL(face_sum_kernel<2>, dim3(grid_for(p.g.faces(2), b)), dim3(b), 0, s, p);
     return hipGetLastError();
 }
 
@@ -1384,11 +1389,12 @@ static bool zero_velocity_cuts_both(const FaceParams &p) {
 }
 
 hipError_t launch_zero_velocity_faces(const FaceParams &p, hipStream_t s) {
-    const dim3 grid(grid_for(p.g.faces(1), kBlock));
+    const int b = block_or_default(p.face_block);
+    const dim3 grid(grid_for(p.g.faces(1), b));
     if (p.interp == IM_WENO6 && !p.integral && zero_velocity_cuts_both(p))
-        hipLaunchKernelGGL(zero_velocity_face_kernel<true>, grid, dim3(kBlock), 0, s, p);
+        hipLaunchKernelGGL(zero_velocity_face_kernel<true>, grid, dim3(b), 0, s, p);
     else
-        hipLaunchKernelGGL(zero_velocity_face_kernel<false>, grid, dim3(kBlock), 0, s, p);
+        hipLaunchKernelGGL(zero_velocity_face_kernel<false>, grid, dim3(b), 0, s, p);
     return hipGetLastError();
 }
 
@@ -1397,9 +1403,10 @@ hipError_t launch_zero_velocity_faces(const FaceParams &p, hipStream_t s) {
 static unsigned list_blocks(int count) { return (unsigned)std::min(1024, (count + 63) / 64); }
 
 hipError_t launch_seg_faces(const FaceParams &p, hipStream_t s) {
-    if (p.fam_mask & 1) hipLaunchKernelGGL(seg_face_kernel<0>, dim3(grid_for(p.g.faces(0), kBlock)), dim3(kBlock), 0, s, p);
-    if (p.fam_mask & 2) hipLaunchKernelGGL(seg_face_kernel<1>, dim3(grid_for(p.g.faces(1), kBlock)), dim3(kBlock), 0, s, p);
-    if (p.fam_mask & 4) hipLaunchKernelGGL(seg_face_kernel<2>, dim3(grid_for(p.g.faces(2), kBlock)), dim3(kBlock), 0, s, p);
+    const int b = block_or_default(p.face_block);
+    if (p.fam_mask & 1) hipLaunchKernelGGL(seg_face_kernel<0>, dim3(grid_for(p.g.faces(0), b)), dim3(b), 0, s, p);
+    if (p.fam_mask & 2) hipLaunchKernelGGL(seg_face_kernel<1>, dim3(grid_for(p.g.faces(1), b)), dim3(b), 0, s, p);
+    if (p.fam_mask & 4) hipLaunchKernelGGL(seg_face_kernel<2>, dim3(grid_for(p.g.faces(2), b)), dim3(b), 0, s, p);
     return hipGetLastError();
 }
 
@@ -1428,7 +1435,8 @@ hipError_t launch_general_pieces(const FaceParams &p, int count, unsigned long l
 }
 
 hipError_t launch_gather(const CompGrid &g, double *const f1[3], double *const c1[3], double *const f2[3],
-                         double *const c2[3], const float *base, float *out, int mode, int fam_mask, hipStream_t s) {
+                         double *const c2[3], const float *base, float *out, int mode, int fam_mask, hipStream_t s,
+                         int block) {
     GatherArgs A;
     A.g = g;
     for (int a = 0; a < 3; ++a) {
@@ -1436,7 +1444,8 @@ hipError_t launch_gather(const CompGrid &g, double *const f1[3], double *const c
         A.f2[a] = f2 ? f2[a] : nullptr; A.c2[a] = c2 ? c2[a] : nullptr;
     }
     A.base = base; A.out = out; A.mode = mode; A.fam_mask = fam_mask;
-    hipLaunchKernelGGL(gather_kernel, dim3(grid_for(g.cells(), kBlock)), dim3(kBlock), 0, s, A);
+    const int b = block_or_default(block);
+    hipLaunchKernelGGL(gather_kernel, dim3(grid_for(g.cells(), b)), dim3(b), 0, s, A);
     return hipGetLastError();
 }
 

@@ -964,18 +964,25 @@ int pcg_blocks(size_t n) {
 
 // Tiling of the direction kernel: kTileZ x kTileY column tiles, x cut into chunks so that the
 // launch has about as many blocks as target_blocks (rounded up to a multiple of 8).
-void pcg_direction_tiling(int nx, int ny, int nz, PcgState &st) {
+void pcg_direction_tiling(int nx, int ny, int nz, PcgState &st, int chunk_x, int recompute) {
     st.dir_tiles_y = (ny + kTileY - 1) / kTileY;
     st.dir_tiles_z = (nz + kTileZ - 1) / kTileZ;
     const long tiles = (long)st.dir_tiles_y * st.dir_tiles_z;
     const long tb = (long)target_blocks((size_t)nx * ny * nz);
     const long want = std::max<long>(1, std::min<long>(nx, (tb + tiles - 1) / tiles));
-    st.dir_chunk_x = (int)((nx + want - 1) / want);
+    st.dir_chunk_x = chunk_x > 0 ? std::min(chunk_x, nx) : (int)((nx + want - 1) / want);
     st.dir_chunks = (nx + st.dir_chunk_x - 1) / st.dir_chunk_x;
-    const long nb = tiles * st.dir_chunks;
+    long nb = tiles * st.dir_chunks;
+    if (nb > (long)1 << 30) {   // a chunk too short for the grid: fall back to the sized one
+        st.dir_chunk_x = (int)((nx + want - 1) / want);
+        st.dir_chunks = (nx + st.dir_chunk_x - 1) / st.dir_chunk_x;
+        nb = tiles * st.dir_chunks;
+    }
     st.dir_blocks = (int)((nb + 7) / 8 * 8);
     // the update recomputes q on these tiles on large grids with long chunks
-    st.recompute_q = (size_t)nx * ny * nz >= kRecomputeCells && st.dir_chunk_x >= kRecomputeChunk;
+    st.recompute_q = recompute == 0 || recompute == 1
+                         ? recompute == 1
+                         : (size_t)nx * ny * nz >= kRecomputeCells && st.dir_chunk_x >= kRecomputeChunk;
     st.upd_blocks = st.recompute_q ? st.dir_blocks : pcg_blocks((size_t)nx * ny * nz);
 }
 

@@ -247,6 +247,34 @@ fia_status fia_ctx_set_pressure_preconditioner(fia_ctx *ctx, fia_precond kind);
  * in the reference (amgcl on the CPU, src/calc_pressure_3d.cpp:141-166).  Synchronises; disabling
  * frees the captured graphs. */
 fia_status fia_ctx_set_graphs(fia_ctx *ctx, int enable);
+
+/* ---- Launch shapes and their autotuner ------------------------------------------------------
+ * No counterpart in the reference (std::thread fan-out on the CPU).  Thread-block shapes of the
+ * advection's face kernels and of the CG sweep, kept per grid shape in the context.  The face,
+ * piece and cell block sizes leave every result bit-identical (each face, piece and cell is one
+ * thread's work, summed in a fixed order); the CG chunking changes the grouping of the FP64
+ * partial sums, so solves agree to their tolerance, not bit for bit. */
+typedef struct fia_tuning {
+    int face_block;      /* faces per block of the count, cutter, sum and zero-velocity kernels:
+                            64, 128 or 256 (0: 256)                                              */
+    int piece_block;     /* pieces per block of the quadrature kernel: 64, 128 or 256 (0: 256)   */
+    int cell_block;      /* cells per block of the gather kernel: 64, 128 or 256 (0: 256)        */
+    int pcg_chunk_x;     /* x planes one 64 x 16 CG tile sweeps (0: about 2048 cells per block)  */
+    int pcg_recompute;   /* the CG update recomputes q = L p (1) or streams it (0); -1: by size  */
+} fia_tuning;
+/* The shapes used on grids of nx x ny x nz cells (their staggered velocity components and
+ * pressure systems included); nx = ny = nz = 0 sets them for shapes without an entry of their
+ * own.  At most 64 shapes per context. */
+fia_status fia_ctx_set_tuning(fia_ctx *ctx, int nx, int ny, int nz, const fia_tuning *t);
+fia_status fia_ctx_get_tuning(fia_ctx *ctx, int nx, int ny, int nz, fia_tuning *t);
+/* Times candidate shapes on copies of grid g (g is left as it was): density steps with opts and
+ * time step dt for the block sizes (64, 128, 256 for each in turn), then iterations of g's
+ * pressure system (obstacle mask solid, may be NULL; the context's preconditioner) for the CG
+ * chunk length (2 .. 128 planes) and q policy.  The fastest of reps timed runs counts; a
+ * candidate must beat the current choice by 2%.  The result is stored for g's shape and
+ * returned in *out (may be NULL).  Allocates its copies; synchronises. */
+fia_status fia_ctx_autotune(fia_ctx *ctx, const fia_grid3d *g, const uint8_t *solid, double dt,
+                            const fia_advect_opts *opts, int reps, fia_tuning *out);
 fia_status fia_ctx_get_stats(fia_ctx *ctx, fia_kernel_stats *out);   /* synchronises */
 fia_status fia_ctx_reset_stats(fia_ctx *ctx);
 

@@ -72,7 +72,7 @@ def test_struct_layouts_match_c(fia, tmp_path):
     # every struct the ctypes mirror declares has the C header's size and field offsets
     import ctypes
     pairs = [("fia_grid3d", fia.GridC), ("fia_advect_opts", fia.AdvectOpts), ("fia_kernel_stats", fia.KernelStats),
-             ("fia_learned_info", fia.LearnedInfo), ("fia_scene", fia.SceneC)]
+             ("fia_learned_info", fia.LearnedInfo), ("fia_scene", fia.SceneC), ("fia_tuning", fia.TuningC)]
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "fia.h"', 'int main(void) {']
     for cname, py in pairs:
         lines.append('printf("%%zu\\n", sizeof(%s));' % cname)
