@@ -103,7 +103,7 @@ class KernelStats(ctypes.Structure):
 class TuningC(ctypes.Structure):
     """fia_tuning: launch shapes of one grid shape (0 / -1 = the built-in choice)."""
     _fields_ = [("face_block", ctypes.c_int), ("piece_block", ctypes.c_int), ("cell_block", ctypes.c_int),
-                ("pcg_chunk_x", ctypes.c_int), ("pcg_recompute", ctypes.c_int)]
+                ("pcg_chunk_x", ctypes.c_int), ("pcg_recompute", ctypes.c_int), ("pcg_fused", ctypes.c_int)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -230,7 +230,7 @@ class Context:
     def set_tuning(self, shape, **shapes):
         """Launch shapes (fia_tuning fields) for grids of shape (nx, ny, nz); shape None sets the
         default for shapes without their own entry.  Unnamed fields keep their built-in choice."""
-        t = TuningC(0, 0, 0, 0, -1)
+        t = TuningC(0, 0, 0, 0, -1, -1)
         for k, v in shapes.items():
             setattr(t, k, int(v))
         nx, ny, nz = shape if shape is not None else (0, 0, 0)

@@ -101,15 +101,15 @@ struct fia_ctx {
     DevBuf<uint8_t> pcg_flags;    // per-cell neighbour flags of the pressure system
     struct PcgBufs {              // one CG solve: x, r, p (two buffers), q (streaming update), z and r.z
                                   // partials (multigrid), partials, scalars
-        DevBuf<double> x, r, q, z, part_mg, part, scal;
+        DevBuf<double> x, r, r_alt, q, z, part_mg, part, scal;
         DevBuf<float> p0, p1;   // the search direction
         size_t bytes() const {
-            return x.bytes() + r.bytes() + p0.bytes() + p1.bytes() + q.bytes() + z.bytes() + part_mg.bytes() +
-                   part.bytes() + scal.bytes();
+            return x.bytes() + r.bytes() + r_alt.bytes() + p0.bytes() + p1.bytes() + q.bytes() + z.bytes() +
+                   part_mg.bytes() + part.bytes() + scal.bytes();
         }
         void release() {
-            x.release(); r.release(); p0.release(); p1.release(); q.release(); z.release(); part_mg.release();
-            part.release(); scal.release();
+            x.release(); r.release(); r_alt.release(); p0.release(); p1.release(); q.release(); z.release();
+            part_mg.release(); part.release(); scal.release();
         }
     } pcgs[2];
     // multigrid preconditioner: the levels' operators and vectors, and the hierarchy's shape
@@ -161,7 +161,7 @@ struct fia_ctx {
         fia_tuning t;
     };
     std::vector<ShapeTuning> tunings;
-    fia_tuning tuning_default{0, 0, 0, 0, -1};
+    fia_tuning tuning_default{0, 0, 0, 0, -1, -1};
     long long graph_captures0 = 0, graph_replays0 = 0;   // the cache's counters at the last stats reset
     // instrumentation
     bool timing = false;
@@ -599,24 +599,32 @@ static fia_status pressure_system(fia_ctx *c, const fia_grid3d *g, const GridVie
 }
 
 // Buffers B (allocated here for the grid of pg) bound into a CG state over the system (b, flags).
+// (batched: the solve joins fia_scenes_step_batched's launches, which run the two-kernel iteration)
 static fia_status bind_solve(fia_ctx::PcgBufs &B, const PcgGrid &pg, const double *b, const uint8_t *flags,
-                             PcgState &ps, bool mg, const fia_tuning &tu) {
+                             PcgState &ps, bool mg, const fia_tuning &tu, bool batched = false) {
     const size_t n = (size_t)pg.nx * pg.ny * pg.nz;
     ps.nblocks = pcg_blocks(n);
     pcg_direction_tiling(pg.nx, pg.ny, pg.nz, ps, tu.pcg_chunk_x, tu.pcg_recompute);
+    // fused by default from 4M cells (256^3: 191 against 203 us per iteration at the best chunking;
+    // 128^3: even)
+    ps.fused = !mg && !batched && (tu.pcg_fused == 1 || (tu.pcg_fused == -1 && n >= ((size_t)1 << 22)));
     HIP_TRY(B.x.ensure(n)); HIP_TRY(B.r.ensure(n)); HIP_TRY(B.p0.ensure(n)); HIP_TRY(B.p1.ensure(n));
     HIP_TRY(B.part.ensure(pcg_partial_words(ps))); HIP_TRY(B.scal.ensure(kPcgScalars));
-    if (!ps.recompute_q) HIP_TRY(B.q.ensure(n));
+    if (!ps.recompute_q && !ps.fused) HIP_TRY(B.q.ensure(n));
+    if (ps.fused) HIP_TRY(B.r_alt.ensure(n));
     if (mg) { HIP_TRY(B.z.ensure(n)); HIP_TRY(B.part_mg.ensure((size_t)ps.dir_blocks)); }
     ps.g = pg; ps.flags = flags; ps.b = b;
     ps.x = B.x.p; ps.r = B.r.p; ps.p[0] = B.p0.p; ps.p[1] = B.p1.p;
-    ps.q = ps.recompute_q ? nullptr : B.q.p;
+    ps.r_alt = ps.fused ? B.r_alt.p : nullptr;
+    ps.q = (ps.recompute_q || ps.fused) ? nullptr : B.q.p;
     ps.z = mg ? B.z.p : nullptr;
     ps.part_mg = mg ? B.part_mg.p : nullptr;
     ps.mg_lv = nullptr;
     ps.mg_levels = 0;
     ps.scal = B.scal.p; ps.part_start = B.part.p; ps.part_dir = B.part.p + 3 * (size_t)ps.nblocks;
-    ps.part_update = ps.part_dir + ps.dir_blocks; ps.it = 0;
+    ps.part_update = ps.part_dir + ps.dir_blocks;
+    ps.part_fused = ps.fused ? ps.part_update + 2 * (size_t)ps.upd_blocks : nullptr;
+    ps.it = 0;
     return FIA_OK;
 }
 
@@ -686,11 +694,11 @@ static fia_status learned_start(fia_ctx *c, fia_unet *u, const PcgGrid &pg, PcgS
     ls.bb = ha[kPcgBB] > 0 ? ha[kPcgBB] : 1.0;
     ls.guess_res = sqrt(ha[kPcgRR] / ls.bb);
     ls.gamma = ha[kPcgGamma];
-    HIP_TRY(launch_pcg_iterations(sa, probe, c->stream, mg));
+    HIP_TRY(launch_pcg_iterations(sa, pcg_launches_for(sa, probe), c->stream, mg));
     HIP_TRY(launch_pcg_check(sa, c->stream));
     HIP_TRY(launch_pcg_guess(sb, nullptr, c->stream));
     if ((st = pcg_begin(c, sb, tol2, mg)) != FIA_OK) return st;
-    HIP_TRY(launch_pcg_iterations(sb, probe, c->stream, mg));
+    HIP_TRY(launch_pcg_iterations(sb, pcg_launches_for(sb, probe), c->stream, mg));
     HIP_TRY(launch_pcg_check(sb, c->stream));
     HIP_TRY(hipMemcpyAsync(ha, sa.scal, kPcgScalars * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipMemcpyAsync(hb, sb.scal, kPcgScalars * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -710,15 +718,16 @@ static fia_status learned_start(fia_ctx *c, fia_unet *u, const PcgGrid &pg, PcgS
 // host round trips with the stream idle, outside the spans).
 static fia_status pcg_run(fia_ctx *c, PcgState &ps, int max_iters, double *hs, const MgHier *mg) {
     const int poll = mg ? 8 : 32;
-    const int it0 = ps.it;
+    const int it0 = pcg_iters_done(ps);
     while (true) {
         HIP_TRY(launch_pcg_check(ps, c->stream));
         HIP_TRY(hipMemcpyAsync(hs, ps.scal, kPcgScalars * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(hipStreamSynchronize(c->stream));
-        if (hs[kPcgDone] != 0.0 || ps.it >= max_iters) break;
+        if (hs[kPcgDone] != 0.0 || pcg_iters_done(ps) >= max_iters) break;
         int sp;
         HIP_TRY(span_begin(c, SK_PCG, sp));
-        HIP_TRY(c->graphs.iterations(ps, std::min(poll, max_iters - ps.it), c->stream, mg));
+        HIP_TRY(c->graphs.iterations(ps, pcg_launches_for(ps, std::min(poll, max_iters - pcg_iters_done(ps))),
+                                     c->stream, mg));
         HIP_TRY(span_end(c, sp));
     }
     if (c->timing) {
@@ -771,7 +780,7 @@ fia_status scene_solve_setup(fia_ctx *cw, fia_ctx *main, int s, fia_scene *sc, f
     const bool use_mg = main->precond == FIA_PRECOND_MG;
     PcgState sa;
     const fia_tuning &tu = tuning_for(main, sv.pg.nx, sv.pg.ny, sv.pg.nz);
-    if ((st = bind_solve(sv.slot[0], sv.pg, sv.b.p, sv.flags.p, sa, use_mg, tu)) != FIA_OK) return st;
+    if ((st = bind_solve(sv.slot[0], sv.pg, sv.b.p, sv.flags.p, sa, use_mg, tu, true)) != FIA_OK) return st;
     const MgHier *mg = nullptr;
     if (use_mg) {
         if ((st = mg_setup(cw->stream, sv.mgb, sv.mgh, sa)) != FIA_OK) return st;
@@ -783,7 +792,7 @@ fia_status scene_solve_setup(fia_ctx *cw, fia_ctx *main, int s, fia_scene *sc, f
     }
     if (net) {
         PcgState sb;
-        if ((st = bind_solve(sv.slot[1], sv.pg, sv.b.p, sv.flags.p, sb, use_mg, tu)) != FIA_OK) return st;
+        if ((st = bind_solve(sv.slot[1], sv.pg, sv.b.p, sv.flags.p, sb, use_mg, tu, true)) != FIA_OK) return st;
         sb.mg_lv = sa.mg_lv;
         sb.mg_levels = sa.mg_levels;
         LearnedStart ls;
@@ -1053,13 +1062,13 @@ fia_status fia_ctx_set_graphs(fia_ctx *c, int enable) {
 static bool tuning_valid(const fia_tuning *t) {
     auto blk = [](int b) { return b == 0 || b == 64 || b == 128 || b == 256; };
     return blk(t->face_block) && blk(t->piece_block) && blk(t->cell_block) && t->pcg_chunk_x >= 0 &&
-           t->pcg_recompute >= -1 && t->pcg_recompute <= 1;
+           t->pcg_recompute >= -1 && t->pcg_recompute <= 1 && t->pcg_fused >= -1 && t->pcg_fused <= 1;
 }
 
 fia_status fia_ctx_set_tuning(fia_ctx *c, int nx, int ny, int nz, const fia_tuning *t) {
     if (!c || !t) return fail(FIA_ERR_INVALID_ARG, "NULL argument");
     if (!tuning_valid(t)) return fail(FIA_ERR_INVALID_ARG, "block sizes must be 0, 64, 128 or 256, the chunk >= 0 "
-                                                           "and pcg_recompute -1, 0 or 1");
+                                                           "and pcg_recompute and pcg_fused -1, 0 or 1");
     if (nx == 0 && ny == 0 && nz == 0) {
         c->tuning_default = *t;
         return FIA_OK;
@@ -1213,25 +1222,44 @@ fia_status fia_ctx_autotune(fia_ctx *c, const fia_grid3d *g, const uint8_t *soli
     };
     double best_cg;
     if ((st = time_cg(best, best_cg)) != FIA_OK) return st;
-    std::vector<std::pair<int, bool>> tried;
+    // a candidate's launches: chunk length, and the q policy of the two-sweep iteration or the
+    // fused sweep (Jacobi only)
+    const size_t ncells = (size_t)pg.nx * pg.ny * pg.nz;
+    auto launch_key = [&](const fia_tuning &t) {
+        PcgState ps;
+        pcg_direction_tiling(pg.nx, pg.ny, pg.nz, ps, t.pcg_chunk_x, t.pcg_recompute);
+        const bool fused = !use_mg && (t.pcg_fused == 1 || (t.pcg_fused == -1 && ncells >= ((size_t)1 << 22)));
+        return std::make_pair(ps.dir_chunk_x, fused ? 2 : (int)ps.recompute_q);
+    };
+    std::vector<std::pair<int, int>> tried{launch_key(best)};
+    // chunk lengths: powers of two, and those that fill the device's resident blocks a half, one,
+    // two or three times
+    std::vector<int> chunks{2, 4, 8, 16, 32, 64};
     {
         PcgState ps;
-        pcg_direction_tiling(pg.nx, pg.ny, pg.nz, ps, best.pcg_chunk_x, best.pcg_recompute);
-        tried.push_back({ps.dir_chunk_x, ps.recompute_q});
+        pcg_direction_tiling(pg.nx, pg.ny, pg.nz, ps, 0, -1);
+        const long tiles = (long)ps.dir_tiles_y * ps.dir_tiles_z, res = std::max(1, pcg_resident_blocks());
+        for (long num : {1L, 2L, 4L, 6L}) {   // half-rounds
+            const long want = std::max<long>(1, std::min<long>(pg.nx, (num * res / 2 + tiles - 1) / tiles));
+            chunks.push_back((int)((pg.nx + want - 1) / want));
+        }
     }
-    for (int chunk : {2, 4, 8, 16, 32, 64, 128}) {
+    std::sort(chunks.begin(), chunks.end());
+    chunks.erase(std::unique(chunks.begin(), chunks.end()), chunks.end());
+    for (int chunk : chunks) {
         if (chunk > pg.nx) break;
-        for (int rq : {0, 1}) {
+        for (int mode : {0, 1, 2}) {   // stream q, recompute q, fused sweep
+            if (mode == 2 && use_mg) continue;
             fia_tuning t = best;
             t.pcg_chunk_x = chunk;
-            t.pcg_recompute = rq;
+            t.pcg_recompute = mode == 2 ? -1 : mode;
+            t.pcg_fused = mode == 2 ? 1 : 0;
             PcgState ps;
-            pcg_direction_tiling(pg.nx, pg.ny, pg.nz, ps, chunk, rq);
+            pcg_direction_tiling(pg.nx, pg.ny, pg.nz, ps, chunk, t.pcg_recompute);
             if (ps.dir_blocks > 4 * kPcgMaxBlocks) continue;   // every block re-sums the partials
-            bool seen = false;
-            for (const auto &p : tried) seen = seen || (p.first == ps.dir_chunk_x && p.second == ps.recompute_q);
-            if (seen) continue;
-            tried.push_back({ps.dir_chunk_x, ps.recompute_q});
+            const auto key = launch_key(t);
+            if (std::find(tried.begin(), tried.end(), key) != tried.end()) continue;
+            tried.push_back(key);
             double ms;
             if ((st = time_cg(t, ms)) != FIA_OK) return st;
             if (ms < 0.98 * best_cg) { best = t; best_cg = ms; }

@@ -213,16 +213,22 @@ struct PcgState {
     const uint8_t *flags;
     const double *b;
     double *x, *r;
+    double *r_alt;        // fused solves: the residual of odd sweeps (a sweep reads the residual of
+                          // neighbouring tiles, so it cannot be updated in place)
     float *p[2];          // the search direction (FP32: it only steers the solve; x and r are FP64)
     double *q;            // L p stored by the direction kernel (streaming update), or null (recompute)
     double *scal;         // kPcgScalars doubles
     double *part_start;   // 3 * nblocks partial sums (start kernel: rz, rr, bb)
     double *part_dir;     // dir_blocks (direction kernel: p.q)
     double *part_update;  // 2 * upd_blocks (update kernel: rz, rr)
+    double *part_fused;   // fused solves: 2 x 5 * dir_blocks (sweeps of even and odd k)
     int nblocks;          // pcg_blocks(n): start kernel, guess scaling, streaming update
     int dir_blocks, dir_tiles_y, dir_tiles_z, dir_chunk_x, dir_chunks;   // pcg_direction_tiling
     bool recompute_q;     // pcg_direction_tiling: the update recomputes q on the direction tiles
     int upd_blocks;       // blocks of the update kernel: dir_blocks (recompute) or nblocks (streaming)
+    bool fused;           // Jacobi only: one sweep per iteration (pcg_fused_kernel) instead of the
+                          // direction and update kernels; launch k is sweep k, and sweep k >= 1
+                          // runs iteration k after testing iteration k - 1
     // multigrid preconditioner (null for Jacobi): z = M r written by the last V-cycle kernel, with
     // dir_blocks partial sums of r.z in part_mg
     double *z;
@@ -232,6 +238,7 @@ struct PcgState {
     int it;             // iterations enqueued since the start (selects the p buffer)
 };
 int pcg_blocks(size_t n);
+int pcg_resident_blocks();
 // chunk_x > 0: the x planes one tile sweeps (else about kPcgCellsPerBlock cells per block);
 // recompute 0 / 1: the update streams q / recomputes it (else chosen by grid size and chunk)
 void pcg_direction_tiling(int nx, int ny, int nz, PcgState &st, int chunk_x = 0, int recompute = -1);
@@ -298,8 +305,15 @@ struct PcgBatch {
     int it;
 };
 // partial-sum words of a solve: 3 nblocks (start) + dir_blocks (p.q) + 2 upd_blocks (rz, rr)
+// (+ 10 dir_blocks for the fused sweeps)
 inline size_t pcg_partial_words(const PcgState &st) {
-    return 3 * (size_t)st.nblocks + (size_t)st.dir_blocks + 2 * (size_t)st.upd_blocks;
+    return 3 * (size_t)st.nblocks + (size_t)st.dir_blocks + 2 * (size_t)st.upd_blocks +
+           (st.fused ? 10 * (size_t)st.dir_blocks : 0);
+}
+// CG iterations completed after st.it launches, and the launches that complete `iters` more
+inline int pcg_iters_done(const PcgState &st) { return st.fused ? std::max(0, st.it - 1) : st.it; }
+inline int pcg_launches_for(const PcgState &st, int iters) {
+    return iters + (st.fused && st.it == 0 && iters > 0 ? 1 : 0);
 }
 hipError_t launch_pcg_batch_iterations(PcgBatch &b, int count, hipStream_t s, const MgBatch *mb = nullptr);
 hipError_t launch_mg_batch_vcycle(const PcgBatch &b, const MgBatch &mb, hipStream_t s);

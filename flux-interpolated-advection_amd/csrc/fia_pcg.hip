@@ -31,6 +31,8 @@
 // depend on scheduling).  The CG scalars are double-buffered by iteration parity, so no block
 // reads a scalar that another block of the same kernel writes.  Once converged every later kernel
 // of the solve returns at once, so the host only polls now and then (after a 1-block check).
+#include <atomic>
+
 #include "fia_internal.hpp"
 #include "fia_math.hpp"
 
@@ -149,6 +151,10 @@ __device__ __forceinline__ void residual_state(const PcgState &st, int k, double
         double tot[3];
         total_of<3>(st.part_start, st.nblocks, tot);
         rz = tot[0]; rr = tot[1]; bb = tot[2];
+    } else if (st.fused) {   // the sums of sweep k - 1 (its residual is iteration k - 1's)
+        double tot[5];
+        total_of<5>(st.part_fused + (size_t)((k - 1) & 1) * 5 * st.dir_blocks, st.dir_blocks, tot);
+        rz = tot[0]; rr = tot[1]; bb = st.scal[kPcgBB];
     } else {
         double tot[2];
         total_of<2>(st.part_update, st.upd_blocks, tot);
@@ -231,7 +237,7 @@ __device__ __forceinline__ void pcg_check_body(const PcgState &st, int k) {
         double *sc = st.scal;
         sc[kPcgRR] = rr;
         if (k == 0) sc[kPcgBB] = bb;
-        sc[kPcgIters] = (double)k;
+        sc[kPcgIters] = (double)(st.fused && k > 0 ? k - 1 : k);
         if (rr <= sc[kPcgTol2] * bb) sc[kPcgDone] = 1.0;
     }
 }
@@ -475,6 +481,261 @@ template <bool RECOMPUTE> __device__ __forceinline__ void pcg_update_body(const 
     if constexpr (RECOMPUTE) tile_sweep(st, blk, UpdOp{st, st.p[(k + 1) & 1], alpha}, acc);
     else update_stream(st, blk, k, alpha, acc);
     write_partials<2>(acc, st.part_update, st.upd_blocks, blk);
+}
+
+// ---- one sweep per iteration (Jacobi) ----------------------------------------------------------
+//
+// With z = r / diag, the residual dot of the next iteration follows from dots over the current
+// vectors: rz_(k+1) = sum (r_k - alpha_k q_k)^2 / d = rz_k - 2 alpha_k (r_k.q_k / d)
+// + alpha_k^2 (q_k.q_k / d).  So one sweep can finish iteration k and start iteration k + 1: with
+// alpha_k = rz_k / p_k.q_k and beta = rz_(k+1) / rz_k from the previous sweep's sums, per cell
+//   q_k = L p_k (recomputed from the stored p_k), r_(k+1) = r_k - alpha_k q_k,
+//   x_(k+1) = x_k + alpha_k p_k, p_(k+1) = r_(k+1) / d + beta p_k (rounded to FP32),
+//   q_(k+1) = L p_(k+1),
+// storing x, r and p and summing rz, rr, p.q, r.q / d and q.q / d of the new vectors (the true
+// rz_(k+1) enters the next alpha; only beta uses the expansion, whose terms come from one sweep, so
+// its rounding does not accumulate).  q_(k+1) of a cell needs p_(k+1) of its neighbours, so the
+// sweep computes r_(k+1) and p_(k+1) on the tile and a one-cell ring around it, whose q_k needs
+// p_k on a two-cell ring; along x, stage A (r, x, p of plane X) runs one plane ahead of stage B
+// (q_(k+1) and the sums of plane X - 1).  The ring reads r_k of the neighbouring tiles, which
+// their own blocks overwrite during the same sweep, so r alternates between two buffers as p does
+// (x is read only by its own cells and stays in place).  41 bytes per cell (flags 1, p 4 + 4,
+// x 8 + 8, r 8 + 8) against 54 for the direction and update sweeps, and one launch per iteration.
+//
+// Sweep 0 (alpha = beta = 0, p_old = 0) forms p_0 = z_0 and the sums of r_0; sweep j >= 1 runs
+// iteration j after testing the residual of iteration j - 1.
+constexpr int kRing1 = 2 * (kTileZ + 2) + 2 * kTileY;         // cells of the one-cell ring
+constexpr int kRing2 = 2 * (kTileZ + 4) + 2 * (kTileY + 2);   // cells of the second ring
+static_assert(kRing1 <= kPB && kRing2 <= kPB, "ring cells: one per thread");
+
+// local (y, z) of ring cell t (tile rows 0 .. kTileY - 1, columns 0 .. kTileZ - 1), as selects
+__device__ __forceinline__ int2 ring1_cell(int t) {
+    const int a = kTileZ + 2, b = 2 * a, c = b + kTileY;
+    const int ly = t < a ? -1 : t < b ? kTileY : t < c ? t - b : t - c;
+    const int lz = t < a ? t - 1 : t < b ? t - a - 1 : t < c ? -1 : kTileZ;
+    return make_int2(ly, lz);
+}
+__device__ __forceinline__ int2 ring2_cell(int t) {
+    const int a = kTileZ + 4, b = 2 * a, c = b + kTileY + 2;
+    const int ly = t < a ? -2 : t < b ? kTileY + 1 : t < c ? t - b - 1 : t - c - 1;
+    const int lz = t < a ? t - 2 : t < b ? t - a - 2 : t < c ? -2 : kTileZ + 1;
+    return make_int2(ly, lz);
+}
+
+// flags and residual of a cell, as stored (zero outside the grid)
+struct FusedRaw {
+    unsigned f;
+    double r;
+};
+
+// The fused sweep of one tile: sums acc = {rz, rr, p.q, r.q / d, q.q / d} of its own cells.
+// Registers per thread: p_k two planes ahead, flags, r and x one plane ahead.  The one-cell ring
+// (a stencil each) goes to the last kRing1 threads and the second ring (a load each) to the first
+// kRing2, so the four waves share the extra work.
+__device__ __forceinline__ void fused_sweep(const PcgState &st, int blk, const float *__restrict__ pold,
+                                            float *__restrict__ pnew, const double *__restrict__ rk,
+                                            double *__restrict__ rw, double alpha, double beta, double (&acc)[5]) {
+    __shared__ float P0[kTileY + 4][kTileZ + 4];      // p_k of plane X, two-cell ring
+    __shared__ float P1[2][kTileY + 2][kTileZ + 2];   // p_(k+1) of planes X and X - 1, one-cell ring
+    const int per = st.dir_blocks / 8;
+    const int L = (blk % 8) * per + blk / 8;
+    if (L >= st.dir_tiles_y * st.dir_tiles_z * st.dir_chunks) return;
+    const int nx = st.g.nx, ny = st.g.ny, nz = st.g.nz;
+    const int y0 = (L % st.dir_tiles_y) * kTileY;
+    const int z0 = ((L / st.dir_tiles_y) % st.dir_tiles_z) * kTileZ;
+    const int xa = (L / (st.dir_tiles_y * st.dir_tiles_z)) * st.dir_chunk_x;
+    const int xb = min(nx, xa + st.dir_chunk_x);
+    constexpr int kRowStep = kTileY / kRows;
+    const int ty = threadIdx.x / kTileZ, tz = threadIdx.x % kTileZ;
+    const long sx = (long)ny * nz;
+    const uint8_t *__restrict__ fl = st.flags;
+    double *__restrict__ xw = st.x;
+    auto inside = [&](int x, int y, int z) { return x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz; };
+    auto load_p = [&](int x, int y, int z) -> float { return inside(x, y, z) ? pold[idx3(x, y, z, ny, nz)] : 0.0f; };
+    auto load_raw = [&](int x, int y, int z) -> FusedRaw {
+        FusedRaw a{0u, 0.0};
+        if (inside(x, y, z)) {
+            const size_t c = idx3(x, y, z, ny, nz);
+            a.f = fl[c];
+            a.r = rk[c];
+        }
+        return a;
+    };
+    auto load_x = [&](int x, int y, int z) -> double { return inside(x, y, z) ? xw[idx3(x, y, z, ny, nz)] : 0.0; };
+    // ring cells of this thread
+    const int t1 = (int)threadIdx.x - (kPB - kRing1);
+    const bool has1 = t1 >= 0, has2 = threadIdx.x < kRing2;
+    const int2 g1 = ring1_cell(has1 ? t1 : 0), g2 = ring2_cell(has2 ? (int)threadIdx.x : 0);
+    const int r1y = g1.x, r1z = g1.y, r2y = g2.x, r2z = g2.y;
+    const int z = z0 + tz;
+    bool act[kRows];
+    size_t c[kRows];   // own cell of plane X (clamped into the grid; used only when active)
+    // own cells: p_k of planes X - 1 .. X + 2; flags, r and x of X and X + 1
+    float pm[kRows], pc[kRows], pn[kRows];
+    FusedRaw oc[kRows];
+    double xc[kRows];
+    // own cells for stage B: p_(k+1) of planes X - 2 and X - 1, r_(k+1) and flags of X - 1
+    float qm[kRows], qc[kRows];
+    double rb[kRows];
+    unsigned fb[kRows];
+    // ring cells: p_k of X - 1 .. X + 1 (ring 1), X and X + 1 (ring 2); flags and r of X (ring 1)
+    float gm = 0.f, gc = 0.f, gn = 0.f, hc = 0.f, hn = 0.f;
+    FusedRaw ac{0u, 0.0};
+    const int xs = xa - 1;   // first plane of stage A
+#pragma unroll
+    for (int w = 0; w < kRows; ++w) {
+        const int y = y0 + ty + w * kRowStep;
+        act[w] = y < ny && z < nz;
+        c[w] = idx3(max(xs, 0), min(y, ny - 1), min(z, nz - 1), ny, nz);
+        pm[w] = load_p(xs - 1, y, z);
+        pc[w] = load_p(xs, y, z);
+        pn[w] = load_p(xs + 1, y, z);
+        oc[w] = load_raw(xs, y, z);
+        xc[w] = load_x(xs, y, z);
+        qm[w] = 0.f; qc[w] = 0.f; rb[w] = 0.0; fb[w] = 0u;
+    }
+    if (has1) {
+        gm = load_p(xs - 1, y0 + r1y, z0 + r1z);
+        gc = load_p(xs, y0 + r1y, z0 + r1z);
+        gn = load_p(xs + 1, y0 + r1y, z0 + r1z);
+        ac = load_raw(xs, y0 + r1y, z0 + r1z);
+    }
+    if (has2) {
+        hc = load_p(xs, y0 + r2y, z0 + r2z);
+        hn = load_p(xs + 1, y0 + r2y, z0 + r2z);
+    }
+    for (int X = xs; X <= xb; ++X) {
+        // loads for the next planes: raw and x of X + 1 while it is a stage-A plane (X + 1 <= xb),
+        // p_k of X + 2 while stage A still needs it (X + 2 <= xb + 1); the same for the whole block
+        const bool ahead = X + 1 <= xb;
+        FusedRaw on[kRows];
+        double xn[kRows];
+        float pn2[kRows];
+#pragma unroll
+        for (int w = 0; w < kRows; ++w) {
+            const int y = y0 + ty + w * kRowStep;
+            on[w] = ahead ? load_raw(X + 1, y, z) : FusedRaw{0u, 0.0};
+            xn[w] = ahead ? load_x(X + 1, y, z) : 0.0;
+            pn2[w] = ahead ? load_p(X + 2, y, z) : 0.f;
+        }
+        FusedRaw an{0u, 0.0};
+        float gn2 = 0.f, hn2 = 0.f;
+        if (has1 && ahead) {
+            an = load_raw(X + 1, y0 + r1y, z0 + r1z);
+            gn2 = load_p(X + 2, y0 + r1y, z0 + r1z);
+        }
+        if (has2 && ahead) hn2 = load_p(X + 2, y0 + r2y, z0 + r2z);
+        // p_k of plane X into LDS
+#pragma unroll
+        for (int w = 0; w < kRows; ++w) P0[ty + w * kRowStep + 2][tz + 2] = pc[w];
+        if (has1) P0[r1y + 2][r1z + 2] = gc;
+        if (has2) P0[r2y + 2][r2z + 2] = hc;
+        __syncthreads();
+        // stage A on plane X: q_k, r_(k+1), p_(k+1) on the tile and the one-cell ring
+        const int b1 = X & 1;
+        auto stage_a = [&](int ly, int lz, unsigned f, double r, float pmv, float pcv, float pnv, double &r1,
+                           float &p1) {
+            double q = (double)__builtin_popcount(f & 63u) * (double)pcv;
+            q -= (f & 1u) ? (double)pmv : 0.0;
+            q -= (f & 2u) ? (double)pnv : 0.0;
+            q -= (f & 4u) ? (double)P0[ly + 1][lz + 2] : 0.0;
+            q -= (f & 8u) ? (double)P0[ly + 3][lz + 2] : 0.0;
+            q -= (f & 16u) ? (double)P0[ly + 2][lz + 1] : 0.0;
+            q -= (f & 32u) ? (double)P0[ly + 2][lz + 3] : 0.0;
+            q = (f & kPcgUnknown) ? q : 0.0;
+            r1 = r - alpha * q;
+            p1 = (float)(r1 * inv_diag(f) + beta * (double)pcv);
+            P1[b1][ly + 1][lz + 1] = p1;
+        };
+        const bool own_plane = X >= xa && X < xb;
+        double r1o[kRows];
+        float p1o[kRows];
+#pragma unroll
+        for (int w = 0; w < kRows; ++w) {
+            const int ly = ty + w * kRowStep;
+            stage_a(ly, tz, oc[w].f, oc[w].r, pm[w], pc[w], pn[w], r1o[w], p1o[w]);
+            if (own_plane && act[w]) {
+                xw[c[w]] = xc[w] + alpha * (double)pc[w];
+                rw[c[w]] = r1o[w];
+                pnew[c[w]] = p1o[w];
+            }
+        }
+        if (has1) {
+            double r1;
+            float p1;
+            stage_a(r1y, r1z, ac.f, ac.r, gm, gc, gn, r1, p1);
+        }
+        __syncthreads();
+        // stage B on plane X - 1 (own cells): q_(k+1) and the sums
+        if (X > xa) {
+            const int b0 = b1 ^ 1;
+#pragma unroll
+            for (int w = 0; w < kRows; ++w) {
+                if (!act[w]) continue;
+                const int ly = ty + w * kRowStep + 1, lz = tz + 1;
+                const unsigned f = fb[w];
+                double q = (double)__builtin_popcount(f & 63u) * (double)qc[w];
+                q -= (f & 1u) ? (double)qm[w] : 0.0;
+                q -= (f & 2u) ? (double)p1o[w] : 0.0;
+                q -= (f & 4u) ? (double)P1[b0][ly - 1][lz] : 0.0;
+                q -= (f & 8u) ? (double)P1[b0][ly + 1][lz] : 0.0;
+                q -= (f & 16u) ? (double)P1[b0][ly][lz - 1] : 0.0;
+                q -= (f & 32u) ? (double)P1[b0][ly][lz + 1] : 0.0;
+                q = (f & kPcgUnknown) ? q : 0.0;
+                const double id = inv_diag(f), r = rb[w], pv = (double)qc[w];
+                acc[0] += r * (r * id);
+                acc[1] += r * r;
+                acc[2] += pv * q;
+                acc[3] += r * (q * id);
+                acc[4] += q * (q * id);
+            }
+        }
+        // advance one plane
+#pragma unroll
+        for (int w = 0; w < kRows; ++w) {
+            qm[w] = qc[w]; qc[w] = p1o[w]; rb[w] = r1o[w]; fb[w] = oc[w].f;
+            pm[w] = pc[w]; pc[w] = pn[w]; pn[w] = pn2[w];
+            oc[w] = on[w]; xc[w] = xn[w];
+            if (X >= 0) c[w] += sx;
+        }
+        gm = gc; gc = gn; gn = gn2; ac = an;
+        hc = hn; hn = hn2;
+    }
+}
+
+// Sweep k of a fused solve (see above): the sums of sweep k - 1 (of the start kernel for k = 0)
+// give the convergence test, alpha and beta; then the sweep.
+__device__ __forceinline__ void pcg_fused_body(const PcgState &st, int blk, int k) {
+    if (solve_done(st.scal)) return;
+    double rz, rr, bb, alpha = 0.0, beta = 0.0;
+    if (k == 0) {
+        double tot[3];
+        total_of<3>(st.part_start, st.nblocks, tot);
+        rz = tot[0]; rr = tot[1]; bb = tot[2];
+    } else {
+        double tot[5];
+        total_of<5>(st.part_fused + (size_t)((k - 1) & 1) * 5 * st.dir_blocks, st.dir_blocks, tot);
+        rz = tot[0]; rr = tot[1]; bb = st.scal[kPcgBB];
+        alpha = tot[2] != 0.0 ? rz / tot[2] : 0.0;
+        const double rz1 = rz - 2.0 * alpha * tot[3] + alpha * alpha * tot[4];
+        beta = rz != 0.0 ? rz1 / rz : 0.0;
+    }
+    const bool done = rr <= st.scal[kPcgTol2] * bb;
+    if (blk == 0 && threadIdx.x == 0) {
+        double *sc = st.scal;
+        sc[kPcgRR] = rr;
+        if (k == 0) sc[kPcgBB] = bb;
+        const double next = sc[kPcgNext];   // sweeps run: iterations done are one fewer
+        sc[kPcgIters] = next > 0.0 ? next - 1.0 : 0.0;
+        sc[kPcgNext] = next + 1.0;
+        if (done) sc[kPcgDone] = 1.0;
+    }
+    if (done) return;
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    // residuals alternate between r (even sweeps read it) and r_alt; the start kernel wrote r
+    fused_sweep(st, blk, st.p[k & 1], st.p[(k + 1) & 1], (k & 1) ? st.r_alt : st.r, (k & 1) ? st.r : st.r_alt,
+                alpha, beta, acc);
+    write_partials<5>(acc, st.part_fused + (size_t)(k & 1) * 5 * st.dir_blocks, st.dir_blocks, blk);
 }
 
 // Energy-optimal scaling of an initial guess: partial sums of x.b and x.(L x) ...
@@ -924,6 +1185,9 @@ template <bool RECOMPUTE> __global__ void __launch_bounds__(kPB) pcg_update_kern
     pcg_update_body<RECOMPUTE>(st, blockIdx.x, k);
 }
 __global__ void __launch_bounds__(kPB) pcg_check_kernel(PcgState st, int k) { pcg_check_body(st, k); }
+__global__ void __launch_bounds__(kPB) __attribute__((amdgpu_waves_per_eu(3))) pcg_fused_kernel(PcgState st, int k) {
+    pcg_fused_body(st, blockIdx.x, k);
+}
 
 // ... or many solves per launch, all at the same iteration: map[b] = (s, j) says that block b of
 // the launch is block j of solve s, so each solve's arithmetic is exactly that of its own launch.
@@ -962,13 +1226,37 @@ int pcg_blocks(size_t n) {
     return (int)((b + 7) / 8 * 8);
 }
 
-// Tiling of the direction kernel: kTileZ x kTileY column tiles, x cut into chunks so that the
-// launch has about as many blocks as target_blocks (rounded up to a multiple of 8).
+// Blocks of the tile sweeps resident on the device at once: blocks per CU that the registers and
+// LDS of the direction, recomputing update and fused kernels allow (the least of them), times the
+// CUs; per device, computed once.  0 when the runtime cannot say.
+int pcg_resident_blocks() {
+    static std::atomic<int> cache[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
+    const int known = cache[dev].load();
+    if (known > 0) return known;
+    int cus = 0, a = 0, b = 0, c = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, pcg_direction_kernel, kPB, 0) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, pcg_update_kernel<true>, kPB, 0) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, pcg_fused_kernel, kPB, 0) != hipSuccess)
+        return 0;
+    const int v = std::max(1, std::min(a, std::min(b, c))) * std::max(1, cus);
+    cache[dev].store(v);
+    return v;
+}
+
+// Tiling of the tile sweeps: kTileZ x kTileY column tiles, x cut into chunks so that the launch
+// has about as many blocks as fit on the device at once (one full round: on 256^3 a 22-plane
+// chunk, 768 blocks at 3 per CU, ran the iteration in 191-203 us against 236-246 us for 16-plane
+// chunks and 1024 blocks, whose second round ran a third full), else as target_blocks; rounded
+// up to a multiple of 8.
 void pcg_direction_tiling(int nx, int ny, int nz, PcgState &st, int chunk_x, int recompute) {
     st.dir_tiles_y = (ny + kTileY - 1) / kTileY;
     st.dir_tiles_z = (nz + kTileZ - 1) / kTileZ;
     const long tiles = (long)st.dir_tiles_y * st.dir_tiles_z;
-    const long tb = (long)target_blocks((size_t)nx * ny * nz);
+    const int resident = pcg_resident_blocks();
+    const long tb = resident > 0 ? (long)resident : (long)target_blocks((size_t)nx * ny * nz);
     const long want = std::max<long>(1, std::min<long>(nx, (tb + tiles - 1) / tiles));
     st.dir_chunk_x = chunk_x > 0 ? std::min(chunk_x, nx) : (int)((nx + want - 1) / want);
     st.dir_chunks = (nx + st.dir_chunk_x - 1) / st.dir_chunk_x;
@@ -1008,6 +1296,14 @@ hipError_t launch_pcg_start(PcgState &st, double tol2, hipStream_t s) {
     return hipGetLastError();
 }
 hipError_t launch_pcg_iterations(PcgState &st, int count, hipStream_t s, const MgHier *mg) {
+    if (st.fused) {
+        if (mg) return hipErrorInvalidValue;   // the fused sweep is the Jacobi iteration
+        for (int k = 0; k < count; ++k) {
+            hipLaunchKernelGGL(pcg_fused_kernel, dim3(st.dir_blocks), dim3(kPB), 0, s, st, st.it);
+            ++st.it;
+        }
+        return hipGetLastError();
+    }
     for (int k = 0; k < count; ++k) {
         hipLaunchKernelGGL(pcg_direction_kernel, dim3(st.dir_blocks), dim3(kPB), 0, s, st, st.it);
         if (st.recompute_q) hipLaunchKernelGGL(pcg_update_kernel<true>, dim3(st.upd_blocks), dim3(kPB), 0, s, st, st.it);
@@ -1052,9 +1348,11 @@ hipError_t launch_pcg_batch_check(const PcgBatch &b, double *out, hipStream_t s)
 // Field-wise equality of everything the iteration launches pass (the iteration index excepted).
 static bool same_launch(const PcgState &a, const PcgState &b) {
     return a.g.nx == b.g.nx && a.g.ny == b.g.ny && a.g.nz == b.g.nz && a.g.solid == b.g.solid && a.g.pin == b.g.pin &&
-           a.flags == b.flags && a.b == b.b && a.x == b.x && a.r == b.r && a.p[0] == b.p[0] && a.p[1] == b.p[1] &&
+           a.flags == b.flags && a.b == b.b && a.x == b.x && a.r == b.r && a.r_alt == b.r_alt && a.p[0] == b.p[0] &&
+           a.p[1] == b.p[1] &&
            a.q == b.q && a.scal == b.scal && a.part_start == b.part_start && a.part_dir == b.part_dir &&
-           a.part_update == b.part_update && a.nblocks == b.nblocks && a.dir_blocks == b.dir_blocks &&
+           a.part_update == b.part_update && a.part_fused == b.part_fused && a.fused == b.fused &&
+           a.nblocks == b.nblocks && a.dir_blocks == b.dir_blocks &&
            a.dir_tiles_y == b.dir_tiles_y && a.dir_tiles_z == b.dir_tiles_z && a.dir_chunk_x == b.dir_chunk_x &&
            a.dir_chunks == b.dir_chunks && a.recompute_q == b.recompute_q && a.upd_blocks == b.upd_blocks &&
            a.z == b.z && a.part_mg == b.part_mg && a.mg_lv == b.mg_lv && a.mg_levels == b.mg_levels;

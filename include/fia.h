@@ -259,8 +259,11 @@ typedef struct fia_tuning {
                             64, 128 or 256 (0: 256)                                              */
     int piece_block;     /* pieces per block of the quadrature kernel: 64, 128 or 256 (0: 256)   */
     int cell_block;      /* cells per block of the gather kernel: 64, 128 or 256 (0: 256)        */
-    int pcg_chunk_x;     /* x planes one 64 x 16 CG tile sweeps (0: about 2048 cells per block)  */
+    int pcg_chunk_x;     /* x planes one 64 x 16 CG tile sweeps (0: one block per resident slot) */
     int pcg_recompute;   /* the CG update recomputes q = L p (1) or streams it (0); -1: by size  */
+    int pcg_fused;       /* Jacobi: one sweep per CG iteration (1) or the direction and update
+                            sweeps (0); -1: fused from 4M cells.  Multigrid and batched solves
+                            use two sweeps. */
 } fia_tuning;
 /* The shapes used on grids of nx x ny x nz cells (their staggered velocity components and
  * pressure systems included); nx = ny = nz = 0 sets them for shapes without an entry of their

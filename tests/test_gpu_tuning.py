@@ -74,7 +74,8 @@ def test_block_sizes_leave_the_advection_bit_identical(fia, scenes, ctx):
 
 def test_invalid_launch_shapes_are_rejected(fia, ctx):
     print("case invalid launch shapes", flush=True)
-    for bad in ({"face_block": 100}, {"piece_block": 512}, {"pcg_chunk_x": -1}, {"pcg_recompute": 2}):
+    for bad in ({"face_block": 100}, {"piece_block": 512}, {"pcg_chunk_x": -1}, {"pcg_recompute": 2},
+                {"pcg_fused": 2}):
         with pytest.raises(fia.FiaError):
             ctx.set_tuning((16, 16, 16), **bad)
     with pytest.raises(fia.FiaError):
@@ -96,8 +97,10 @@ def test_cg_chunking_reaches_the_same_solution(fia, scenes, precond):
     try:
         c.set_pressure_preconditioner(precond)
         out = []
-        for shapes in ({}, {"pcg_chunk_x": 2, "pcg_recompute": 1}, {"pcg_chunk_x": 16, "pcg_recompute": 0},
-                       {"pcg_chunk_x": 64, "pcg_recompute": 1}):
+        for shapes in ({}, {"pcg_chunk_x": 2, "pcg_recompute": 1, "pcg_fused": 0},
+                       {"pcg_chunk_x": 16, "pcg_recompute": 0, "pcg_fused": 0},
+                       {"pcg_chunk_x": 64, "pcg_recompute": 1, "pcg_fused": 0}, {"pcg_chunk_x": 3, "pcg_fused": 1},
+                       {"pcg_chunk_x": 1, "pcg_fused": 1}):
             c.set_tuning((n, n, n), **shapes)
             g = fia.Grid3D(n, n, n, 1.0 / n, None, *[x.clone() for x in base])
             g.density = None
@@ -136,7 +139,7 @@ def test_autotune_keeps_the_grid_and_stores_its_choice(fia, scenes):
         assert c.get_tuning((n, n, n)) == best
         for k in ("face_block", "piece_block", "cell_block"):
             assert best[k] in (0, 64, 128, 256)
-        assert best["pcg_chunk_x"] >= 0 and best["pcg_recompute"] in (-1, 0, 1)
+        assert best["pcg_chunk_x"] >= 0 and best["pcg_recompute"] in (-1, 0, 1) and best["pcg_fused"] in (-1, 0, 1)
         # the step under the picked shapes is the same step
         got = _density_step(fia, c, t, n, h, dt, od)
         assert got.density.equal(ref.density)
