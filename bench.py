@@ -72,6 +72,8 @@ TRAFFIC_BYTES_256_CFL4 = 12.123e9
 # kernel reads x, r (2 x 8), p (4) and the flag byte, writes x, r (2 x 8); q = L p is recomputed,
 # not stored
 PCG_BYTES_PER_CELL_ITER = 54
+# the one-sweep Jacobi iteration (pcg_fused_kernel): flags 1, p 4 + 4, x 8 + 8, r 8 + 8
+PCG_FUSED_BYTES_PER_CELL_ITER = 41
 # one multigrid V-cycle per iteration, per fine cell: the down kernel reads r (8) and the flag
 # byte, writes x (4) and the aggregate's b (0.5); the up kernel reads x (4), r (8), the flag byte
 # and the correction (0.5), writes z (8); the coarser levels add about a seventh of a fine level's
@@ -284,14 +286,25 @@ class PlumeRun:
         self.ctx.close()
 
 
-def pcg_roofline(st, n, precond):
+def pcg_is_fused(ctx, n, precond):
+    """Whether the context's solves of an n^3 grid run the one-sweep Jacobi iteration (its launch
+    shapes for that grid: fused on request, or by default from 4M cells)."""
+    if precond != "jacobi":
+        return False
+    f = ctx.get_tuning((n, n, n))["pcg_fused"]
+    return f == 1 or (f == -1 and n ** 3 >= 1 << 22)
+
+
+def pcg_roofline(st, n, precond, fused=False):
     """Roofline entry of the CG iteration from the library's HIP-event spans around its launches:
     algorithmic bytes of one iteration over all cells, divided by the measured time per iteration."""
-    per_cell = PCG_BYTES_PER_CELL_ITER + (MG_BYTES_PER_CELL_VCYCLE if precond == "mg" else 0)
+    per_cell = (PCG_FUSED_BYTES_PER_CELL_ITER if fused else PCG_BYTES_PER_CELL_ITER) + \
+        (MG_BYTES_PER_CELL_VCYCLE if precond == "mg" else 0)
     iters = max(st["pcg_iters"], 1)
     ms_iter = st["pcg_ms"] / iters
     gbs = per_cell * n ** 3 / (ms_iter / 1e3) / 1e9 if ms_iter > 0 else 0.0
-    kernels = ("pcg_direction_kernel + pcg_update_kernel<true> (Jacobi-PCG iteration, FP64 vectors)"
+    kernels = ("pcg_fused_kernel (one-sweep Jacobi-PCG iteration, FP64 x and r, FP32 p)" if fused else
+               "pcg_direction_kernel + pcg_update_kernel<true> (Jacobi-PCG iteration, FP64 vectors)"
                if precond == "jacobi" else "multigrid-preconditioned CG iteration (direction, update, V-cycle)")
     return {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
             "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": None, "kernel": kernels,
@@ -334,6 +347,7 @@ def run_plume(args, fia, scenes, torch):
     state = plume_scene(scenes, n)
     run = PlumeRun(fia, torch, n, state, "jacobi", args.cfl)
     sec, st = run.timed(args.steps, args.warmup, log)
+    fused = pcg_is_fused(run.ctx, n, "jacobi")
     mass0 = fia.grid_totals(run.ctx, run.grid())[0]
     main = {"iters": list(run.iters), "res": list(run.res), "dts": list(run.dts), "shapes": run.shapes}
     # the mass change of one density step on the final state (no source, no forcing)
@@ -367,7 +381,7 @@ def run_plume(args, fia, scenes, torch):
                    "launch_shapes": main["shapes"],
                    "piece_rule": "the reference's: one psi point per piece plus 1-point Gauss along y for the "
                                  "density term; the 4x4 per-piece Gauss rule the configuration names is not built"},
-        "roofline": pcg_roofline(st, n, "jacobi"),
+        "roofline": pcg_roofline(st, n, "jacobi", fused),
         "pcg": {"iterations": main["iters"], "max_rel_residual": max(main["res"]),
                 "dt": [round(d, 7) for d in main["dts"]]},
         "kernels_ms_per_step": {"pcg_ms": round(st["pcg_ms"] / args.steps, 3),
@@ -443,7 +457,9 @@ def run_projection(args, fia, scenes, torch):
             times.append(time.perf_counter() - t0); iters.append(it); res.append(rr)
     sec = sum(times)
     it_mean = float(np.mean(iters))
-    per_iter = PCG_BYTES_PER_CELL_ITER + (MG_BYTES_PER_CELL_VCYCLE if args.precond == "mg" else 0)
+    fused = pcg_is_fused(ctx, n, args.precond)
+    per_iter = (PCG_FUSED_BYTES_PER_CELL_ITER if fused else PCG_BYTES_PER_CELL_ITER) + \
+        (MG_BYTES_PER_CELL_VCYCLE if args.precond == "mg" else 0)
     gbs = per_iter * n ** 3 * it_mean / (sec / args.steps) / 1e9
     ctx.close()
     solver = "Jacobi-PCG" if args.precond == "jacobi" else "multigrid-preconditioned CG (V(1,1) per iteration)"
@@ -476,10 +492,12 @@ def run_warmstart(args, fia, scenes, torch):
     rows = learned.evaluate(fia, scenes, ctx, net, n, [5000], args.warmup + args.steps, log=lambda s: None)
     timed = rows[args.warmup:]
     s = learned.summary(timed)
+    fused = pcg_is_fused(ctx, n, args.precond)
     net.close()
     ctx.close()
     l_ms, p_ms = s["learned_ms"], s["plain_ms"]
-    per_iter = PCG_BYTES_PER_CELL_ITER + (MG_BYTES_PER_CELL_VCYCLE if args.precond == "mg" else 0)
+    per_iter = (PCG_FUSED_BYTES_PER_CELL_ITER if fused else PCG_BYTES_PER_CELL_ITER) + \
+        (MG_BYTES_PER_CELL_VCYCLE if args.precond == "mg" else 0)
     solver = "Jacobi-PCG" if args.precond == "jacobi" else "multigrid-preconditioned CG"
     gbs = per_iter * n ** 3 * s["learned_iters"] / (l_ms / 1e3) / 1e9
     return {
