@@ -74,6 +74,10 @@ TRAFFIC_BYTES_256_CFL4 = 12.123e9
 PCG_BYTES_PER_CELL_ITER = 54
 # the one-sweep Jacobi iteration (pcg_fused_kernel): flags 1, p 4 + 4, x 8 + 8, r 8 + 8
 PCG_FUSED_BYTES_PER_CELL_ITER = 41
+# HBM bytes per pcg_fused_kernel launch on the 256^3 plume (8 obstacles), from two rocprofv3
+# passes (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0, scaled by the counters' reading of a 1 GiB copy):
+# 444.7 MB read + 337.2 MB written
+PCG_FUSED_TRAFFIC_BYTES_256 = 781.9e6
 # one multigrid V-cycle per iteration, per fine cell: the down kernel reads r (8) and the flag
 # byte, writes x (4) and the aggregate's b (0.5); the up kernel reads x (4), r (8), the flag byte
 # and the correction (0.5), writes z (8); the coarser levels add about a seventh of a fine level's
@@ -306,8 +310,9 @@ def pcg_roofline(st, n, precond, fused=False):
     kernels = ("pcg_fused_kernel (one-sweep Jacobi-PCG iteration, FP64 x and r, FP32 p)" if fused else
                "pcg_direction_kernel + pcg_update_kernel<true> (Jacobi-PCG iteration, FP64 vectors)"
                if precond == "jacobi" else "multigrid-preconditioned CG iteration (direction, update, V-cycle)")
+    traffic = PCG_FUSED_TRAFFIC_BYTES_256 if (fused and n == 256) else None
     return {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-            "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": None, "kernel": kernels,
+            "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": traffic, "kernel": kernels,
             "bytes_per_cell_iteration": per_cell, "avg_iteration_ms": round(ms_iter, 4),
             "iterations": st["pcg_iters"], "solves": st["pcg_solves"]}
 
