@@ -155,6 +155,7 @@ struct fia_ctx {
     double *host_batch = nullptr;   // pinned: the scalars of every batched solve
     size_t host_batch_n = 0;
     PcgGraphCache graphs;           // captured CG iteration batches (fia_ctx_set_graphs)
+    bool shared_device = false;     // inside a step of several scenes: other solves share the device
     // launch shapes per cell-grid shape (fia_ctx_set_tuning, fia_ctx_autotune) and for the rest
     struct ShapeTuning {
         int nx, ny, nz;
@@ -183,6 +184,7 @@ namespace fia {
 int ctx_device(const fia_ctx *c) { return c->device; }
 hipStream_t ctx_stream(const fia_ctx *c) { return c->stream; }
 int ctx_precond(const fia_ctx *c) { return c->precond; }
+void ctx_set_shared(fia_ctx *c, bool shared) { c->shared_device = shared; }
 fia_status set_error(fia_status status, const std::string &msg) { return fail(status, msg); }
 }  // namespace fia
 
@@ -599,15 +601,19 @@ static fia_status pressure_system(fia_ctx *c, const fia_grid3d *g, const GridVie
 }
 
 // Buffers B (allocated here for the grid of pg) bound into a CG state over the system (b, flags).
-// (batched: the solve joins fia_scenes_step_batched's launches, which run the two-kernel iteration)
+// batched: the solve joins fia_scenes_step_batched's launches, which run the two-kernel iteration.
+// shared: other scenes' solves share the device (a step of several scenes): the tiles are sized by
+// cells per block, not by one round of the device, and the iteration is the two-kernel one, so a
+// scene's solve has the same bits whether its launches are batched or its own.
 static fia_status bind_solve(fia_ctx::PcgBufs &B, const PcgGrid &pg, const double *b, const uint8_t *flags,
-                             PcgState &ps, bool mg, const fia_tuning &tu, bool batched = false) {
+                             PcgState &ps, bool mg, const fia_tuning &tu, bool batched, bool shared) {
     const size_t n = (size_t)pg.nx * pg.ny * pg.nz;
     ps.nblocks = pcg_blocks(n);
-    pcg_direction_tiling(pg.nx, pg.ny, pg.nz, ps, tu.pcg_chunk_x, tu.pcg_recompute);
+    pcg_direction_tiling(pg.nx, pg.ny, pg.nz, ps, tu.pcg_chunk_x, tu.pcg_recompute, shared);
     // fused by default from 4M cells (256^3: 191 against 203 us per iteration at the best chunking;
     // 128^3: even)
-    ps.fused = !mg && !batched && (tu.pcg_fused == 1 || (tu.pcg_fused == -1 && n >= ((size_t)1 << 22)));
+    ps.fused = !mg && !batched && !shared &&
+               (tu.pcg_fused == 1 || (tu.pcg_fused == -1 && n >= ((size_t)1 << 22)));
     HIP_TRY(B.x.ensure(n)); HIP_TRY(B.r.ensure(n)); HIP_TRY(B.p0.ensure(n)); HIP_TRY(B.p1.ensure(n));
     HIP_TRY(B.part.ensure(pcg_partial_words(ps))); HIP_TRY(B.scal.ensure(kPcgScalars));
     if (!ps.recompute_q && !ps.fused) HIP_TRY(B.q.ensure(n));
@@ -664,7 +670,7 @@ static fia_status mg_prepare(fia_ctx *c, const PcgState &ps, const MgHier **out)
 // Solve k's buffers bound into a CG state over the context's system.
 static fia_status pcg_bind(fia_ctx *c, int k, const PcgGrid &pg, PcgState &ps) {
     return bind_solve(c->pcgs[k], pg, c->pcg_b.p, c->pcg_flags.p, ps, c->precond == FIA_PRECOND_MG,
-                      tuning_for(c, pg.nx, pg.ny, pg.nz));
+                      tuning_for(c, pg.nx, pg.ny, pg.nz), false, c->shared_device);
 }
 
 // The start of a solve: r = b - L x and the scalars, then z = M r under multigrid.
@@ -780,7 +786,8 @@ fia_status scene_solve_setup(fia_ctx *cw, fia_ctx *main, int s, fia_scene *sc, f
     const bool use_mg = main->precond == FIA_PRECOND_MG;
     PcgState sa;
     const fia_tuning &tu = tuning_for(main, sv.pg.nx, sv.pg.ny, sv.pg.nz);
-    if ((st = bind_solve(sv.slot[0], sv.pg, sv.b.p, sv.flags.p, sa, use_mg, tu, true)) != FIA_OK) return st;
+    if ((st = bind_solve(sv.slot[0], sv.pg, sv.b.p, sv.flags.p, sa, use_mg, tu, true, cw->shared_device)) != FIA_OK)
+        return st;
     const MgHier *mg = nullptr;
     if (use_mg) {
         if ((st = mg_setup(cw->stream, sv.mgb, sv.mgh, sa)) != FIA_OK) return st;
@@ -792,7 +799,9 @@ fia_status scene_solve_setup(fia_ctx *cw, fia_ctx *main, int s, fia_scene *sc, f
     }
     if (net) {
         PcgState sb;
-        if ((st = bind_solve(sv.slot[1], sv.pg, sv.b.p, sv.flags.p, sb, use_mg, tu, true)) != FIA_OK) return st;
+        if ((st = bind_solve(sv.slot[1], sv.pg, sv.b.p, sv.flags.p, sb, use_mg, tu, true, cw->shared_device)) !=
+            FIA_OK)
+            return st;
         sb.mg_lv = sa.mg_lv;
         sb.mg_levels = sa.mg_levels;
         LearnedStart ls;
@@ -1203,7 +1212,8 @@ fia_status fia_ctx_autotune(fia_ctx *c, const fia_grid3d *g, const uint8_t *soli
     auto time_cg = [&](const fia_tuning &t, double &ms) -> fia_status {
         PcgState ps;
         fia_status s2;
-        if ((s2 = bind_solve(c->pcgs[0], pg, c->pcg_b.p, c->pcg_flags.p, ps, use_mg, t)) != FIA_OK) return s2;
+        if ((s2 = bind_solve(c->pcgs[0], pg, c->pcg_b.p, c->pcg_flags.p, ps, use_mg, t, false, false)) != FIA_OK)
+            return s2;
         const MgHier *mg = nullptr;
         if ((s2 = mg_prepare(c, ps, &mg)) != FIA_OK) return s2;
         HIP_TRY(launch_pcg_guess(ps, nullptr, c->stream));

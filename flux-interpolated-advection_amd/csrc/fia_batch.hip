@@ -107,6 +107,18 @@ std::vector<int> largest_first(const fia_scene *scenes, int n) {
     return order;
 }
 
+// Marks the contexts of a step of several scenes as sharing the device for the duration of the step.
+struct SharedDevice {
+    fia_ctx *const *ctxs;
+    int workers;
+    SharedDevice(fia_ctx *const *c, int w, bool shared) : ctxs(c), workers(w) {
+        for (int i = 0; i < workers; ++i) fia::ctx_set_shared(ctxs[i], shared);
+    }
+    ~SharedDevice() {
+        for (int i = 0; i < workers; ++i) fia::ctx_set_shared(ctxs[i], false);
+    }
+};
+
 fia_status check_step_args(fia_ctx *const *ctxs, int workers, const fia_scene *scenes, int n_scenes,
                            const fia_advect_opts *od, const fia_advect_opts *ov) {
     if (!ctxs || workers <= 0 || (!scenes && n_scenes > 0) || n_scenes < 0 || !od || !ov)
@@ -169,6 +181,7 @@ fia_status fia_scenes_step(fia_ctx *const *ctxs, int workers, fia_scene *scenes,
                            int max_iters, fia_unet *const *nets) {
     fia_status st;
     if ((st = check_step_args(ctxs, workers, scenes, n_scenes, density_opts, velocity_opts)) != FIA_OK) return st;
+    const SharedDevice shared(ctxs, workers, n_scenes > 1);
     return for_scenes(workers, scenes, largest_first(scenes, n_scenes), [&](int w, int i) {
         return scene_step(ctxs[w], &scenes[i], density_opts, velocity_opts, rel_tol, max_iters,
                           nets ? nets[w] : nullptr);
@@ -190,6 +203,7 @@ fia_status fia_scenes_step_batched(fia_ctx *const *ctxs, int workers, fia_scene 
             return fia::set_error(FIA_ERR_INVALID_ARG, "a scene needs a device grid with density");
     }
     fia_ctx *main = ctxs[0];
+    const SharedDevice shared(ctxs, workers, n_scenes > 1);
     if ((st = fia::scenes_solve_alloc(main, scenes, n_scenes)) != FIA_OK) return st;
     const std::vector<int> order = largest_first(scenes, n_scenes);
     std::vector<double> dt(n_scenes, 0.0);

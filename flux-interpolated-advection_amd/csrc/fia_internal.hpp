@@ -239,9 +239,11 @@ struct PcgState {
 };
 int pcg_blocks(size_t n);
 int pcg_resident_blocks();
-// chunk_x > 0: the x planes one tile sweeps (else about kPcgCellsPerBlock cells per block);
+// chunk_x > 0: the x planes one tile sweeps (else one round of the device's resident blocks, or
+// about 2048 cells per block for a solve sharing its launches with others: shared);
 // recompute 0 / 1: the update streams q / recomputes it (else chosen by grid size and chunk)
-void pcg_direction_tiling(int nx, int ny, int nz, PcgState &st, int chunk_x = 0, int recompute = -1);
+void pcg_direction_tiling(int nx, int ny, int nz, PcgState &st, int chunk_x = 0, int recompute = -1,
+                          bool shared = false);
 
 // Geometric multigrid preconditioner for the pressure system: level 0 is the system itself (flag
 // bytes), level l + 1 aggregates 2 x 2 x 2 cells of level l with the exact Galerkin operator of
@@ -368,6 +370,9 @@ namespace fia {
 int ctx_device(const fia_ctx *c);
 hipStream_t ctx_stream(const fia_ctx *c);
 int ctx_precond(const fia_ctx *c);
+// marks a context as one of several sharing the device for a step of several scenes (its solves
+// then tile and iterate the same way batched or not)
+void ctx_set_shared(fia_ctx *c, bool shared);
 fia_status set_error(fia_status status, const std::string &msg);   // returns status
 // batched pressure solves of many scenes (fia_api.hip): buffers on the main context, set-up of
 // one scene on a worker context, then the joint solve and projection on the main context

@@ -1254,11 +1254,12 @@ int pcg_resident_blocks() {
 // chunk, 768 blocks at 3 per CU, ran the iteration in 191-203 us against 236-246 us for 16-plane
 // chunks and 1024 blocks, whose second round ran a third full), else as target_blocks; rounded
 // up to a multiple of 8.
-void pcg_direction_tiling(int nx, int ny, int nz, PcgState &st, int chunk_x, int recompute) {
+void pcg_direction_tiling(int nx, int ny, int nz, PcgState &st, int chunk_x, int recompute, bool shared) {
     st.dir_tiles_y = (ny + kTileY - 1) / kTileY;
     st.dir_tiles_z = (nz + kTileZ - 1) / kTileZ;
     const long tiles = (long)st.dir_tiles_y * st.dir_tiles_z;
-    const int resident = pcg_resident_blocks();
+    // a solve that shares its launches with others (batched scenes) has no round of its own to fill
+    const int resident = shared ? 0 : pcg_resident_blocks();
     // chunks per tile column: as many as fit in one round of resident blocks (rounded down: a few
     // blocks over the round would run alone in a second one)
     const long want = resident > 0 ? std::max<long>(1, std::min<long>(nx, resident / tiles))
