@@ -609,11 +609,11 @@ static fia_status bind_solve(fia_ctx::PcgBufs &B, const PcgGrid &pg, const doubl
                              PcgState &ps, bool mg, const fia_tuning &tu, bool batched, bool shared) {
     const size_t n = (size_t)pg.nx * pg.ny * pg.nz;
     ps.nblocks = pcg_blocks(n);
-    pcg_direction_tiling(pg.nx, pg.ny, pg.nz, ps, tu.pcg_chunk_x, tu.pcg_recompute, shared);
     // fused by default from 4M cells (256^3: 191 against 203 us per iteration at the best chunking;
     // 128^3: even)
     ps.fused = !mg && !batched && !shared &&
                (tu.pcg_fused == 1 || (tu.pcg_fused == -1 && n >= ((size_t)1 << 22)));
+    pcg_direction_tiling(pg.nx, pg.ny, pg.nz, ps, tu.pcg_chunk_x, tu.pcg_recompute, shared);
     HIP_TRY(B.x.ensure(n)); HIP_TRY(B.r.ensure(n)); HIP_TRY(B.p0.ensure(n)); HIP_TRY(B.p1.ensure(n));
     HIP_TRY(B.part.ensure(pcg_partial_words(ps))); HIP_TRY(B.scal.ensure(kPcgScalars));
     if (!ps.recompute_q && !ps.fused) HIP_TRY(B.q.ensure(n));
@@ -1237,20 +1237,21 @@ fia_status fia_ctx_autotune(fia_ctx *c, const fia_grid3d *g, const uint8_t *soli
     const size_t ncells = (size_t)pg.nx * pg.ny * pg.nz;
     auto launch_key = [&](const fia_tuning &t) {
         PcgState ps;
+        ps.fused = !use_mg && (t.pcg_fused == 1 || (t.pcg_fused == -1 && ncells >= ((size_t)1 << 22)));
         pcg_direction_tiling(pg.nx, pg.ny, pg.nz, ps, t.pcg_chunk_x, t.pcg_recompute);
-        const bool fused = !use_mg && (t.pcg_fused == 1 || (t.pcg_fused == -1 && ncells >= ((size_t)1 << 22)));
-        return std::make_pair(ps.dir_chunk_x, fused ? 2 : (int)ps.recompute_q);
+        return std::make_pair(ps.dir_chunk_x, ps.fused ? 2 : (int)ps.recompute_q);
     };
     std::vector<std::pair<int, int>> tried{launch_key(best)};
     // chunk lengths: powers of two, and those that fill the device's resident blocks a half, one,
     // two or three times
     std::vector<int> chunks{2, 4, 8, 16, 32, 64};
-    {
+    for (bool fused : {false, true}) {
         PcgState ps;
+        ps.fused = fused;
         pcg_direction_tiling(pg.nx, pg.ny, pg.nz, ps, 0, -1);
-        const long tiles = (long)ps.dir_tiles_y * ps.dir_tiles_z, res = std::max(1, pcg_resident_blocks());
+        const long tiles = (long)ps.dir_tiles_y * ps.dir_tiles_z, res = std::max(1, pcg_resident_blocks(fused));
         for (long num : {1L, 2L, 4L, 6L}) {   // half-rounds
-            const long want = std::max<long>(1, std::min<long>(pg.nx, (num * res / 2 + tiles - 1) / tiles));
+            const long want = std::max<long>(1, std::min<long>(pg.nx, num * res / 2 / tiles));
             chunks.push_back((int)((pg.nx + want - 1) / want));
         }
     }
@@ -1265,6 +1266,7 @@ fia_status fia_ctx_autotune(fia_ctx *c, const fia_grid3d *g, const uint8_t *soli
             t.pcg_recompute = mode == 2 ? -1 : mode;
             t.pcg_fused = mode == 2 ? 1 : 0;
             PcgState ps;
+            ps.fused = mode == 2;
             pcg_direction_tiling(pg.nx, pg.ny, pg.nz, ps, chunk, t.pcg_recompute);
             if (ps.dir_blocks > 4 * kPcgMaxBlocks) continue;   // every block re-sums the partials
             const auto key = launch_key(t);

@@ -238,9 +238,9 @@ struct PcgState {
     int it;             // iterations enqueued since the start (selects the p buffer)
 };
 int pcg_blocks(size_t n);
-int pcg_resident_blocks();
-// chunk_x > 0: the x planes one tile sweeps (else one round of the device's resident blocks, or
-// about 2048 cells per block for a solve sharing its launches with others: shared);
+int pcg_resident_blocks(bool fused);
+// chunk_x > 0: the x planes one tile sweeps (else one round of the device's resident blocks of
+// st.fused's form, or about 2048 cells per block for a solve sharing the device: shared);
 // recompute 0 / 1: the update streams q / recomputes it (else chosen by grid size and chunk)
 void pcg_direction_tiling(int nx, int ny, int nz, PcgState &st, int chunk_x = 0, int recompute = -1,
                           bool shared = false);

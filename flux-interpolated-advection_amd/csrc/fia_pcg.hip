@@ -78,30 +78,34 @@ __global__ void pcg_setup_kernel(PcgGrid g, const float *u, const float *v, cons
     flags[c] = (uint8_t)(f | kPcgUnknown);
 }
 
-// Sums NV values over the block (fixed tree order); thread 0 gets the totals.
+// Sums NV values over the block in a fixed order: a shuffle tree in each wave, then the waves'
+// sums in order.  Every thread gets the same totals, and every block sums in the same order.
 template <int NV>
-__device__ __forceinline__ void block_sum(double (&v)[NV]) {
-    __shared__ double sh[NV][kPB];
-    const int t = threadIdx.x;
+__device__ __forceinline__ void block_total(double (&v)[NV]) {
+    constexpr int kWaves = kPB / 64;
+    __shared__ double sh[NV][kWaves];
 #pragma unroll
-    for (int a = 0; a < NV; ++a) sh[a][t] = v[a];
-    __syncthreads();
-    for (int off = kPB / 2; off > 0; off >>= 1) {
-        if (t < off) {
+    for (int a = 0; a < NV; ++a)
+        for (int off = 32; off > 0; off >>= 1) v[a] += __shfl_down(v[a], off, 64);
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
 #pragma unroll
-            for (int a = 0; a < NV; ++a) sh[a][t] += sh[a][t + off];
-        }
-        __syncthreads();
+        for (int a = 0; a < NV; ++a) sh[a][wave] = v[a];
     }
+    __syncthreads();
 #pragma unroll
-    for (int a = 0; a < NV; ++a) v[a] = sh[a][0];
+    for (int a = 0; a < NV; ++a) {
+        double t = sh[a][0];
+        for (int w = 1; w < kWaves; ++w) t += sh[a][w];
+        v[a] = t;
+    }
     __syncthreads();
 }
 
 // Writes block blk's sums of v into part[a * nb + blk].
 template <int NV>
 __device__ __forceinline__ void write_partials(double (&v)[NV], double *part, int nb, int blk) {
-    block_sum<NV>(v);
+    block_total<NV>(v);
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int a = 0; a < NV; ++a) part[(size_t)a * nb + blk] = v[a];
@@ -111,28 +115,13 @@ __device__ __forceinline__ void write_partials(double (&v)[NV], double *part, in
 // Totals of partials written by an earlier kernel, identical in every thread of every block.
 template <int NV>
 __device__ __forceinline__ void total_of(const double *part, int nb, double (&tot)[NV]) {
-    double v[NV];
 #pragma unroll
     for (int a = 0; a < NV; ++a) {
         double s = 0.0;
         for (int i = threadIdx.x; i < nb; i += kPB) s += part[(size_t)a * nb + i];
-        v[a] = s;
+        tot[a] = s;
     }
-    __shared__ double sh[NV][kPB];
-    const int t = threadIdx.x;
-#pragma unroll
-    for (int a = 0; a < NV; ++a) sh[a][t] = v[a];
-    __syncthreads();
-    for (int off = kPB / 2; off > 0; off >>= 1) {
-        if (t < off) {
-#pragma unroll
-            for (int a = 0; a < NV; ++a) sh[a][t] += sh[a][t + off];
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int a = 0; a < NV; ++a) tot[a] = sh[a][0];
-    __syncthreads();
+    block_total<NV>(tot);
 }
 
 // The converged flag, read by one thread and shared, so that every thread of the block takes the
@@ -1230,22 +1219,25 @@ int pcg_blocks(size_t n) {
 }
 
 // Blocks of the tile sweeps resident on the device at once: blocks per CU that the registers and
-// LDS of the direction, recomputing update and fused kernels allow (the least of them), times the
-// CUs; per device, computed once.  0 when the runtime cannot say.
-int pcg_resident_blocks() {
-    static std::atomic<int> cache[64];
+// LDS allow (the fused kernel's, or the lesser of the direction and recomputing update kernels'),
+// times the CUs; per device and form, computed once.  0 when the runtime cannot say.
+int pcg_resident_blocks(bool fused) {
+    static std::atomic<int> cache[64][2];
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
-    const int known = cache[dev].load();
+    const int known = cache[dev][fused].load();
     if (known > 0) return known;
-    int cus = 0, a = 0, b = 0, c = 0;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, pcg_direction_kernel, kPB, 0) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, pcg_update_kernel<true>, kPB, 0) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, pcg_fused_kernel, kPB, 0) != hipSuccess)
+    int cus = 0, a = 0, b = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    if (fused) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, pcg_fused_kernel, kPB, 0) != hipSuccess) return 0;
+        b = a;
+    } else if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, pcg_direction_kernel, kPB, 0) != hipSuccess ||
+               hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, pcg_update_kernel<true>, kPB, 0) != hipSuccess) {
         return 0;
-    const int v = std::max(1, std::min(a, std::min(b, c))) * std::max(1, cus);
-    cache[dev].store(v);
+    }
+    const int v = std::max(1, std::min(a, b)) * std::max(1, cus);
+    cache[dev][fused].store(v);
     return v;
 }
 
@@ -1259,7 +1251,7 @@ void pcg_direction_tiling(int nx, int ny, int nz, PcgState &st, int chunk_x, int
     st.dir_tiles_z = (nz + kTileZ - 1) / kTileZ;
     const long tiles = (long)st.dir_tiles_y * st.dir_tiles_z;
     // a solve that shares its launches with others (batched scenes) has no round of its own to fill
-    const int resident = shared ? 0 : pcg_resident_blocks();
+    const int resident = shared ? 0 : pcg_resident_blocks(st.fused);
     // chunks per tile column: as many as fit in one round of resident blocks (rounded down: a few
     // blocks over the round would run alone in a second one)
     const long want = resident > 0 ? std::max<long>(1, std::min<long>(nx, resident / tiles))
