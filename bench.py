@@ -115,7 +115,7 @@ def cpu_baseline(scenes, n, cfl, threads):
                       "reference path (per-cell faces, std::thread over x-slabs), %.1f s" % (n, cfl, sec)}
 
 
-def quad_roofline(st, steps, n, cfl, density):
+def quad_roofline(st, steps, n, cfl, density, optimized=False):
     """Roofline entry of the WENO6 integral quadrature kernel (piece_kernel<1>) from the
     library's HIP-event spans and device work counters of the timed steps.  achieved counts the
     flops the kernel executes (its 1D WENO6 evaluations x 73); the reference rule's count
@@ -127,13 +127,15 @@ def quad_roofline(st, steps, n, cfl, density):
     qlaunches = max(st["quad_launches"], 1)
     executed = st["weno_evals"] * FLOP_PER_WENO6_EXECUTED / quad_s / 1e12 if quad_s > 0 else 0.0
     rule = points * FLOP_PER_WENO3D_POINT / quad_s / 1e12 if quad_s > 0 else 0.0
-    traffic = TRAFFIC_BYTES_256_CFL4 if (density and n == 256 and cfl == 4.0) else None
+    traffic = TRAFFIC_BYTES_256_CFL4 if (density and not optimized and n == 256 and cfl == 4.0) else None
     avg_quad_s = quad_s / qlaunches
     return {"bound": "valu", "achieved": round(executed, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
             "frac": round(executed / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
             "peak_nonpacked": PEAK_FP32_NONPACKED_TFLOPS,
             "frac_nonpacked": round(executed / PEAK_FP32_NONPACKED_TFLOPS, 4),
-            "kernel": "piece_kernel<1> (WENO6 integral quadrature)", "compute": "FP32 VALU, no MFMA",
+            "kernel": ("piece_kernel<1, true> (WENO6-optimized integral quadrature; flops priced as the WENO6 "
+                       "form's, without its two extra reciprocals)" if optimized else
+                       "piece_kernel<1> (WENO6 integral quadrature)"), "compute": "FP32 VALU, no MFMA",
             "avg_launch_ms": round(st["quad_ms"] / qlaunches, 3),
             "weno6_evals_per_launch": st["weno_evals"] / qlaunches, "flop_per_weno6": FLOP_PER_WENO6_EXECUTED,
             "reference_rule": {"TFLOPs": round(rule, 3), "weno3d_points_per_launch": points / qlaunches,
@@ -165,7 +167,7 @@ def run_advection(args, fia, scenes, torch, velocity):
             torch.cuda.synchronize()
             fia.advect_velocity_xyz_flux_advection_3d(ctx, grid, dt, **opts)
     else:
-        opts = dict(split_method="xz-CellFaceAligned", interpolation_method="WENO6", use_integral=True,
+        opts = dict(split_method="xz-CellFaceAligned", interpolation_method=args.interp, use_integral=True,
                     num_gauss_quadrature_point_for_integrate_density=1,
                     enable_eliminate_zero_velocity_false_diffusion=True)
         step = lambda i: fia.move_substances_3d(ctx, grid, i, dt, **opts)
@@ -190,7 +192,8 @@ def run_advection(args, fia, scenes, torch, velocity):
                                                      w_d.abs().max().item()))}
     else:
         workload = ("move_substances_3D density step, %d^3 plume blob + sphere source, curl-noise velocity, CFL %g, "
-                    "WENO6, integral psi (Gauss 1), xz-CellFaceAligned, zero-velocity correction on" % (n, args.cfl))
+                    "%s, integral psi (Gauss 1), xz-CellFaceAligned, zero-velocity correction on" %
+                    (n, args.cfl, args.interp))
         extra = {"mass_after": float(rho_d.double().sum().item())}
     out = {
         "metric": METRIC, "value": round(n ** 3 * args.steps / sec / 1e6, 3), "unit": "Mcell-steps/s", "n_gpus": 1,
@@ -198,7 +201,8 @@ def run_advection(args, fia, scenes, torch, velocity):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": workload, "grid": [n, n, n], "cfl": args.cfl,
                    "passes_per_step": 9 if velocity else 3},
-        "roofline": quad_roofline(st, args.steps, n, args.cfl, not velocity),
+        "roofline": quad_roofline(st, args.steps, n, args.cfl, not velocity,
+                                  optimized=not velocity and args.interp == "WENO6-optimized"),
         "kernels_ms_per_step": {k: round(st[k] / args.steps, 3)
                                 for k in ("face_ms", "quad_ms", "vertex_ms", "psi_ms", "gather_ms")},
         "face_work": {"launches": st["face_launches"], "weno_pieces_per_step": st["pieces"] / args.steps,
@@ -583,6 +587,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=None, help="grid edge (256; warmstart: 128)")
     ap.add_argument("--cfl", type=float, default=4.0)
+    ap.add_argument("--interp", choices=("WENO6", "WENO6-optimized"), default="WENO6",
+                    help="density workload: interpolation method of the step")
     ap.add_argument("--workload", choices=("plume", "density", "velocity", "projection", "warmstart", "scenes"),
                     default="plume")
     ap.add_argument("--verbose", action="store_true", help="progress lines on stderr")

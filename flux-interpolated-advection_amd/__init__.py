@@ -27,7 +27,7 @@ STATUS = {0: "FIA_OK", 1: "FIA_ERR_INVALID_ARG", 2: "FIA_ERR_UNSUPPORTED", 3: "F
           8: "FIA_ERR_CUT"}
 
 # reference strings -> enums (reference src/split_face_3D.cpp:26-43, src/main.cpp:60,79)
-INTERP = {"linear": 0, "WENO6": 1, "1Dy_linear": 2, "1Dy_WENO6": 3}
+INTERP = {"linear": 0, "WENO6": 1, "1Dy_linear": 2, "1Dy_WENO6": 3, "WENO6-optimized": 4}
 SPLIT = {"xz-CellFaceAligned": 1, "xz-AxisAligned": 2, "x-CellFaceAligned_z-AxisAligned": 3,
          "x-AxisAligned_z-CellFaceAligned": 4}
 PSI = {"const": 0, "linear": 1, "WENO6": 2}

@@ -302,7 +302,9 @@ fia_status do_vertices(fia_ctx *c, const CompGrid &g, const float *u, const floa
     for (int a = 0; a < 3; ++a) HIP_TRY(c->disp[a].ensure(g.lattice()));
     VertexParams p;
     memset(&p, 0, sizeof(p));
-    p.g = g; p.u = u; p.v = v; p.w = w; p.dt_over_h = (float)dt_over_h; p.interp = interp;
+    // the vertex velocities of "WENO6-optimized" are WENO6's (reference src/grid_3d.cpp:557)
+    p.g = g; p.u = u; p.v = v; p.w = w; p.dt_over_h = (float)dt_over_h;
+    p.interp = interp == FIA_INTERP_WENO6_OPTIMIZED ? FIA_INTERP_WENO6 : interp;
     for (int a = 0; a < 3; ++a) p.disp[a] = c->disp[a].p;
     int sp;
     HIP_TRY(span_begin(c, SK_VERTEX, sp));
@@ -334,7 +336,11 @@ fia_status do_faces(fia_ctx *c, const CompGrid &g, const float *q, bool moving, 
     }
     p.twisted = c->twisted.p; p.twisted_count = c->counters.p; p.error = c->counters.p + 1;
     p.degen = c->degen.p; p.degen_count = c->counters.p + 2;
-    p.split = s.split; p.interp = s.interp; p.integral = s.integral; p.fam_mask = fam_mask;
+    p.split = s.split; p.integral = s.integral; p.fam_mask = fam_mask;
+    // "WENO6-optimized": the WENO6 step whose quadrature points are interpolated by the ISPC variant
+    // (reference src/calc_density_in_cell_from_quadrature_point_list*.cpp)
+    p.interp = s.interp == FIA_INTERP_WENO6_OPTIMIZED ? FIA_INTERP_WENO6 : s.interp;
+    p.weno_opt = s.interp == FIA_INTERP_WENO6_OPTIMIZED ? 1 : 0;
     p.general_all = s.min_area < 1.0;
     p.fuse_value = !p.integral && (p.interp == FIA_INTERP_LINEAR || p.interp == FIA_INTERP_1DY_LINEAR);
     p.kappa_q = 1.0f;
@@ -513,7 +519,7 @@ fia_status check_opts(const fia_advect_opts *o) {
     if (!o->use_flux_advection) return fail(FIA_ERR_UNSUPPORTED, "semi-Lagrangian advection is not implemented");
     if (o->use_maccormack) return fail(FIA_ERR_UNSUPPORTED, "MacCormack is not implemented");
     if (o->enable_cell_volume_correction) return fail(FIA_ERR_UNSUPPORTED, "cell volume correction is not implemented");
-    if (o->interpolation_method < FIA_INTERP_LINEAR || o->interpolation_method > FIA_INTERP_1DY_WENO6)
+    if (o->interpolation_method < FIA_INTERP_LINEAR || o->interpolation_method > FIA_INTERP_WENO6_OPTIMIZED)
         return fail(FIA_ERR_INVALID_ARG, "unknown interpolation method");
     if (o->interpolation_method == FIA_INTERP_1DY_WENO6)
         return fail(FIA_ERR_UNSUPPORTED, "1Dy_WENO6 has no psi rule in the step (reference leaves it undefined)");
@@ -1361,7 +1367,8 @@ fia_status fia_advect_flux_pass(fia_ctx *c, const fia_grid3d *g, float *q, doubl
     if ((st = check_grid(g, false)) != FIA_OK) return st;
     if (!q) return fail(FIA_ERR_INVALID_ARG, "q is NULL");
     if (!gauss_supported(num_gauss_density)) return fail(FIA_ERR_INVALID_ARG, "unsupported Gauss rule");
-    if (interp < FIA_INTERP_LINEAR || interp > FIA_INTERP_1DY_WENO6) return fail(FIA_ERR_INVALID_ARG, "unknown interpolation");
+    if (interp < FIA_INTERP_LINEAR || interp > FIA_INTERP_WENO6_OPTIMIZED)
+        return fail(FIA_ERR_INVALID_ARG, "unknown interpolation");
     if (split < FIA_SPLIT_NONE || split > FIA_SPLIT_X_AXIS_Z_CELLFACE) return fail(FIA_ERR_INVALID_ARG, "unknown split");
     if (psi_mode < FIA_PSI_CONST || psi_mode > FIA_PSI_WENO6) return fail(FIA_ERR_INVALID_ARG, "unknown psi mode");
     if (!(minimal_area > 0.0)) return fail(FIA_ERR_INVALID_ARG, "minimal_area must be > 0");
@@ -1391,6 +1398,9 @@ fia_status fia_advect_velocity_3d(fia_ctx *c, fia_grid3d *g, int dim, double dt,
     if ((st = check_grid(g, false)) != FIA_OK) return st;
     if ((st = check_opts(o)) != FIA_OK) return st;
     if (dim < -1 || dim > 2) return fail(FIA_ERR_INVALID_ARG, "dim must be -1, 0, 1 or 2");
+    if (o->interpolation_method == FIA_INTERP_WENO6_OPTIMIZED)
+        return fail(FIA_ERR_UNSUPPORTED, "WENO6-optimized velocity: the reference's velocity quadrature has no branch "
+                                         "for it (src/calc_velocity_from_quadrature_point_list_use_integral.cpp:85-111)");
     if (!isfinite(dt)) return fail(FIA_ERR_NONFINITE, "dt is not finite");
     HIP_TRY(set_device(c));
     if ((st = reset_error(c)) != FIA_OK) return st;
@@ -1454,7 +1464,7 @@ fia_status fia_interpolate(fia_ctx *c, const float *field, int mx, int my, int m
                            int on_device) {
     if (!c || !field || (!points_xyz && n) || (!out && n)) return fail(FIA_ERR_INVALID_ARG, "NULL argument");
     if (mx < 1 || my < 1 || mz < 1 || !(dx > 0)) return fail(FIA_ERR_INVALID_ARG, "bad field");
-    if (method < FIA_INTERP_LINEAR || method > FIA_INTERP_1DY_WENO6) return fail(FIA_ERR_INVALID_ARG, "unknown method");
+    if (method < FIA_INTERP_LINEAR || method > FIA_INTERP_WENO6_OPTIMIZED) return fail(FIA_ERR_INVALID_ARG, "unknown method");
     HIP_TRY(set_device(c));
     const size_t nf = (size_t)mx * my * mz;
     HIP_TRY(c->gen[0].ensure(nf)); HIP_TRY(c->gen[1].ensure(3 * n)); HIP_TRY(c->gen[2].ensure(n));
@@ -1481,7 +1491,7 @@ fia_status fia_backtrace_vertices(fia_ctx *c, const fia_grid3d *g, int dim, doub
     fia_status st;
     if ((st = check_grid(g, false)) != FIA_OK) return st;
     if (dim < -1 || dim > 2) return fail(FIA_ERR_INVALID_ARG, "dim must be -1..2");
-    if (interp < FIA_INTERP_LINEAR || interp > FIA_INTERP_1DY_WENO6) return fail(FIA_ERR_INVALID_ARG, "unknown interp");
+    if (interp < FIA_INTERP_LINEAR || interp > FIA_INTERP_WENO6_OPTIMIZED) return fail(FIA_ERR_INVALID_ARG, "unknown interp");
     HIP_TRY(set_device(c));
     GridView gv;
     if ((st = upload_grid(c, g, gv, false)) != FIA_OK) return st;

@@ -111,6 +111,7 @@ struct FaceParams {
     int *degen_count;
     int *error;              // first error code seen (0 = none)
     int split, interp, integral;
+    int weno_opt;            // interp IM_WENO6 with the "WENO6-optimized" quadrature interpolation
     int fam_mask;            // families computed (bit 0 x, 1 y, 2 z)
     int general_all;         // every face goes through the general cutter (minimal_area < 1)
     int fuse_value;          // the cutters store piece values, not records (non-integral rule with

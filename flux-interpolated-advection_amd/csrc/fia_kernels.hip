@@ -314,15 +314,17 @@ FIA_HD double piece_flux(const FaceParams &P, const double c[3], double ay, int 
     const double oqx = 0.5 - ex, oqy = 0.5 - ey, oqz = 0.5 - ez;
     const double opx = 0.5 - ex, opy = -ey, opz = 0.5 - ez;
     const Loader<double> PS{P.psi, mx, my + 1, mz};
+    // the quadrature points' interpolation ("WENO6-optimized": the ISPC variant)
+    const int im = (P.interp == IM_WENO6 && P.weno_opt) ? IM_WENO6_OPT : P.interp;
     if (!P.integral) {
-        return ay * interp_point<double>(PS, P.interp, c[0], c[1], c[2], opx, opy, opz, P.kappa_psi);
+        return ay * interp_point<double>(PS, im, c[0], c[1], c[2], opx, opy, opz, P.kappa_psi);
     }
     const Loader<float> Q{P.q, mx, my, mz};
     const double gy = c[1] + ey;             // G coordinate along y
     const double jf = floor(gy);
     const double f = gy - jf;
     const double level = jf - ey;            // physical-frame y of G face level j
-    const double psi = interp_point<double>(PS, P.interp, c[0], level, c[2], opx, opy, opz, P.kappa_psi);
+    const double psi = interp_point<double>(PS, im, c[0], level, c[2], opx, opy, opz, P.kappa_psi);
     float dens = 0.0f;
     const int nh = f < 0.5 ? 1 : 2;
     for (int half = 0; half < nh; ++half) {
@@ -332,7 +334,7 @@ FIA_HD double piece_flux(const FaceParams &P, const double c[3], double ay, int 
         else { a = level + 0.5; len = f - 0.5; frac = f - 0.5; }
         for (int gq = 0; gq < P.gauss.n; ++gq) {
             const double y = (0.5 * len) * P.gauss.x[gq] + 0.5 * len + a;
-            const float val = interp_point<float>(Q, P.interp, c[0], y, c[2], oqx, oqy, oqz, P.kappa_q);
+            const float val = interp_point<float>(Q, im, c[0], y, c[2], oqx, oqy, oqz, P.kappa_q);
             dens += val * (float)(frac * P.gauss.w[gq]);
         }
     }
@@ -349,15 +351,23 @@ FIA_HD double piece_flux(const FaceParams &P, const double c[3], double ay, int 
 // seven rows J-3..J+3 and each point adds its own y and x passes.  Pass order z, y, x and every
 // 1D evaluation are those of weno3d, so each value equals the per-point rule's (up to the
 // t == 0 skips, which return sample f_2 where this evaluates WENO6 at t = 0).
-template <int NG>
+// OPT: the "WENO6-optimized" interpolation (IM_WENO6_OPT): clamped x and z splits and weights;
+// a density point beyond the first or last sample row takes that row's value (the clamped rule's
+// result there), read from the z pass's window.
+template <int NG, bool OPT>
 FIA_HD double piece_flux_weno_integral(const FaceParams &P, const double c[3], double ay, int &npts,
                                                            int &nevals) {
     const int mx = P.g.m[0], my = P.g.m[1], mz = P.g.m[2];
     const double ex = 0.5 * P.g.e[0], ey = 0.5 * P.g.e[1], ez = 0.5 * P.g.e[2];
     int bx, bz;
     float tx, tz;
-    split_coord(c[0] - (0.5 - ex), bx, tx);
-    split_coord(c[2] - (0.5 - ez), bz, tz);
+    if constexpr (OPT) {
+        split_coord_clamped(c[0] - (0.5 - ex), mx, bx, tx);
+        split_coord_clamped(c[2] - (0.5 - ez), mz, bz, tz);
+    } else {
+        split_coord(c[0] - (0.5 - ex), bx, tx);
+        split_coord(c[2] - (0.5 - ez), bz, tz);
+    }
     const double gy = c[1] + ey;
     const double jf = floor(gy);
     const double f = gy - jf;
@@ -382,9 +392,9 @@ FIA_HD double piece_flux_weno_integral(const FaceParams &P, const double c[3], d
             double fz[6];
             if (in) load_row6<double>(P.psi, o0 + (unsigned)ix * sx, fz);
             else load_z6<double>(PS, bx - 2 + ix, J, bz - 2, zin, fz);
-            xr[ix] = weno6<double>(fz, wz);
+            xr[ix] = weno6<double, OPT>(fz, wz);
         }
-        psi = weno6<double>(xr, wx);
+        psi = weno6<double, OPT>(xr, wx);
     }
     // density points: half h, Gauss node g -> point p = h * NG + g
     const Loader<float> Q{P.q, mx, my, mz};
@@ -392,6 +402,7 @@ FIA_HD double piece_flux_weno_integral(const FaceParams &P, const double c[3], d
     const double level = jf - ey, oqy = 0.5 - ey;
     const int B = J - 1;   // sample row base of the lower-half points
     float ty[2 * NG], wt[2 * NG];
+    int edge[2 * NG];   // OPT: window row of a point beyond the sample rows, else -1
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
 #pragma unroll
@@ -401,8 +412,16 @@ FIA_HD double piece_flux_weno_integral(const FaceParams &P, const double c[3], d
             else if (h == 0) { a = level; len = 0.5; frac = 0.5; }
             else { a = level + 0.5; len = f - 0.5; frac = f - 0.5; }
             const double y = (0.5 * len) * P.gauss.x[g] + 0.5 * len + a;
-            ty[h * NG + g] = (float)((y - oqy) - (double)(B + h));
+            const double s = y - oqy;
+            ty[h * NG + g] = (float)(s - (double)(B + h));
             wt[h * NG + g] = (float)(frac * P.gauss.w[g]);
+            edge[h * NG + g] = -1;
+            if constexpr (OPT) {
+                if (s < 0.0 || s > (double)(my - 1)) {
+                    const int row = s < 0.0 ? 0 : my - 1;
+                    edge[h * NG + g] = clampi(row - (B - 2), 0, 6);
+                }
+            }
         }
     }
     const WenoCoef wz(tz, P.kappa_q), wx(tx, P.kappa_q);
@@ -420,18 +439,21 @@ FIA_HD double piece_flux_weno_integral(const FaceParams &P, const double c[3], d
             float fz[6];
             if (qin) load_row6<float>(P.q, q0 + (unsigned)ix * qsx + (unsigned)r * (unsigned)mz, fz);
             else load_z6<float>(Q, bx - 2 + ix, B - 2 + r, bz - 2, zin, fz);
-            zr[r] = weno6<float>(fz, wz);
+            zr[r] = weno6<float, OPT>(fz, wz);
         }
 #pragma unroll
         for (int p = 0; p < 2 * NG; ++p) {
             const int h = p / NG;
-            if (h < nh) yv[p][ix] = weno6<float>(&zr[h], WenoCoef(ty[p], P.kappa_q));
+            if (h < nh) {
+                if (OPT && edge[p] >= 0) yv[p][ix] = zr[edge[p]];
+                else yv[p][ix] = weno6<float, OPT>(&zr[h], WenoCoef(ty[p], P.kappa_q));
+            }
         }
     }
     float dens = 0.0f;
 #pragma unroll
     for (int p = 0; p < 2 * NG; ++p)
-        if (p / NG < nh) dens += weno6<float>(yv[p], wx) * wt[p];
+        if (p / NG < nh) dens += weno6<float, OPT>(yv[p], wx) * wt[p];
     npts += nh * NG;
     nevals += 7 + 6 * (nh == 2 ? 7 : 6) + 7 * nh * NG;
     return ay * (psi + (double)dens);
@@ -770,26 +792,27 @@ __device__ __forceinline__ void add_stats(unsigned long long *stats, int a, int 
 // Phase 3: quadrature of every piece slot, one thread per slot (the expensive WENO work runs
 // converged regardless of how many pieces each face has).  QM selects the quadrature: the
 // WENO6 integral rule with NG = QM Gauss points per half (1 or 2), or any mode (QM = 0).
-template <int QM>
+template <int QM, bool OPT = false>
 FIA_HD void piece_body(const FaceParams &P, unsigned long long q, int &npieces, int &npts, int &nevals) {
     const double ay = P.piece_ay[q];
     double val = 0.0;
     if (ay != 0.0) {
         const double c3[3] = {P.piece_c[0][q], P.piece_c[1][q], P.piece_c[2][q]};
         if constexpr (QM == 0) val = piece_flux(P, c3, ay, npts);
-        else val = piece_flux_weno_integral<QM>(P, c3, ay, npts, nevals);
+        else val = piece_flux_weno_integral<QM, OPT>(P, c3, ay, npts, nevals);
         npieces = 1;
     }
     P.piece_val[q] = val;
 }
 // The one-point WENO6 integral rule (QM = 1) is held to 128 VGPRs, four waves per SIMD, which its
-// VALU-bound quadrature needs to hide the row loads.
-template <int QM>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(QM == 1 ? 4 : 1)))
+// VALU-bound quadrature needs to hide the row loads.  OPT: the "WENO6-optimized" interpolation
+// (its three reciprocals per evaluation do not fit 128 VGPRs; it is left to the compiler).
+template <int QM, bool OPT>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(QM == 1 && !OPT ? 4 : 1)))
 piece_kernel(FaceParams P, unsigned long long nslots) {
     const unsigned long long q = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
     int npieces = 0, npts = 0, nevals = 0;
-    if (q < nslots) piece_body<QM>(P, q, npieces, npts, nevals);
+    if (q < nslots) piece_body<QM, OPT>(P, q, npieces, npts, nevals);
     if (QM != 0 && P.stats) add_stats(P.stats, npieces, npts, nevals);   // counted for the WENO6 rule only
 }
 
@@ -1220,9 +1243,17 @@ hipError_t launch_pieces(const FaceParams &p, unsigned long long nslots, hipStre
     const int b = block_or_default(p.piece_block);
     const dim3 grid(grid_for(nslots, b));
     const int qm = piece_quadrature_mode(p);
-    if (qm == 1) hipLaunchKernelGGL(piece_kernel<1>, grid, dim3(b), 0, s, p, nslots);
-    else if (qm == 2) hipLaunchKernelGGL(piece_kernel<2>, grid, dim3(b), 0, s, p, nslots);
-    else hipLaunchKernelGGL(piece_kernel<0>, grid, dim3(b), 0, s, p, nslots);
+    if (p.weno_opt) {
+        if (qm == 1) hipLaunchKernelGGL((piece_kernel<1, true>), grid, dim3(b), 0, s, p, nslots);
+        else if (qm == 2) hipLaunchKernelGGL((piece_kernel<2, true>), grid, dim3(b), 0, s, p, nslots);
+        else hipLaunchKernelGGL((piece_kernel<0, true>), grid, dim3(b), 0, s, p, nslots);
+    } else if (qm == 1) {
+        hipLaunchKernelGGL((piece_kernel<1, false>), grid, dim3(b), 0, s, p, nslots);
+    } else if (qm == 2) {
+        hipLaunchKernelGGL((piece_kernel<2, false>), grid, dim3(b), 0, s, p, nslots);
+    } else {
+        hipLaunchKernelGGL((piece_kernel<0, false>), grid, dim3(b), 0, s, p, nslots);
+    }
     return hipGetLastError();
 }
 
@@ -1391,7 +1422,9 @@ static bool zero_velocity_cuts_both(const FaceParams &p) {
 hipError_t launch_zero_velocity_faces(const FaceParams &p, hipStream_t s) {
     const int b = block_or_default(p.face_block);
     const dim3 grid(grid_for(p.g.faces(1), b));
-    if (p.interp == IM_WENO6 && !p.integral && zero_velocity_cuts_both(p))
+    // (the closed form's shared stencils are the WENO6 rule's; "WENO6-optimized" takes the
+    // general quadrature per piece)
+    if (p.interp == IM_WENO6 && !p.weno_opt && !p.integral && zero_velocity_cuts_both(p))
         hipLaunchKernelGGL(zero_velocity_face_kernel<true>, grid, dim3(b), 0, s, p);
     else
         hipLaunchKernelGGL(zero_velocity_face_kernel<false>, grid, dim3(b), 0, s, p);

@@ -43,8 +43,10 @@ FIA_HD float fast_rcp(float x) {
 // h_k = g_{k+1} - g_k, D_k = h_{k+1} - h_k, A = t (t-1) / 2 and B_0 = t (t-1) (t+4) / 6,
 // B_1 = t (t-1) (t+1) / 6, B_2 = t (t-1) (t-2) / 6.  eps is the reference's 1e-6 in the units of
 // the indicators below (S_phys = kappa S / 180); kappa in (0, 1e3] keeps eps^4 a normal float.
+// eps2 is the 1e-30 that the "WENO6-optimized" variant adds to S_phys^2 (reference
+// src/weno.ispc:10-51), in the squared units of the same indicators.
 struct WenoCoef {
-    float t, A, B0, B1, B2, c0, c1, c2, eps;
+    float t, A, B0, B1, B2, c0, c1, c2, eps, eps2;
     FIA_HD explicit WenoCoef(float tt, float kappa) {
         const float p = tt * (tt - 1.0f);
         t = tt;
@@ -56,6 +58,8 @@ struct WenoCoef {
         c1 = (3.0f - tt) * (tt + 2.0f) * (1.0f / 10.0f);
         c2 = (tt + 2.0f) * (tt + 1.0f) * (1.0f / 20.0f);
         eps = 1.8e-4f * fast_rcp(kappa);
+        const float s = 180.0f * fast_rcp(kappa);
+        eps2 = 1e-30f * (s * s);
     }
 };
 
@@ -67,19 +71,27 @@ struct WenoCoef {
 //   S_1 = 180 g_2^2 + 244 h_1^2 - 293 h_1 h_2 + 244 h_2^2
 //   S_2 = 180 g_2^2 + 634 h_2^2 - 683 h_2 h_3 + 244 h_3^2
 // The weights C_r / (1e-6 + S_phys,r)^2 are normalised, so the factor kappa / 180 of S_phys
-// cancels and e_r = S_r + eps stands in for 1e-6 + S_phys,r.
+// cancels and e_r = S_r + eps stands in for 1e-6 + S_phys,r.  OPT: the "WENO6-optimized" weights
+// C_r / (1e-30 + S_phys,r^2) (reference src/weno.ispc:46-50, whose rcp is taken as exact here),
+// with three reciprocals: e_r^2 + eps2 can be as small as eps2.
+template <bool OPT = false>
 FIA_HD float weno6_delta(float g0, float g1, float g2, float g3, float g4, const WenoCoef &w) {
     const float h0 = g1 - g0, h1 = g2 - g1, h2 = g3 - g2, h3 = g4 - g3;
     const float D0 = h1 - h0, D1 = h2 - h1, D2 = h3 - h2;
     const float u0 = w.A * h0 + w.B0 * D0, u1 = w.A * h1 + w.B1 * D1, u2 = w.A * h2 + w.B2 * D2;
     const float p00 = h0 * h0, p11 = h1 * h1, p22 = h2 * h2, p33 = h3 * h3;
     const float p01 = h0 * h1, p12 = h1 * h2, p23 = h2 * h3;
-    const float base = 180.0f * (g2 * g2) + w.eps;
+    const float base = 180.0f * (g2 * g2) + (OPT ? 0.0f : w.eps);
     const float e0 = base + 244.0f * p00 - 683.0f * p01 + 634.0f * p11;
     const float e1 = base + 244.0f * p11 - 293.0f * p12 + 244.0f * p22;
     const float e2 = base + 634.0f * p22 - 683.0f * p23 + 244.0f * p33;
-    const float E0 = e0 * e0, E1 = e1 * e1, E2 = e2 * e2;
     const float lin = w.t * g2;
+    if constexpr (OPT) {
+        const float a0 = w.c0 * fast_rcp(e0 * e0 + w.eps2), a1 = w.c1 * fast_rcp(e1 * e1 + w.eps2),
+                    a2 = w.c2 * fast_rcp(e2 * e2 + w.eps2);
+        return lin + (a0 * u0 + a1 * u1 + a2 * u2) * fast_rcp(a0 + a1 + a2);
+    }
+    const float E0 = e0 * e0, E1 = e1 * e1, E2 = e2 * e2;
     if (fmaxf(e0, fmaxf(e1, e2)) < 1e9f) {
         // weights C_r / e_r^2 scaled by E0 E1 E2 (one reciprocal): for eps <= e < 1e9 every
         // product stays between eps^4 and 1e36
@@ -91,12 +103,14 @@ FIA_HD float weno6_delta(float g0, float g1, float g2, float g3, float g4, const
 }
 
 // WENO6 of six samples of type V (float, or double for the y-prefix psi).
-template <class V> FIA_HD V weno6(const V f[6], const WenoCoef &w) {
+template <class V, bool OPT = false> FIA_HD V weno6(const V f[6], const WenoCoef &w) {
     const float g0 = (float)(f[1] - f[0]), g1 = (float)(f[2] - f[1]), g2 = (float)(f[3] - f[2]);
     const float g3 = (float)(f[4] - f[3]), g4 = (float)(f[5] - f[4]);
-    return f[2] + (V)weno6_delta(g0, g1, g2, g3, g4, w);
+    return f[2] + (V)weno6_delta<OPT>(g0, g1, g2, g3, g4, w);
 }
-template <class V> FIA_HD V weno6(const V f[6], float t, float kappa) { return weno6<V>(f, WenoCoef(t, kappa)); }
+template <class V, bool OPT = false> FIA_HD V weno6(const V f[6], float t, float kappa) {
+    return weno6<V, OPT>(f, WenoCoef(t, kappa));
+}
 
 // Clamped loader of a staggered field.
 template <class V> struct Loader {
@@ -138,7 +152,7 @@ template <class V> FIA_HD void load_row6(const V *__restrict__ base, unsigned of
 // 3D WENO6, passes z then y then x (reference src/WENO6_interpolation_3d.cpp:7-258).  A pass
 // whose sample coordinate is on a grid node (SX/SY/SZ) is skipped: WENO6 at t = 0 returns f_2
 // exactly (every candidate interpolates node 2 and the weights sum to 1).
-template <bool SX, bool SY, bool SZ, class V, class L>
+template <bool SX, bool SY, bool SZ, bool OPT, class V, class L>
 FIA_HD V weno3d_t(const L &ld, int bx, int by, int bz, float tx, float ty, float tz, float kappa) {
     V yv[6];
 #pragma unroll
@@ -154,26 +168,26 @@ FIA_HD V weno3d_t(const L &ld, int bx, int by, int bz, float tx, float ty, float
                 V f[6];
 #pragma unroll
                 for (int iz = 0; iz < 6; ++iz) f[iz] = ld(bx - 2 + ix, by - 2 + iy, bz - 2 + iz);
-                zr[iy] = weno6<V>(f, tz, kappa);
+                zr[iy] = weno6<V, OPT>(f, tz, kappa);
             }
         }
-        yv[ix] = SY ? zr[2] : weno6<V>(zr, ty, kappa);
+        yv[ix] = SY ? zr[2] : weno6<V, OPT>(zr, ty, kappa);
     }
-    return SX ? yv[2] : weno6<V>(yv, tx, kappa);
+    return SX ? yv[2] : weno6<V, OPT>(yv, tx, kappa);
 }
 
 // Runtime dispatch on which passes sit on a node (uniform inside a wave for lattice vertices).
-template <class V, class L>
+template <class V, bool OPT = false, class L>
 FIA_HD V weno3d(const L &ld, int bx, int by, int bz, float tx, float ty, float tz, float kappa) {
     const int m = (tx == 0.0f ? 4 : 0) | (ty == 0.0f ? 2 : 0) | (tz == 0.0f ? 1 : 0);
     switch (m) {
-        case 0: return weno3d_t<false, false, false, V>(ld, bx, by, bz, tx, ty, tz, kappa);
-        case 1: return weno3d_t<false, false, true, V>(ld, bx, by, bz, tx, ty, tz, kappa);
-        case 2: return weno3d_t<false, true, false, V>(ld, bx, by, bz, tx, ty, tz, kappa);
-        case 3: return weno3d_t<false, true, true, V>(ld, bx, by, bz, tx, ty, tz, kappa);
-        case 4: return weno3d_t<true, false, false, V>(ld, bx, by, bz, tx, ty, tz, kappa);
-        case 5: return weno3d_t<true, false, true, V>(ld, bx, by, bz, tx, ty, tz, kappa);
-        case 6: return weno3d_t<true, true, false, V>(ld, bx, by, bz, tx, ty, tz, kappa);
+        case 0: return weno3d_t<false, false, false, OPT, V>(ld, bx, by, bz, tx, ty, tz, kappa);
+        case 1: return weno3d_t<false, false, true, OPT, V>(ld, bx, by, bz, tx, ty, tz, kappa);
+        case 2: return weno3d_t<false, true, false, OPT, V>(ld, bx, by, bz, tx, ty, tz, kappa);
+        case 3: return weno3d_t<false, true, true, OPT, V>(ld, bx, by, bz, tx, ty, tz, kappa);
+        case 4: return weno3d_t<true, false, false, OPT, V>(ld, bx, by, bz, tx, ty, tz, kappa);
+        case 5: return weno3d_t<true, false, true, OPT, V>(ld, bx, by, bz, tx, ty, tz, kappa);
+        case 6: return weno3d_t<true, true, false, OPT, V>(ld, bx, by, bz, tx, ty, tz, kappa);
         default: return ld(bx, by, bz);
     }
 }
@@ -221,7 +235,22 @@ FIA_HD void split_coord(double s, int &b, float &t) {
     t = (float)(s - f);
 }
 
-enum InterpMethod { IM_LINEAR = 0, IM_WENO6 = 1, IM_1DY_LINEAR = 2, IM_1DY_WENO6 = 3 };
+// The "WENO6-optimized" rule's split along one axis of m samples (reference
+// src/sginterp3.cpp:254-258 and 299-301): the coordinate held to the samples' range [0, m - 1],
+// the base to [0, m - 2] and the fraction to [0, 1].  Beyond the last sample the value is that
+// sample's (the candidates all pass through it at t = 1), where the WENO6 rule extrapolates from
+// the clamped samples.
+FIA_HD void split_coord_clamped(double s, int m, int &b, float &t) {
+    const double c = fmin(fmax(s, 0.0), (double)(m - 1));
+    const int f = (int)floor(c);
+    b = f < m - 2 ? f : (m >= 2 ? m - 2 : 0);
+    t = (float)(c - (double)b);
+}
+
+// IM_WENO6_OPT: "WENO6-optimized", the quadrature-point interpolation of the reference's ISPC path
+// (src/sginterp3.cpp:203-383, src/sginterp3.ispc:79-115): passes z, y, x as WENO6, with the
+// clamped split above and the weights of weno6_delta<true>.
+enum InterpMethod { IM_LINEAR = 0, IM_WENO6 = 1, IM_1DY_LINEAR = 2, IM_1DY_WENO6 = 3, IM_WENO6_OPT = 4 };
 
 // Interpolation of a staggered field at p (cell units, in the same frame as the field's
 // offsets o, which are in cells).  o_col is the physical offset used by the 1Dy-WENO6 column
@@ -231,6 +260,12 @@ FIA_HD V interp_point(const L &ld, int method, double px, double py, double pz, 
                       float kappa) {
     int bx, by, bz;
     float tx, ty, tz;
+    if (method == IM_WENO6_OPT) {
+        split_coord_clamped(px - ox, ld.mx, bx, tx);
+        split_coord_clamped(py - oy, ld.my, by, ty);
+        split_coord_clamped(pz - oz, ld.mz, bz, tz);
+        return weno3d<V, true>(ld, bx, by, bz, tx, ty, tz, kappa);
+    }
     split_coord(py - oy, by, ty);
     if (method == IM_1DY_LINEAR) {
         // column of the containing cell: floor(p - (o - 1/2))

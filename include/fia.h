@@ -46,7 +46,14 @@ typedef enum fia_interp {
     FIA_INTERP_LINEAR = 0,      /* "linear"     : trilinear, clamp-to-edge                    */
     FIA_INTERP_WENO6 = 1,       /* "WENO6"      : 6^3 WENO6, z then y then x passes           */
     FIA_INTERP_1DY_LINEAR = 2,  /* "1Dy_linear" : linear in y inside the containing column    */
-    FIA_INTERP_1DY_WENO6 = 3    /* "1Dy_WENO6"  : WENO6 in y inside the containing column     */
+    FIA_INTERP_1DY_WENO6 = 3,   /* "1Dy_WENO6"  : WENO6 in y inside the containing column     */
+    FIA_INTERP_WENO6_OPTIMIZED = 4  /* "WENO6-optimized": the density step's quadrature points
+                                       interpolated by the reference's ISPC WENO6
+                                       (src/sginterp3.cpp:203-383, src/weno.ispc:10-51):
+                                       alpha = C / (1e-30 + S^2), coordinates clamped to the
+                                       samples' range; vertices and psi as "WENO6".  Density only:
+                                       the reference's velocity path has no branch for it
+                                       (src/calc_velocity_from_quadrature_point_list_use_integral.cpp:85-111) */
 } fia_interp;
 
 /* Face split methods (reference src/split_face_3D.cpp:26-43).  Any other string of the

@@ -92,6 +92,7 @@ inline fia_interp interp_of(const std::string &s) {
     if (s == "linear") return FIA_INTERP_LINEAR;
     if (s == "1Dy_linear") return FIA_INTERP_1DY_LINEAR;
     if (s == "1Dy_WENO6") return FIA_INTERP_1DY_WENO6;
+    if (s == "WENO6-optimized") return FIA_INTERP_WENO6_OPTIMIZED;   // density only (velocity throws)
     throw std::runtime_error("libfia: interpolation_method \"" + s + "\" is not implemented");
 }
 

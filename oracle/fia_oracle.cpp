@@ -82,7 +82,7 @@ template <class R> struct Field {
 // (reference src/WENO6_interpolation_3d.cpp:63-123): Newton-form cubic candidates, ideal weights
 // C0=d4 d5/20h^2, C1=-d5 d0/10h^2, C2=d0 d1/20h^2, smoothness indicators as integer quadratic
 // forms /180, alpha = C/(1e-6+S)^2.
-template <class R> static R weno1d(const R f[6], const R d[6], R h) {
+template <class R> static R weno1d(const R f[6], const R d[6], R h, bool opt = false) {
     R q[3];
     for (int i = 0; i < 3; ++i) {
         q[i] = f[i] + (f[i + 1] - f[i]) * d[i] / h
@@ -108,7 +108,10 @@ template <class R> static R weno1d(const R f[6], const R d[6], R h) {
             + 1854 * f[5] * f[3]) / R(180.0);
     R a[3];
     const R eps = R(0.000001);
-    for (int i = 0; i < 3; ++i) a[i] = c[i] / ((eps + s[i]) * (eps + s[i]));
+    // "WENO6-optimized" (reference src/weno.ispc:46-50, its rcp taken as exact):
+    // alpha = C / (1e-30 + S^2)
+    if (opt) for (int i = 0; i < 3; ++i) a[i] = c[i] / (R(1e-30) + s[i] * s[i]);
+    else for (int i = 0; i < 3; ++i) a[i] = c[i] / ((eps + s[i]) * (eps + s[i]));
     R sum = a[0] + a[1] + a[2];
     R w0 = a[0] / sum, w1 = a[1] / sum, w2 = a[2] / sum;
     return w0 * q[0] + w1 * q[1] + w2 * q[2];
@@ -140,6 +143,45 @@ template <class R> static R weno3d(const Field<R> &F, R px, R py, R pz, R h) {
     R yv[6];
     for (int ix = 0; ix < 6; ++ix) yv[ix] = weno1d<R>(zv[ix], dy, h);
     return weno1d<R>(yv, dxv, h);
+}
+
+// "WENO6-optimized": the reference's ISPC interpolation of quadrature points
+// (src/sginterp3.cpp:203-383 with presort, src/sginterp3.ispc:79-115).  Per axis the coordinate in
+// sample units is held to [0, m - 1], the base cell to [0, m - 2] (src/sginterp3.cpp:254-258) and
+// the fraction to [0, 1] (:299-301); samples base-2 .. base+3 clamp to the field; passes run along
+// the fastest axis first (z, then y, then x, as the reference hands it z as its x); the 1D rule
+// is weno1d with alpha = C / (1e-30 + S^2).
+template <class R> static void clamped_split(R s, int m, int &b, R &t) {
+    const R c = std::min(std::max(s, R(0)), R(m - 1));
+    const int f = (int)std::floor(c);
+    b = f < m - 2 ? f : (m >= 2 ? m - 2 : 0);
+    t = c - R(b);
+}
+template <class R> static R weno3d_opt(const Field<R> &F, R px, R py, R pz, R h) {
+    int bx, by, bz;
+    R tx, ty, tz;
+    clamped_split<R>(px / h - R(F.ox), F.mx, bx, tx);
+    clamped_split<R>(py / h - R(F.oy), F.my, by, ty);
+    clamped_split<R>(pz / h - R(F.oz), F.mz, bz, tz);
+    R dz[6], dy[6], dxv[6];
+    for (int i = 0; i < 6; ++i) {
+        dz[i] = (tz - R(i - 2)) * h;
+        dy[i] = (ty - R(i - 2)) * h;
+        dxv[i] = (tx - R(i - 2)) * h;
+    }
+    R zv[6][6];
+    for (int ix = 0; ix < 6; ++ix) {
+        const int jx = clampi(bx - 2 + ix, 0, F.mx - 1);
+        for (int iy = 0; iy < 6; ++iy) {
+            const int jy = clampi(by - 2 + iy, 0, F.my - 1);
+            R v[6];
+            for (int iz = 0; iz < 6; ++iz) v[iz] = F.at(jx, jy, clampi(bz - 2 + iz, 0, F.mz - 1));
+            zv[ix][iy] = weno1d<R>(v, dz, h, true);
+        }
+    }
+    R yv[6];
+    for (int ix = 0; ix < 6; ++ix) yv[ix] = weno1d<R>(zv[ix], dy, h, true);
+    return weno1d<R>(yv, dxv, h, true);
 }
 
 // WENO6 along y inside the column floor(x/h), floor(z/h)
@@ -191,7 +233,7 @@ template <class R> static R linear1dy(const Field<R> &F, R px, R py, R pz, R h) 
     return b1 * v0 + b0 * v1;
 }
 
-enum Interp { I_LINEAR = 0, I_WENO6 = 1, I_1DY_LINEAR = 2, I_1DY_WENO6 = 3 };
+enum Interp { I_LINEAR = 0, I_WENO6 = 1, I_1DY_LINEAR = 2, I_1DY_WENO6 = 3, I_WENO6_OPT = 4 };
 enum PsiMode { P_CONST = 0, P_LINEAR = 1, P_WENO6 = 2 };
 enum Split { S_NONE = 0, S_XZ_CELL = 1, S_XZ_AXIS = 2, S_XCELL_ZAXIS = 3, S_XAXIS_ZCELL = 4 };
 
@@ -201,6 +243,7 @@ template <class R> static R interp(const Field<R> &F, int method, R px, R py, R 
         case I_WENO6: return weno3d(F, px, py, pz, h);
         case I_1DY_LINEAR: return linear1dy(F, px, py, pz, h);
         case I_1DY_WENO6: return weno1dy(F, px, py, pz, h);
+        case I_WENO6_OPT: return weno3d_opt(F, px, py, pz, h);
     }
     return R(0);
 }
@@ -585,7 +628,9 @@ template <class R, class C> struct FaceIntegrator {
     bool face(const int lat[4][3], R &acc_density, R &acc_psi, long &pieces) const {
         R vert[4][3];
         for (int k = 0; k < 4; ++k)
-            backtrace_vertex<R>(G, vel, o.interp, h, dt, lat[k][0], lat[k][1], lat[k][2], o.zero_velocity != 0,
+            // "WENO6-optimized" backtraces with WENO6 (reference src/grid_3d.cpp:557)
+            backtrace_vertex<R>(G, vel, o.interp == I_WENO6_OPT ? I_WENO6 : o.interp, h, dt, lat[k][0], lat[k][1],
+                                lat[k][2], o.zero_velocity != 0,
                                 vert[k]);
         C col[4][2];
         face_colours<C, R>(vert, h, o.split, col);
@@ -907,6 +952,7 @@ extern "C" double orc_weno1d(int prec, const double *f, const double *d, double 
     if (prec) { float ff[6], dd[6]; for (int i = 0; i < 6; ++i) { ff[i] = (float)f[i]; dd[i] = (float)d[i]; } return weno1d<float>(ff, dd, (float)h); }
     return weno1d<double>(f, d, h);
 }
+extern "C" double orc_weno1d_opt(const double *f, const double *d, double h) { return weno1d<double>(f, d, h, true); }
 
 template <class R>
 static void interp_t(const double *field, int mx, int my, int mz, double ox, double oy, double oz, double h,
@@ -1108,6 +1154,7 @@ extern "C" int orc_advect_velocity(int prec, int cprec, int dim, int nx, int ny,
                         double *stats) {
     if (!gauss_n_supported(ng)) return 1;
     if (method == I_1DY_WENO6) return 2;
+    if (method == I_WENO6_OPT) return 2;   // the reference's velocity path has no branch for it
     if (prec) return cprec ? vel_t<float, float>(dim, nx, ny, nz, h, u, v, w, dt, method, integ, corr, ng, min_area, nthreads, stats)
                            : vel_t<float, double>(dim, nx, ny, nz, h, u, v, w, dt, method, integ, corr, ng, min_area, nthreads, stats);
     return cprec ? vel_t<double, float>(dim, nx, ny, nz, h, u, v, w, dt, method, integ, corr, ng, min_area, nthreads, stats)

@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "libfia_oracle.so")
 REF_CUTTER_PATH = os.path.join(HERE, "_ref", "libref_facecutter.so")
 
-INTERP = {"linear": 0, "WENO6": 1, "1Dy_linear": 2, "1Dy_WENO6": 3}
+INTERP = {"linear": 0, "WENO6": 1, "1Dy_linear": 2, "1Dy_WENO6": 3, "WENO6-optimized": 4}
 PSI = {"const": 0, "linear": 1, "WENO6": 2}
 SPLIT = {"none": 0, "xz-CellFaceAligned": 1, "xz-AxisAligned": 2,
          "x-CellFaceAligned_z-AxisAligned": 3, "x-AxisAligned_z-CellFaceAligned": 4}
@@ -46,6 +46,8 @@ def lib():
         L.orc_gauss_legendre.argtypes = [ctypes.c_int, _dp, _dp]
         L.orc_weno1d.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_double]
         L.orc_weno1d.restype = ctypes.c_double
+        L.orc_weno1d_opt.argtypes = [_dp, _dp, ctypes.c_double]
+        L.orc_weno1d_opt.restype = ctypes.c_double
         L.orc_interpolate.argtypes = [ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                       ctypes.c_int, _dp, ctypes.c_long, _dp]
@@ -89,7 +91,11 @@ def gauss_legendre(n):
     return x, w
 
 
-def weno1d(f, d, h, prec=0):
+def weno1d(f, d, h, prec=0, opt=False):
+    """1D WENO6 of six samples at signed distances d; opt: the "WENO6-optimized" weights
+    C / (1e-30 + S^2) (FP64 only)."""
+    if opt:
+        return lib().orc_weno1d_opt(_p(_f64(f)), _p(_f64(d)), h)
     return lib().orc_weno1d(prec, _p(_f64(f)), _p(_f64(d)), h)
 
 

@@ -70,6 +70,10 @@ class HostKernels:
         L.host_face_pieces.restype = ctypes.c_int
         L.host_weno6.argtypes = [dp, ctypes.c_double, ctypes.c_double, ctypes.c_int]
         L.host_weno6.restype = ctypes.c_double
+        L.host_weno6_opt.argtypes = [dp, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+        L.host_weno6_opt.restype = ctypes.c_double
+        L.host_split_clamped.argtypes = [ctypes.c_double, ctypes.c_int, dp]
+        L.host_split_clamped.restype = ctypes.c_int
         L.host_pipeline_pieces.argtypes = [dp, dp, dp, ctypes.c_int]
         L.host_pipeline_pieces.restype = ctypes.c_int
         L.host_pipeline_records.argtypes = [dp, dp, dp, ctypes.c_int]
@@ -98,9 +102,17 @@ class HostKernels:
         k = self.lib.host_face_class(col.ctypes.data_as(self._dp))
         return bool(k & 1), bool(k & 2)
 
-    def weno6(self, f, t, kappa=1.0, dbl=False):
+    def weno6(self, f, t, kappa=1.0, dbl=False, opt=False):
         f = np.ascontiguousarray(f, dtype=np.float64)
-        return self.lib.host_weno6(f.ctypes.data_as(self._dp), float(t), float(kappa), int(dbl))
+        fn = self.lib.host_weno6_opt if opt else self.lib.host_weno6
+        return fn(f.ctypes.data_as(self._dp), float(t), float(kappa), int(dbl))
+
+    def split_clamped(self, s, m):
+        """The "WENO6-optimized" split of sample coordinate s over m samples: (base, fraction)."""
+        import ctypes
+        t = ctypes.c_double(0.0)
+        b = self.lib.host_split_clamped(float(s), int(m), ctypes.byref(t))
+        return b, t.value
 
 
 @pytest.fixture(scope="session")

@@ -113,6 +113,24 @@ extern "C" double host_weno6(const double *f6, double t, double kappa, int dbl) 
     return (double)fia::weno6<float>(f, w);
 }
 
+// The same with the "WENO6-optimized" weights (weno6<V, true>).
+extern "C" double host_weno6_opt(const double *f6, double t, double kappa, int dbl) {
+    const fia::WenoCoef w((float)t, (float)kappa);
+    if (dbl) return fia::weno6<double, true>(f6, w);
+    float f[6];
+    for (int i = 0; i < 6; ++i) f[i] = (float)f6[i];
+    return (double)fia::weno6<float, true>(f, w);
+}
+
+// The "WENO6-optimized" split of a sample coordinate s over m samples: base (return) and fraction.
+extern "C" int host_split_clamped(double s, int m, double *t) {
+    int b;
+    float tf;
+    fia::split_coord_clamped(s, m, b, tf);
+    *t = tf;
+    return b;
+}
+
 // The clipping pipeline's pieces of one face (the degenerate-face path): every (strip, band) slot
 // in the order s * (z bands) + m, as xi-eta centroid and area (0 for an empty slot).  Returns the
 // slot count, or -1 if the pipeline saw a twisted cut.

@@ -29,7 +29,7 @@ def smooth_field(shape, seed):
     return f
 
 
-@pytest.mark.parametrize("method", ["WENO6", "linear", "1Dy_linear", "1Dy_WENO6"])
+@pytest.mark.parametrize("method", ["WENO6", "linear", "1Dy_linear", "1Dy_WENO6", "WENO6-optimized"])
 @pytest.mark.parametrize("kind", list(FIELD_OFFSETS))
 def test_interpolate_matches_oracle(fia, oracle, ctx, method, kind):
     print("case interpolate", method, kind, flush=True)

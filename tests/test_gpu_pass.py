@@ -35,6 +35,10 @@ PASS_CASES = [
     ("xz-CellFaceAligned", "linear", "linear", False, True, 1),
     ("none", "WENO6", "WENO6", False, True, 1),
     ("x-CellFaceAligned_z-AxisAligned", "WENO6", "const", False, False, 1),
+    ("xz-CellFaceAligned", "WENO6-optimized", "WENO6", False, True, 1),
+    ("xz-CellFaceAligned", "WENO6-optimized", "WENO6", False, True, 3),
+    ("xz-CellFaceAligned", "WENO6-optimized", "const", True, False, 1),
+    ("none", "WENO6-optimized", "WENO6", False, False, 1),
 ]
 
 
@@ -57,7 +61,7 @@ def test_flux_pass_matches_oracle(fia, oracle, ctx, scenes, case, cfl):
     assert rel_linf(q, ref) < TOL
 
 
-@pytest.mark.parametrize("interp,split", [("WENO6", "xz-CellFaceAligned"), ("WENO6", "y-CellFaceAligned"),
+@pytest.mark.parametrize("interp,split", [("WENO6", "xz-CellFaceAligned"), ("WENO6", "y-CellFaceAligned"), ("WENO6-optimized", "xz-CellFaceAligned"),
                                           ("linear", "xz-CellFaceAligned"), ("1Dy_linear", "xz-AxisAligned")])
 @pytest.mark.parametrize("n,cfl", [(16, 1.0), (20, 2.0), (24, 4.0)])
 def test_move_substances_matches_oracle(fia, oracle, ctx, scenes, interp, split, n, cfl):

@@ -75,6 +75,8 @@ def weno6_numpy(f, t, eps_form="reference"):
     S = _jiang_shu(f)
     if eps_form == "reference":
         a = C / (1e-6 + S) ** 2
+    elif eps_form == "optimized":   # the ISPC variant (src/weno.ispc:46-50): 1e-30 added to S^2
+        a = C / (1e-30 + S ** 2)
     else:   # a plausible mistake: epsilon added to S^2
         a = C / (1e-6 + S ** 2)
     return float((a * q).sum() / a.sum())
@@ -134,6 +136,80 @@ def test_product_weno6_matches_jiang_shu(cutter_host):
             got = cutter_host.weno6(f, t, 1.0)
             worst = max(worst, abs(got - want) / np.abs(f).max())
     assert worst < 2e-5
+
+
+# ---------------------------------------------------------------------------------------------
+# "WENO6-optimized" (the reference's ISPC interpolation of quadrature points): the same candidates,
+# ideal weights and indicators, alpha = C / (1e-30 + S^2), and coordinates clamped to the samples
+
+@pytest.mark.parametrize("scale", [3e-4, 3e-3, 1.0, 50.0])
+def test_weno1d_opt_matches_jiang_shu(oracle, scale):
+    rng = np.random.default_rng(13)
+    worst, sep = 0.0, 0.0
+    for _ in range(150):
+        t = rng.uniform(0, 1)
+        f = scale * rng.normal(size=6)
+        f[rng.integers(0, 6)] += scale * 4.0 * rng.choice([-1, 1])
+        want = weno6_numpy(f, t, "optimized")
+        got = oracle.weno1d(f, _dist(t), 1.0, opt=True)
+        worst = max(worst, abs(got - want) / np.abs(f).max())
+        sep = max(sep, abs(weno6_numpy(f, t) - want) / np.abs(f).max())
+    assert worst < 1e-11
+    if scale <= 3e-3:   # where S is comparable to the reference form's 1e-6 the two rules differ
+        assert sep > 1e-4
+
+
+def test_product_weno6_opt_matches_jiang_shu(cutter_host):
+    # the library's FP32 difference form with the optimized weights (weno6<V, true>), on rough
+    # data, on data whose indicators vanish (constant samples: the ideal weights), and with the
+    # units factor kappa of the psi field (S in units of kappa / 180)
+    rng = np.random.default_rng(17)
+    worst = 0.0
+    for scale in (3e-2, 1.0, 50.0):
+        for _ in range(200):
+            t = float(np.float32(rng.uniform(0, 1)))
+            f = (scale * rng.normal(size=6)).astype(np.float32).astype(np.float64)
+            f[rng.integers(0, 6)] += scale * 3.0
+            f = f.astype(np.float32).astype(np.float64)
+            want = weno6_numpy(f, t, "optimized")
+            worst = max(worst, abs(cutter_host.weno6(f, t, 1.0, opt=True) - want) / np.abs(f).max())
+            # kappa: samples in cells^2-scaled units, S rescaled back (S_phys = kappa S / 180)
+            kap = 1.0 / 4096.0
+            g = f / kap
+            worst = max(worst, abs(cutter_host.weno6(g, t, kap, dbl=True, opt=True) * kap - want) / np.abs(f).max())
+    assert worst < 2e-5
+    for t in (0.0, 0.3, 1.0):
+        assert cutter_host.weno6(np.full(6, 2.5), t, 1.0, opt=True) == 2.5
+
+
+def test_split_clamped_holds_the_coordinate_to_the_samples(cutter_host):
+    m = 10
+    for s, b, t in [(-3.0, 0, 0.0), (-0.25, 0, 0.0), (0.0, 0, 0.0), (3.25, 3, 0.25), (8.5, 8, 0.5), (9.0, 8, 1.0),
+                    (9.75, 8, 1.0), (40.0, 8, 1.0)]:
+        got_b, got_t = cutter_host.split_clamped(s, m)
+        assert (got_b, got_t) == (b, t), s
+
+
+def test_interpolate_opt_clamps_to_the_samples(oracle):
+    # a field cubic along each axis: inside the samples both rules are exact; beyond the first or
+    # last sample the optimized rule returns that sample's value exactly (the WENO6 rule, on the
+    # clamped samples, comes to the same value only up to its epsilon)
+    n = 12
+    h = 1.0 / n
+    c = lambda s: 0.3 + 0.7 * s - 0.05 * s ** 2 + 0.004 * s ** 3
+    idx = np.arange(n, dtype=np.float64)
+    f = c(idx)[:, None, None] + 2.0 * c(idx)[None, :, None] - c(idx)[None, None, :]
+    off = (0.5, 0.5, 0.5)
+    inside = np.array([[3.3, 5.7, 7.1], [4.02, 6.5, 5.5], [5.5, 4.25, 6.75]]) * h
+    want = c(inside[:, 0] / h - 0.5) + 2.0 * c(inside[:, 1] / h - 0.5) - c(inside[:, 2] / h - 0.5)
+    for method in ("WENO6", "WENO6-optimized"):
+        got = oracle.interpolate(f, off, h, method, inside)
+        assert np.abs(got - want).max() < 1e-12, method
+    beyond = np.array([[n + 0.4, 5.7, 7.1], [-1.3, 6.5, 5.5], [5.5, 4.25, n + 2.0]]) * h
+    s = np.clip(beyond / h - 0.5, 0.0, n - 1.0)
+    want = c(s[:, 0]) + 2.0 * c(s[:, 1]) - c(s[:, 2])
+    got = oracle.interpolate(f, off, h, "WENO6-optimized", beyond)
+    assert np.abs(got - want).max() < 1e-12
 
 
 # ---------------------------------------------------------------------------------------------
