@@ -72,12 +72,13 @@ TRAFFIC_BYTES_256_CFL4 = 12.123e9
 # kernel reads x, r (2 x 8), p (4) and the flag byte, writes x, r (2 x 8); q = L p is recomputed,
 # not stored
 PCG_BYTES_PER_CELL_ITER = 54
-# the one-sweep Jacobi iteration (pcg_fused_kernel): flags 1, p 4 + 4, x 8 + 8, r 8 + 8
-PCG_FUSED_BYTES_PER_CELL_ITER = 41
-# HBM bytes per pcg_fused_kernel launch on the 256^3 plume (8 obstacles), from two rocprofv3
-# passes (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0, scaled by the counters' reading of a 1 GiB copy):
-# 444.7 MB read + 337.2 MB written
-PCG_FUSED_TRAFFIC_BYTES_256 = 781.9e6
+# the one-sweep Jacobi iteration (pcg_fused_kernel), x stepped on every other sweep: odd sweeps
+# flags 1, p 4 + 4, r 8 + 8 (25); even sweeps also x 8 + 8 and p_(k-2) 4 (45); 35 on average
+PCG_FUSED_BYTES_PER_CELL_ITER = 35
+# HBM bytes per pcg_fused_kernel launch (mean of an odd and an even sweep) on the 256^3 plume
+# (8 obstacles), from two rocprofv3 passes (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0, scaled by the
+# counters' reading of a 1 GiB copy); None until measured for the current kernel
+PCG_FUSED_TRAFFIC_BYTES_256 = None
 # one multigrid V-cycle per iteration, per fine cell: the down kernel reads r (8) and the flag
 # byte, writes x (4) and the aggregate's b (0.5); the up kernel reads x (4), r (8), the flag byte
 # and the correction (0.5), writes z (8); the coarser levels add about a seventh of a fine level's
@@ -311,7 +312,8 @@ def pcg_roofline(st, n, precond, fused=False):
     iters = max(st["pcg_iters"], 1)
     ms_iter = st["pcg_ms"] / iters
     gbs = per_cell * n ** 3 / (ms_iter / 1e3) / 1e9 if ms_iter > 0 else 0.0
-    kernels = ("pcg_fused_kernel (one-sweep Jacobi-PCG iteration, FP64 x and r, FP32 p)" if fused else
+    kernels = ("pcg_fused_kernel<false|true> (one-sweep Jacobi-PCG iteration, FP64 r, FP32 p, FP64 x stepped "
+               "on every other sweep)" if fused else
                "pcg_direction_kernel + pcg_update_kernel<true> (Jacobi-PCG iteration, FP64 vectors)"
                if precond == "jacobi" else "multigrid-preconditioned CG iteration (direction, update, V-cycle)")
     traffic = PCG_FUSED_TRAFFIC_BYTES_256 if (fused and n == 256) else None

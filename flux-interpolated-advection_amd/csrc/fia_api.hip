@@ -742,6 +742,7 @@ static fia_status pcg_run(fia_ctx *c, PcgState &ps, int max_iters, double *hs, c
                                      c->stream, mg));
         HIP_TRY(span_end(c, sp));
     }
+    HIP_TRY(launch_pcg_fused_flush(ps, c->stream));
     if (c->timing) {
         c->acc.pcg_iters += std::max(0LL, (long long)hs[kPcgIters] - (long long)it0);
         c->acc.pcg_solves += 1;

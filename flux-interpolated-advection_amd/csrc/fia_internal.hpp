@@ -202,10 +202,10 @@ constexpr int kPcgMaxBlocks = 1024;
 // device scalars of one solve: rz of even and odd iterations, last residual r.r, b.b, tol^2,
 // converged flag, iterations done, scale applied to the initial guess, direction launches that
 // have run (the iteration index as the device counts it: a replayed graph passes the index it was
-// captured with)
+// captured with), alpha of the x step an odd fused sweep leaves to the next one
 enum {
     kPcgRZ = 0, kPcgRR = 3, kPcgBB = 4, kPcgTol2 = 5, kPcgDone = 6, kPcgIters = 7, kPcgGamma = 8, kPcgNext = 9,
-    kPcgScalars = 16
+    kPcgAlpha2 = 10, kPcgScalars = 16
 };
 struct MgLevel;
 // one CG solve in flight: its vectors and device scalars
@@ -296,6 +296,8 @@ hipError_t launch_pcg_scale_guess(const PcgState &st, hipStream_t s);
 hipError_t launch_pcg_start(PcgState &st, double tol2, hipStream_t s);
 hipError_t launch_pcg_iterations(PcgState &st, int count, hipStream_t s, const MgHier *mg = nullptr);
 hipError_t launch_pcg_check(const PcgState &st, hipStream_t s);
+// after a fused solve's last check: the x step it owes (nothing for the two-sweep form)
+hipError_t launch_pcg_fused_flush(const PcgState &st, hipStream_t s);
 // Many started solves advanced together (all at the same iteration), one launch per kernel and
 // iteration: their states and, per block of the launches, (solve, block of that solve), all in
 // device memory.

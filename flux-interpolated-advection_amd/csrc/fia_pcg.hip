@@ -481,21 +481,32 @@ template <bool RECOMPUTE> __device__ __forceinline__ void pcg_update_body(const 
 //   q_k = L p_k (recomputed from the stored p_k), r_(k+1) = r_k - alpha_k q_k,
 //   x_(k+1) = x_k + alpha_k p_k, p_(k+1) = r_(k+1) / d + beta p_k (rounded to FP32),
 //   q_(k+1) = L p_(k+1),
-// storing x, r and p and summing rz, rr, p.q, r.q / d and q.q / d of the new vectors (the true
+// storing r and p and summing rz, rr, p.q, r.q / d and q.q / d of the new vectors (the true
 // rz_(k+1) enters the next alpha; only beta uses the expansion, whose terms come from one sweep, so
 // its rounding does not accumulate).  q_(k+1) of a cell needs p_(k+1) of its neighbours, so the
 // sweep computes r_(k+1) and p_(k+1) on the tile and a one-cell ring around it, whose q_k needs
 // p_k on a two-cell ring; along x, stage A (r, x, p of plane X) runs one plane ahead of stage B
 // (q_(k+1) and the sums of plane X - 1).  The ring reads r_k of the neighbouring tiles, which
 // their own blocks overwrite during the same sweep, so r alternates between two buffers as p does
-// (x is read only by its own cells and stays in place).  41 bytes per cell (flags 1, p 4 + 4,
-// x 8 + 8, r 8 + 8) against 54 for the direction and update sweeps, and one launch per iteration.
+// (x is read only by its own cells and stays in place).
+//
+// x takes two steps at once: even sweeps k >= 2 store x_k = x_(k-2) + (alpha_(k-2) p_(k-2) +
+// alpha_(k-1) p_(k-1)), odd sweeps leave x alone.  p_(k-1) is the sweep's p_old; p_(k-2) is still
+// in the buffer the sweep writes p_k to, and a block reads it there only at its own cells, each
+// before the same thread overwrites it; alpha_(k-2) is stored by the odd sweep k - 1.  x is
+// exactly the sum of the stored p times the same alphas, so the recurrence r = b - L x holds as
+// before, up to the grouping of two FP64 additions.  Per cell, odd sweeps move 25 bytes (flags 1,
+// p 4 + 4, r 8 + 8), even ones 45 (x 8 + 8, p_(k-2) 4 more): 35 on average, against 41 with x
+// every sweep and 54 for the direction and update sweeps, and one launch per iteration.  A solve
+// that stops after an odd iteration I leaves alpha_(I-1) p_(I-1) unadded; pcg_fused_flush_kernel
+// adds it.
 //
 // Sweep 0 (alpha = beta = 0, p_old = 0) forms p_0 = z_0 and the sums of r_0; sweep j >= 1 runs
 // iteration j after testing the residual of iteration j - 1.
 constexpr int kRing1 = 2 * (kTileZ + 2) + 2 * kTileY;         // cells of the one-cell ring
 constexpr int kRing2 = 2 * (kTileZ + 4) + 2 * (kTileY + 2);   // cells of the second ring
 static_assert(kRing1 <= kPB && kRing2 <= kPB, "ring cells: one per thread");
+static_assert(kRows == 4, "the fused sweep packs the rows' flag bytes into one word");
 
 // local (y, z) of ring cell t (tile rows 0 .. kTileY - 1, columns 0 .. kTileZ - 1), as selects
 __device__ __forceinline__ int2 ring1_cell(int t) {
@@ -517,18 +528,24 @@ struct FusedRaw {
     double r;
 };
 
-// The fused sweep of one tile: sums acc = {rz, rr, p.q, r.q / d, q.q / d} of its own cells.
-// Registers per thread: p_k two planes ahead, flags, r and x one plane ahead.  The one-cell ring
-// (a stencil each) goes to the last kRing1 threads and the second ring (a load each) to the first
-// kRing2, so the four waves share the extra work.
+// The fused sweep of one tile: sums acc = {rz, rr, p.q, r.q / d, q.q / d} of its own cells; with
+// XUPD it also adds a2 p_(k-2) + alpha p_(k-1) to x.  Registers per thread: p_k two planes ahead,
+// flags and r one plane ahead, and with XUPD x and p_(k-2) of the next plane, whose x is stored
+// at the end of the plane's iteration.  The one-cell ring (a stencil each) goes to the last kRing1 threads and the
+// second ring (a load each) to the first kRing2, so the four waves share the extra work.
+template <bool XUPD>
 __device__ __forceinline__ void fused_sweep(const PcgState &st, int blk, const float *__restrict__ pold,
                                             float *__restrict__ pnew, const double *__restrict__ rk,
-                                            double *__restrict__ rw, double alpha, double beta, double (&acc)[5]) {
+                                            double *__restrict__ rw, double alpha, double beta, double a2,
+                                            double (&acc)[5]) {
     // p_k of plane X (two-cell ring) and p_(k+1) of planes X and X - 1 (one-cell ring, by the
     // parity of the plane).  Two barriers per plane: P0 is rewritten every plane.  (A second P0
     // buffer and one barrier measured 2% slower on 256^3: more LDS and a register spill.)
     __shared__ float P0[kTileY + 4][kTileZ + 4];
     __shared__ float P1[2][kTileY + 2][kTileZ + 2];
+    // with XUPD, r_(k+1) of the own cells waits for stage B here (per thread, by plane parity)
+    // rather than in registers, which x and p_(k-2) need
+    __shared__ double R1[XUPD ? 2 : 1][kRows][XUPD ? kPB : 1];
     const int per = st.dir_blocks / 8;
     const int L = (blk % 8) * per + blk / 8;
     if (L >= st.dir_tiles_y * st.dir_tiles_z * st.dir_chunks) return;
@@ -542,18 +559,25 @@ __device__ __forceinline__ void fused_sweep(const PcgState &st, int blk, const f
     const long sx = (long)ny * nz;
     const uint8_t *__restrict__ fl = st.flags;
     double *__restrict__ xw = st.x;
+    // Every load of the plane loop is unconditional, from an address clamped into the grid, and a
+    // select zeroes what lies outside: a load under a branch that all lanes may skip leaves the
+    // compiler unable to count the loads issued after an earlier one, and it then waits for all of
+    // them (vmcnt(0)) before the plane's first LDS store, which undoes the one-plane prefetch.
     auto inside = [&](int x, int y, int z) { return x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz; };
-    auto load_p = [&](int x, int y, int z) -> float { return inside(x, y, z) ? pold[idx3(x, y, z, ny, nz)] : 0.0f; };
-    auto load_raw = [&](int x, int y, int z) -> FusedRaw {
-        FusedRaw a{0u, 0.0};
-        if (inside(x, y, z)) {
-            const size_t c = idx3(x, y, z, ny, nz);
-            a.f = fl[c];
-            a.r = rk[c];
-        }
-        return a;
+    auto clamped = [&](int x, int y, int z) {
+        return idx3(min(max(x, 0), nx - 1), min(max(y, 0), ny - 1), min(max(z, 0), nz - 1), ny, nz);
     };
-    auto load_x = [&](int x, int y, int z) -> double { return inside(x, y, z) ? xw[idx3(x, y, z, ny, nz)] : 0.0; };
+    auto load_p = [&](int x, int y, int z) -> float {
+        const float v = pold[clamped(x, y, z)];
+        return inside(x, y, z) ? v : 0.0f;
+    };
+    auto load_raw = [&](int x, int y, int z) -> FusedRaw {
+        const size_t c = clamped(x, y, z);
+        const unsigned f = fl[c];
+        const double r = rk[c];
+        const bool in = inside(x, y, z);
+        return FusedRaw{in ? f : 0u, in ? r : 0.0};
+    };
     // ring cells of this thread
     const int t1 = (int)threadIdx.x - (kPB - kRing1);
     const bool has1 = t1 >= 0, has2 = threadIdx.x < kRing2;
@@ -561,15 +585,16 @@ __device__ __forceinline__ void fused_sweep(const PcgState &st, int blk, const f
     const int r1y = g1.x, r1z = g1.y, r2y = g2.x, r2z = g2.y;
     const int z = z0 + tz;
     bool act[kRows];
-    size_t c[kRows];   // own cell of plane X (clamped into the grid; used only when active)
-    // own cells: p_k of planes X - 1 .. X + 2; flags, r and x of X and X + 1
+    size_t c[kRows];   // own cell of plane max(X, 0), y and z clamped into the grid
+    // own cells: p_k of planes X - 1 .. X + 2; flags and r of X and X + 1
     float pm[kRows], pc[kRows], pn[kRows];
-    FusedRaw oc[kRows];
-    double xc[kRows];
+    // flags of the kRows rows are packed one byte each (ocf: plane X, fbf: plane X - 1)
+    double ocr[kRows];
+    unsigned ocf = 0u;
     // own cells for stage B: p_(k+1) of planes X - 2 and X - 1, r_(k+1) and flags of X - 1
     float qm[kRows], qc[kRows];
     double rb[kRows];
-    unsigned fb[kRows];
+    unsigned fbf = 0u;
     // ring cells: p_k of X - 1 .. X + 1 (ring 1), X and X + 1 (ring 2); flags and r of X (ring 1)
     float gm = 0.f, gc = 0.f, gn = 0.f, hc = 0.f, hn = 0.f;
     FusedRaw ac{0u, 0.0};
@@ -582,9 +607,10 @@ __device__ __forceinline__ void fused_sweep(const PcgState &st, int blk, const f
         pm[w] = load_p(xs - 1, y, z);
         pc[w] = load_p(xs, y, z);
         pn[w] = load_p(xs + 1, y, z);
-        oc[w] = load_raw(xs, y, z);
-        xc[w] = load_x(xs, y, z);
-        qm[w] = 0.f; qc[w] = 0.f; rb[w] = 0.0; fb[w] = 0u;
+        const FusedRaw o = load_raw(xs, y, z);
+        ocr[w] = o.r;
+        ocf |= o.f << (8 * w);
+        qm[w] = 0.f; qc[w] = 0.f; rb[w] = 0.0;
     }
     if (has1) {
         gm = load_p(xs - 1, y0 + r1y, z0 + r1z);
@@ -597,26 +623,21 @@ __device__ __forceinline__ void fused_sweep(const PcgState &st, int blk, const f
         hn = load_p(xs + 1, y0 + r2y, z0 + r2z);
     }
     for (int X = xs; X <= xb; ++X) {
-        // loads for the next planes: raw and x of X + 1 while it is a stage-A plane (X + 1 <= xb),
-        // p_k of X + 2 while stage A still needs it (X + 2 <= xb + 1); the same for the whole block
-        const bool ahead = X + 1 <= xb;
+        const bool own_plane = X >= xa && X < xb;
+        // loads for the next planes: raw of X + 1 (a stage-A plane while X + 1 <= xb), p_k of X + 2
+        // (needed by stage A while X + 2 <= xb + 1); past those, loaded and not used
         FusedRaw on[kRows];
-        double xn[kRows];
         float pn2[kRows];
 #pragma unroll
         for (int w = 0; w < kRows; ++w) {
             const int y = y0 + ty + w * kRowStep;
-            on[w] = ahead ? load_raw(X + 1, y, z) : FusedRaw{0u, 0.0};
-            xn[w] = ahead ? load_x(X + 1, y, z) : 0.0;
-            pn2[w] = ahead ? load_p(X + 2, y, z) : 0.f;
+            on[w] = load_raw(X + 1, y, z);
+            pn2[w] = load_p(X + 2, y, z);
         }
-        FusedRaw an{0u, 0.0};
-        float gn2 = 0.f, hn2 = 0.f;
-        if (has1 && ahead) {
-            an = load_raw(X + 1, y0 + r1y, z0 + r1z);
-            gn2 = load_p(X + 2, y0 + r1y, z0 + r1z);
-        }
-        if (has2 && ahead) hn2 = load_p(X + 2, y0 + r2y, z0 + r2z);
+        // ring cells (threads without one load a cell of the first ring and do not use it)
+        const FusedRaw an = load_raw(X + 1, y0 + r1y, z0 + r1z);
+        const float gn2 = load_p(X + 2, y0 + r1y, z0 + r1z);
+        const float hn2 = load_p(X + 2, y0 + r2y, z0 + r2z);
         // p_k of plane X into LDS
         const int b1 = X & 1;
 #pragma unroll
@@ -639,15 +660,14 @@ __device__ __forceinline__ void fused_sweep(const PcgState &st, int blk, const f
             p1 = (float)(r1 * inv_diag(f) + beta * (double)pcv);
             P1[b1][ly + 1][lz + 1] = p1;
         };
-        const bool own_plane = X >= xa && X < xb;
         double r1o[kRows];
         float p1o[kRows];
 #pragma unroll
         for (int w = 0; w < kRows; ++w) {
             const int ly = ty + w * kRowStep;
-            stage_a(ly, tz, oc[w].f, oc[w].r, pm[w], pc[w], pn[w], r1o[w], p1o[w]);
+            stage_a(ly, tz, (ocf >> (8 * w)) & 0xffu, ocr[w], pm[w], pc[w], pn[w], r1o[w], p1o[w]);
+            if constexpr (XUPD) R1[b1][w][threadIdx.x] = r1o[w];
             if (own_plane && act[w]) {
-                xw[c[w]] = xc[w] + alpha * (double)pc[w];
                 rw[c[w]] = r1o[w];
                 pnew[c[w]] = p1o[w];
             }
@@ -657,6 +677,20 @@ __device__ __forceinline__ void fused_sweep(const PcgState &st, int blk, const f
             float p1;
             stage_a(r1y, r1z, ac.f, ac.r, gm, gc, gn, r1, p1);
         }
+        // x and p_(k-2) of the own cells of plane X + 1, stored after stage B (the cell's p_(k-2)
+        // is read before this thread overwrites it, in stage A of the next plane)
+        const bool own_next = X + 1 < xb;
+        double xo[kRows];
+        float po[kRows];
+        // from plane max(X, 0) of c to plane min(X + 1, nx - 1) (past xb: in the grid, not used)
+        const long xoff = (long)(min(X + 1, nx - 1) - max(X, 0)) * sx;
+        if constexpr (XUPD) {
+#pragma unroll
+            for (int w = 0; w < kRows; ++w) {
+                xo[w] = xw[(long)c[w] + xoff];
+                po[w] = pnew[(long)c[w] + xoff];
+            }
+        }
         __syncthreads();
         // stage B on plane X - 1 (own cells): q_(k+1) and the sums
         if (X > xa) {
@@ -665,7 +699,7 @@ __device__ __forceinline__ void fused_sweep(const PcgState &st, int blk, const f
             for (int w = 0; w < kRows; ++w) {
                 if (!act[w]) continue;
                 const int ly = ty + w * kRowStep + 1, lz = tz + 1;
-                const unsigned f = fb[w];
+                const unsigned f = (fbf >> (8 * w)) & 0xffu;
                 double q = (double)__builtin_popcount(f & 63u) * (double)qc[w];
                 q -= (f & 1u) ? (double)qm[w] : 0.0;
                 q -= (f & 2u) ? (double)p1o[w] : 0.0;
@@ -674,7 +708,10 @@ __device__ __forceinline__ void fused_sweep(const PcgState &st, int blk, const f
                 q -= (f & 16u) ? (double)P1[b0][ly][lz - 1] : 0.0;
                 q -= (f & 32u) ? (double)P1[b0][ly][lz + 1] : 0.0;
                 q = (f & kPcgUnknown) ? q : 0.0;
-                const double id = inv_diag(f), r = rb[w], pv = (double)qc[w];
+                double r;
+                if constexpr (XUPD) r = R1[b0][w][threadIdx.x];
+                else r = rb[w];
+                const double id = inv_diag(f), pv = (double)qc[w];
                 acc[0] += r * (r * id);
                 acc[1] += r * r;
                 acc[2] += pv * q;
@@ -682,14 +719,23 @@ __device__ __forceinline__ void fused_sweep(const PcgState &st, int blk, const f
                 acc[4] += q * (q * id);
             }
         }
+        // x of plane X + 1: p_(k-1) there is pn
+        if constexpr (XUPD) {
+#pragma unroll
+            for (int w = 0; w < kRows; ++w)
+                if (own_next && act[w]) xw[(long)c[w] + xoff] = xo[w] + (a2 * (double)po[w] + alpha * (double)pn[w]);
+        }
         // advance one plane
 #pragma unroll
         for (int w = 0; w < kRows; ++w) {
-            qm[w] = qc[w]; qc[w] = p1o[w]; rb[w] = r1o[w]; fb[w] = oc[w].f;
+            qm[w] = qc[w]; qc[w] = p1o[w];
+            if constexpr (!XUPD) rb[w] = r1o[w];
             pm[w] = pc[w]; pc[w] = pn[w]; pn[w] = pn2[w];
-            oc[w] = on[w]; xc[w] = xn[w];
+            ocr[w] = on[w].r;
             if (X >= 0) c[w] += sx;
         }
+        fbf = ocf;
+        ocf = on[0].f | on[1].f << 8 | on[2].f << 16 | on[3].f << 24;
         gm = gc; gc = gn; gn = gn2; ac = an;
         hc = hn; hn = hn2;
     }
@@ -697,7 +743,7 @@ __device__ __forceinline__ void fused_sweep(const PcgState &st, int blk, const f
 
 // Sweep k of a fused solve (see above): the sums of sweep k - 1 (of the start kernel for k = 0)
 // give the convergence test, alpha and beta; then the sweep.
-__device__ __forceinline__ void pcg_fused_body(const PcgState &st, int blk, int k) {
+template <bool XUPD> __device__ __forceinline__ void pcg_fused_body(const PcgState &st, int blk, int k) {
     if (solve_done(st.scal)) return;
     double rz, rr, bb, alpha = 0.0, beta = 0.0;
     if (k == 0) {
@@ -721,12 +767,13 @@ __device__ __forceinline__ void pcg_fused_body(const PcgState &st, int blk, int 
         sc[kPcgIters] = next > 0.0 ? next - 1.0 : 0.0;
         sc[kPcgNext] = next + 1.0;
         if (done) sc[kPcgDone] = 1.0;
+        else if (k & 1) sc[kPcgAlpha2] = alpha;   // alpha_(k-1), for the x step of sweep k + 1
     }
     if (done) return;
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     // residuals alternate between r (even sweeps read it) and r_alt; the start kernel wrote r
-    fused_sweep(st, blk, st.p[k & 1], st.p[(k + 1) & 1], (k & 1) ? st.r_alt : st.r, (k & 1) ? st.r : st.r_alt,
-                alpha, beta, acc);
+    fused_sweep<XUPD>(st, blk, st.p[k & 1], st.p[(k + 1) & 1], (k & 1) ? st.r_alt : st.r, (k & 1) ? st.r : st.r_alt,
+                      alpha, beta, XUPD ? st.scal[kPcgAlpha2] : 0.0, acc);
     write_partials<5>(acc, st.part_fused + (size_t)(k & 1) * 5 * st.dir_blocks, st.dir_blocks, blk);
 }
 
@@ -1177,8 +1224,22 @@ template <bool RECOMPUTE> __global__ void __launch_bounds__(kPB) pcg_update_kern
     pcg_update_body<RECOMPUTE>(st, blockIdx.x, k);
 }
 __global__ void __launch_bounds__(kPB) pcg_check_kernel(PcgState st, int k) { pcg_check_body(st, k); }
+template <bool XUPD>
 __global__ void __launch_bounds__(kPB) __attribute__((amdgpu_waves_per_eu(3))) pcg_fused_kernel(PcgState st, int k) {
-    pcg_fused_body(st, blockIdx.x, k);
+    pcg_fused_body<XUPD>(st, blockIdx.x, k);
+}
+
+// The x step a fused solve still owes when it stopped after an odd iteration I (device count):
+// x += alpha_(I-1) p_(I-1), with p_(I-1) in p[1] and alpha_(I-1) stored by sweep I.
+__global__ void __launch_bounds__(kPB) pcg_fused_flush_kernel(PcgState st) {
+    const long long iters = (long long)st.scal[kPcgIters];
+    if (!(iters & 1)) return;
+    const double a = st.scal[kPcgAlpha2];
+    const float *__restrict__ p = st.p[1];
+    double *__restrict__ x = st.x;
+    const size_t n = (size_t)st.g.nx * st.g.ny * st.g.nz;
+    const CellRange cr = block_cells(n, st.nblocks, blockIdx.x);
+    for (size_t c = cr.first; c < cr.end; c += cr.step) x[c] += a * (double)p[c];
 }
 
 // ... or many solves per launch, all at the same iteration: map[b] = (s, j) says that block b of
@@ -1230,8 +1291,9 @@ int pcg_resident_blocks(bool fused) {
     int cus = 0, a = 0, b = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
     if (fused) {
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, pcg_fused_kernel, kPB, 0) != hipSuccess) return 0;
-        b = a;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, pcg_fused_kernel<true>, kPB, 0) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, pcg_fused_kernel<false>, kPB, 0) != hipSuccess)
+            return 0;
     } else if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, pcg_direction_kernel, kPB, 0) != hipSuccess ||
                hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, pcg_update_kernel<true>, kPB, 0) != hipSuccess) {
         return 0;
@@ -1298,7 +1360,11 @@ hipError_t launch_pcg_iterations(PcgState &st, int count, hipStream_t s, const M
     if (st.fused) {
         if (mg) return hipErrorInvalidValue;   // the fused sweep is the Jacobi iteration
         for (int k = 0; k < count; ++k) {
-            hipLaunchKernelGGL(pcg_fused_kernel, dim3(st.dir_blocks), dim3(kPB), 0, s, st, st.it);
+            // x steps on even sweeps from 2 on (a captured batch never holds sweep 0)
+            if (st.it >= 2 && !(st.it & 1))
+                hipLaunchKernelGGL(pcg_fused_kernel<true>, dim3(st.dir_blocks), dim3(kPB), 0, s, st, st.it);
+            else
+                hipLaunchKernelGGL(pcg_fused_kernel<false>, dim3(st.dir_blocks), dim3(kPB), 0, s, st, st.it);
             ++st.it;
         }
         return hipGetLastError();
@@ -1313,6 +1379,11 @@ hipError_t launch_pcg_iterations(PcgState &st, int count, hipStream_t s, const M
         }
         ++st.it;
     }
+    return hipGetLastError();
+}
+hipError_t launch_pcg_fused_flush(const PcgState &st, hipStream_t s) {
+    if (!st.fused) return hipSuccess;
+    hipLaunchKernelGGL(pcg_fused_flush_kernel, dim3(st.nblocks), dim3(kPB), 0, s, st);
     return hipGetLastError();
 }
 hipError_t launch_pcg_check(const PcgState &st, hipStream_t s) {
