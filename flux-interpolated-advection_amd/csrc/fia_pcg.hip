@@ -247,6 +247,17 @@ __device__ __forceinline__ void pcg_check_body(const PcgState &st, int k) {
 // x += alpha p, r -= alpha q (alpha = rz / p.q), partial rz, rr.
 // Raw records hold the loaded values as stored (FP32 p included): converting at the load would
 // make the sweep wait for the plane it prefetches two steps ahead.
+// Every load of a sweep is issued unconditionally, from an address clamped into the grid, and a
+// select zeroes what lies outside (also for the planes past the chunk and for threads without a
+// halo cell, whose values are not used): a load under a branch that all lanes may skip leaves the
+// compiler unable to count the loads issued after an earlier one, and it then waits for all of
+// them (vmcnt(0)), which undoes the prefetch.
+__device__ __forceinline__ bool in_grid(const PcgGrid &g, int x, int y, int z) {
+    return x >= 0 && x < g.nx && y >= 0 && y < g.ny && z >= 0 && z < g.nz;
+}
+__device__ __forceinline__ size_t clamped_cell(const PcgGrid &g, int x, int y, int z) {
+    return idx3(min(max(x, 0), g.nx - 1), min(max(y, 0), g.ny - 1), min(max(z, 0), g.nz - 1), g.ny, g.nz);
+}
 struct DirRaw {
     unsigned f;
     double r;
@@ -260,14 +271,13 @@ struct DirOp {
     double beta;
     // Jacobi: r (z = r / diag on the fly); multigrid: the V-cycle's z
     __device__ __forceinline__ Raw load(int x, int y, int z, bool) const {
-        Raw a{0u, 0.0, 0.0f};
-        if (x >= 0 && x < st.g.nx && y >= 0 && y < st.g.ny && z >= 0 && z < st.g.nz) {
-            const size_t c = idx3(x, y, z, st.g.ny, st.g.nz);
-            a.f = st.flags[c];
-            a.r = st.z ? st.z[c] : st.r[c];
-            a.p = pold[c];
-        }
-        return a;
+        const size_t c = clamped_cell(st.g, x, y, z);
+        const double *rz = st.z ? st.z : st.r;
+        const unsigned f = st.flags[c];
+        const double r = rz[c];
+        const float p = pold[c];
+        const bool in = in_grid(st.g, x, y, z);
+        return Raw{in ? f : 0u, in ? r : 0.0, in ? p : 0.0f};
     }
     // the direction rounded to FP32 here, before the stencil: q = L p and p.q are those of the
     // stored p, which the update reads back
@@ -292,12 +302,15 @@ struct UpdOp {
     double alpha;
     // x and r only for the tile's own cells (own), not for the halo
     __device__ __forceinline__ Raw load(int x, int y, int z, bool own) const {
-        Raw a{0u, 0.0f, 0.0, 0.0};
-        if (x >= 0 && x < st.g.nx && y >= 0 && y < st.g.ny && z >= 0 && z < st.g.nz) {
-            const size_t c = idx3(x, y, z, st.g.ny, st.g.nz);
-            a.f = st.flags[c];
-            a.p = p[c];
-            if (own) { a.x = st.x[c]; a.r = st.r[c]; }
+        const size_t c = clamped_cell(st.g, x, y, z);
+        const bool in = in_grid(st.g, x, y, z);
+        const unsigned f = st.flags[c];
+        const float pv = p[c];
+        Raw a{in ? f : 0u, in ? pv : 0.0f, 0.0, 0.0};
+        if (own) {   // a constant at every call
+            const double xv = st.x[c], rv = st.r[c];
+            a.x = in ? xv : 0.0;
+            a.r = in ? rv : 0.0;
         }
         return a;
     }
@@ -356,19 +369,18 @@ __device__ __forceinline__ void tile_sweep(const PcgState &st, int blk, const Op
         plane[0][ty + w * kRowStep + 1][tz + 1] = dc[w];
         c[w] = idx3(xa, min(y, st.g.ny - 1), min(z, st.g.nz - 1), st.g.ny, st.g.nz);
     }
-    const Raw hc = has_halo ? op.load(xa, hy, hz, false) : Raw{};
-    Raw hn = has_halo ? op.load(xa + 1, hy, hz, false) : Raw{};
+    // a thread without a halo cell loads (hy, hz) = (0, 0) and does not use it
+    const Raw hc = op.load(xa, hy, hz, false);
+    Raw hn = op.load(xa + 1, hy, hz, false);
     if (has_halo) plane[0][hly][hlz] = op.value(hc);
     int cur = 0;
     __syncthreads();
     for (int x = xa; x < xb; ++x) {
-        // plane x + 2 is loaded while it is still this chunk's or its +x neighbour plane (the same
-        // for the whole block: no divergence)
-        const bool ahead = x + 2 <= xb;
+        // plane x + 2: used while it is still this chunk's or its +x neighbour plane (x + 2 <= xb)
         Raw on2[kRows];
 #pragma unroll
-        for (int w = 0; w < kRows; ++w) on2[w] = ahead ? op.load(x + 2, y0 + ty + w * kRowStep, z, true) : Raw{};
-        const Raw hn2 = (has_halo && ahead) ? op.load(x + 2, hy, hz, false) : Raw{};
+        for (int w = 0; w < kRows; ++w) on2[w] = op.load(x + 2, y0 + ty + w * kRowStep, z, true);
+        const Raw hn2 = op.load(x + 2, hy, hz, false);
         double dn[kRows];
 #pragma unroll
         for (int w = 0; w < kRows; ++w) {
@@ -1084,13 +1096,11 @@ struct MgDown0Op {
     const PcgState &st;
     float *x, *res;
     __device__ __forceinline__ Raw load(int x0, int y, int z, bool) const {
-        Raw a{0u, 0.0};
-        if (x0 >= 0 && x0 < st.g.nx && y >= 0 && y < st.g.ny && z >= 0 && z < st.g.nz) {
-            const size_t c = idx3(x0, y, z, st.g.ny, st.g.nz);
-            a.f = st.flags[c];
-            a.r = st.r[c];
-        }
-        return a;
+        const size_t c = clamped_cell(st.g, x0, y, z);
+        const unsigned f = st.flags[c];
+        const double r = st.r[c];
+        const bool in = in_grid(st.g, x0, y, z);
+        return Raw{in ? f : 0u, in ? r : 0.0};
     }
     __device__ __forceinline__ double value(const Raw &a) const {
         return (a.f & kPcgUnknown) ? (double)(kMgOmega * (float)a.r / (float)__builtin_popcount(a.f & 63u)) : 0.0;
@@ -1111,10 +1121,11 @@ struct MgUp0Op {
     const float *x, *corr;
     int cny, cnz;
     // the loaded values as stored (x and the correction exist for every cell: loaded with the flag
-    // byte, not after it); combined only where the sweep uses them
+    // byte, not after it); combined only where the sweep uses them.  Unlike the other sweeps' loads
+    // this one keeps its branch: the clamped form needs 140 VGPRs (3 waves per SIMD instead of 4).
     __device__ __forceinline__ Raw load(int x0, int y, int z, bool own) const {
         Raw a{0u, 0.0f, 0.0f, 0.0};
-        if (x0 >= 0 && x0 < st.g.nx && y >= 0 && y < st.g.ny && z >= 0 && z < st.g.nz) {
+        if (in_grid(st.g, x0, y, z)) {
             const size_t c = idx3(x0, y, z, st.g.ny, st.g.nz);
             a.x = x[c];
             a.corr = corr[idx3(x0 >> 1, y >> 1, z >> 1, cny, cnz)];
