@@ -65,10 +65,23 @@ GaussTable make_gauss(int n) {
 }  // namespace
 
 // Per-family face fluxes of one pass and the corrections of faces whose two evaluations differ.
+// Between passes every correction is zero and every mask bit clear: a pass's general cutter sets
+// the bits of its faces (dirty), and the next pass on the set clears them first.
 struct FluxSet {
     DevBuf<double> f[3], c[3];
-    size_t bytes() const { size_t b = 0; for (int a = 0; a < 3; ++a) b += f[a].bytes() + c[a].bytes(); return b; }
-    void release() { for (int a = 0; a < 3; ++a) { f[a].release(); c[a].release(); } }
+    DevBuf<unsigned> m[3];
+    bool dirty = false;
+    size_t bytes() const {
+        size_t b = 0;
+        for (int a = 0; a < 3; ++a) b += f[a].bytes() + c[a].bytes() + m[a].bytes();
+        return b;
+    }
+    void release() { for (int a = 0; a < 3; ++a) { f[a].release(); c[a].release(); m[a].release(); } dirty = false; }
+    FluxView view() const {
+        FluxView v;
+        for (int a = 0; a < 3; ++a) { v.f[a] = f[a].p; v.c[a] = c[a].p; v.m[a] = m[a].p; }
+        return v;
+    }
 };
 
 struct fia_ctx {
@@ -319,7 +332,17 @@ fia_status do_faces(fia_ctx *c, const CompGrid &g, const float *q, bool moving, 
     for (int f = 0; f < 3; ++f) {
         if (g.faces(f) >= (size_t(1) << 30)) return fail(FIA_ERR_INVALID_ARG, "grid too large for the face index");
         HIP_TRY(fs.f[f].ensure(g.faces(f)));
-        HIP_TRY(fs.c[f].ensure(g.faces(f)));
+        if (!fs.c[f].p || fs.c[f].n < g.faces(f)) {   // new corrections and masks start at zero
+            HIP_TRY(fs.c[f].ensure(g.faces(f)));
+            HIP_TRY(fs.m[f].ensure((fs.c[f].n + 31) / 32));
+            HIP_TRY(hipMemsetAsync(fs.c[f].p, 0, fs.c[f].bytes(), c->stream));
+            HIP_TRY(hipMemsetAsync(fs.m[f].p, 0, fs.m[f].bytes(), c->stream));
+        }
+    }
+    if (fs.dirty) {
+        const size_t words[3] = {fs.m[0].n, fs.m[1].n, fs.m[2].n};
+        HIP_TRY(launch_corr_reset(fs.view(), words, c->stream));
+        fs.dirty = false;
     }
     HIP_TRY(c->twisted.ensure(g.faces(0) + g.faces(1) + g.faces(2)));
     HIP_TRY(c->degen.ensure(g.faces(0) + g.faces(1) + g.faces(2)));
@@ -333,6 +356,7 @@ fia_status do_faces(fia_ctx *c, const CompGrid &g, const float *q, bool moving, 
         p.disp[a] = moving ? c->disp[a].p : nullptr;
         p.flux[a] = fs.f[a].p;
         p.corr[a] = fs.c[a].p;
+        p.corr_mask[a] = fs.m[a].p;
     }
     p.twisted = c->twisted.p; p.twisted_count = c->counters.p; p.error = c->counters.p + 1;
     p.degen = c->degen.p; p.degen_count = c->counters.p + 2;
@@ -426,6 +450,7 @@ fia_status do_faces(fia_ctx *c, const CompGrid &g, const float *q, bool moving, 
         for (int a = 0; a < 3; ++a) p.gen_c[a] = c->gen_c[a].p;
         p.gen_ay = c->gen_ay.p;
         p.gen_val = c->gen_val.p;
+        fs.dirty = true;   // from here on the set may hold corrections
         HIP_TRY(launch_general_pieces(p, n_twisted, npieces, c->stream));
         if (c->timing) { c->acc.twisted_faces += n_twisted; c->acc.general_pieces += (long long)npieces; }
     }
@@ -439,11 +464,10 @@ hipError_t timed_gather(fia_ctx *c, const CompGrid &g, FluxSet &s1, FluxSet *s2,
     int sp;
     hipError_t e = span_begin(c, SK_GATHER, sp);
     if (e != hipSuccess) return e;
-    double *f1[3] = {s1.f[0].p, s1.f[1].p, s1.f[2].p}, *c1[3] = {s1.c[0].p, s1.c[1].p, s1.c[2].p};
-    double *f2[3] = {nullptr, nullptr, nullptr}, *c2[3] = {nullptr, nullptr, nullptr};
-    if (s2)
-        for (int a = 0; a < 3; ++a) { f2[a] = s2->f[a].p; c2[a] = s2->c[a].p; }
-    e = launch_gather(g, f1, c1, s2 ? f2 : nullptr, s2 ? c2 : nullptr, base, out, mode, fm, c->stream,
+    const FluxView v1 = s1.view();
+    FluxView v2;
+    if (s2) v2 = s2->view();
+    e = launch_gather(g, v1, s2 ? &v2 : nullptr, base, out, mode, fm, c->stream,
                       tuning_for(c, g.n[0], g.n[1], g.n[2]).cell_block);
     if (e != hipSuccess) return e;
     if (c->timing) c->acc.gather_cells += (long long)g.cells();

@@ -103,8 +103,9 @@ struct FaceParams {
     const double *psi;       // psi on the y-faces of G, in units of q * cells
     const float *disp[3];    // vertex displacements (cells); null = zero velocity
     double *flux[3];         // per family face fluxes (normalised: q * cells^3 / cells^3)
-    double *corr[3];         // per family: (upper cell's own evaluation) + flux, nonzero only for
-                             // faces the general cutter handled (kept zero elsewhere)
+    double *corr[3];         // per family: (upper cell's own evaluation) + flux for the faces the
+                             // general cutter handled, whose bit is set in corr_mask; zero elsewhere
+    unsigned *corr_mask[3];  // per family: bit id of word id / 32 set where corr holds a correction
     unsigned *twisted;       // list of faces for the general cutter: fam << 30 | index (unsigned)
     int *twisted_count;
     unsigned *degen;         // degenerate faces (segment cutter): cut per face by the clipping pipeline
@@ -179,9 +180,16 @@ hipError_t launch_zero_velocity_faces(const FaceParams &p, hipStream_t s);
 // out = sum over families of flux differences (scaled); mode 0: out = G1; 1: out = G1 + G2;
 // 2: out = base - G1 (y family only).
 // block: cells per block (64, 128 or 256, anything else 256)
-hipError_t launch_gather(const CompGrid &g, double *const f1[3], double *const c1[3], double *const f2[3],
-                         double *const c2[3], const float *base, float *out, int mode, int fam_mask, hipStream_t s,
-                         int block = 0);
+// a flux set's fluxes, corrections and correction masks per family (FluxSet on the device)
+struct FluxView {
+    double *f[3], *c[3];
+    unsigned *m[3];
+};
+hipError_t launch_gather(const CompGrid &g, const FluxView &s1, const FluxView *s2, const float *base, float *out,
+                         int mode, int fam_mask, hipStream_t s, int block = 0);
+// the corrections of the last pass back to zero (the entries whose mask bit is set) and the masks
+// cleared: words[a] mask words of family a
+hipError_t launch_corr_reset(const FluxView &v, const size_t words[3], hipStream_t s);
 hipError_t launch_interpolate(const float *field, int mx, int my, int mz, double ox, double oy, double oz,
                               int method, float kappa, const float *pts_cells, size_t n, float *out,
                               hipStream_t s);

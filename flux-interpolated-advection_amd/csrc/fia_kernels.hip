@@ -37,6 +37,13 @@ FIA_HD int atomic_add_i32(int *p, int v) {
     return __atomic_fetch_add(p, v, __ATOMIC_RELAXED);
 #endif
 }
+FIA_HD void atomic_or_u32(unsigned *p, unsigned v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    atomicOr(p, v);
+#else
+    __atomic_fetch_or(p, v, __ATOMIC_RELAXED);
+#endif
+}
 FIA_HD void atomic_cas_i32(int *p, int cmp, int v) {
 #if defined(__HIP_DEVICE_COMPILE__)
     atomicCAS(p, cmp, v);
@@ -746,7 +753,6 @@ template <bool FAST> FIA_HD void zero_velocity_face_body(const FaceParams &P, lo
     const int nx = unit_interval_pieces(x0, cx, ox, xl, xh), nz = unit_interval_pieces(z0, cz, oz, zl, zh);
     if constexpr (FAST) {
         P.flux[1][id] = (nx == 2 && nz == 2) ? zero_velocity_weno4(P, j, xl, xh, zl, zh) : 0.0;
-        P.corr[1][id] = 0.0;
         return;
     } else {
         double f = 0.0;
@@ -758,7 +764,6 @@ template <bool FAST> FIA_HD void zero_velocity_face_body(const FaceParams &P, lo
             f += piece_flux(P, c3, (xh[a] - xl[a]) * (zh[b] - zl[b]), npts);
         }
         P.flux[1][id] = f;
-        P.corr[1][id] = 0.0;
     }
 }
 template <bool FAST> __global__ void __launch_bounds__(kBlock) zero_velocity_face_kernel(FaceParams P) {
@@ -824,7 +829,6 @@ template <int FAM> FIA_HD void face_sum_body(const FaceParams &P, long id) {
     double f = 0.0;
     for (unsigned q = 0; q < n; ++q) f += P.piece_val[s0 + q];
     P.flux[FAM][id] = f;
-    P.corr[FAM][id] = 0.0;
 }
 template <int FAM> __global__ void __launch_bounds__(kBlock) face_sum_kernel(FaceParams P) {
     const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1030,6 +1034,7 @@ FIA_HD void general_sum_body(const FaceParams &P, int t) {
     const long id = (long)(code & ((1u << 30) - 1u));
     P.flux[fam][id] = f[0];
     P.corr[fam][id] = f[1] + f[0];
+    atomic_or_u32(&P.corr_mask[fam][id >> 5], 1u << (id & 31));
 }
 __global__ void __launch_bounds__(kBlock) general_sum_kernel(FaceParams P) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1040,39 +1045,43 @@ __global__ void __launch_bounds__(kBlock) general_sum_kernel(FaceParams P) {
 // gather: q(i,j,k) = sum over the six faces of +-flux (the '+' face of a cell is its own
 // orientation, the '-' face is the neighbour's '+' face reversed).
 // ---------------------------------------------------------------------------------------------
-FIA_HD double cell_sum(const CompGrid &g, double *const f[3], double *const c[3], int i, int j,
-                                           int k, int fm) {
-    const int my = g.m[1], mz = g.m[2];
-    double s = 0.0;
-    if (fm & 1) {
-        const size_t lo = idx3(i, j, k, my, mz);
-        s += f[0][idx3(i + 1, j, k, my, mz)] - f[0][lo] + c[0][lo];
-    }
-    if (fm & 2) {
-        const size_t lo = idx3(i, j, k, my + 1, mz);
-        s += f[1][idx3(i, j + 1, k, my + 1, mz)] - f[1][lo] + c[1][lo];
-    }
-    if (fm & 4) {
-        const size_t lo = idx3(i, j, k, my, mz + 1);
-        s += f[2][idx3(i, j, k + 1, my, mz + 1)] - f[2][lo] + c[2][lo];
-    }
-    return s;
+// The correction of the lower face is read only where its mask bit is set (faces of the general
+// cutter); elsewhere it is zero and the sum adds +0.0, as it would from a stored zero.
+FIA_HD double corr_at(const FluxView &v, int fam, size_t lo) {
+    return ((v.m[fam][lo >> 5] >> (lo & 31)) & 1u) ? v.c[fam][lo] : 0.0;
 }
 
 struct GatherArgs {
     CompGrid g;
-    double *f1[3], *c1[3];
-    double *f2[3], *c2[3];
+    FluxView s1, s2;
     const float *base;
     float *out;
     int mode, fam_mask;
 };
 
+FIA_HD double cell_sum(const CompGrid &g, const FluxView &v, int i, int j, int k, int fm) {
+    const int my = g.m[1], mz = g.m[2];
+    double s = 0.0;
+    if (fm & 1) {
+        const size_t lo = idx3(i, j, k, my, mz);
+        s += v.f[0][idx3(i + 1, j, k, my, mz)] - v.f[0][lo] + corr_at(v, 0, lo);
+    }
+    if (fm & 2) {
+        const size_t lo = idx3(i, j, k, my + 1, mz);
+        s += v.f[1][idx3(i, j + 1, k, my + 1, mz)] - v.f[1][lo] + corr_at(v, 1, lo);
+    }
+    if (fm & 4) {
+        const size_t lo = idx3(i, j, k, my, mz + 1);
+        s += v.f[2][idx3(i, j, k + 1, my, mz + 1)] - v.f[2][lo] + corr_at(v, 2, lo);
+    }
+    return s;
+}
+
 FIA_HD void gather_body(const GatherArgs &A, long id) {
     const int mz = A.g.m[2], my = A.g.m[1];
     const int k = (int)(id % mz), j = (int)((id / mz) % my), i = (int)(id / ((long)my * mz));
-    double s = cell_sum(A.g, A.f1, A.c1, i, j, k, A.fam_mask);
-    if (A.mode == 1) s += cell_sum(A.g, A.f2, A.c2, i, j, k, 7);
+    double s = cell_sum(A.g, A.s1, i, j, k, A.fam_mask);
+    if (A.mode == 1) s += cell_sum(A.g, A.s2, i, j, k, 7);
     if (A.mode == 2) s = (double)A.base[id] - s;
     A.out[id] = (float)s;
 }
@@ -1080,6 +1089,31 @@ __global__ void __launch_bounds__(kBlock) gather_kernel(GatherArgs A) {
     const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= (long)A.g.cells()) return;
     gather_body(A, id);
+}
+
+// Back to the invariant of a flux set between passes: every correction zero, every mask bit clear.
+// Word w of a family's mask: each set bit's correction to zero, then the word.
+FIA_HD void corr_reset_word(unsigned *mask, double *corr, size_t w) {
+    unsigned b = mask[w];
+    if (b == 0u) return;
+    while (b) {
+        const int t = __builtin_ctz(b);
+        corr[w * 32 + (size_t)t] = 0.0;
+        b &= b - 1u;
+    }
+    mask[w] = 0u;
+}
+struct CorrResetArgs {
+    FluxView v;
+    size_t words[3];
+};
+__global__ void __launch_bounds__(kBlock) corr_reset_kernel(CorrResetArgs A) {
+    const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t w = id;
+    for (int a = 0; a < 3; ++a) {
+        if (w < A.words[a]) { corr_reset_word(A.v.m[a], A.v.c[a], w); return; }
+        w -= A.words[a];
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1467,18 +1501,25 @@ hipError_t launch_general_pieces(const FaceParams &p, int count, unsigned long l
     return hipGetLastError();
 }
 
-hipError_t launch_gather(const CompGrid &g, double *const f1[3], double *const c1[3], double *const f2[3],
-                         double *const c2[3], const float *base, float *out, int mode, int fam_mask, hipStream_t s,
-                         int block) {
+hipError_t launch_gather(const CompGrid &g, const FluxView &s1, const FluxView *s2, const float *base, float *out,
+                         int mode, int fam_mask, hipStream_t s, int block) {
     GatherArgs A;
     A.g = g;
-    for (int a = 0; a < 3; ++a) {
-        A.f1[a] = f1[a]; A.c1[a] = c1[a];
-        A.f2[a] = f2 ? f2[a] : nullptr; A.c2[a] = c2 ? c2[a] : nullptr;
-    }
+    A.s1 = s1;
+    A.s2 = s2 ? *s2 : FluxView{};
     A.base = base; A.out = out; A.mode = mode; A.fam_mask = fam_mask;
     const int b = block_or_default(block);
     hipLaunchKernelGGL(gather_kernel, dim3(grid_for(g.cells(), b)), dim3(b), 0, s, A);
+    return hipGetLastError();
+}
+
+hipError_t launch_corr_reset(const FluxView &v, const size_t words[3], hipStream_t s) {
+    CorrResetArgs A;
+    A.v = v;
+    for (int a = 0; a < 3; ++a) A.words[a] = words[a];
+    const size_t n = words[0] + words[1] + words[2];
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(corr_reset_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, A);
     return hipGetLastError();
 }
 

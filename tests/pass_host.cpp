@@ -61,6 +61,13 @@ template <class T> struct HostBuf {
 
 struct FluxSetH {
     HostBuf<double> f[3], c[3];
+    HostBuf<unsigned> m[3];
+    bool dirty = false;
+    FluxView view() {
+        FluxView v;
+        for (int a = 0; a < 3; ++a) { v.f[a] = f[a].p; v.c[a] = c[a].p; v.m[a] = m[a].p; }
+        return v;
+    }
 };
 
 // replay counters per pass (stats rows of kStatCols doubles)
@@ -138,7 +145,17 @@ int do_faces(HostCtx &c, const CompGrid &g, const float *q, bool moving, FluxSet
     double st[kStatCols] = {0};
     for (int f = 0; f < 3; ++f) {
         fs.f[f].ensure(g.faces(f));
-        fs.c[f].ensure(g.faces(f));
+        if (!fs.c[f].p || fs.c[f].n < g.faces(f)) {   // new corrections and masks start at zero
+            fs.c[f].ensure(g.faces(f));
+            fs.m[f].ensure((fs.c[f].n + 31) / 32);
+            memset((void *)fs.c[f].p, 0, fs.c[f].n * sizeof(double));
+            memset((void *)fs.m[f].p, 0, fs.m[f].n * sizeof(unsigned));
+        }
+    }
+    if (fs.dirty) {   // the last pass's corrections back to zero (corr_reset_kernel)
+        for (int f = 0; f < 3; ++f)
+            parallel_for((long)fs.m[f].n, [&](long w) { corr_reset_word(fs.m[f].p, fs.c[f].p, (size_t)w); });
+        fs.dirty = false;
     }
     const size_t nf = g.faces(0) + g.faces(1) + g.faces(2);
     c.twisted.ensure(nf);
@@ -151,6 +168,7 @@ int do_faces(HostCtx &c, const CompGrid &g, const float *q, bool moving, FluxSet
         p.disp[a] = moving ? c.disp[a].p : nullptr;
         p.flux[a] = fs.f[a].p;
         p.corr[a] = fs.c[a].p;
+        p.corr_mask[a] = fs.m[a].p;
     }
     p.twisted = c.twisted.p; p.twisted_count = &c.counters[0]; p.error = &c.counters[1];
     p.degen = c.degen.p; p.degen_count = &c.counters[2];
@@ -269,6 +287,7 @@ int do_faces(HostCtx &c, const CompGrid &g, const float *q, bool moving, FluxSet
         p.gen_val = c.gen_val.p;
         parallel_for(n_twisted, [&](long t) { general_emit_body(p, (int)t); });
         parallel_for((long)run2, [&](long q) { general_piece_body(p, (unsigned long long)q); });
+        fs.dirty = true;
         parallel_for(n_twisted, [&](long t) { general_sum_body(p, (int)t); });
     }
     st[ST_TWISTED] = n_twisted; st[ST_DEGEN] = n_degen; st[ST_SLOTS] = (double)nslots;
@@ -281,10 +300,9 @@ int do_faces(HostCtx &c, const CompGrid &g, const float *q, bool moving, FluxSet
 void gather(const CompGrid &g, FluxSetH &s1, FluxSetH *s2, const float *base, float *out, int mode, int fm) {
     GatherArgs A;
     A.g = g;
-    for (int a = 0; a < 3; ++a) {
-        A.f1[a] = s1.f[a].p; A.c1[a] = s1.c[a].p;
-        A.f2[a] = s2 ? s2->f[a].p : nullptr; A.c2[a] = s2 ? s2->c[a].p : nullptr;
-    }
+    A.s1 = s1.view();
+    if (s2) A.s2 = s2->view();
+    else memset(&A.s2, 0, sizeof(A.s2));
     A.base = base; A.out = out; A.mode = mode; A.fam_mask = fm;
     parallel_for((long)g.cells(), [&](long id) { gather_body(A, id); });
 }
