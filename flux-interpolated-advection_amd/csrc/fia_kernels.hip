@@ -221,6 +221,65 @@ template <bool HELD> __global__ void __launch_bounds__(64 * kPsiSegs) psi_kernel
     }
 }
 
+// The WENO6 integral rule in held form (segments of at most kPsiHeld cells, at most
+// kPsiWenoGauss Gauss points per half cell): the rows of the segment and its six neighbours are
+// loaded at once, the increments are computed once (not again for the prefix) and kept in
+// registers.  Each increment is psi_weno_segment's, with the same operations in the same order.
+constexpr int kPsiWenoGauss = 2;
+FIA_HD bool psi_weno_held(const PsiParams &P) {
+    return P.integral && P.mode == 2 && (P.g.m[1] + kPsiSegs - 1) / kPsiSegs <= kPsiHeld &&
+           P.gauss.n <= kPsiWenoGauss;
+}
+__global__ void __launch_bounds__(64 * kPsiSegs) psi_weno_held_kernel(PsiParams P) {
+    const int my = P.g.m[1], mz = P.g.m[2];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i = blockIdx.y, k = blockIdx.x * 64 + lane;
+    const bool own = k < mz;
+    const Loader<float> Q{P.q, P.g.m[0], my, mz};
+    int j0, j1;
+    psi_segment(my, w, j0, j1);
+    __shared__ double tot[kPsiSegs][64];
+    // rows j0 - 3 .. j0 + kPsiHeld + 2 of the column (clamped), all loads issued before any use
+    float rows[kPsiHeld + 6];
+#pragma unroll
+    for (int r = 0; r < kPsiHeld + 6; ++r) rows[r] = Q(i, j0 - 3 + r, k);
+    float inc[kPsiHeld];
+    double s = 0.0;
+#pragma unroll
+    for (int r = 0; r < kPsiHeld; ++r) {
+        const int jj = j0 + r;
+        float v = 0.0f;
+        if (own && jj < j1) {
+#pragma unroll
+            for (int half = 0; half < 2; ++half) {
+                const double a = jj + 0.5 * half;
+                for (int gq = 0; gq < P.gauss.n; ++gq) {
+                    const double y = 0.25 * P.gauss.x[gq] + 0.25 + a;   // G coords
+                    int by; float ty;
+                    split_coord(y - 0.5, by, ty);
+                    v += 0.5f * (float)P.gauss.w[gq] * weno6<float>(&rows[r + half], WenoCoef(ty, P.kappa));
+                }
+            }
+        }
+        inc[r] = v;
+        s += (double)v;
+    }
+    tot[w][lane] = s;
+    __syncthreads();
+    if (!own) return;
+    double acc = 0.0;
+    for (int b = 0; b < w; ++b) acc += tot[b][lane];
+    double *col = P.psi + idx3(i, 0, k, my + 1, mz);
+    if (j0 == 0) col[0] = 0.0;
+#pragma unroll
+    for (int r = 0; r < kPsiHeld; ++r) {
+        if (j0 + r < j1) {
+            acc += (double)inc[r];
+            col[(size_t)(j0 + r + 1) * mz] = acc;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------------------------
 // vertex backtrace (reference src/calc_backtraced_face_3D.cpp:9-159)
 // ---------------------------------------------------------------------------------------------
@@ -1249,6 +1308,7 @@ hipError_t launch_psi(const PsiParams &p, hipStream_t s) {
     if (p.g.m[0] > 65535) return hipErrorInvalidValue;   // one grid row of blocks per x plane
     const dim3 grid((unsigned)((p.g.m[2] + 63) / 64), (unsigned)p.g.m[0]);
     if (psi_held(p)) hipLaunchKernelGGL(psi_kernel<true>, grid, dim3(64 * kPsiSegs), 0, s, p);
+    else if (psi_weno_held(p)) hipLaunchKernelGGL(psi_weno_held_kernel, grid, dim3(64 * kPsiSegs), 0, s, p);
     else hipLaunchKernelGGL(psi_kernel<false>, grid, dim3(64 * kPsiSegs), 0, s, p);
     return hipGetLastError();
 }
