@@ -3,6 +3,7 @@
 // One flux pass over a component grid G (density, or one staggered velocity component):
 //   psi_kernel        y-prefix psi on the y-faces of G: 64 z-columns of one x plane per block, the
 //                     column's y range in eight segments (one wave each), FP64 sums
+//                     (psi_weno_held_kernel: the WENO6 integral rule with the segment held)
 //   vertex_kernel     backtraced displacement of every lattice vertex, one thread per vertex
 //   face_count_kernel slots of every face (x strips x z bands); twisted and degenerate faces listed
 //   scan_*            exclusive scan of the slot counts (three kernels over tiles)
@@ -11,7 +12,9 @@
 //   piece_kernel      quadrature of every piece slot (WENO6 integral rule: piece_kernel<1>)
 //   face_sum_kernel   face flux = ordered sum of its slots
 //   general_*         the exact recursive cutter for the few twisted faces: pieces counted and
-//                     stored per face, integrated one thread per piece, summed per face
+//                     stored per face, integrated one thread per piece, summed per face; their
+//                     corrections are marked in a bit mask per family (corr_reset_kernel clears
+//                     them before the flux set's next pass)
 //   gather_kernel     per cell: sum of its six signed face fluxes (deterministic, no atomics)
 // Each interior face is computed once; its flux enters the two neighbouring cells with opposite
 // signs, so nothing is lost or created inside the domain.
