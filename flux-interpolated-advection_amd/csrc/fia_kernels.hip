@@ -224,11 +224,11 @@ template <bool HELD> __global__ void __launch_bounds__(64 * kPsiSegs) psi_kernel
     }
 }
 
-// The WENO6 integral rule in held form (segments of at most kPsiHeld cells, at most
-// kPsiWenoGauss Gauss points per half cell): the rows of the segment and its six neighbours are
+// The WENO6 integral rule in held form (segments of at most kPsiHeld cells, kPsiWenoGauss Gauss
+// point per half cell, the default rule): the rows of the segment and its six neighbours are
 // loaded at once, the increments are computed once (not again for the prefix) and kept in
 // registers.  Each increment is psi_weno_segment's, with the same operations in the same order.
-constexpr int kPsiWenoGauss = 2;
+constexpr int kPsiWenoGauss = 1;
 FIA_HD bool psi_weno_held(const PsiParams &P) {
     return P.integral && P.mode == 2 && (P.g.m[1] + kPsiSegs - 1) / kPsiSegs <= kPsiHeld &&
            P.gauss.n <= kPsiWenoGauss;

@@ -55,6 +55,22 @@ def test_psi_matches_oracle(fia, oracle, ctx, scenes, mode, integral, ng):
     assert rel_linf(got, ref) < 1e-6
 
 
+# Columns of 250 cells (eight segments of 32, the most the held forms take) and 300 (segments of
+# 38: the sliding-window WENO6 form and the unheld cheap rules), ragged in x and z; the WENO6
+# integral with 3 Gauss points per half cell takes the sliding-window form at any length.
+@pytest.mark.parametrize("ny", [250, 300])
+@pytest.mark.parametrize("mode,ng", [("WENO6", 1), ("WENO6", 3), ("linear", 1)])
+def test_psi_long_columns_match_oracle(fia, oracle, ctx, ny, mode, ng):
+    print("case psi long columns", ny, mode, ng, flush=True)
+    h = 1.0 / 64
+    rng = np.random.default_rng(ny + ng)
+    y = (np.arange(ny) + 0.5) / ny
+    q = (0.5 + 0.4 * np.sin(7.0 * y)[None, :, None] + 0.05 * rng.standard_normal((5, ny, 70))).astype(np.float32)
+    got = fia.calc_psi_y(ctx, q, h, mode=mode, use_integral=True, num_gauss=ng)
+    ref = oracle.calc_psi(q.astype(np.float64), h, mode=mode, use_integral=True, num_gauss=ng)
+    assert rel_linf(got, ref) < 1e-6
+
+
 @pytest.mark.parametrize("method", ["WENO6", "linear", "1Dy_linear"])
 @pytest.mark.parametrize("dim", [-1, 0, 1, 2])
 def test_backtrace_matches_oracle(fia, oracle, ctx, scenes, method, dim):
