@@ -50,3 +50,25 @@ def test_velocity_box_matches_oracle(fia, oracle, ctx, scenes, shape):
         got = np.asarray(got, dtype=np.float64)
         err = np.min([np.abs(got - r[a]) for r in refs], axis=0).max() / np.abs(refs[0][a]).max()
         assert err < TOL, (a, err)
+
+
+@pytest.mark.parametrize("shape,seed,cfl", [((4, 3, 5), 3, 1.5), ((2, 7, 3), 5, 2.0), ((6, 6, 1), 7, 1.0),
+                                             ((1, 5, 6), 9, 1.0), ((3, 3, 3), 4, 3.0)])
+def test_move_substances_tiny_grid_matches_oracle(fia, oracle, ctx, shape, seed, cfl):
+    # grids narrower than the six-sample WENO6 stencil (every stencil clamped) and one cell thick;
+    # the same cases run through the kernels' host bodies in tests/test_pass_host.py
+    print("case move tiny", shape, cfl, flush=True)
+    nx, ny, nz = shape
+    h = 1.0 / 16
+    rng = np.random.default_rng(seed)
+    rho = rng.uniform(0, 1, size=shape).astype(np.float32)
+    u = rng.normal(size=(nx + 1, ny, nz)); u[0] = u[-1] = 0
+    v = rng.normal(size=(nx, ny + 1, nz)); v[:, 0] = v[:, -1] = 0
+    w = rng.normal(size=(nx, ny, nz + 1)); w[:, :, 0] = w[:, :, -1] = 0
+    u, v, w = [np.ascontiguousarray(a, dtype=np.float32) for a in (u, v, w)]
+    dt = cfl * h / max(abs(u).max(), abs(v).max(), abs(w).max())
+    grid = fia.Grid3D(nx, ny, nz, h, rho.copy(), u, v, w)
+    fia.move_substances_3d(ctx, grid, 0, dt, split_method="xz-CellFaceAligned", interpolation_method="WENO6")
+    ref = oracle.move_substances(rho.astype(np.float64), u, v, w, h, dt, split="xz-CellFaceAligned")
+    assert np.isfinite(grid.density).all()
+    assert rel_linf(grid.density, ref) < TOL
