@@ -66,7 +66,7 @@ PEAK_HBM_GBS = 8000.0
 # HBM bytes per launch of piece_kernel<1> at the default workload (256^3, CFL 4), from two
 # rocprofv3 counter passes (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a 1 GiB device copy);
 # recorded in profiles/density_step_256_cfl4_hbm_counters.md
-TRAFFIC_BYTES_256_CFL4 = 12.026e9
+TRAFFIC_BYTES_256_CFL4 = 12.124e9
 # Jacobi-PCG algorithmic traffic per cell and iteration (FP64 x and r, FP32 search direction p,
 # two kernels): direction kernel reads r (8), p_old (4) and the flag byte, writes p (4); update
 # kernel reads x, r (2 x 8), p (4) and the flag byte, writes x, r (2 x 8); q = L p is recomputed,
