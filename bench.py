@@ -505,6 +505,8 @@ def run_warmstart(args, fia, scenes, torch):
     timed = rows[args.warmup:]
     s = learned.summary(timed)
     fused = pcg_is_fused(ctx, n, args.precond)
+    # how good a guess must be: iterations from guesses (1 - eps) p* of the same system
+    gq = learned.guess_quality(fia, scenes, ctx, n, 5000)
     net.close()
     ctx.close()
     l_ms, p_ms = s["learned_ms"], s["plain_ms"]
@@ -524,7 +526,9 @@ def run_warmstart(args, fia, scenes, torch):
                        "speedup": round(p_ms / l_ms, 4), "plain_iters": s["plain_iters"],
                        "learned_iters": s["learned_iters"], "network_used": s["network_used"],
                        "projections": s["projections"], "guess_rel_residual_mean": s["guess_rel_residual_mean"],
-                       "max_rel_residual": s["max_rel_residual"]},
+                       "max_rel_residual": s["max_rel_residual"],
+                       "guess_quality": [{k: (round(v, 6) if isinstance(v, float) else v) for k, v in r.items()}
+                                         for r in gq]},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": None,
                      "kernel": "warm-started projection: PCG iterations at %d B per cell over its whole time "

@@ -172,6 +172,49 @@ def evaluate(fia, scenes, ctx, net, n, scene_seeds, steps, warmup=3, rel_tol=1e-
     return rows
 
 
+def guess_quality(fia, scenes, ctx, n, seed, eps_list=(1e-1, 1e-2, 1e-3), warmup=3, rel_tol=1e-6):
+    """How good an initial guess must be to cut the CG's iterations, on the velocity state of one
+    plume step: p* from a solve to 1e-10, then guesses (1 - eps) p*, whose error is a scaled copy
+    of the solution (the slow modes CG spends its iterations on) and whose residual is eps |b| up
+    to p*'s FP32 rounding, and the iterations to rel_tol from each, beside the zero guess.  Each
+    row: eps, the guess's measured relative residual, iterations, their ratio to the zero guess's."""
+    import ctypes
+    sim = PlumeSim(fia, scenes, ctx, n, seed)
+    for _ in range(warmup):
+        sim.before_projection()
+        sim.project(rel_tol)
+        sim.after_projection()
+    sim.before_projection()
+    g = sim.grid
+    sp = sim.solid.data_ptr() if sim.solid is not None else None
+
+    def fresh():
+        return fia.Grid3D(n, n, n, g.dx, None, g.u.clone(), g.v.clone(), g.w.clone())
+
+    def solve(guess, tol, max_iters):
+        # through the C ABI: a capped solve reports FIA_ERR_NOT_CONVERGED with its count and residual
+        gr = fresh()
+        gc = gr._c(with_density=False)
+        gr.pressure = guess.clone()
+        it, res = ctypes.c_int(0), ctypes.c_double(0.0)
+        st = fia.lib().fia_pressure_project(ctx._h, ctypes.byref(gc), sp, gr.pressure.data_ptr(), 1, float(tol),
+                                            int(max_iters), ctypes.byref(it), ctypes.byref(res))
+        if fia.STATUS[st] not in ("FIA_OK", "FIA_ERR_NOT_CONVERGED"):
+            raise fia.FiaError(st, fia.lib().fia_last_error().decode())
+        return it.value, res.value, gr.pressure
+
+    zero = g.u.new_zeros((n, n, n))
+    _, _, p_star = solve(zero, 1e-10, 50000)
+    it0, _, _ = solve(zero, rel_tol, 20000)
+    rows = [{"eps": 1.0, "guess_rel_residual": 1.0, "iters": it0, "iters_ratio": 1.0}]
+    for eps in eps_list:
+        guess = (1.0 - eps) * p_star
+        _, r0, _ = solve(guess, rel_tol, 0)
+        it, _, _ = solve(guess, rel_tol, 20000)
+        rows.append({"eps": eps, "guess_rel_residual": r0, "iters": it, "iters_ratio": it / max(it0, 1)})
+    return rows
+
+
 def summary(rows):
     p_it = float(np.mean([r["plain_iters"] for r in rows]))
     l_it = float(np.mean([r["learned_iters"] for r in rows]))
