@@ -175,9 +175,10 @@ FIA_HD void psi_body(const PsiParams &P, long colid) {
 // of one thread per column.  Each wave sums its segment's increments, the totals meet in LDS, and
 // each wave writes its segment's prefix from the sum of the segments below it.  Segments of at
 // most kPsiHeld cells under the cheap rules (const, linear) keep their increments in registers
-// between the two steps (q is read once); otherwise they are computed again (the WENO6 integral
-// rule, whose increments are arithmetic-bound, or long columns).
-constexpr int kPsiHeld = 32;
+// between the two steps (q is read once), and so does the WENO6 integral rule with one Gauss point
+// per half cell (psi_weno_held_kernel, below); otherwise (longer segments, more Gauss points) they
+// are computed again.
+constexpr int kPsiHeld = 33;   // 33 cells: segments of 257-row columns (a y-velocity grid of 256 cells)
 // HELD: the increments stay in registers (cheap rules, segments of at most kPsiHeld cells); a
 // template parameter, so the other form keeps its own register budget
 FIA_HD bool psi_held(const PsiParams &P) {

@@ -55,10 +55,10 @@ def test_psi_matches_oracle(fia, oracle, ctx, scenes, mode, integral, ng):
     assert rel_linf(got, ref) < 1e-6
 
 
-# Columns of 250 cells (eight segments of 32, the most the held forms take) and 300 (segments of
-# 38: the sliding-window WENO6 form and the unheld cheap rules), ragged in x and z; the WENO6
-# integral with 3 Gauss points per half cell takes the sliding-window form at any length.
-@pytest.mark.parametrize("ny", [250, 300])
+# Columns of 250 and 257 cells (eight segments of 32 and 33, the most the held forms take) and 300
+# (segments of 38: the sliding-window WENO6 form and the unheld cheap rules), ragged in x and z;
+# the WENO6 integral with 3 Gauss points per half cell takes the sliding-window form at any length.
+@pytest.mark.parametrize("ny", [250, 257, 300])
 @pytest.mark.parametrize("mode,ng", [("WENO6", 1), ("WENO6", 3), ("linear", 1)])
 def test_psi_long_columns_match_oracle(fia, oracle, ctx, ny, mode, ng):
     print("case psi long columns", ny, mode, ng, flush=True)
