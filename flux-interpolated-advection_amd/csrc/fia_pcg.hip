@@ -558,9 +558,19 @@ __device__ __forceinline__ void fused_sweep(const PcgState &st, int blk, const f
     // with XUPD, r_(k+1) of the own cells waits for stage B here (per thread, by plane parity)
     // rather than in registers, which x and p_(k-2) need
     __shared__ double R1[XUPD ? 2 : 1][kRows][XUPD ? kPB : 1];
+    // +0.0f, read in place of a neighbour that is not fluid, and 1 / diag by neighbour count (index
+    // 7: not an unknown).  The stencils' LDS reads take their address from the flag bit instead of
+    // sitting under a select of the value, which the compiler turns into a branch with its own wait
+    // per read; inv_diag's chain of selects likewise becomes a tree of branches.
+    __shared__ float zero_slot;
+    __shared__ double inv_tab[8];
     const int per = st.dir_blocks / 8;
     const int L = (blk % 8) * per + blk / 8;
     if (L >= st.dir_tiles_y * st.dir_tiles_z * st.dir_chunks) return;
+    if (threadIdx.x < 8) inv_tab[threadIdx.x] = inv_diag(threadIdx.x < 7 ? kPcgUnknown | ((1u << threadIdx.x) - 1u) : 0u);
+    if (threadIdx.x == 0) zero_slot = 0.0f;
+    // read after the first barrier of the plane loop
+    auto inv_of = [&](unsigned f) { return inv_tab[(f & kPcgUnknown) ? __builtin_popcount(f & 63u) : 7]; };
     const int nx = st.g.nx, ny = st.g.ny, nz = st.g.nz;
     const int y0 = (L % st.dir_tiles_y) * kTileY;
     const int z0 = ((L / st.dir_tiles_y) % st.dir_tiles_z) * kTileZ;
@@ -660,16 +670,21 @@ __device__ __forceinline__ void fused_sweep(const PcgState &st, int blk, const f
         // stage A on plane X: q_k, r_(k+1), p_(k+1) on the tile and the one-cell ring
         auto stage_a = [&](int ly, int lz, unsigned f, double r, float pmv, float pcv, float pnv, double &r1,
                            float &p1) {
+            const float n4 = *((f & 4u) ? &P0[ly + 1][lz + 2] : &zero_slot);
+            const float n8 = *((f & 8u) ? &P0[ly + 3][lz + 2] : &zero_slot);
+            const float n16 = *((f & 16u) ? &P0[ly + 2][lz + 1] : &zero_slot);
+            const float n32 = *((f & 32u) ? &P0[ly + 2][lz + 3] : &zero_slot);
+            const double id = inv_of(f);
             double q = (double)__builtin_popcount(f & 63u) * (double)pcv;
             q -= (f & 1u) ? (double)pmv : 0.0;
             q -= (f & 2u) ? (double)pnv : 0.0;
-            q -= (f & 4u) ? (double)P0[ly + 1][lz + 2] : 0.0;
-            q -= (f & 8u) ? (double)P0[ly + 3][lz + 2] : 0.0;
-            q -= (f & 16u) ? (double)P0[ly + 2][lz + 1] : 0.0;
-            q -= (f & 32u) ? (double)P0[ly + 2][lz + 3] : 0.0;
+            q -= (double)n4;
+            q -= (double)n8;
+            q -= (double)n16;
+            q -= (double)n32;
             q = (f & kPcgUnknown) ? q : 0.0;
             r1 = r - alpha * q;
-            p1 = (float)(r1 * inv_diag(f) + beta * (double)pcv);
+            p1 = (float)(r1 * id + beta * (double)pcv);
             P1[b1][ly + 1][lz + 1] = p1;
         };
         double r1o[kRows];
@@ -712,18 +727,23 @@ __device__ __forceinline__ void fused_sweep(const PcgState &st, int blk, const f
                 if (!act[w]) continue;
                 const int ly = ty + w * kRowStep + 1, lz = tz + 1;
                 const unsigned f = (fbf >> (8 * w)) & 0xffu;
+                const float n4 = *((f & 4u) ? &P1[b0][ly - 1][lz] : &zero_slot);
+                const float n8 = *((f & 8u) ? &P1[b0][ly + 1][lz] : &zero_slot);
+                const float n16 = *((f & 16u) ? &P1[b0][ly][lz - 1] : &zero_slot);
+                const float n32 = *((f & 32u) ? &P1[b0][ly][lz + 1] : &zero_slot);
+                const double id = inv_of(f);
                 double q = (double)__builtin_popcount(f & 63u) * (double)qc[w];
                 q -= (f & 1u) ? (double)qm[w] : 0.0;
                 q -= (f & 2u) ? (double)p1o[w] : 0.0;
-                q -= (f & 4u) ? (double)P1[b0][ly - 1][lz] : 0.0;
-                q -= (f & 8u) ? (double)P1[b0][ly + 1][lz] : 0.0;
-                q -= (f & 16u) ? (double)P1[b0][ly][lz - 1] : 0.0;
-                q -= (f & 32u) ? (double)P1[b0][ly][lz + 1] : 0.0;
+                q -= (double)n4;
+                q -= (double)n8;
+                q -= (double)n16;
+                q -= (double)n32;
                 q = (f & kPcgUnknown) ? q : 0.0;
                 double r;
                 if constexpr (XUPD) r = R1[b0][w][threadIdx.x];
                 else r = rb[w];
-                const double id = inv_diag(f), pv = (double)qc[w];
+                const double pv = (double)qc[w];
                 acc[0] += r * (r * id);
                 acc[1] += r * r;
                 acc[2] += pv * q;
