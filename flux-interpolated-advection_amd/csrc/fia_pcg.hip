@@ -333,6 +333,11 @@ __device__ __forceinline__ bool halo_cell(int t, int y0, int z0, int &y, int &z,
     return false;
 }
 
+// An Op with kSelectReads = true keeps the plane reads of tile_sweep under selects of the value
+// (fewer registers, more branches); the others read a zero slot for a missing neighbour.
+template <class Op> constexpr bool select_reads(long) { return false; }
+template <class Op> constexpr auto select_reads(int) -> decltype((void)Op::kSelectReads, bool()) { return Op::kSelectReads; }
+
 // Tile blk of a solve's sweep (dir_blocks tiles; blocks are dispatched to the 8 XCDs round-robin,
 // so consecutive logical tiles go to one XCD, which sweeps whole y-z slabs of neighbouring tiles).
 template <class Op>
@@ -340,6 +345,11 @@ __device__ __forceinline__ void tile_sweep(const PcgState &st, int blk, const Op
     static_assert(kTileZ * kTileY == kPB * kRows && 2 * (kTileZ + kTileY) <= kPB, "tile shape");
     using Raw = typename Op::Raw;
     __shared__ double plane[2][kTileY + 2][kTileZ + 2];
+    // +0.0, read in place of a neighbour that is not fluid: the plane reads take their address from
+    // the flag bit instead of sitting under a select of the value, which the compiler turns into a
+    // branch with its own wait per read
+    __shared__ double zero_slot;
+    if (threadIdx.x == 0) zero_slot = 0.0;
     const int per = st.dir_blocks / 8;
     const int L = (blk % 8) * per + blk / 8;
     constexpr int kRowStep = kTileY / kRows;
@@ -396,10 +406,21 @@ __device__ __forceinline__ void tile_sweep(const PcgState &st, int blk, const Op
             double s = (double)__builtin_popcount(f & 63u) * dc[w];
             s -= (f & 1u) ? dm[w] : 0.0;
             s -= (f & 2u) ? dn[w] : 0.0;
-            s -= (f & 4u) ? plane[cur][ly - 1][tz + 1] : 0.0;
-            s -= (f & 8u) ? plane[cur][ly + 1][tz + 1] : 0.0;
-            s -= (f & 16u) ? plane[cur][ly][tz] : 0.0;
-            s -= (f & 32u) ? plane[cur][ly][tz + 2] : 0.0;
+            if constexpr (select_reads<Op>(0)) {
+                s -= (f & 4u) ? plane[cur][ly - 1][tz + 1] : 0.0;
+                s -= (f & 8u) ? plane[cur][ly + 1][tz + 1] : 0.0;
+                s -= (f & 16u) ? plane[cur][ly][tz] : 0.0;
+                s -= (f & 32u) ? plane[cur][ly][tz + 2] : 0.0;
+            } else {
+                const double n4 = *((f & 4u) ? &plane[cur][ly - 1][tz + 1] : &zero_slot);
+                const double n8 = *((f & 8u) ? &plane[cur][ly + 1][tz + 1] : &zero_slot);
+                const double n16 = *((f & 16u) ? &plane[cur][ly][tz] : &zero_slot);
+                const double n32 = *((f & 32u) ? &plane[cur][ly][tz + 2] : &zero_slot);
+                s -= n4;
+                s -= n8;
+                s -= n16;
+                s -= n32;
+            }
             op.cell(c[w], oc[w], dc[w], (f & kPcgUnknown) ? s : 0.0, acc);
         }
         __syncthreads();
@@ -1143,6 +1164,8 @@ struct MgUp0Op {
     // the loaded values as stored (x and the correction exist for every cell: loaded with the flag
     // byte, not after it); combined only where the sweep uses them.  Unlike the other sweeps' loads
     // this one keeps its branch: the clamped form needs 140 VGPRs (3 waves per SIMD instead of 4).
+    // For the same reason its plane reads stay under selects (the zero-slot reads need 130).
+    static constexpr bool kSelectReads = true;
     __device__ __forceinline__ Raw load(int x0, int y, int z, bool own) const {
         Raw a{0u, 0.0f, 0.0f, 0.0};
         if (in_grid(st.g, x0, y, z)) {
