@@ -626,6 +626,12 @@ __device__ __forceinline__ void fused_sweep(const PcgState &st, int blk, const f
     const bool has1 = t1 >= 0, has2 = threadIdx.x < kRing2;
     const int2 g1 = ring1_cell(has1 ? t1 : 0), g2 = ring2_cell(has2 ? (int)threadIdx.x : 0);
     const int r1y = g1.x, r1z = g1.y, r2y = g2.x, r2z = g2.y;
+    // the ring cells' (y, z) address parts, clamped into the grid, and whether they lie in it; the
+    // plane loop adds the plane part, which is the same for every thread
+    auto yz_part = [&](int y, int z) { return (size_t)min(max(y, 0), ny - 1) * nz + min(max(z, 0), nz - 1); };
+    auto yz_in = [&](int y, int z) { return y >= 0 && y < ny && z >= 0 && z < nz; };
+    const size_t r1yz = yz_part(y0 + r1y, z0 + r1z), r2yz = yz_part(y0 + r2y, z0 + r2z);
+    const bool r1in = yz_in(y0 + r1y, z0 + r1z), r2in = yz_in(y0 + r2y, z0 + r2z);
     const int z = z0 + tz;
     bool act[kRows];
     size_t c[kRows];   // own cell of plane max(X, 0), y and z clamped into the grid
@@ -668,19 +674,29 @@ __device__ __forceinline__ void fused_sweep(const PcgState &st, int blk, const f
     for (int X = xs; X <= xb; ++X) {
         const bool own_plane = X >= xa && X < xb;
         // loads for the next planes: raw of X + 1 (a stage-A plane while X + 1 <= xb), p_k of X + 2
-        // (needed by stage A while X + 2 <= xb + 1); past those, loaded and not used
-        FusedRaw on[kRows];
+        // (needed by stage A while X + 2 <= xb + 1); past those, loaded and not used.  X + 1 >= 0,
+        // so only the top of the grid clamps the planes; an own row's cell of plane min(X + k,
+        // nx - 1) is its cell of plane max(X, 0) plus a plane offset.
+        const int x1 = min(X + 1, nx - 1), x2 = min(X + 2, nx - 1);
+        const bool in1 = X + 1 < nx, in2 = X + 2 < nx;
+        const long o1 = (long)(x1 - max(X, 0)) * sx, o2 = (long)(x2 - max(X, 0)) * sx;
+        // As loaded; the selects for cells outside the grid come at the end of the plane: a select
+        // next to its load, on a condition uniform across the block (in1, in2), is compiled into a
+        // branch around the load with a wait for it.
+        unsigned on_f[kRows];
+        double on_r[kRows];
         float pn2[kRows];
 #pragma unroll
         for (int w = 0; w < kRows; ++w) {
-            const int y = y0 + ty + w * kRowStep;
-            on[w] = load_raw(X + 1, y, z);
-            pn2[w] = load_p(X + 2, y, z);
+            on_f[w] = fl[(long)c[w] + o1];
+            on_r[w] = rk[(long)c[w] + o1];
+            pn2[w] = pold[(long)c[w] + o2];
         }
         // ring cells (threads without one load a cell of the first ring and do not use it)
-        const FusedRaw an = load_raw(X + 1, y0 + r1y, z0 + r1z);
-        const float gn2 = load_p(X + 2, y0 + r1y, z0 + r1z);
-        const float hn2 = load_p(X + 2, y0 + r2y, z0 + r2z);
+        const size_t a1 = r1yz + (size_t)x1 * sx;
+        const unsigned an_f = fl[a1];
+        const double an_r = rk[a1];
+        const float gn2 = pold[r1yz + (size_t)x2 * sx], hn2 = pold[r2yz + (size_t)x2 * sx];
         // p_k of plane X into LDS
         const int b1 = X & 1;
 #pragma unroll
@@ -715,7 +731,12 @@ __device__ __forceinline__ void fused_sweep(const PcgState &st, int blk, const f
             const int ly = ty + w * kRowStep;
             stage_a(ly, tz, (ocf >> (8 * w)) & 0xffu, ocr[w], pm[w], pc[w], pn[w], r1o[w], p1o[w]);
             if constexpr (XUPD) R1[b1][w][threadIdx.x] = r1o[w];
-            if (own_plane && act[w]) {
+        }
+        // the stores after all rows' stencils, so that the rows' LDS reads share one block
+        if (own_plane) {
+#pragma unroll
+            for (int w = 0; w < kRows; ++w) {
+                if (!act[w]) continue;
                 rw[c[w]] = r1o[w];
                 pnew[c[w]] = p1o[w];
             }
@@ -731,7 +752,7 @@ __device__ __forceinline__ void fused_sweep(const PcgState &st, int blk, const f
         double xo[kRows];
         float po[kRows];
         // from plane max(X, 0) of c to plane min(X + 1, nx - 1) (past xb: in the grid, not used)
-        const long xoff = (long)(min(X + 1, nx - 1) - max(X, 0)) * sx;
+        const long xoff = o1;
         if constexpr (XUPD) {
 #pragma unroll
             for (int w = 0; w < kRows; ++w) {
@@ -779,18 +800,20 @@ __device__ __forceinline__ void fused_sweep(const PcgState &st, int blk, const f
                 if (own_next && act[w]) xw[(long)c[w] + xoff] = xo[w] + (a2 * (double)po[w] + alpha * (double)pn[w]);
         }
         // advance one plane
+        fbf = ocf;
+        ocf = 0u;
 #pragma unroll
         for (int w = 0; w < kRows; ++w) {
             qm[w] = qc[w]; qc[w] = p1o[w];
             if constexpr (!XUPD) rb[w] = r1o[w];
-            pm[w] = pc[w]; pc[w] = pn[w]; pn[w] = pn2[w];
-            ocr[w] = on[w].r;
+            pm[w] = pc[w]; pc[w] = pn[w]; pn[w] = in2 && act[w] ? pn2[w] : 0.0f;
+            ocr[w] = in1 && act[w] ? on_r[w] : 0.0;
+            ocf |= (in1 && act[w] ? on_f[w] : 0u) << (8 * w);
             if (X >= 0) c[w] += sx;
         }
-        fbf = ocf;
-        ocf = on[0].f | on[1].f << 8 | on[2].f << 16 | on[3].f << 24;
-        gm = gc; gc = gn; gn = gn2; ac = an;
-        hc = hn; hn = hn2;
+        gm = gc; gc = gn; gn = in2 && r1in ? gn2 : 0.0f;
+        ac = FusedRaw{in1 && r1in ? an_f : 0u, in1 && r1in ? an_r : 0.0};
+        hc = hn; hn = in2 && r2in ? hn2 : 0.0f;
     }
 }
 
