@@ -9,6 +9,8 @@
 //   scan_*            exclusive scan of the slot counts (three kernels over tiles)
 //   seg_face_kernel   boundary-segment cutter, one thread per face (multi-cut faces piece by piece)
 //   degenerate_kernel clipping pipeline for faces with a corner colour on an interior integer
+//   zero_velocity_face_kernel  the zero-velocity correction pass: the undeformed y faces' pieces
+//                     in closed form (at most 2 x 2 axis-aligned rectangles per face)
 //   piece_kernel      quadrature of every piece slot (WENO6 integral rule: piece_kernel<1>)
 //   face_sum_kernel   face flux = ordered sum of its slots
 //   general_*         the exact recursive cutter for the few twisted faces: pieces counted and

@@ -56,7 +56,11 @@ inline void check(fia_status s) {
 
 // One library context per process (device scratch is kept between calls of the same size).  Its
 // pressure solves use the multigrid preconditioner, the counterpart of the reference's AMG
-// (amgcl smoothed aggregation, src/calc_pressure_3d.cpp:141-166).
+// (amgcl smoothed aggregation, src/calc_pressure_3d.cpp:141-166).  Velocity advection, projection
+// and density advection share it, in that order every step.  An earlier version of the library
+// once stalled on a 256^3 density step that followed a velocity step in one context, and no cause
+// was found (host replays of the same sequence complete); the sequence is the library's own
+// benchmark step and has run there since without a stall.
 inline fia_ctx *context() {
     static fia_ctx *ctx = nullptr;
     if (!ctx) {
