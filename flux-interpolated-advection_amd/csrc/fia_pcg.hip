@@ -571,10 +571,12 @@ __device__ __forceinline__ void fused_sweep(const PcgState &st, int blk, const f
                                             float *__restrict__ pnew, const double *__restrict__ rk,
                                             double *__restrict__ rw, double alpha, double beta, double a2,
                                             double (&acc)[5]) {
-    // p_k of plane X (two-cell ring) and p_(k+1) of planes X and X - 1 (one-cell ring, by the
-    // parity of the plane).  Two barriers per plane: P0 is rewritten every plane.  (A second P0
-    // buffer and one barrier measured 2% slower on 256^3: more LDS and a register spill.)
-    __shared__ float P0[kTileY + 4][kTileZ + 4];
+    // p_k of plane X (two-cell ring) and p_(k+1) of planes X and X - 1 (one-cell ring), both by
+    // the parity of the plane, so one barrier per plane suffices: stage A of plane X reads P0[X & 1]
+    // and writes P1[X & 1], stage B reads P1 of plane X - 1, and a buffer is written again two
+    // planes later, after the next plane's barrier, which every thread passes only when done with
+    // it.  (R1 is per thread.)
+    __shared__ float P0[2][kTileY + 4][kTileZ + 4];
     __shared__ float P1[2][kTileY + 2][kTileZ + 2];
     // with XUPD, r_(k+1) of the own cells waits for stage B here (per thread, by plane parity)
     // rather than in registers, which x and p_(k-2) need
@@ -700,17 +702,17 @@ __device__ __forceinline__ void fused_sweep(const PcgState &st, int blk, const f
         // p_k of plane X into LDS
         const int b1 = X & 1;
 #pragma unroll
-        for (int w = 0; w < kRows; ++w) P0[ty + w * kRowStep + 2][tz + 2] = pc[w];
-        if (has1) P0[r1y + 2][r1z + 2] = gc;
-        if (has2) P0[r2y + 2][r2z + 2] = hc;
+        for (int w = 0; w < kRows; ++w) P0[b1][ty + w * kRowStep + 2][tz + 2] = pc[w];
+        if (has1) P0[b1][r1y + 2][r1z + 2] = gc;
+        if (has2) P0[b1][r2y + 2][r2z + 2] = hc;
         __syncthreads();
         // stage A on plane X: q_k, r_(k+1), p_(k+1) on the tile and the one-cell ring
         auto stage_a = [&](int ly, int lz, unsigned f, double r, float pmv, float pcv, float pnv, double &r1,
                            float &p1) {
-            const float n4 = *((f & 4u) ? &P0[ly + 1][lz + 2] : &zero_slot);
-            const float n8 = *((f & 8u) ? &P0[ly + 3][lz + 2] : &zero_slot);
-            const float n16 = *((f & 16u) ? &P0[ly + 2][lz + 1] : &zero_slot);
-            const float n32 = *((f & 32u) ? &P0[ly + 2][lz + 3] : &zero_slot);
+            const float n4 = *((f & 4u) ? &P0[b1][ly + 1][lz + 2] : &zero_slot);
+            const float n8 = *((f & 8u) ? &P0[b1][ly + 3][lz + 2] : &zero_slot);
+            const float n16 = *((f & 16u) ? &P0[b1][ly + 2][lz + 1] : &zero_slot);
+            const float n32 = *((f & 32u) ? &P0[b1][ly + 2][lz + 3] : &zero_slot);
             const double id = inv_of(f);
             double q = (double)__builtin_popcount(f & 63u) * (double)pcv;
             q -= (f & 1u) ? (double)pmv : 0.0;
@@ -760,7 +762,6 @@ __device__ __forceinline__ void fused_sweep(const PcgState &st, int blk, const f
                 po[w] = pnew[(long)c[w] + xoff];
             }
         }
-        __syncthreads();
         // stage B on plane X - 1 (own cells): q_(k+1) and the sums
         if (X > xa) {
             const int b0 = b1 ^ 1;
