@@ -23,7 +23,9 @@
 // streaming; the update reads 4 (p) + 8 x 2 (x, r) + 1, plus 8 (q) when streaming, and writes 16
 // (x, r): 70 bytes streaming, 54 recomputing.  Recomputing pays where the tiles sweep
 // long x chunks (large grids); on small grids, whose chunks are a few planes, the halo planes
-// cost more than q and the streaming form is faster.
+// cost more than q and the streaming form is faster.  Under Jacobi on grids of at least 4 M
+// cells one kernel per iteration replaces the two (pcg_fused_kernel, below: 35 bytes per cell
+// and iteration), and a geometric multigrid V-cycle can replace the Jacobi preconditioner.
 //
 // Reductions without fences: every block writes its partial sums with plain stores, and every
 // block of the NEXT kernel sums all partials itself, in block order, before its own work (the
