@@ -345,6 +345,20 @@ FIA_HD void face_corner_offsets(int fam, int c, int o[3]) {
 FIA_HD bool load_face(const FaceParams &P, int fam, int i, int j, int k, double v[4][3],
                                           double col[4][2]) {
     const int ly = P.g.m[1] + 1, lz = P.g.m[2] + 1;
+    // the twelve displacements first, in one block (loaded corner by corner under the test of
+    // the pointer, each corner's loads were waited for before the next corner's were issued)
+    const bool moving = P.disp[0] != nullptr;
+    float d[4][3];
+    if (moving) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            int o[3];
+            face_corner_offsets(fam, c, o);
+            const size_t vi = idx3(i + o[0], j + o[1], k + o[2], ly, lz);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) d[c][a] = P.disp[a][vi];
+        }
+    }
     bool finite = true;
     for (int c = 0; c < 4; ++c) {
         int o[3];
@@ -353,13 +367,12 @@ FIA_HD bool load_face(const FaceParams &P, int fam, int i, int j, int k, double 
         v[c][0] = li - 0.5 * P.g.e[0];
         v[c][1] = lj - 0.5 * P.g.e[1];
         v[c][2] = lk - 0.5 * P.g.e[2];
-        if (P.disp[0]) {
-            const size_t vi = idx3(li, lj, lk, ly, lz);
-            v[c][0] += (double)P.disp[0][vi];
-            v[c][1] += (double)P.disp[1][vi];
-            v[c][2] += (double)P.disp[2][vi];
+        if (moving) {
+            v[c][0] += (double)d[c][0];
+            v[c][1] += (double)d[c][1];
+            v[c][2] += (double)d[c][2];
         }
-        finite = finite && std::isfinite(v[c][0]) && std::isfinite(v[c][1]) && std::isfinite(v[c][2]);
+        finite = finite & std::isfinite(v[c][0]) & std::isfinite(v[c][1]) & std::isfinite(v[c][2]);
         // colours (reference src/split_face_3D.cpp:26-43)
         const double x = v[c][0], z = v[c][2];
         double cx = 0.0, cz = 0.0;
@@ -1402,10 +1415,20 @@ __global__ void __launch_bounds__(kBlock) scan_tile_sums_kernel(const unsigned *
     // the tile read striped over the block (coalesced); the sum does not depend on the order
     const size_t tile0 = (size_t)blockIdx.x * kScanTile;
     unsigned long long s = 0;
+    if (tile0 + kScanTile <= n) {
+        // a full tile (all but the last): no bounds tests, so the loads are issued together (a
+        // load under a test is compiled into a branch and waited for at once)
+        unsigned v[kScanItems];
 #pragma unroll
-    for (int i = 0; i < kScanItems; ++i) {
-        const size_t k = tile0 + (size_t)i * kBlock + threadIdx.x;
-        if (k < n) s += in[k];
+        for (int i = 0; i < kScanItems; ++i) v[i] = in[tile0 + (size_t)i * kBlock + threadIdx.x];
+#pragma unroll
+        for (int i = 0; i < kScanItems; ++i) s += v[i];
+    } else {
+#pragma unroll
+        for (int i = 0; i < kScanItems; ++i) {
+            const size_t k = tile0 + (size_t)i * kBlock + threadIdx.x;
+            if (k < n) s += in[k];
+        }
     }
     unsigned long long total;
     (void)block_exclusive_scan(s, total);
@@ -1461,11 +1484,20 @@ __global__ void __launch_bounds__(kBlock) scan_tiles_kernel(const unsigned *in, 
     __shared__ unsigned sv[kScanTile + kScanTile / kScanItems];
     __shared__ unsigned long long so[kScanTile + kScanTile / kScanItems];
     const size_t tile0 = (size_t)blockIdx.x * kScanTile;
+    const bool full = tile0 + kScanTile <= n;   // all tiles but the last: loads without tests
+    if (full) {
+        unsigned w[kScanItems];
 #pragma unroll
-    for (int i = 0; i < kScanItems; ++i) {
-        const int e = i * kBlock + threadIdx.x;
-        const size_t k = tile0 + e;
-        sv[scan_pad(e)] = k < n ? in[k] : 0u;
+        for (int i = 0; i < kScanItems; ++i) w[i] = in[tile0 + (size_t)i * kBlock + threadIdx.x];
+#pragma unroll
+        for (int i = 0; i < kScanItems; ++i) sv[scan_pad(i * kBlock + threadIdx.x)] = w[i];
+    } else {
+#pragma unroll
+        for (int i = 0; i < kScanItems; ++i) {
+            const int e = i * kBlock + threadIdx.x;
+            const size_t k = tile0 + e;
+            sv[scan_pad(e)] = k < n ? in[k] : 0u;
+        }
     }
     __syncthreads();
     unsigned v[kScanItems];
