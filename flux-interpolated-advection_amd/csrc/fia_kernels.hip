@@ -472,12 +472,23 @@ FIA_HD double piece_flux_weno_integral(const FaceParams &P, const double c[3], d
         const unsigned o0 = in ? ((unsigned)(bx - 2) * (unsigned)(my + 1) + (unsigned)J) * (unsigned)mz + (unsigned)(bz - 2)
                                : 0u;
         double xr[6];
+        if (__all(in)) {
+            // every lane's stencil inside the field (the common case): the rows' loads carry no
+            // test, so they need not each be waited for before the next row's are issued
 #pragma unroll
-        for (int ix = 0; ix < 6; ++ix) {
-            double fz[6];
-            if (in) load_row6<double>(P.psi, o0 + (unsigned)ix * sx, fz);
-            else load_z6<double>(PS, bx - 2 + ix, J, bz - 2, zin, fz);
-            xr[ix] = weno6<double, OPT>(fz, wz);
+            for (int ix = 0; ix < 6; ++ix) {
+                double fz[6];
+                load_row6<double>(P.psi, o0 + (unsigned)ix * sx, fz);
+                xr[ix] = weno6<double, OPT>(fz, wz);
+            }
+        } else {
+#pragma unroll
+            for (int ix = 0; ix < 6; ++ix) {
+                double fz[6];
+                if (in) load_row6<double>(P.psi, o0 + (unsigned)ix * sx, fz);
+                else load_z6<double>(PS, bx - 2 + ix, J, bz - 2, zin, fz);
+                xr[ix] = weno6<double, OPT>(fz, wz);
+            }
         }
         psi = weno6<double, OPT>(xr, wx);
     }
