@@ -236,7 +236,9 @@ FIA_HD bool psi_weno_held(const PsiParams &P) {
     return P.integral && P.mode == 2 && (P.g.m[1] + kPsiSegs - 1) / kPsiSegs <= kPsiHeld &&
            P.gauss.n <= kPsiWenoGauss;
 }
-__global__ void __launch_bounds__(64 * kPsiSegs) psi_weno_held_kernel(PsiParams P) {
+// bound to 128 VGPRs (three values spilled): two 8-wave blocks per CU instead of one (0.127 ->
+// 0.106 ms per launch on the 256^3 velocity grids)
+__global__ void __launch_bounds__(64 * kPsiSegs) __attribute__((amdgpu_waves_per_eu(4))) psi_weno_held_kernel(PsiParams P) {
     const int my = P.g.m[1], mz = P.g.m[2];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int i = blockIdx.y, k = blockIdx.x * 64 + lane;
