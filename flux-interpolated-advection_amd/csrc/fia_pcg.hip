@@ -115,13 +115,36 @@ __device__ __forceinline__ void write_partials(double (&v)[NV], double *part, in
 }
 
 // Totals of partials written by an earlier kernel, identical in every thread of every block.
+// Every block runs this before its own work, so up to kPcgMaxBlocks partials its loads are all
+// issued at once: each thread takes the partials threadIdx.x + j kPB in order of j from clamped
+// indices, and one past nb counts as 0 through a multiply by 0 (a select next to the load, or a
+// loop to nb, became one round trip per partial).  The sums are those of the loop that larger
+// counts take: x * 1 = x, and s + (+-0) = s for the finite partials.
 template <int NV>
 __device__ __forceinline__ void total_of(const double *part, int nb, double (&tot)[NV]) {
+    if (nb <= kPcgMaxBlocks) {
+        constexpr int kPer = kPcgMaxBlocks / kPB;
+        double v[kPer][NV];
 #pragma unroll
-    for (int a = 0; a < NV; ++a) {
-        double s = 0.0;
-        for (int i = threadIdx.x; i < nb; i += kPB) s += part[(size_t)a * nb + i];
-        tot[a] = s;
+        for (int j = 0; j < kPer; ++j) {
+            const int i = min((int)threadIdx.x + j * kPB, nb - 1);
+#pragma unroll
+            for (int a = 0; a < NV; ++a) v[j][a] = part[(size_t)a * nb + i];
+        }
+#pragma unroll
+        for (int a = 0; a < NV; ++a) {
+            double s = 0.0;
+#pragma unroll
+            for (int j = 0; j < kPer; ++j) s += v[j][a] * (double)((int)threadIdx.x + j * kPB < nb);
+            tot[a] = s;
+        }
+    } else {
+#pragma unroll
+        for (int a = 0; a < NV; ++a) {
+            double s = 0.0;
+            for (int i = threadIdx.x; i < nb; i += kPB) s += part[(size_t)a * nb + i];
+            tot[a] = s;
+        }
     }
     block_total<NV>(tot);
 }
