@@ -157,6 +157,13 @@ __device__ __forceinline__ bool solve_done(const double *scal) {
     __syncthreads();
     return flag != 0;
 }
+// the same with a flag value each thread has loaded: thread 0's decides
+__device__ __forceinline__ bool block_flag(double v) {
+    __shared__ int flag;
+    if (threadIdx.x == 0) flag = v != 0.0;
+    __syncthreads();
+    return flag != 0;
+}
 
 // The residual state after k iterations: rz, rr (and bb) from the partials of the start kernel
 // (k = 0) or of update kernel k - 1.
@@ -846,7 +853,10 @@ __device__ __forceinline__ void fused_sweep(const PcgState &st, int blk, const f
 // Sweep k of a fused solve (see above): the sums of sweep k - 1 (of the start kernel for k = 0)
 // give the convergence test, alpha and beta; then the sweep.
 template <bool XUPD> __device__ __forceinline__ void pcg_fused_body(const PcgState &st, int blk, int k) {
-    if (solve_done(st.scal)) return;
+    // the converged flag is read before the previous sweep's partials and tested after them, so
+    // the two round trips overlap; thread 0's value decides for the block (as in solve_done), and
+    // the test comes before block 0 updates the counters
+    const double flag = st.scal[kPcgDone];
     double rz, rr, bb, alpha = 0.0, beta = 0.0;
     if (k == 0) {
         double tot[3];
@@ -860,6 +870,7 @@ template <bool XUPD> __device__ __forceinline__ void pcg_fused_body(const PcgSta
         const double rz1 = rz - 2.0 * alpha * tot[3] + alpha * alpha * tot[4];
         beta = rz != 0.0 ? rz1 / rz : 0.0;
     }
+    if (block_flag(flag)) return;
     const bool done = rr <= st.scal[kPcgTol2] * bb;
     if (blk == 0 && threadIdx.x == 0) {
         double *sc = st.scal;
