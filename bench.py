@@ -77,9 +77,9 @@ PCG_BYTES_PER_CELL_ITER = 54
 PCG_FUSED_BYTES_PER_CELL_ITER = 35
 # HBM bytes per pcg_fused_kernel launch (mean of an odd and an even sweep) on the 256^3 plume
 # (8 obstacles), from two rocprofv3 passes (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0, scaled by the
-# counters' reading of a 1 GiB copy): odd 512.3 MB, even 867.1 MB; recorded in
+# counters' reading of a 1 GiB copy): odd 512.2 MB, even 865.8 MB; recorded in
 # profiles/plume_step_256_cfl4_hbm_counters.md
-PCG_FUSED_TRAFFIC_BYTES_256 = 689.7e6
+PCG_FUSED_TRAFFIC_BYTES_256 = 689.0e6
 # one multigrid V-cycle per iteration, per fine cell: the down kernel reads r (8) and the flag
 # byte, writes x (4) and the aggregate's b (0.5); the up kernel reads x (4), r (8), the flag byte
 # and the correction (0.5), writes z (8); the coarser levels add about a seventh of a fine level's
