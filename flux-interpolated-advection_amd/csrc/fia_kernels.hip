@@ -474,7 +474,7 @@ FIA_HD double piece_flux_weno_integral(const FaceParams &P, const double c[3], d
         const unsigned o0 = in ? ((unsigned)(bx - 2) * (unsigned)(my + 1) + (unsigned)J) * (unsigned)mz + (unsigned)(bz - 2)
                                : 0u;
         double xr[6];
-        if (__all(in)) {
+        if (wave_all(in)) {
             // every lane's stencil inside the field (the common case): the rows' loads carry no
             // test, so they need not each be waited for before the next row's are issued
 #pragma unroll

@@ -36,6 +36,15 @@ FIA_HD float fast_rcp(float x) {
 #endif
 }
 
+// p for every lane of the wave (the host replays run one lane at a time: p itself)
+FIA_HD bool wave_all(bool p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __all(p);
+#else
+    return p;
+#endif
+}
+
 // Coefficients of one WENO6 evaluation that depend only on the fraction t (shared by every
 // evaluation of a pass).  With d_i = t + 2 - i the ideal weights are C0 = d4 d5 / 20,
 // C1 = -d5 d0 / 10, C2 = d0 d1 / 20 (all positive on [0,1]).  Candidate r (the Newton-form cubic
