@@ -1058,6 +1058,46 @@ __device__ __forceinline__ float mg_b(const PcgState &st, const MgLevel &L, int 
 // l + 1 along z): every unknown cell gets the first smoothing sweep from zero, x = w b / d, and
 // its residual b - L x (the neighbours' x recomputed from their b); each aggregate's right-hand
 // side is the sum of its eight children's residuals, added in a fixed order through LDS.
+// The down step's residual of cell (i, j, k) of a level l >= 1 (x = w b / d on every unknown),
+// with every load issued at once: the cell's operator record and those of its -x, -y, -z
+// neighbours (the weights), and each neighbour's record and right-hand side, from indices clamped
+// into the grid.  A neighbour that is outside, not an unknown, or joined by a zero weight adds
+// (its term) * 0 with a denominator that cannot be 0; the others add what the tested form adds.
+__device__ __forceinline__ float mg_down_cell_coarse(const MgLevel &F, int i, int j, int k) {
+    const int sx = F.ny * F.nz, sy = F.nz;
+    const int c = (i * F.ny + j) * F.nz + k;
+    const float4 o = F.op[c];
+    const float4 oxm = F.op[i > 0 ? c - sx : c], oym = F.op[j > 0 ? c - sy : c], ozm = F.op[k > 0 ? c - 1 : c];
+    const float bc = F.b[c];
+    const float w[6] = {oxm.y * (float)(i > 0), o.y, oym.z * (float)(j > 0), o.z, ozm.w * (float)(k > 0), o.w};
+    float dn[6], bn[6];
+    bool in[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        int ni, nj, nk;
+        mg_neighbour(q, i, j, k, ni, nj, nk);
+        in[q] = ni >= 0 && nj >= 0 && nk >= 0 && ni < F.nx && nj < F.ny && nk < F.nz;
+        const int cn = (clampi(ni, 0, F.nx - 1) * F.ny + clampi(nj, 0, F.ny - 1)) * F.nz + clampi(nk, 0, F.nz - 1);
+        dn[q] = F.op[cn].x;
+        bn[q] = F.b[cn];
+    }
+    if (!(o.x > 0.0f)) {
+        F.x[c] = 0.0f;
+        return 0.0f;
+    }
+    const float d = o.x;
+    const float xc = kMgOmega * bc / d;
+    F.x[c] = xc;
+    float lx = d * xc;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        const bool use = w[q] != 0.0f && in[q] && dn[q] > 0.0f;
+        const float dq = dn[q] + (use ? 0.0f : 1.0f);
+        lx -= (w[q] * (kMgOmega * bn[q] / dq)) * (float)use;
+    }
+    return bc - lx;
+}
+
 __device__ __forceinline__ void mg_down_body(const PcgState &st, const MgLevel &F, const MgLevel &C, int l, int blk) {
     if (solve_done(st.scal)) return;
     __shared__ float res[kPB];
@@ -1066,7 +1106,9 @@ __device__ __forceinline__ void mg_down_body(const PcgState &st, const MgLevel &
     const int bz = blk % tiles_z, by = (blk / tiles_z) % tiles_y, bx = blk / (tiles_z * tiles_y);
     const int i = 2 * bx + tx, j = 2 * by + ty, k = bz * kMgTileZ + tz;
     float rc = 0.0f;
-    if (i < F.nx && j < F.ny && k < F.nz) {
+    if (l >= 1) {
+        if (i < F.nx && j < F.ny && k < F.nz) rc = mg_down_cell_coarse(F, i, j, k);
+    } else if (i < F.nx && j < F.ny && k < F.nz) {
         const size_t c = idx3(i, j, k, F.ny, F.nz);
         const MgCell a = mg_get(st, F, l, i, j, k);
         if (a.unknown) {
@@ -1171,13 +1213,50 @@ __device__ __forceinline__ float mg_up_cell(const PcgState &st, const MgLevel &F
     }
     return xc + kMgOmega * (bc - lx) / a.d;
 }
+// The same on a level l >= 1 with every load issued at once (as mg_down_cell_coarse): the
+// operator records for the cell's weights, its x and correction, and each neighbour's record, x
+// and correction from indices clamped into the grid; a neighbour that is outside, not an unknown
+// or joined by a zero weight adds its term times 0 (every value finite).
+__device__ __forceinline__ float mg_up_cell_coarse(const MgLevel &F, const float *corr, int cny, int cnz, int c,
+                                                   float bc) {
+    const int k = c % F.nz, j = (c / F.nz) % F.ny, i = c / (F.ny * F.nz);
+    const int sx = F.ny * F.nz, sy = F.nz;
+    const float4 o = F.op[c];
+    const float4 oxm = F.op[i > 0 ? c - sx : c], oym = F.op[j > 0 ? c - sy : c], ozm = F.op[k > 0 ? c - 1 : c];
+    const float w[6] = {oxm.y * (float)(i > 0), o.y, oym.z * (float)(j > 0), o.z, ozm.w * (float)(k > 0), o.w};
+    const float x0 = F.x[c], cc = corr[((i >> 1) * cny + (j >> 1)) * cnz + (k >> 1)];
+    float dn[6], xn[6], crn[6];
+    bool in[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        int ni, nj, nk;
+        mg_neighbour(q, i, j, k, ni, nj, nk);
+        in[q] = ni >= 0 && nj >= 0 && nk >= 0 && ni < F.nx && nj < F.ny && nk < F.nz;
+        ni = clampi(ni, 0, F.nx - 1);
+        nj = clampi(nj, 0, F.ny - 1);
+        nk = clampi(nk, 0, F.nz - 1);
+        const int cn = (ni * F.ny + nj) * F.nz + nk;
+        dn[q] = F.op[cn].x;
+        xn[q] = F.x[cn];
+        crn[q] = corr[((ni >> 1) * cny + (nj >> 1)) * cnz + (nk >> 1)];
+    }
+    if (!(o.x > 0.0f)) return 0.0f;
+    const float xc = x0 + kMgCorrScale * cc;
+    float lx = o.x * xc;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        const bool use = w[q] != 0.0f && in[q] && dn[q] > 0.0f;
+        lx -= (w[q] * (xn[q] + kMgCorrScale * crn[q])) * (float)use;
+    }
+    return xc + kMgOmega * (bc - lx) / o.x;
+}
 // levels >= 1: one thread per cell, writes xs
 __device__ __forceinline__ void mg_up_body(const PcgState &st, const MgLevel &F, const float *corr, int cny, int cnz,
                                            int l, int blk) {
     if (solve_done(st.scal)) return;
     const size_t n = (size_t)F.nx * F.ny * F.nz;
     const size_t c = (size_t)blk * kPB + threadIdx.x;
-    if (c < n) F.xs[c] = mg_up_cell(st, F, corr, cny, cnz, l, c, F.b[c]);
+    if (c < n) F.xs[c] = l >= 1 ? mg_up_cell_coarse(F, corr, cny, cnz, (int)c, F.b[c]) : mg_up_cell(st, F, corr, cny, cnz, l, c, F.b[c]);
 }
 __global__ void __launch_bounds__(kPB) mg_up_kernel(PcgState st, MgLevel F, const float *corr, int cny, int cnz,
                                                     int l) {
