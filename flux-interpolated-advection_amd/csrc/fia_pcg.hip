@@ -1342,13 +1342,18 @@ __device__ __forceinline__ void mg_restrict0_body(const PcgState &st, const MgLe
     const size_t I = (size_t)blk * kPB + threadIdx.x;
     if (I >= n) return;
     const int K = (int)(I % C.nz), J = (int)((I / C.nz) % C.ny), II = (int)(I / ((size_t)C.ny * C.nz));
+    // the eight children loaded at once from clamped indices, a child outside the grid added
+    // times 0 (a load under the bounds test was waited for before the next was issued)
+    float v[8], m[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int i = 2 * II + (e >> 2), j = 2 * J + ((e >> 1) & 1), k = 2 * K + (e & 1);
+        m[e] = (float)(i < F.nx && j < F.ny && k < F.nz);
+        v[e] = F.b[idx3(min(i, F.nx - 1), min(j, F.ny - 1), min(k, F.nz - 1), F.ny, F.nz)];
+    }
     float sum = 0.0f;
-    for (int ci = 0; ci < 2; ++ci)
-        for (int cj = 0; cj < 2; ++cj)
-            for (int ck = 0; ck < 2; ++ck) {
-                const int i = 2 * II + ci, j = 2 * J + cj, k = 2 * K + ck;
-                if (i < F.nx && j < F.ny && k < F.nz) sum += F.b[idx3(i, j, k, F.ny, F.nz)];
-            }
+#pragma unroll
+    for (int e = 0; e < 8; ++e) sum += v[e] * m[e];
     C.b[I] = C.op[I].x > 0.0f ? sum : 0.0f;
 }
 __global__ void __launch_bounds__(kPB) mg_restrict0_kernel(PcgState st, MgLevel F, MgLevel C) {
