@@ -468,9 +468,11 @@ __device__ __forceinline__ void tile_sweep(const PcgState &st, int blk, const Op
 __device__ __forceinline__ void pcg_direction_body(const PcgState &st, int blk, int k) {
     const float *__restrict__ pold = st.p[k & 1];
     float *__restrict__ pnew = st.p[(k + 1) & 1];
-    if (solve_done(st.scal)) return;
+    // the converged flag read with the partials (one round trip), tested before any write
+    const double flag = st.scal[kPcgDone];
     double rz, rr, bb;
     residual_state(st, k, rz, rr, bb);
+    if (block_flag(flag)) return;
     const bool done = rr <= st.scal[kPcgTol2] * bb;
     const double beta = k == 0 ? 0.0 : rz / st.scal[kPcgRZ + ((k - 1) & 1)];
     if (blk == 0 && threadIdx.x == 0) {
@@ -527,9 +529,10 @@ __device__ __forceinline__ void update_stream(const PcgState &st, int blk, int k
 
 // Iteration k's update: alpha from the direction kernel's partials, then the solve's form.
 template <bool RECOMPUTE> __device__ __forceinline__ void pcg_update_body(const PcgState &st, int blk, int k) {
-    if (solve_done(st.scal)) return;
+    const double flag = st.scal[kPcgDone];   // read with the partials, as in the direction step
     double pq[1];
     total_of<1>(st.part_dir, st.dir_blocks, pq);
+    if (block_flag(flag)) return;
     const double alpha = pq[0] != 0.0 ? st.scal[kPcgRZ + (k & 1)] / pq[0] : 0.0;
     double acc[2] = {0.0, 0.0};
     if constexpr (RECOMPUTE) tile_sweep(st, blk, UpdOp{st, st.p[(k + 1) & 1], alpha}, acc);
